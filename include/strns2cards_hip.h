@@ -1,0 +1,186 @@
+/*
+ * strns2cards_hip.h — C ABI of the MI355X (gfx950) HIP library behind
+ * strns2cards_amd.  Plain pointers and sizes only: every array argument is a
+ * device pointer (HIP device memory, C-contiguous), every size an int, and
+ * `stream` is a hipStream_t passed as void* (NULL = the legacy default stream).
+ *
+ * Each entry point below replaces one launcher of the reference's
+ * `strns2cards::cuda` namespace, declared at
+ * pystrns2cards/src/pystrns2cards.cpp:13-192 (kenji-tojo/strns2cards) and
+ * bound to Python by the nanobind `_core` module at
+ * pystrns2cards/src/pystrns2cards.cpp:204-924.  Argument order and meaning
+ * are the reference's; two things differ on purpose:
+ *   - every function returns a status (S2C_OK, a negative S2C_ERR_*, or a
+ *     positive hipError_t) instead of calling exit() on a CUDA error
+ *     (cuda_common.cuh:11-14); s2c_last_error() gives the message;
+ *   - launchers that return a count in the reference write it through a
+ *     trailing `int* out` host pointer (the call then synchronizes `stream`,
+ *     as the reference's cudaMemcpy does).
+ * Temporaries are taken from HIP's stream-ordered allocator on `stream`.
+ * List outputs are produced in the stable (primitive-major) order.
+ */
+#ifndef STRNS2CARDS_HIP_H
+#define STRNS2CARDS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define S2C_OK 0
+#define S2C_ERR_ARG (-1)      /* invalid argument (shape, size, null) */
+#define S2C_ERR_OVERFLOW (-2) /* a count exceeds INT32_MAX */
+
+/* Library version as major*10000 + minor*100 + patch. */
+int s2c_version(void);
+/* Message of the last failing call on this host thread ("" if none). */
+const char* s2c_last_error(void);
+
+/* ---- triangle rasterization: rasterize.cu -------------------------------- */
+
+/* rasterize.cu:61-88 (compute_triangle_rects). Writes rects (B*T,4) =
+ * [h0,h_len,w0,w_len] and the inclusive prefix of h_len*w_len; *num_frags_out
+ * receives the candidate total (0 when B*T == 0). */
+int s2c_compute_triangle_rects(int H, int W, int B, int V, const float* pos,
+                               int T, const int* tri, int* triangle_rects,
+                               int* frag_prefix_sum, int* num_frags_out,
+                               void* stream);
+
+/* rasterize.cu:174-195 (compute_fragments). Candidates that fail the
+ * coverage / depth tests are not written (the caller pre-fills frag_pix with
+ * -1, rasterize.py:179). */
+int s2c_compute_fragments(int H, int W, int V, const float* pos, int T,
+                          const int* tri, int num_tris, int num_frags,
+                          const int* frag_prefix_sum, const int* triangle_rects,
+                          int* frag_pix, float* frag_attrs, void* stream);
+
+/* rasterize.cu:258-278 (depth_test). rast_out (B,H,W,4) is zero-filled by
+ * the caller; key = (bits(z+2) << 32) | fragment index, smallest wins. */
+int s2c_depth_test(int B, int H, int W, int num_fragments, const int* frag_pix,
+                   const float* frag_attrs, float* rast_out, void* stream);
+
+/* rasterize.cu:336-365 (filter_valid_fragments), stable order. */
+int s2c_filter_valid_fragments(int num_frags, const int* frag_pix,
+                               const float* frag_attrs, int* frag_pix_out,
+                               float* frag_attrs_out, int* valid_count_out,
+                               void* stream);
+
+/* rasterize.cu:440-455 (interpolate_triangle_attributes). */
+int s2c_interpolate_triangle_attributes(int B, int H, int W, int attr_dim,
+                                        const float* rast, const float* attr,
+                                        const int* tri, float* image,
+                                        void* stream);
+
+/* rasterize.cu:457-472 (backward_interpolate_triangle_attributes);
+ * accumulates into d_attr. */
+int s2c_backward_interpolate_triangle_attributes(int B, int H, int W,
+                                                 int attr_dim,
+                                                 const float* rast,
+                                                 const float* d_image,
+                                                 const int* tri, float* d_attr,
+                                                 void* stream);
+
+/* rasterize.cu:500-520 (cluster_mask_from_fragments); ORs into bitset. */
+int s2c_cluster_mask_from_fragments(int num_frags, const int* frag_pix,
+                                    const float* frag_attrs,
+                                    const int* prim2cluster, int H, int W,
+                                    int num_slots, int* bitset, void* stream);
+
+/* ---- bitset utilities: bitset_utils.cu ----------------------------------- */
+
+/* bitset_utils.cu:55-68 (accumulate_bitset_rgb). */
+int s2c_accumulate_bitset_rgb(int B, int H, int W, int num_slots,
+                              const int* bitset, const float* cluster_rgb,
+                              float* accum_color, void* stream);
+
+/* bitset_utils.cu:70-84 (popcount_bitset). */
+int s2c_popcount_bitset(int B, int H, int W, int num_slots, const int* bitset,
+                        float* count_image, void* stream);
+
+/* ---- DDA line rasterization: dda.cu --------------------------------------- */
+
+/* dda.cu:89-122 (dda_compute_span). edge_ids[i] = b*E + e. */
+int s2c_dda_compute_span(int num_prims, const int* edge_ids, int H, int W,
+                         int V, const float* pos, int E, const int* edges,
+                         int* frag_prefix_sum, float* frag_slopes,
+                         float* frag_spans, int* num_frags_out, void* stream);
+
+/* dda.cu:227-256 (dda_compute_fragments). Failing candidates not written. */
+int s2c_dda_compute_fragments(int num_prims, int num_frags,
+                              const int* frag_prefix_sum, const int* edge_ids,
+                              const float* frag_slopes, const float* frag_spans,
+                              int H, int W, int V, const float* pos, int E,
+                              const int* edges, int* frag_pix,
+                              float* frag_attrs, void* stream);
+
+/* dda.cu:286-298 (dda_interpolate_attributes). */
+int s2c_dda_interpolate_attributes(int B, int H, int W, int C,
+                                   const float* rast_dda, const float* attr,
+                                   const int* edges, float* image,
+                                   void* stream);
+
+/* dda.cu:329-342 (backward_dda_interpolate_attributes); accumulates. */
+int s2c_backward_dda_interpolate_attributes(int B, int H, int W,
+                                            const float* rast_dda, int C,
+                                            const float* d_image,
+                                            const int* edges, float* d_attr,
+                                            void* stream);
+
+/* ---- antialias: antialias.cu, compact_utils.cu ---------------------------- */
+
+/* antialias.cu:97-119 (mark_discontinuity_edges). prim_ids (B*E) is
+ * -1-filled and normals (B*E,2) zero-filled by the caller. */
+int s2c_mark_discontinuity_edges(int H, int W, int B, int V, const float* pos,
+                                 const int* tri, int E, const int* edges,
+                                 const int* edge2tri, int* prim_ids,
+                                 float* normals, void* stream);
+
+/* antialias.cu:282-317 (backward_antialiased_cluster_bitset); accumulates
+ * into d_pos (B,V,4). */
+int s2c_backward_antialiased_cluster_bitset(
+    int num_frags, const int* frag_pix, const float* frag_attrs_dda, int H,
+    int W, int num_slots, const int* bitset, const int* target_bitset, int V,
+    const float* pos, int E, const int* edges, const float* normals,
+    const int* edge2cluster, float* d_pos, float kernel_radius, float rho,
+    void* stream);
+
+/* compact_utils.cu:54-77 (compact_valid_ints), stable order. */
+int s2c_compact_valid_ints(int N, const int* input, int* output,
+                           int* valid_count_out, void* stream);
+
+/* ---- strands: curve_utils.cu ---------------------------------------------- */
+
+/* curve_utils.cu:135-147 (smooth_strands). Entries whose weight sum is
+ * <= 1e-6 are left untouched (the caller zero-fills, curve_utils.py:203). */
+int s2c_smooth_strands(int num_strands, int num_points, const float* strands,
+                       const float* arclength, float* smoothed_strands,
+                       float beta, void* stream);
+
+/* curve_utils.cu:149-166 (skinning). */
+int s2c_skinning(int num_strands, int num_points, const float* strands,
+                 const float* arclength, const float* handle_strands,
+                 const float* canonical_handles, const float* R,
+                 float* skinned_strands, float beta, void* stream);
+
+/* ---- fused forms of the reference's call sequences ------------------------
+ * No single reference launcher corresponds; each equals the sequence cited,
+ * with no fragment list in memory and no host synchronization. */
+
+/* dda.py:5-76 then rasterize.py:170-214, as used at 3_fitting.py:254-261:
+ * all B*E edges DDA-rasterized and OR-ed into bitset (B,H,W,num_slots),
+ * which this call clears first.  Equal to the reference's mask on every view
+ * (views without fragments are all zero, where the reference shrinks B). */
+int s2c_dda_cluster_mask(int B, int H, int W, int V, const float* pos, int E,
+                         const int* edges, const int* edge2cluster,
+                         int num_slots, int* bitset, void* stream);
+
+/* rasterize.py:6-53 then rasterize.py:170-214, as used at 3_fitting.py:154-163
+ * (render_bitset_from_triangles); clears bitset first. */
+int s2c_triangle_cluster_mask(int B, int H, int W, int V, const float* pos,
+                              int T, const int* tri, const int* tri2cluster,
+                              int num_slots, int* bitset, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STRNS2CARDS_HIP_H */

@@ -1,0 +1,336 @@
+"""CPU oracle for the pystrns2cards hot path — TEST INFRASTRUCTURE ONLY.
+
+numpy front end over ``liboracle.so`` (built from ``s2c_oracle.c`` by
+``oracle/Makefile``), a plain-C, element-for-element restatement of the
+reference CUDA kernels (kenji-tojo/strns2cards, pystrns2cards/src/*.cu).
+Only tests, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg may
+use it, and only as the checker / the timed CPU path; the product never does.
+
+Functions take and return numpy arrays and follow the reference's *Python*
+API semantics (allocation, -1 fill, stable compaction), e.g.
+``dda_compute_fragments`` = dda.py:5-76 on the CPU.
+
+Parity pins live in tests/test_oracle_*.py.  Pinned by closed forms / brute
+force / invariants: DDA span+fragments, triangle raster, depth test, cluster
+masks, bitset RGB/popcount, silhouette marking, compaction, smoothing,
+skinning.  The antialias backward is pinned by hand-computable cases (one
+straight edge, translation equivariance, rho scaling) — see its tests.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_I = ctypes.c_int
+_F = ctypes.c_float
+
+_SIGS = {
+    "orc_dda_compute_span": (_I, [_I, _i32p, _I, _I, _I, _f32p, _I, _i32p, _i32p, _f32p, _f32p]),
+    "orc_dda_compute_fragments": (None, [_I, _I, _i32p, _i32p, _f32p, _f32p, _I, _I, _I, _f32p,
+                                         _I, _i32p, _i32p, _f32p]),
+    "orc_dda_interpolate_attributes": (None, [_I, _I, _I, _I, _f32p, _f32p, _i32p, _f32p]),
+    "orc_backward_dda_interpolate_attributes": (None, [_I, _I, _I, _f32p, _I, _f32p, _i32p, _f32p]),
+    "orc_compute_triangle_rects": (_I, [_I, _I, _I, _I, _f32p, _I, _i32p, _i32p, _i32p]),
+    "orc_compute_fragments": (None, [_I, _I, _I, _f32p, _I, _i32p, _I, _I, _i32p, _i32p, _i32p,
+                                     _f32p]),
+    "orc_depth_test": (None, [_I, _I, _I, _I, _i32p, _f32p, _f32p]),
+    "orc_filter_valid_fragments": (_I, [_I, _i32p, _f32p, _i32p, _f32p]),
+    "orc_interpolate_triangle_attributes": (None, [_I, _I, _I, _I, _f32p, _f32p, _i32p, _f32p]),
+    "orc_backward_interpolate_triangle_attributes": (None, [_I, _I, _I, _I, _f32p, _f32p, _i32p,
+                                                            _f32p]),
+    "orc_cluster_mask_from_fragments": (None, [_I, _i32p, _f32p, _i32p, _I, _I, _I, _i32p]),
+    "orc_accumulate_bitset_rgb": (None, [_I, _I, _I, _I, _i32p, _f32p, _f32p]),
+    "orc_popcount_bitset": (None, [_I, _I, _I, _I, _i32p, _f32p]),
+    "orc_mark_discontinuity_edges": (None, [_I, _I, _I, _I, _f32p, _i32p, _I, _i32p, _i32p,
+                                            _i32p, _f32p]),
+    "orc_compact_valid_ints": (_I, [_I, _i32p, _i32p]),
+    "orc_backward_antialiased_cluster_bitset": (None, [_I, _i32p, _f32p, _I, _I, _I, _i32p, _i32p,
+                                                       _I, _f32p, _I, _i32p, _f32p, _i32p, _f32p,
+                                                       _F, _F]),
+    "orc_smooth_strands": (None, [_I, _I, _f32p, _f32p, _f32p, _F]),
+    "orc_skinning": (None, [_I, _I, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _F]),
+    "orc_dda_cluster_mask": (None, [_I, _I, _I, _I, _f32p, _I, _i32p, _i32p, _I, _i32p, _I]),
+    "orc_triangle_cluster_mask": (None, [_I, _I, _I, _I, _f32p, _I, _i32p, _i32p, _I, _i32p]),
+}
+
+
+def build(force: bool = False) -> str:
+    """Compiles liboracle.so with oracle/Makefile (gcc)."""
+    src = os.path.join(HERE, "s2c_oracle.c")
+    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def _f(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# ------------------------------------------------------------------ DDA
+
+def dda_compute_span(resolution, pos, edges, edge_ids=None):
+    """dda.cu:89-122 → (prefix, slopes, spans, total).  Entries of slopes /
+    spans for zero-length edges are NaN (the reference leaves them unset)."""
+    H, W = resolution
+    pos, edges = _f(pos), _i(edges)
+    B, V, _ = pos.shape
+    E = edges.shape[0]
+    ids = _i(np.arange(B * E) if edge_ids is None else edge_ids)
+    P = ids.shape[0]
+    prefix = np.zeros(P, np.int32)
+    slopes = np.full((P, 2), np.nan, np.float32)
+    spans = np.full((P, 4), np.nan, np.float32)
+    total = lib().orc_dda_compute_span(P, ids, H, W, V, pos, E, edges, prefix, slopes, spans)
+    return prefix, slopes, spans, int(total)
+
+
+def dda_compute_fragments(resolution, pos, edges, edge_ids=None):
+    """dda.py:5-76: (frag_pix (N,3) int32, frag_attrs (N,4) float32), stable."""
+    H, W = resolution
+    pos, edges = _f(pos), _i(edges)
+    B, V, _ = pos.shape
+    E = edges.shape[0]
+    ids = _i(np.arange(B * E) if edge_ids is None else edge_ids)
+    prefix, slopes, spans, n = dda_compute_span(resolution, pos, edges, ids)
+    fp = np.full((n, 3), -1, np.int32)
+    fa = np.zeros((n, 4), np.float32)
+    lib().orc_dda_compute_fragments(ids.shape[0], n, prefix, ids, slopes, spans, H, W, V, pos, E,
+                                    edges, fp, fa)
+    return filter_valid_fragments(fp, fa)
+
+
+def dda_interpolate_attributes(rast, attr, edges):
+    rast, attr, edges = _f(rast), _f(attr), _i(edges)
+    B, H, W, _ = rast.shape
+    C = attr.shape[1]
+    img = np.zeros((B, H, W, C), np.float32)
+    lib().orc_dda_interpolate_attributes(B, H, W, C, rast, attr, edges, img)
+    return img
+
+
+def backward_dda_interpolate_attributes(rast, d_image, edges, num_vertices):
+    rast, d_image, edges = _f(rast), _f(d_image), _i(edges)
+    B, H, W, _ = rast.shape
+    C = d_image.shape[3]
+    d_attr = np.zeros((num_vertices, C), np.float32)
+    lib().orc_backward_dda_interpolate_attributes(B, H, W, rast, C, d_image, edges, d_attr)
+    return d_attr
+
+
+def dda_cluster_mask(resolution, pos, edges, edge2cluster, cluster_count, threads=1):
+    """dda.py:5-76 + rasterize.py:170-214 as one pass; B = pos.shape[0]."""
+    H, W = resolution
+    pos, edges, e2c = _f(pos), _i(edges), _i(edge2cluster)
+    B, V, _ = pos.shape
+    S = (cluster_count + 31) // 32
+    bits = np.zeros((B, H, W, S), np.int32)
+    lib().orc_dda_cluster_mask(B, H, W, V, pos, edges.shape[0], edges, e2c, S, bits, int(threads))
+    return bits
+
+
+# ------------------------------------------------------------- triangles
+
+def compute_triangle_rects(resolution, pos, tri):
+    H, W = resolution
+    pos, tri = _f(pos), _i(tri)
+    B, V, _ = pos.shape
+    T = tri.shape[0]
+    rects = np.zeros((B * T, 4), np.int32)
+    prefix = np.zeros(B * T, np.int32)
+    n = lib().orc_compute_triangle_rects(H, W, B, V, pos, T, tri, rects, prefix)
+    return rects, prefix, int(n)
+
+
+def compute_fragments(resolution, pos, tri):
+    """rasterize.py:6-53 on the CPU, stable order."""
+    H, W = resolution
+    pos, tri = _f(pos), _i(tri)
+    B, V, _ = pos.shape
+    T = tri.shape[0]
+    rects, prefix, n = compute_triangle_rects(resolution, pos, tri)
+    fp = np.full((n, 3), -1, np.int32)
+    fa = np.zeros((n, 4), np.float32)
+    lib().orc_compute_fragments(H, W, V, pos, T, tri, B * T, n, prefix, rects, fp, fa)
+    return filter_valid_fragments(fp, fa)
+
+
+def filter_valid_fragments(fp, fa):
+    fp, fa = _i(fp), _f(fa)
+    n = fp.shape[0]
+    fp_out = np.zeros((n, 3), np.int32)
+    fa_out = np.zeros((n, 4), np.float32)
+    m = lib().orc_filter_valid_fragments(n, fp, fa, fp_out, fa_out)
+    return fp_out[:m].copy(), fa_out[:m].copy()
+
+
+def depth_test(B, resolution, frag_pix, frag_attrs):
+    H, W = resolution
+    fp, fa = _i(frag_pix), _f(frag_attrs)
+    rast = np.zeros((B, H, W, 4), np.float32)
+    lib().orc_depth_test(B, H, W, fp.shape[0], fp, fa, rast)
+    return rast
+
+
+def rasterize(resolution, pos, tri):
+    fp, fa = compute_fragments(resolution, pos, tri)
+    return depth_test(np.asarray(pos).shape[0], resolution, fp, fa)
+
+
+def interpolate_triangle_attributes(rast, attr, tri):
+    rast, attr, tri = _f(rast), _f(attr), _i(tri)
+    B, H, W, _ = rast.shape
+    C = attr.shape[1]
+    img = np.zeros((B, H, W, C), np.float32)
+    lib().orc_interpolate_triangle_attributes(B, H, W, C, rast, attr, tri, img)
+    return img
+
+
+def backward_interpolate_triangle_attributes(rast, d_image, tri, num_vertices):
+    rast, d_image, tri = _f(rast), _f(d_image), _i(tri)
+    B, H, W, _ = rast.shape
+    C = d_image.shape[3]
+    d_attr = np.zeros((num_vertices, C), np.float32)
+    lib().orc_backward_interpolate_triangle_attributes(B, H, W, C, rast, d_image, tri, d_attr)
+    return d_attr
+
+
+def cluster_mask_from_fragments(resolution, frag_pix, frag_attrs, prim2cluster, cluster_count=None,
+                                B=None):
+    """rasterize.py:170-214; B defaults to frag_pix[:,0].max()+1."""
+    H, W = resolution
+    fp, fa, p2c = _i(frag_pix), _f(frag_attrs), _i(prim2cluster)
+    if B is None:
+        B = int(fp[:, 0].max()) + 1
+    if cluster_count is None:
+        cluster_count = int(p2c.max()) + 1
+    S = (cluster_count + 31) // 32
+    bits = np.zeros((B, H, W, S), np.int32)
+    lib().orc_cluster_mask_from_fragments(fp.shape[0], fp, fa, p2c, H, W, S, bits)
+    return bits
+
+
+def triangle_cluster_mask(resolution, pos, tri, tri2cluster, cluster_count):
+    H, W = resolution
+    pos, tri, t2c = _f(pos), _i(tri), _i(tri2cluster)
+    B, V, _ = pos.shape
+    S = (cluster_count + 31) // 32
+    bits = np.zeros((B, H, W, S), np.int32)
+    lib().orc_triangle_cluster_mask(B, H, W, V, pos, tri.shape[0], tri, t2c, S, bits)
+    return bits
+
+
+# ---------------------------------------------------------------- bitset
+
+def accumulate_bitset_rgb(bitset, rgb):
+    bits, rgb = _i(bitset), _f(rgb)
+    B, H, W, S = bits.shape
+    out = np.zeros((B, H, W, 3), np.float32)
+    lib().orc_accumulate_bitset_rgb(B, H, W, S, bits, rgb, out)
+    return out
+
+
+def popcount_bitset(bitset):
+    bits = _i(bitset)
+    B, H, W, S = bits.shape
+    out = np.zeros((B, H, W), np.float32)
+    lib().orc_popcount_bitset(B, H, W, S, bits, out)
+    return out
+
+
+# ------------------------------------------------------------- antialias
+
+def mark_discontinuity_edges(resolution, pos, tri, edges, edge2tri):
+    """antialias.py:8-52 → (prim_ids, normals)."""
+    H, W = resolution
+    pos, tri, edges, e2t = _f(pos), _i(tri), _i(edges), _i(edge2tri)
+    B, V, _ = pos.shape
+    E = edges.shape[0]
+    prim = np.full(B * E, -1, np.int32)
+    normals = np.zeros((B * E, 2), np.float32)
+    lib().orc_mark_discontinuity_edges(H, W, B, V, pos, tri, E, edges, e2t, prim, normals)
+    return compact_valid_ints(prim), normals
+
+
+def compact_valid_ints(a):
+    a = _i(a)
+    out = np.zeros_like(a)
+    n = lib().orc_compact_valid_ints(a.shape[0], a, out)
+    return out[:n].copy()
+
+
+def backward_antialiased_cluster_bitset(frag_pix, frag_attrs, bitset, target, pos, edges, normals,
+                                        edge2cluster, kernel_radius=7.0, rho=1.0):
+    fp, fa = _i(frag_pix), _f(frag_attrs)
+    bits, tgt = _i(bitset), _i(target)
+    pos, edges, normals, e2c = _f(pos), _i(edges), _f(normals), _i(edge2cluster)
+    B, H, W, S = bits.shape
+    V = pos.shape[1]
+    d_pos = np.zeros_like(pos)
+    lib().orc_backward_antialiased_cluster_bitset(fp.shape[0], fp, fa, H, W, S, bits, tgt, V, pos,
+                                                  edges.shape[0], edges, normals, e2c, d_pos,
+                                                  float(kernel_radius), float(rho))
+    return d_pos
+
+
+def antialiased_cluster_bitset_grad(bitset, target, pos, tri, edges, edge2tri, edge2cluster,
+                                    kernel_radius=7.0, rho=1.0):
+    """d_pos of antialias.py:66-130 (the gradient its loss carries)."""
+    H, W = np.asarray(target).shape[1:3]
+    prim, normals = mark_discontinuity_edges((H, W), pos, tri, edges, edge2tri)
+    fp, fa = dda_compute_fragments((H, W), pos, edges, prim)
+    return backward_antialiased_cluster_bitset(fp, fa, bitset, target, pos, edges, normals,
+                                               edge2cluster, kernel_radius, rho)
+
+
+# ------------------------------------------------------------------ curves
+
+def compute_arclength(strands):
+    """curve_utils.py:167-180 in float32."""
+    x = _f(strands)
+    seg = np.sqrt(((x[:, 1:] - x[:, :-1]) ** 2).sum(-1, dtype=np.float32)).astype(np.float32)
+    return np.concatenate([np.zeros((x.shape[0], 1), np.float32), np.cumsum(seg, 1, dtype=np.float32)],
+                          1).astype(np.float32)
+
+
+def smooth_strands(strands, arclength, beta):
+    x, s = _f(strands), _f(arclength)
+    N, P, _ = x.shape
+    out = np.zeros_like(x)
+    lib().orc_smooth_strands(N, P, x, s, out, float(beta))
+    return out
+
+
+def skinning(strands, arclength, handle, canonical, R, beta):
+    x, s, h, c, r = _f(strands), _f(arclength), _f(handle), _f(canonical), _f(R)
+    N, P, _ = x.shape
+    out = np.zeros_like(x)
+    lib().orc_skinning(N, P, x, s, h, c, r, out, float(beta))
+    return out

@@ -1,0 +1,34 @@
+"""strns2cards_amd — the pystrns2cards hot path on AMD MI355X (gfx950).
+
+The public API mirrors the reference package ``pystrns2cards``
+(kenji-tojo/strns2cards, pystrns2cards/__init__.py:1-14): every function of
+``dda``, ``rasterize``, ``antialias``, ``bitset_utils`` and ``curve_utils`` is
+re-exported at the top level.  All GPU work runs in hand-written HIP kernels
+in ``libstrns2cards_hip.so`` (C ABI: include/strns2cards_hip.h), reached
+through the ``_core`` extension; there is no CPU fallback, and importing the
+package fails if the native libraries are not built
+(``python -m strns2cards_amd.build``).
+"""
+import os as _os
+
+_HERE = _os.path.dirname(_os.path.abspath(__file__))
+LIBRARY_PATH = _os.path.join(_HERE, "libstrns2cards_hip.so")
+
+try:
+    import torch as _torch  # noqa: F401  (loads the HIP runtime torch links)
+    from . import _core
+except ImportError as _e:  # pragma: no cover - exercised only on broken installs
+    raise ImportError(
+        "strns2cards_amd: native extension not built or not loadable "
+        f"({_e}); run `python -m strns2cards_amd.build`") from _e
+
+from .antialias import *  # noqa: E402,F401,F403
+from .bitset_utils import *  # noqa: E402,F401,F403
+from .curve_utils import *  # noqa: E402,F401,F403
+from .dda import *  # noqa: E402,F401,F403
+from .rasterize import *  # noqa: E402,F401,F403
+
+
+def print_version() -> None:
+    """Prints the native library version (reference: __init__.py:16-17)."""
+    _core.print_version()

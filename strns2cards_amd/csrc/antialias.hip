@@ -1,0 +1,205 @@
+// Silhouette-edge marking and the antialiased cluster-bitset edge gradient
+// (reference: pystrns2cards/src/antialias.cu).
+#include "common.hpp"
+
+namespace s2c {
+
+// antialias.cu:8-15.
+__device__ __forceinline__ int third_vertex(const int* tri, int t, int v0, int v1) {
+    const int i0 = tri[t * 3 + 0], i1 = tri[t * 3 + 1], i2 = tri[t * 3 + 2];
+    return (i0 != v0 && i0 != v1) ? i0 : (i1 != v0 && i1 != v1) ? i1
+         : (i2 != v0 && i2 != v1) ? i2 : -1;
+}
+
+// antialias.cu:18-35.
+__device__ __forceinline__ void project(const float* p, int v, float fW, float fH, float& x,
+                                        float& y) {
+    const float4 q = reinterpret_cast<const float4*>(p)[v];
+    const float nx = q.x / q.w;
+    const float ny = q.y / q.w;
+    x = fW * 0.5f * (1.f + nx);
+    y = fH * 0.5f * (1.f - ny);
+}
+
+// antialias.cu:38-94.
+__global__ void __launch_bounds__(kBlock)
+mark_edges_kernel(int H, int W, long long BE, int V, const float* __restrict__ pos,
+                  const int* __restrict__ tri, int E, const int* __restrict__ edges,
+                  const int* __restrict__ edge2tri, int* __restrict__ prim_ids,
+                  float* __restrict__ normals) {
+    const long long idx = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (idx >= BE) return;
+    const int b = static_cast<int>(idx / E);
+    const int e = static_cast<int>(idx - static_cast<long long>(b) * E);
+    const float* p = pos + static_cast<long long>(b) * V * 4;
+    const float fH = static_cast<float>(H), fW = static_cast<float>(W);
+    const int v0 = edges[e * 2 + 0], v1 = edges[e * 2 + 1];
+    float x0, y0, x1, y1;
+    project(p, v0, fW, fH, x0, y0);
+    project(p, v1, fW, fH, x1, y1);
+    const float ex = x1 - x0, ey = y1 - y0;
+    const float len = sqrtf(ex * ex + ey * ey);
+    if (len == 0.f) return;
+    const float nx = -ey / len;
+    const float ny = ex / len;
+    float d0 = 0.f, d1 = 0.f;
+    const int t0 = edge2tri[e * 2 + 0], t1 = edge2tri[e * 2 + 1];
+    if (t0 >= 0) {
+        float x2, y2;
+        project(p, third_vertex(tri, t0, v0, v1), fW, fH, x2, y2);
+        d0 = nx * (x2 - x0) + ny * (y2 - y0);
+    }
+    if (t1 >= 0) {
+        float x2, y2;
+        project(p, third_vertex(tri, t1, v0, v1), fW, fH, x2, y2);
+        d1 = nx * (x2 - x0) + ny * (y2 - y0);
+    }
+    if (d0 * d1 >= 0.f) prim_ids[idx] = static_cast<int>(idx);
+    const float sign = 2.f * (.5f - static_cast<float>(d0 > 0.f || d1 > 0.f));
+    reinterpret_cast<float2*>(normals)[idx] = make_float2(sign * nx, sign * ny);
+}
+
+// antialias.cu:121-132 with HIP's accurate expf.
+__device__ __forceinline__ float gauss_w(float x, float y, int w, int h, float R) {
+    constexpr float pi = 3.14159265358979323846f;
+    const float sigma = R / 4.f;
+    const float c = 1.f / (sigma * sigma * 2.f * pi);
+    const float alpha = 0.5f / (sigma * sigma);
+    const float dx = static_cast<float>(w) + 0.5f - x;
+    const float dy = static_cast<float>(h) + 0.5f - y;
+    const float n2 = dx * dx + dy * dy;
+    return c * expf(-alpha * n2) * static_cast<float>(n2 < R * R);
+}
+
+__device__ __forceinline__ float bit_at(const int* bs, long long pix, int S, int k) {
+    return static_cast<float>((static_cast<unsigned>(bs[pix * S + (k >> 5)]) >> (k & 31)) & 1u);
+}
+
+// antialias.cu:147-176.
+__device__ float adjoint_sum(int b, float x, float y, int k, int H, int W, int S,
+                             const int* __restrict__ bs, const int* __restrict__ tg, float R,
+                             float rho) {
+    const int Ri = f2i(floorf(R));
+    const int cw = f2i(floorf(x)), ch = f2i(floorf(y));
+    const int h0 = dmax(ch - Ri, 0), h1 = dmin(ch + Ri, H - 1) + 1;
+    const int w0 = dmax(cw - Ri, 0), w1 = dmin(cw + Ri, W - 1) + 1;
+    const long long plane = static_cast<long long>(H) * W * b;
+    float dc = 0.f;
+    for (int h = h0; h < h1; ++h) {
+        for (int w = w0; w < w1; ++w) {
+            const long long pix = plane + w + static_cast<long long>(h) * W;
+            const float c0 = bit_at(bs, pix, S, k);
+            const float c1 = bit_at(tg, pix, S, k);
+            const float wt = gauss_w(x, y, w, h, R);
+            dc += wt * (c0 - c1) * (c1 == 0.f ? rho : 1.f);
+        }
+    }
+    return dc;
+}
+
+// antialias.cu:179-279.
+__global__ void __launch_bounds__(kBlock)
+antialias_bwd_kernel(int N, const int* __restrict__ frag_pix, const float* __restrict__ fa,
+                     int H, int W, int S, const int* __restrict__ bitset,
+                     const int* __restrict__ target, int V, const float* __restrict__ pos, int E,
+                     const int* __restrict__ edges, const float* __restrict__ normals,
+                     const int* __restrict__ edge2cluster, float* __restrict__ d_pos, float R,
+                     float rho) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const int b = frag_pix[i * 3 + 0];
+    const int h = frag_pix[i * 3 + 1];
+    const int w = frag_pix[i * 3 + 2];
+    const float4 a = reinterpret_cast<const float4*>(fa)[i];
+    const float axis = a.x;
+    const float t1 = a.y;
+    const float t0 = 1.f - t1;
+    const int e = f2i(a.w) - 1;
+    if (e < 0) return;
+    const float* p = pos + static_cast<long long>(b) * V * 4;
+    const float* nb = normals + static_cast<long long>(b) * E * 2;
+    float* dp = d_pos + static_cast<long long>(b) * V * 4;
+    const float fH = static_cast<float>(H), fW = static_cast<float>(W);
+    const int v0 = edges[e * 2 + 0], v1 = edges[e * 2 + 1];
+    const float4 P0 = reinterpret_cast<const float4*>(p)[v0];
+    const float4 P1 = reinterpret_cast<const float4*>(p)[v1];
+    const float px = t0 * P0.x + t1 * P1.x;
+    const float py = t0 * P0.y + t1 * P1.y;
+    const float pw = t0 * P0.w + t1 * P1.w;
+    const float fx = 0.5f * (1.f + px / pw) * fW;
+    const float fy = 0.5f * (1.f - py / pw) * fH;
+    const float2 n = reinterpret_cast<const float2*>(nb)[e];
+    int nh = h, nw = w;
+    const float fxr = fx - floorf(fx);
+    const float fyr = fy - floorf(fy);
+    constexpr float eps = 1e-4f;
+    float dot = 0.f;
+    if (axis == 0.f) {
+        if (!isfinite(fxr) || fabsf(fxr - 0.5f) >= eps) return;
+        const int d = 2 * static_cast<int>(fyr > 0.5f) - 1;
+        nh += d;
+        dot += static_cast<float>(d) * n.y;
+    } else {
+        if (!isfinite(fyr) || fabsf(fyr - 0.5f) >= eps) return;
+        const int d = 2 * static_cast<int>(fxr > 0.5f) - 1;
+        nw += d;
+        dot += static_cast<float>(d) * n.x;
+    }
+    if (nw < 0 || nw >= W || nh < 0 || nh >= H || dot == 0.f) return;
+    const int k = edge2cluster[e];
+    const long long plane = static_cast<long long>(H) * W * b;
+    const float c01 = (dot > 0.f)
+        ? 1.f - bit_at(bitset, plane + nw + static_cast<long long>(nh) * W, S, k)
+        : 1.f - bit_at(bitset, plane + w + static_cast<long long>(h) * W, S, k);
+    const float dc = c01 * adjoint_sum(b, fx, fy, k, H, W, S, bitset, target, R, rho);
+    if (dc == 0.f) return;
+    const float dx = dc * 0.5f * fW * n.x;
+    const float dy = dc * -0.5f * fH * n.y;
+    const float dpx = dx / pw;
+    const float dpy = dy / pw;
+    const float dpw = (dx * px + dy * py) / (pw * pw);
+    if (isfinite(dpx) && isfinite(dpy) && isfinite(dpw)) {
+        atomicAdd(&dp[v0 * 4 + 0], dpx * t0);
+        atomicAdd(&dp[v0 * 4 + 1], dpy * t0);
+        atomicAdd(&dp[v0 * 4 + 3], dpw * t0);
+        atomicAdd(&dp[v1 * 4 + 0], dpx * t1);
+        atomicAdd(&dp[v1 * 4 + 1], dpy * t1);
+        atomicAdd(&dp[v1 * 4 + 3], dpw * t1);
+    }
+}
+
+}  // namespace s2c
+
+using namespace s2c;
+
+extern "C" int s2c_mark_discontinuity_edges(int H, int W, int B, int V, const float* pos,
+                                            const int* tri, int E, const int* edges,
+                                            const int* edge2tri, int* prim_ids, float* normals,
+                                            void* stream) {
+    S2C_CHECK(H >= 0 && W >= 0 && B >= 0 && V >= 0 && E >= 0,
+              "mark_discontinuity_edges: negative size");
+    const long long BE = static_cast<long long>(B) * E;
+    S2C_CHECK(BE <= INT_MAX, "mark_discontinuity_edges: B*E exceeds INT32_MAX");
+    if (BE == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    mark_edges_kernel<<<grid_for(BE, kBlock), kBlock, 0, s>>>(H, W, BE, V, pos, tri, E, edges,
+                                                              edge2tri, prim_ids, normals);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
+
+extern "C" int s2c_backward_antialiased_cluster_bitset(
+    int num_frags, const int* frag_pix, const float* frag_attrs_dda, int H, int W, int num_slots,
+    const int* bitset, const int* target_bitset, int V, const float* pos, int E,
+    const int* edges, const float* normals, const int* edge2cluster, float* d_pos,
+    float kernel_radius, float rho, void* stream) {
+    S2C_CHECK(num_frags >= 0 && H >= 0 && W >= 0 && num_slots >= 0 && V >= 0 && E >= 0,
+              "backward_antialiased_cluster_bitset: negative size");
+    if (num_frags == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    antialias_bwd_kernel<<<grid_for(num_frags, kBlock), kBlock, 0, s>>>(
+        num_frags, frag_pix, frag_attrs_dda, H, W, num_slots, bitset, target_bitset, V, pos, E,
+        edges, normals, edge2cluster, d_pos, kernel_radius, rho);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}

@@ -1,0 +1,541 @@
+// `_core`: the PyTorch-ROCm binding of the C ABI in include/strns2cards_hip.h.
+//
+// Same module name, function names and argument order as the reference's
+// nanobind module (pystrns2cards/src/pystrns2cards.cpp:204-924), so the
+// Python wrappers read like the reference's.  Each function validates
+// dtype / rank / contiguity (TypeError, as nanobind's casters do) and
+// device / shape agreement (RuntimeError, as the reference's explicit
+// checks do), then calls the C ABI on the current HIP stream of the
+// tensor's device.  HIP failures raise RuntimeError instead of exit().
+#include <torch/extension.h>
+#include <c10/hip/HIPStream.h>
+#include <c10/hip/HIPGuard.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/strns2cards_hip.h"
+
+namespace py = pybind11;
+using at::Tensor;
+
+namespace {
+
+const char* dtype_name(at::ScalarType t) {
+    return t == at::kFloat ? "float32" : t == at::kInt ? "int32" : "other";
+}
+
+// Shape spec: -1 = any extent.
+void need(const Tensor& t, const char* fn, const char* name, at::ScalarType dt,
+          std::initializer_list<int64_t> shape) {
+    if (t.scalar_type() != dt)
+        throw py::type_error(std::string(fn) + ": `" + name + "` must be " + dtype_name(dt));
+    if (t.dim() != static_cast<int64_t>(shape.size()))
+        throw py::type_error(std::string(fn) + ": `" + name + "` must have " +
+                             std::to_string(shape.size()) + " dimensions");
+    int i = 0;
+    for (int64_t s : shape) {
+        if (s >= 0 && t.size(i) != s)
+            throw py::type_error(std::string(fn) + ": `" + name + "` has a wrong extent in dim " +
+                                 std::to_string(i));
+        ++i;
+    }
+    if (!t.is_contiguous())
+        throw py::type_error(std::string(fn) + ": `" + name + "` must be C-contiguous");
+    if (!t.is_cuda())
+        throw std::runtime_error(std::string(fn) + ": all tensors must be GPU tensors");
+}
+
+void same_device(const char* fn, std::initializer_list<const Tensor*> ts) {
+    const Tensor* first = *ts.begin();
+    for (const Tensor* t : ts)
+        if (t->device() != first->device())
+            throw std::runtime_error(std::string(fn) + ": tensors are on different devices");
+}
+
+void check(int rc, const char* fn) {
+    if (rc != S2C_OK) throw std::runtime_error(std::string(fn) + ": " + s2c_last_error());
+}
+
+void* stream_of(const Tensor& t) {
+    return static_cast<void*>(c10::hip::getCurrentHIPStream(t.device().index()).stream());
+}
+
+int I(int64_t v, const char* fn) {
+    if (v > INT32_MAX) throw std::runtime_error(std::string(fn) + ": size exceeds INT32_MAX");
+    return static_cast<int>(v);
+}
+
+template <class T> T* P(const Tensor& t) { return static_cast<T*>(t.data_ptr()); }
+
+}  // namespace
+
+PYBIND11_MODULE(_core, m) {
+    m.doc() = "strns2cards_amd native core (gfx950 HIP)";
+
+    m.def("print_version", []() {
+        const int v = s2c_version();
+        py::print("pystrns2cards version " + std::to_string(v / 10000) + "." +
+                  std::to_string((v / 100) % 100) + "." + std::to_string(v % 100) +
+                  " (strns2cards_amd, gfx950 HIP)");
+    });
+
+    // pystrns2cards.cpp:245-281
+    m.def("compute_triangle_rects", [](int H, int W, Tensor pos, Tensor tri, Tensor rects,
+                                       Tensor prefix) -> int {
+        const char* fn = "compute_triangle_rects";
+        need(pos, fn, "pos", at::kFloat, {-1, -1, 4});
+        need(tri, fn, "tri", at::kInt, {-1, 3});
+        need(rects, fn, "triangle_rects", at::kInt, {-1, 4});
+        need(prefix, fn, "frag_prefix_sum", at::kInt, {-1});
+        same_device(fn, {&pos, &tri, &rects, &prefix});
+        const int B = I(pos.size(0), fn), V = I(pos.size(1), fn), T = I(tri.size(0), fn);
+        if (rects.size(0) != static_cast<int64_t>(B) * T)
+            throw std::runtime_error("`triangle_rects` must have shape [B * T, 4].");
+        if (prefix.size(0) != static_cast<int64_t>(B) * T)
+            throw std::runtime_error("`frag_prefix_sum` must have shape [B * T].");
+        c10::hip::HIPGuard g(pos.device());
+        int n = 0;
+        {
+            py::gil_scoped_release nogil;
+            check(s2c_compute_triangle_rects(H, W, B, V, P<float>(pos), T, P<int>(tri),
+                                             P<int>(rects), P<int>(prefix), &n, stream_of(pos)),
+                  fn);
+        }
+        return n;
+    });
+
+    // pystrns2cards.cpp:283-326
+    m.def("compute_fragments", [](int H, int W, Tensor pos, Tensor tri, Tensor prefix,
+                                  Tensor rects, Tensor frag_pix, Tensor frag_attrs) {
+        const char* fn = "compute_fragments";
+        need(pos, fn, "pos", at::kFloat, {-1, -1, 4});
+        need(tri, fn, "tri", at::kInt, {-1, 3});
+        need(prefix, fn, "frag_prefix_sum", at::kInt, {-1});
+        need(rects, fn, "triangle_rects", at::kInt, {-1, 4});
+        need(frag_pix, fn, "frag_pix", at::kInt, {-1, 3});
+        need(frag_attrs, fn, "frag_attrs", at::kFloat, {-1, 4});
+        same_device(fn, {&pos, &tri, &prefix, &rects, &frag_pix, &frag_attrs});
+        const int B = I(pos.size(0), fn), V = I(pos.size(1), fn), T = I(tri.size(0), fn);
+        const int num_tris = I(static_cast<int64_t>(B) * T, fn);
+        const int N = I(frag_pix.size(0), fn);
+        if (frag_attrs.size(0) != N || prefix.size(0) != num_tris)
+            throw std::runtime_error("Mismatch in fragment count or prefix sum length.");
+        if (rects.size(0) != num_tris)
+            throw std::runtime_error("`triangle_rects` must have shape [B * T, 4].");
+        c10::hip::HIPGuard g(pos.device());
+        check(s2c_compute_fragments(H, W, V, P<float>(pos), T, P<int>(tri), num_tris, N,
+                                    P<int>(prefix), P<int>(rects), P<int>(frag_pix),
+                                    P<float>(frag_attrs), stream_of(pos)),
+              fn);
+    });
+
+    // pystrns2cards.cpp:328-356
+    m.def("depth_test", [](int H, int W, Tensor frag_pix, Tensor frag_attrs, Tensor rast_out) {
+        const char* fn = "depth_test";
+        need(frag_pix, fn, "frag_pix", at::kInt, {-1, 3});
+        need(frag_attrs, fn, "frag_attrs", at::kFloat, {-1, 4});
+        need(rast_out, fn, "rast_out", at::kFloat, {-1, -1, -1, 4});
+        same_device(fn, {&frag_pix, &frag_attrs, &rast_out});
+        const int N = I(frag_pix.size(0), fn);
+        if (frag_attrs.size(0) != N)
+            throw std::runtime_error("frag_attrs must match frag_pix in size.");
+        if (rast_out.size(1) != H || rast_out.size(2) != W)
+            throw std::runtime_error("rast_out must have shape (B, H, W, 4).");
+        c10::hip::HIPGuard g(rast_out.device());
+        check(s2c_depth_test(I(rast_out.size(0), fn), H, W, N, P<int>(frag_pix),
+                             P<float>(frag_attrs), P<float>(rast_out), stream_of(rast_out)),
+              fn);
+    });
+
+    // pystrns2cards.cpp:358-392
+    m.def("filter_valid_fragments", [](Tensor fp, Tensor fa, Tensor fp_out, Tensor fa_out) -> int {
+        const char* fn = "filter_valid_fragments";
+        need(fp, fn, "frag_pix", at::kInt, {-1, 3});
+        need(fa, fn, "frag_attrs", at::kFloat, {-1, 4});
+        need(fp_out, fn, "frag_pix_out", at::kInt, {-1, 3});
+        need(fa_out, fn, "frag_attrs_out", at::kFloat, {-1, 4});
+        same_device(fn, {&fp, &fa, &fp_out, &fa_out});
+        const int N = I(fp.size(0), fn);
+        if (fa.size(0) != N || fp_out.size(0) != N || fa_out.size(0) != N)
+            throw std::runtime_error("Mismatch in fragment counts between input/output tensors.");
+        c10::hip::HIPGuard g(fp.device());
+        int n = 0;
+        {
+            py::gil_scoped_release nogil;
+            check(s2c_filter_valid_fragments(N, P<int>(fp), P<float>(fa), P<int>(fp_out),
+                                             P<float>(fa_out), &n, stream_of(fp)),
+                  fn);
+        }
+        return n;
+    });
+
+    // pystrns2cards.cpp:394-415
+    m.def("interpolate_triangle_attributes", [](Tensor rast, Tensor attr, Tensor tri, Tensor image) {
+        const char* fn = "interpolate_triangle_attributes";
+        need(rast, fn, "rast", at::kFloat, {-1, -1, -1, 4});
+        need(attr, fn, "attr", at::kFloat, {-1, -1});
+        need(tri, fn, "tri", at::kInt, {-1, 3});
+        need(image, fn, "image", at::kFloat, {-1, -1, -1, -1});
+        same_device(fn, {&rast, &attr, &tri, &image});
+        const int B = I(rast.size(0), fn), H = I(rast.size(1), fn), W = I(rast.size(2), fn);
+        const int C = I(attr.size(1), fn);
+        if (image.size(0) != B || image.size(1) != H || image.size(2) != W || image.size(3) != C)
+            throw std::runtime_error("image must have shape [B, H, W, C]");
+        c10::hip::HIPGuard g(rast.device());
+        check(s2c_interpolate_triangle_attributes(B, H, W, C, P<float>(rast), P<float>(attr),
+                                                  P<int>(tri), P<float>(image), stream_of(rast)),
+              fn);
+    });
+
+    // pystrns2cards.cpp:417-438
+    m.def("backward_interpolate_triangle_attributes",
+          [](Tensor rast, Tensor d_image, Tensor tri, Tensor d_attr) {
+              const char* fn = "backward_interpolate_triangle_attributes";
+              need(rast, fn, "rast", at::kFloat, {-1, -1, -1, 4});
+              need(d_image, fn, "d_image", at::kFloat, {-1, -1, -1, -1});
+              need(tri, fn, "tri", at::kInt, {-1, 3});
+              need(d_attr, fn, "d_attr", at::kFloat, {-1, -1});
+              same_device(fn, {&rast, &d_image, &tri, &d_attr});
+              const int B = I(rast.size(0), fn), H = I(rast.size(1), fn), W = I(rast.size(2), fn);
+              const int C = I(d_attr.size(1), fn);
+              if (d_image.size(0) != B || d_image.size(1) != H || d_image.size(2) != W ||
+                  d_image.size(3) != C)
+                  throw std::runtime_error("`d_image` must have shape [B, H, W, C] matching rast");
+              c10::hip::HIPGuard g(rast.device());
+              check(s2c_backward_interpolate_triangle_attributes(
+                        B, H, W, C, P<float>(rast), P<float>(d_image), P<int>(tri),
+                        P<float>(d_attr), stream_of(rast)),
+                    fn);
+          });
+
+    // pystrns2cards.cpp:440-478
+    m.def("cluster_mask_from_fragments",
+          [](Tensor fp, Tensor fa, Tensor prim2cluster, Tensor bitset) {
+              const char* fn = "cluster_mask_from_fragments";
+              need(fp, fn, "frag_pix", at::kInt, {-1, 3});
+              need(fa, fn, "frag_attrs", at::kFloat, {-1, 4});
+              need(prim2cluster, fn, "prim2cluster", at::kInt, {-1});
+              need(bitset, fn, "bitset", at::kInt, {-1, -1, -1, -1});
+              same_device(fn, {&fp, &fa, &prim2cluster, &bitset});
+              const int N = I(fp.size(0), fn);
+              if (fa.size(0) != N)
+                  throw std::runtime_error(
+                      "frag_pix and frag_attrs must have the same number of fragments.");
+              c10::hip::HIPGuard g(bitset.device());
+              check(s2c_cluster_mask_from_fragments(N, P<int>(fp), P<float>(fa),
+                                                    P<int>(prim2cluster), I(bitset.size(1), fn),
+                                                    I(bitset.size(2), fn), I(bitset.size(3), fn),
+                                                    P<int>(bitset), stream_of(bitset)),
+                    fn);
+          });
+
+    // pystrns2cards.cpp:480-505
+    m.def("accumulate_bitset_rgb", [](Tensor bitset, Tensor rgb, Tensor out) {
+        const char* fn = "accumulate_bitset_rgb";
+        need(bitset, fn, "bitset", at::kInt, {-1, -1, -1, -1});
+        need(rgb, fn, "cluster_rgb", at::kFloat, {-1, 3});
+        need(out, fn, "accum_color", at::kFloat, {-1, -1, -1, 3});
+        same_device(fn, {&bitset, &rgb, &out});
+        const int B = I(bitset.size(0), fn), H = I(bitset.size(1), fn), W = I(bitset.size(2), fn);
+        const int S = I(bitset.size(3), fn);
+        if (static_cast<int64_t>(S) * 32 < rgb.size(0))
+            throw std::runtime_error("Too few slots for the number of clusters.");
+        if (out.size(0) != B || out.size(1) != H || out.size(2) != W)
+            throw std::runtime_error("accum_color must have shape [B, H, W, 3]");
+        c10::hip::HIPGuard g(bitset.device());
+        check(s2c_accumulate_bitset_rgb(B, H, W, S, P<int>(bitset), P<float>(rgb), P<float>(out),
+                                        stream_of(bitset)),
+              fn);
+    });
+
+    // pystrns2cards.cpp:507-529
+    m.def("popcount_bitset", [](Tensor bitset, Tensor out) {
+        const char* fn = "popcount_bitset";
+        need(bitset, fn, "bitset", at::kInt, {-1, -1, -1, -1});
+        need(out, fn, "count_image", at::kFloat, {-1, -1, -1});
+        same_device(fn, {&bitset, &out});
+        const int B = I(bitset.size(0), fn), H = I(bitset.size(1), fn), W = I(bitset.size(2), fn);
+        if (out.size(0) != B || out.size(1) != H || out.size(2) != W)
+            throw std::runtime_error("count_image must have shape [B, H, W]");
+        c10::hip::HIPGuard g(bitset.device());
+        check(s2c_popcount_bitset(B, H, W, I(bitset.size(3), fn), P<int>(bitset), P<float>(out),
+                                  stream_of(bitset)),
+              fn);
+    });
+
+    // pystrns2cards.cpp:531-565
+    m.def("dda_compute_span", [](Tensor edge_ids, int H, int W, Tensor pos, Tensor edges,
+                                 Tensor prefix, Tensor slopes, Tensor spans) -> int {
+        const char* fn = "dda_compute_span";
+        need(edge_ids, fn, "edge_ids", at::kInt, {-1});
+        need(pos, fn, "pos", at::kFloat, {-1, -1, 4});
+        need(edges, fn, "edges", at::kInt, {-1, 2});
+        need(prefix, fn, "frag_prefix_sum", at::kInt, {-1});
+        need(slopes, fn, "frag_slopes", at::kFloat, {-1, 2});
+        need(spans, fn, "frag_spans", at::kFloat, {-1, 4});
+        same_device(fn, {&edge_ids, &pos, &edges, &prefix, &slopes, &spans});
+        const int P_ = I(edge_ids.size(0), fn);
+        if (prefix.size(0) != P_ || slopes.size(0) != P_ || spans.size(0) != P_)
+            throw std::runtime_error(
+                "Invalid output tensor shapes for frag_prefix_sum, frag_slopes, or frag_spans.");
+        c10::hip::HIPGuard g(pos.device());
+        int n = 0;
+        {
+            py::gil_scoped_release nogil;
+            check(s2c_dda_compute_span(P_, P<int>(edge_ids), H, W, I(pos.size(1), fn),
+                                       P<float>(pos), I(edges.size(0), fn), P<int>(edges),
+                                       P<int>(prefix), P<float>(slopes), P<float>(spans), &n,
+                                       stream_of(pos)),
+                  fn);
+        }
+        return n;
+    });
+
+    // pystrns2cards.cpp:567-611
+    m.def("dda_compute_fragments", [](Tensor prefix, Tensor edge_ids, Tensor slopes, Tensor spans,
+                                      int H, int W, Tensor pos, Tensor edges, Tensor frag_pix,
+                                      Tensor frag_attrs) {
+        const char* fn = "dda_compute_fragments";
+        need(prefix, fn, "frag_prefix_sum", at::kInt, {-1});
+        need(edge_ids, fn, "edge_ids", at::kInt, {-1});
+        need(slopes, fn, "frag_slopes", at::kFloat, {-1, 2});
+        need(spans, fn, "frag_spans", at::kFloat, {-1, 4});
+        need(pos, fn, "pos", at::kFloat, {-1, -1, 4});
+        need(edges, fn, "edges", at::kInt, {-1, 2});
+        need(frag_pix, fn, "frag_pix", at::kInt, {-1, 3});
+        need(frag_attrs, fn, "frag_attrs", at::kFloat, {-1, 4});
+        same_device(fn, {&prefix, &edge_ids, &slopes, &spans, &pos, &edges, &frag_pix, &frag_attrs});
+        const int N = I(frag_pix.size(0), fn), P_ = I(edge_ids.size(0), fn);
+        if (prefix.size(0) != P_ || slopes.size(0) != P_ || spans.size(0) != P_ ||
+            frag_attrs.size(0) != N)
+            throw std::runtime_error("Invalid shape for input or output arrays.");
+        c10::hip::HIPGuard g(pos.device());
+        check(s2c_dda_compute_fragments(P_, N, P<int>(prefix), P<int>(edge_ids), P<float>(slopes),
+                                        P<float>(spans), H, W, I(pos.size(1), fn), P<float>(pos),
+                                        I(edges.size(0), fn), P<int>(edges), P<int>(frag_pix),
+                                        P<float>(frag_attrs), stream_of(pos)),
+              fn);
+    });
+
+    // pystrns2cards.cpp:613-647
+    m.def("dda_interpolate_attributes", [](Tensor rast, Tensor attr, Tensor edges, Tensor image) {
+        const char* fn = "dda_interpolate_attributes";
+        need(rast, fn, "rast_dda", at::kFloat, {-1, -1, -1, 4});
+        need(attr, fn, "attr", at::kFloat, {-1, -1});
+        need(edges, fn, "edges", at::kInt, {-1, 2});
+        need(image, fn, "image", at::kFloat, {-1, -1, -1, -1});
+        same_device(fn, {&rast, &attr, &edges, &image});
+        const int B = I(rast.size(0), fn), H = I(rast.size(1), fn), W = I(rast.size(2), fn);
+        const int C = I(attr.size(1), fn);
+        if (image.size(0) != B || image.size(1) != H || image.size(2) != W || image.size(3) != C)
+            throw std::runtime_error("image must have shape [B, H, W, C]");
+        c10::hip::HIPGuard g(rast.device());
+        check(s2c_dda_interpolate_attributes(B, H, W, C, P<float>(rast), P<float>(attr),
+                                             P<int>(edges), P<float>(image), stream_of(rast)),
+              fn);
+    });
+
+    // pystrns2cards.cpp:649-680
+    m.def("backward_dda_interpolate_attributes",
+          [](Tensor rast, Tensor d_image, Tensor edges, Tensor d_attr) {
+              const char* fn = "backward_dda_interpolate_attributes";
+              need(rast, fn, "rast_dda", at::kFloat, {-1, -1, -1, 4});
+              need(d_image, fn, "d_image", at::kFloat, {-1, -1, -1, -1});
+              need(edges, fn, "edges", at::kInt, {-1, 2});
+              need(d_attr, fn, "d_attr", at::kFloat, {-1, -1});
+              same_device(fn, {&rast, &d_image, &edges, &d_attr});
+              const int B = I(rast.size(0), fn), H = I(rast.size(1), fn), W = I(rast.size(2), fn);
+              const int C = I(d_image.size(3), fn);
+              if (d_image.size(0) != B || d_image.size(1) != H || d_image.size(2) != W)
+                  throw std::runtime_error(
+                      "`d_image` must have shape [B, H, W, C] matching rast_dda");
+              if (d_attr.size(1) != C)
+                  throw std::runtime_error(
+                      "`d_attr` must have shape [V, C] where C matches d_image.shape(3)");
+              c10::hip::HIPGuard g(rast.device());
+              check(s2c_backward_dda_interpolate_attributes(B, H, W, P<float>(rast), C,
+                                                            P<float>(d_image), P<int>(edges),
+                                                            P<float>(d_attr), stream_of(rast)),
+                    fn);
+          });
+
+    // pystrns2cards.cpp:682-715
+    m.def("mark_discontinuity_edges", [](int H, int W, Tensor tri, Tensor edges, Tensor edge2tri,
+                                         Tensor pos, Tensor prim_ids, Tensor normals) {
+        const char* fn = "mark_discontinuity_edges";
+        need(tri, fn, "tri", at::kInt, {-1, 3});
+        need(edges, fn, "edges", at::kInt, {-1, 2});
+        need(edge2tri, fn, "edge2tri", at::kInt, {-1, 2});
+        need(pos, fn, "pos", at::kFloat, {-1, -1, 4});
+        need(prim_ids, fn, "prim_ids", at::kInt, {-1});
+        need(normals, fn, "normals", at::kFloat, {-1, 2});
+        same_device(fn, {&tri, &edges, &edge2tri, &pos, &prim_ids, &normals});
+        const int B = I(pos.size(0), fn), V = I(pos.size(1), fn), E = I(edges.size(0), fn);
+        if (edge2tri.size(0) != E)
+            throw std::runtime_error(
+                "mark_discontinuity_edges: edge2tri must have the same number of edges.");
+        if (prim_ids.size(0) != static_cast<int64_t>(B) * E)
+            throw std::runtime_error("mark_discontinuity_edges: prim_ids must have shape (B * E).");
+        if (normals.size(0) != static_cast<int64_t>(B) * E)
+            throw std::runtime_error(
+                "mark_discontinuity_edges: normals must have shape (B * E, 2).");
+        c10::hip::HIPGuard g(pos.device());
+        check(s2c_mark_discontinuity_edges(H, W, B, V, P<float>(pos), P<int>(tri), E,
+                                           P<int>(edges), P<int>(edge2tri), P<int>(prim_ids),
+                                           P<float>(normals), stream_of(pos)),
+              fn);
+    });
+
+    // pystrns2cards.cpp:717-772
+    m.def("backward_antialiased_cluster_bitset",
+          [](Tensor fp, Tensor fa, Tensor bitset, Tensor target, Tensor pos, Tensor edges,
+             Tensor normals, Tensor edge2cluster, Tensor d_pos, float kernel_radius, float rho) {
+              const char* fn = "backward_antialiased_cluster_bitset";
+              need(fp, fn, "frag_pix", at::kInt, {-1, 3});
+              need(fa, fn, "frag_attrs_dda", at::kFloat, {-1, 4});
+              need(bitset, fn, "bitset", at::kInt, {-1, -1, -1, -1});
+              need(target, fn, "target_bitset", at::kInt, {-1, -1, -1, -1});
+              need(pos, fn, "pos", at::kFloat, {-1, -1, 4});
+              need(edges, fn, "edges", at::kInt, {-1, 2});
+              need(normals, fn, "normals", at::kFloat, {-1, 2});
+              need(edge2cluster, fn, "edge2cluster", at::kInt, {-1});
+              need(d_pos, fn, "d_pos", at::kFloat, {-1, -1, 4});
+              same_device(fn, {&fp, &fa, &bitset, &target, &pos, &edges, &normals, &edge2cluster,
+                               &d_pos});
+              const int N = I(fp.size(0), fn);
+              const int B = I(bitset.size(0), fn), H = I(bitset.size(1), fn);
+              const int W = I(bitset.size(2), fn), S = I(bitset.size(3), fn);
+              const int V = I(pos.size(1), fn), E = I(edges.size(0), fn);
+              if (fa.size(0) != N)
+                  throw std::runtime_error(
+                      "backward_antialiased_bitset: frag_pix and frag_attrs_dda must match in "
+                      "number of fragments.");
+              if (target.size(0) != B || target.size(1) != H || target.size(2) != W ||
+                  target.size(3) != S)
+                  throw std::runtime_error(
+                      "backward_antialiased_bitset: target_bitset must match bitset shape.");
+              if (d_pos.size(0) != B || d_pos.size(1) != V)
+                  throw std::runtime_error(
+                      "backward_antialiased_bitset: d_pos must match pos shape.");
+              c10::hip::HIPGuard g(pos.device());
+              check(s2c_backward_antialiased_cluster_bitset(
+                        N, P<int>(fp), P<float>(fa), H, W, S, P<int>(bitset), P<int>(target), V,
+                        P<float>(pos), E, P<int>(edges), P<float>(normals), P<int>(edge2cluster),
+                        P<float>(d_pos), kernel_radius, rho, stream_of(pos)),
+                    fn);
+          });
+
+    // pystrns2cards.cpp:774-786
+    m.def("compact_valid_ints", [](Tensor in, Tensor out) -> int {
+        const char* fn = "compact_valid_ints";
+        need(in, fn, "input", at::kInt, {-1});
+        need(out, fn, "output", at::kInt, {-1});
+        same_device(fn, {&in, &out});
+        if (in.size(0) != out.size(0))
+            throw std::runtime_error(
+                "compact_valid_ints: input and output must have the same size.");
+        c10::hip::HIPGuard g(in.device());
+        int n = 0;
+        {
+            py::gil_scoped_release nogil;
+            check(s2c_compact_valid_ints(I(in.size(0), fn), P<int>(in), P<int>(out), &n,
+                                         stream_of(in)),
+                  fn);
+        }
+        return n;
+    });
+
+    // pystrns2cards.cpp:841-871
+    m.def("smooth_strands", [](Tensor strands, Tensor arclength, Tensor out, float beta) {
+        const char* fn = "smooth_strands";
+        need(strands, fn, "strands", at::kFloat, {-1, -1, 3});
+        need(arclength, fn, "arclength", at::kFloat, {-1, -1});
+        need(out, fn, "smoothed_strands", at::kFloat, {-1, -1, 3});
+        same_device(fn, {&strands, &arclength, &out});
+        const int N = I(strands.size(0), fn), Pn = I(strands.size(1), fn);
+        if (arclength.size(0) != N || arclength.size(1) != Pn)
+            throw std::runtime_error(
+                "smooth_strands: arclength must match (num_strands, num_points).");
+        if (out.size(0) != N || out.size(1) != Pn)
+            throw std::runtime_error(
+                "smooth_strands: smoothed_strands must match (num_strands, num_points, 3).");
+        c10::hip::HIPGuard g(strands.device());
+        check(s2c_smooth_strands(N, Pn, P<float>(strands), P<float>(arclength), P<float>(out),
+                                 beta, stream_of(strands)),
+              fn);
+    });
+
+    // pystrns2cards.cpp:873-923
+    m.def("skinning", [](Tensor strands, Tensor arclength, Tensor handle, Tensor canonical,
+                         Tensor R, Tensor out, float beta) {
+        const char* fn = "skinning";
+        need(strands, fn, "strands", at::kFloat, {-1, -1, 3});
+        need(arclength, fn, "arclength", at::kFloat, {-1, -1});
+        need(handle, fn, "handle_strands", at::kFloat, {-1, -1, 3});
+        need(canonical, fn, "canonical_handles", at::kFloat, {-1, -1, 3});
+        need(R, fn, "R", at::kFloat, {-1, -1, 3, 3});
+        need(out, fn, "skinned_strands", at::kFloat, {-1, -1, 3});
+        same_device(fn, {&strands, &arclength, &handle, &canonical, &R, &out});
+        const int N = I(strands.size(0), fn), Pn = I(strands.size(1), fn);
+        auto match = [&](const Tensor& t) { return t.size(0) == N && t.size(1) == Pn; };
+        if (!match(arclength))
+            throw std::runtime_error("skinning: arclength must match (num_strands, num_points).");
+        if (!match(handle))
+            throw std::runtime_error(
+                "skinning: handle_strands must match (num_strands, num_points, 3).");
+        if (!match(canonical))
+            throw std::runtime_error(
+                "skinning: canonical_handles must match (num_strands, num_points, 3).");
+        if (!match(R))
+            throw std::runtime_error("skinning: R must match (num_strands, num_points, 3, 3).");
+        if (!match(out))
+            throw std::runtime_error(
+                "skinning: skinned_strands must match (num_strands, num_points, 3).");
+        c10::hip::HIPGuard g(strands.device());
+        check(s2c_skinning(N, Pn, P<float>(strands), P<float>(arclength), P<float>(handle),
+                           P<float>(canonical), P<float>(R), P<float>(out), beta,
+                           stream_of(strands)),
+              fn);
+    });
+
+    // Fused forms (no single reference binding; see the C ABI header).
+    m.def("dda_cluster_mask", [](Tensor pos, Tensor edges, Tensor edge2cluster, Tensor bitset) {
+        const char* fn = "dda_cluster_mask";
+        need(pos, fn, "pos", at::kFloat, {-1, -1, 4});
+        need(edges, fn, "edges", at::kInt, {-1, 2});
+        need(edge2cluster, fn, "edge2cluster", at::kInt, {-1});
+        need(bitset, fn, "bitset", at::kInt, {-1, -1, -1, -1});
+        same_device(fn, {&pos, &edges, &edge2cluster, &bitset});
+        if (bitset.size(0) != pos.size(0))
+            throw std::runtime_error("dda_cluster_mask: bitset must have B = pos.shape[0] views");
+        if (edge2cluster.size(0) != edges.size(0))
+            throw std::runtime_error("dda_cluster_mask: edge2cluster must have E entries");
+        c10::hip::HIPGuard g(pos.device());
+        check(s2c_dda_cluster_mask(I(pos.size(0), fn), I(bitset.size(1), fn), I(bitset.size(2), fn),
+                                   I(pos.size(1), fn), P<float>(pos), I(edges.size(0), fn),
+                                   P<int>(edges), P<int>(edge2cluster), I(bitset.size(3), fn),
+                                   P<int>(bitset), stream_of(pos)),
+              fn);
+    });
+
+    m.def("triangle_cluster_mask", [](Tensor pos, Tensor tri, Tensor tri2cluster, Tensor bitset) {
+        const char* fn = "triangle_cluster_mask";
+        need(pos, fn, "pos", at::kFloat, {-1, -1, 4});
+        need(tri, fn, "tri", at::kInt, {-1, 3});
+        need(tri2cluster, fn, "tri2cluster", at::kInt, {-1});
+        need(bitset, fn, "bitset", at::kInt, {-1, -1, -1, -1});
+        same_device(fn, {&pos, &tri, &tri2cluster, &bitset});
+        if (bitset.size(0) != pos.size(0))
+            throw std::runtime_error(
+                "triangle_cluster_mask: bitset must have B = pos.shape[0] views");
+        if (tri2cluster.size(0) != tri.size(0))
+            throw std::runtime_error("triangle_cluster_mask: tri2cluster must have T entries");
+        c10::hip::HIPGuard g(pos.device());
+        check(s2c_triangle_cluster_mask(I(pos.size(0), fn), I(bitset.size(1), fn),
+                                        I(bitset.size(2), fn), I(pos.size(1), fn), P<float>(pos),
+                                        I(tri.size(0), fn), P<int>(tri), P<int>(tri2cluster),
+                                        I(bitset.size(3), fn), P<int>(bitset), stream_of(pos)),
+              fn);
+    });
+}

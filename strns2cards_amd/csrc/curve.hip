@@ -1,0 +1,148 @@
+// Gaussian arclength smoothing and rotation skinning of strands
+// (reference: pystrns2cards/src/curve_utils.cu).
+//
+// The reference runs one thread per strand and loops over all n_v^2 pairs
+// (curve_utils.cu:24-131), which leaves a 256-CU part mostly idle for tens of
+// thousands of strands.  Here one thread owns one output vertex (strand,
+// i); the strand's arclength row is staged in LDS by the workgroup, so the
+// inner j-loop reads LDS and the per-vertex sum keeps the reference's
+// summation order (j ascending), which keeps the result within fp32
+// round-off of the oracle.
+#include "common.hpp"
+
+namespace s2c {
+
+constexpr int kCurveMaxLds = 4096;  // arclength floats staged per workgroup
+
+// Each workgroup covers kBlock consecutive (strand, vertex) outputs; the
+// strands they touch are contiguous, so their arclength rows form one
+// contiguous range that is copied to LDS when it fits.
+__global__ void __launch_bounds__(kBlock)
+smooth_kernel(int N, int P, const float* __restrict__ x, const float* __restrict__ s,
+              float* __restrict__ out, float beta) {
+    __shared__ float sl[kCurveMaxLds];
+    const long long total = static_cast<long long>(N) * P;
+    const long long g0 = static_cast<long long>(blockIdx.x) * kBlock;
+    const long long n_first = g0 / P;
+    const long long g_last = dmin<long long>(g0 + kBlock, total) - 1;
+    const long long n_last = g_last / P;
+    const long long span = (n_last - n_first + 1) * P;
+    const bool staged = span <= kCurveMaxLds;
+    if (staged) {
+        for (long long k = threadIdx.x; k < span; k += kBlock) sl[k] = s[n_first * P + k];
+    }
+    __syncthreads();
+    const long long g = g0 + threadIdx.x;
+    if (g >= total) return;
+    const long long n = g / P;
+    const int i = static_cast<int>(g - n * P);
+    const float* row = staged ? sl + (n - n_first) * P : s + n * P;
+    const float* xs = x + n * P * 3;
+    const float inv_b2 = 1.0f / (beta * beta);
+    const float cut = 3.0f * beta;
+    const float si = row[i];
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, sw = 0.f;
+    for (int j = 0; j < P; ++j) {
+        const float dist = fabsf(si - row[j]);
+        if (dist > cut) continue;
+        const float w = expf(-(dist * dist) * inv_b2);
+        a0 += w * xs[j * 3 + 0];
+        a1 += w * xs[j * 3 + 1];
+        a2 += w * xs[j * 3 + 2];
+        sw += w;
+    }
+    if (sw > 1e-6f) {
+        float* o = out + g * 3;
+        o[0] = a0 / sw;
+        o[1] = a1 / sw;
+        o[2] = a2 / sw;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+skinning_kernel(int N, int P, const float* __restrict__ x, const float* __restrict__ s,
+                const float* __restrict__ hnd, const float* __restrict__ can,
+                const float* __restrict__ R, float* __restrict__ out, float beta) {
+    __shared__ float sl[kCurveMaxLds];
+    const long long total = static_cast<long long>(N) * P;
+    const long long g0 = static_cast<long long>(blockIdx.x) * kBlock;
+    const long long n_first = g0 / P;
+    const long long g_last = dmin<long long>(g0 + kBlock, total) - 1;
+    const long long n_last = g_last / P;
+    const long long span = (n_last - n_first + 1) * P;
+    const bool staged = span <= kCurveMaxLds;
+    if (staged) {
+        for (long long k = threadIdx.x; k < span; k += kBlock) sl[k] = s[n_first * P + k];
+    }
+    __syncthreads();
+    const long long g = g0 + threadIdx.x;
+    if (g >= total) return;
+    const long long n = g / P;
+    const int i = static_cast<int>(g - n * P);
+    const float* row = staged ? sl + (n - n_first) * P : s + n * P;
+    const long long o3 = n * P * 3;
+    const float inv_b2 = 1.0f / (beta * beta);
+    const float cut = 3.0f * beta;
+    const float si = row[i];
+    const float xi0 = x[o3 + i * 3 + 0], xi1 = x[o3 + i * 3 + 1], xi2 = x[o3 + i * 3 + 2];
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, sw = 0.f;
+    for (int j = 0; j < P; ++j) {
+        const float dist = fabsf(si - row[j]);
+        if (dist > cut) continue;
+        const float w = expf(-(dist * dist) * inv_b2);
+        const float* hj = hnd + o3 + j * 3;
+        const float* cj = can + o3 + j * 3;
+        const float* Rj = R + (n * P + j) * 9;
+        const float l0 = xi0 - hj[0], l1 = xi1 - hj[1], l2 = xi2 - hj[2];
+        const float r0 = Rj[0] * l0 + Rj[1] * l1 + Rj[2] * l2;
+        const float r1 = Rj[3] * l0 + Rj[4] * l1 + Rj[5] * l2;
+        const float r2 = Rj[6] * l0 + Rj[7] * l1 + Rj[8] * l2;
+        a0 += w * (r0 + cj[0]);
+        a1 += w * (r1 + cj[1]);
+        a2 += w * (r2 + cj[2]);
+        sw += w;
+    }
+    float* o = out + g * 3;
+    if (sw > 1e-8f) {
+        o[0] = a0 / sw;
+        o[1] = a1 / sw;
+        o[2] = a2 / sw;
+    } else {
+        o[0] = xi0;
+        o[1] = xi1;
+        o[2] = xi2;
+    }
+}
+
+}  // namespace s2c
+
+using namespace s2c;
+
+extern "C" int s2c_smooth_strands(int num_strands, int num_points, const float* strands,
+                                  const float* arclength, float* smoothed_strands, float beta,
+                                  void* stream) {
+    S2C_CHECK(num_strands >= 0 && num_points >= 0, "smooth_strands: negative size");
+    const long long n = static_cast<long long>(num_strands) * num_points;
+    if (n == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    smooth_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(num_strands, num_points, strands,
+                                                         arclength, smoothed_strands, beta);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
+
+extern "C" int s2c_skinning(int num_strands, int num_points, const float* strands,
+                            const float* arclength, const float* handle_strands,
+                            const float* canonical_handles, const float* R,
+                            float* skinned_strands, float beta, void* stream) {
+    S2C_CHECK(num_strands >= 0 && num_points >= 0, "skinning: negative size");
+    const long long n = static_cast<long long>(num_strands) * num_points;
+    if (n == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    skinning_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(num_strands, num_points, strands,
+                                                           arclength, handle_strands,
+                                                           canonical_handles, R,
+                                                           skinned_strands, beta);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}

@@ -1,0 +1,120 @@
+// Per-primitive and per-fragment DDA arithmetic, shared by the drop-in
+// kernels (dda.hip), the fused DDA->mask kernel and the antialias pipeline.
+// Every expression follows the reference's order of operations so that,
+// with contraction off, pixels / ids are bit-exact with the CPU oracle.
+#pragma once
+
+#include "common.hpp"
+
+namespace s2c {
+
+struct DdaSpan {
+    float sx, sy, off0, off1;  // frag_spans row (dda.cu:62-65, 82-85)
+    float axis, slope;         // frag_slopes row (dda.cu:48-49, 68-69)
+};
+
+// dda.cu:9-87 for one primitive.  Returns the candidate count; `sp` is
+// written only for non-degenerate edges (the reference writes nothing for a
+// zero-length edge).  `p0`/`p1` receive the two clip-space vertices.
+__device__ __forceinline__ int dda_span(int H, int W, float4 p0, float4 p1, DdaSpan& sp,
+                                        bool& wrote) {
+    const float x0n = p0.x / p0.w;
+    const float y0n = p0.y / p0.w;
+    const float x1n = p1.x / p1.w;
+    const float y1n = p1.y / p1.w;
+    const float fH = static_cast<float>(H);
+    const float fW = static_cast<float>(W);
+    float x0 = 0.5f * (x0n + 1.f) * fW;
+    float y0 = 0.5f * (1.f - y0n) * fH;
+    float x1 = 0.5f * (x1n + 1.f) * fW;
+    float y1 = 0.5f * (1.f - y1n) * fH;
+    const float dx = x1 - x0;
+    const float dy = y1 - y0;
+    wrote = false;
+    if (dx == 0.f && dy == 0.f) return 0;
+    wrote = true;
+    if (fabsf(dx) >= fabsf(dy)) {
+        const float s = dy / dx;
+        sp.axis = 0.f;
+        sp.slope = s;
+        if (x0 > x1) {
+            float t = x0; x0 = x1; x1 = t;
+            t = y0; y0 = y1; y1 = t;
+        }
+        const float sx = dmax(floorf(x0), 0.f);
+        const float sy = y0 + s * (sx - x0);
+        const int i0 = f2i(sx);
+        const int i1 = dmin(f2i(floorf(x1)), W - 1) + 1;
+        sp.sx = sx; sp.sy = sy; sp.off0 = x0 - sx; sp.off1 = x1 - sx;
+        return dmax(i1 - i0, 0);
+    } else {
+        const float s = dx / dy;
+        sp.axis = 1.f;
+        sp.slope = s;
+        if (y0 > y1) {
+            float t = x0; x0 = x1; x1 = t;
+            t = y0; y0 = y1; y1 = t;
+        }
+        const float sy = dmax(floorf(y0), 0.f);
+        const float sx = x0 + s * (sy - y0);
+        const int i0 = f2i(sy);
+        const int i1 = dmin(f2i(floorf(y1)), H - 1) + 1;
+        sp.sx = sx; sp.sy = sy; sp.off0 = y0 - sy; sp.off1 = y1 - sy;
+        return dmax(i1 - i0, 0);
+    }
+}
+
+// dda.cu:124-144.
+__device__ __forceinline__ float dda_t1(int H, int W, float4 p0, float4 p1, float x, float y,
+                                        float axis) {
+    float t0, t1;
+    if (axis == 0.f) {
+        const float qx = -1.f + 2.f * (x / static_cast<float>(W));
+        const float a = p0.x - qx * p0.w;
+        const float c = p1.x - qx * p1.w;
+        t0 = fabsf(c);
+        t1 = fabsf(a);
+    } else {
+        const float qy = -1.f + 2.f * (y / static_cast<float>(H));
+        const float a = -p0.y - qy * p0.w;
+        const float c = -p1.y - qy * p1.w;
+        t0 = fabsf(c);
+        t1 = fabsf(a);
+    }
+    const float denom = t0 + t1;
+    if (denom < 1e-6f) return 0.5f;
+    return t1 / denom;
+}
+
+struct DdaFrag {
+    int h, w;
+    float t1, zw;
+};
+
+// dda.cu:185-216 for candidate k of a primitive.  Returns true when the
+// fragment is on screen and its depth lies in [-1, 1].
+__device__ __forceinline__ bool dda_frag(int H, int W, const DdaSpan& sp, int k, float4 p0,
+                                         float4 p1, DdaFrag& f) {
+    const float d = fminf(fmaxf(static_cast<float>(k) + 0.5f, sp.off0), sp.off1);
+    const float ddx = (sp.axis == 0.f) ? d : d * sp.slope;
+    const float ddy = (sp.axis == 0.f) ? d * sp.slope : d;
+    const float x = sp.sx + ddx;
+    const float y = sp.sy + ddy;
+    const int h = f2i(floorf(y));
+    const int w = f2i(floorf(x));
+    if (h < 0 || h >= H || w < 0 || w >= W) return false;
+    const float t1 = dda_t1(H, W, p0, p1, x, y, sp.axis);
+    const float t0 = 1.f - t1;
+    const float pz = t0 * p0.z + t1 * p1.z;
+    const float pw = t0 * p0.w + t1 * p1.w;
+    const float zw = pz / pw;
+    if (!isfinite(zw) || zw < -1.f || zw > 1.f) return false;
+    f.h = h; f.w = w; f.t1 = t1; f.zw = zw;
+    return true;
+}
+
+__device__ __forceinline__ float4 load_pos(const float* pos, long long row) {
+    return reinterpret_cast<const float4*>(pos)[row];
+}
+
+}  // namespace s2c

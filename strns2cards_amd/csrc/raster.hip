@@ -1,0 +1,398 @@
+// Triangle rasterization, depth test, barycentric interpolation and cluster
+// masks (reference: pystrns2cards/src/rasterize.cu), plus the fused
+// triangle -> cluster-bitset kernel used by card fitting.
+#include "common.hpp"
+
+namespace s2c {
+
+__device__ __forceinline__ float4 ld4(const float* p, long long row) {
+    return reinterpret_cast<const float4*>(p)[row];
+}
+
+// rasterize.cu:9-59 for one (b, t).  Returns h_len * w_len.
+__device__ __forceinline__ int tri_rect(int H, int W, float4 P0, float4 P1, float4 P2,
+                                        int4& r) {
+    const float fH = static_cast<float>(H);
+    const float fW = static_cast<float>(W);
+    float x0 = fW, y0 = fH, x1 = 0.f, y1 = 0.f;
+    const float4 P[3] = {P0, P1, P2};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float inv_w = 1.f / P[i].w;
+        const float x = 0.5f * (1.f + P[i].x * inv_w) * fW;
+        const float y = 0.5f * (1.f - P[i].y * inv_w) * fH;
+        x0 = isnan(x) ? x0 : fminf(x, x0);
+        x1 = isnan(x) ? x1 : fmaxf(x, x1);
+        y0 = isnan(y) ? y0 : fminf(y, y0);
+        y1 = isnan(y) ? y1 : fmaxf(y, y1);
+    }
+    const int h0 = dmax(f2i(floorf(y0)), 0);
+    const int h1 = dmin(f2i(floorf(y1)), H - 1) + 1;
+    const int w0 = dmax(f2i(floorf(x0)), 0);
+    const int w1 = dmin(f2i(floorf(x1)), W - 1) + 1;
+    const int hl = dmax(h1 - h0, 0);
+    const int wl = dmax(w1 - w0, 0);
+    r = make_int4(h0, hl, w0, wl);
+    return hl * wl;
+}
+
+// rasterize.cu:132-163 for pixel (x, y).  Returns true when covered.
+__device__ __forceinline__ bool tri_cover(int H, int W, int x, int y, float4 P0, float4 P1,
+                                          float4 P2, float& b0, float& b1, float& z) {
+    if (x < 0 || x >= W || y < 0 || y >= H) return false;
+    const float px = -1.f + 2.f * (static_cast<float>(x) + 0.5f) / static_cast<float>(W);
+    const float py = -1.f + 2.f * (static_cast<float>(y) + 0.5f) / static_cast<float>(H);
+    const float q0x = P0.x - px * P0.w, q0y = -P0.y - py * P0.w;
+    const float q1x = P1.x - px * P1.w, q1y = -P1.y - py * P1.w;
+    const float q2x = P2.x - px * P2.w, q2y = -P2.y - py * P2.w;
+    const float A0 = q1x * q2y - q1y * q2x;
+    const float A1 = q2x * q0y - q2y * q0x;
+    const float A2 = q0x * q1y - q0y * q1x;
+    const float A = A0 + A1 + A2;
+    if (A == 0.f) return false;
+    const float invA = 1.f / A;
+    b0 = A0 * invA;
+    b1 = A1 * invA;
+    const float zc = A0 * P0.z + A1 * P1.z + A2 * P2.z;
+    const float wc = A0 * P0.w + A1 * P1.w + A2 * P2.w;
+    z = zc / wc;
+    if (!isfinite(b0) || !isfinite(b1) || !isfinite(z)) return false;
+    if (b0 < 0.f || b1 < 0.f || b0 + b1 > 1.f || z < -1.f || z > 1.f) return false;
+    return true;
+}
+
+__global__ void __launch_bounds__(kBlock)
+tri_rects_kernel(int H, int W, int B, int V, const float* __restrict__ pos, int T,
+                 const int* __restrict__ tri, int* __restrict__ rects, int* __restrict__ counts) {
+    const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= static_cast<long long>(B) * T) return;
+    const int b = static_cast<int>(i / T);
+    const int t = static_cast<int>(i - static_cast<long long>(b) * T);
+    const long long vb = static_cast<long long>(b) * V;
+    const float4 P0 = ld4(pos, vb + tri[t * 3 + 0]);
+    const float4 P1 = ld4(pos, vb + tri[t * 3 + 1]);
+    const float4 P2 = ld4(pos, vb + tri[t * 3 + 2]);
+    int4 r;
+    counts[i] = tri_rect(H, W, P0, P1, P2, r);
+    reinterpret_cast<int4*>(rects)[i] = r;
+}
+
+constexpr int kExpandItems = 4;
+constexpr int kExpand = kBlock * kExpandItems;
+
+__device__ __forceinline__ int upper_bound_i(const int* prefix, int lo, int hi, int key) {
+    while (lo < hi) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if (prefix[mid] <= key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// rasterize.cu:90-172, with the per-workgroup primitive range found once
+// (see dda_fragments_kernel).
+__global__ void __launch_bounds__(kBlock)
+tri_fragments_kernel(int H, int W, int V, const float* __restrict__ pos, int T,
+                     const int* __restrict__ tri, int num_tris, int N,
+                     const int* __restrict__ prefix, const int* __restrict__ rects,
+                     int* __restrict__ frag_pix, float* __restrict__ frag_attrs) {
+    __shared__ int range[2];
+    const long long f0 = static_cast<long long>(blockIdx.x) * kExpand;
+    if (threadIdx.x == 0) {
+        const long long f1 = dmin<long long>(f0 + kExpand, N) - 1;
+        range[0] = upper_bound_i(prefix, 0, num_tris, static_cast<int>(f0));
+        range[1] = dmin(upper_bound_i(prefix, range[0], num_tris, static_cast<int>(f1)) + 1,
+                        num_tris);
+    }
+    __syncthreads();
+    const int lo = range[0], hi = range[1];
+#pragma unroll 1
+    for (int j = 0; j < kExpandItems; ++j) {
+        const long long f = f0 + j * kBlock + threadIdx.x;
+        if (f >= N) break;
+        const int fi = static_cast<int>(f);
+        const int ti = upper_bound_i(prefix, lo, hi, fi);
+        if (ti >= num_tris) continue;
+        const int4 r = reinterpret_cast<const int4*>(rects)[ti];
+        if (r.w <= 0 || r.y <= 0) continue;
+        const int b = ti / T;
+        const int t = ti % T;
+        const int l = fi - (ti == 0 ? 0 : prefix[ti - 1]);
+        const int y = r.x + l / r.w;
+        const int x = r.z + l % r.w;
+        const long long vb = static_cast<long long>(b) * V;
+        const float4 P0 = ld4(pos, vb + tri[t * 3 + 0]);
+        const float4 P1 = ld4(pos, vb + tri[t * 3 + 1]);
+        const float4 P2 = ld4(pos, vb + tri[t * 3 + 2]);
+        float b0, b1, z;
+        if (!tri_cover(H, W, x, y, P0, P1, P2, b0, b1, z)) continue;
+        frag_pix[f * 3 + 0] = b;
+        frag_pix[f * 3 + 1] = y;
+        frag_pix[f * 3 + 2] = x;
+        reinterpret_cast<float4*>(frag_attrs)[f] = make_float4(b0, b1, z, static_cast<float>(t + 1));
+    }
+}
+
+// ------------------------------------------------------------ depth test
+// rasterize.cu:197-251.
+
+__global__ void __launch_bounds__(kBlock)
+depth_key_kernel(int N, const int* __restrict__ frag_pix, const float* __restrict__ frag_attrs,
+                 int H, int W, long long* __restrict__ keys) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const int b = frag_pix[i * 3 + 0];
+    const int h = frag_pix[i * 3 + 1];
+    const int w = frag_pix[i * 3 + 2];
+    const float zz = frag_attrs[static_cast<long long>(i) * 4 + 2] + 2.f;
+    const unsigned long long k =
+        (static_cast<unsigned long long>(__float_as_uint(zz)) << 32) | static_cast<unsigned>(i);
+    atomicMin(&keys[(static_cast<long long>(b) * H + h) * W + w], static_cast<long long>(k));
+}
+
+__global__ void __launch_bounds__(kBlock)
+depth_gather_kernel(long long npix, const long long* __restrict__ keys,
+                    const float* __restrict__ frag_attrs, float* __restrict__ rast) {
+    const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= npix) return;
+    const int f = static_cast<int>(static_cast<unsigned>(keys[i] & 0xFFFFFFFFll));
+    if (f < 0) return;
+    reinterpret_cast<float4*>(rast)[i] = reinterpret_cast<const float4*>(frag_attrs)[f];
+}
+
+// --------------------------------------------------- barycentric interp
+
+// rasterize.cu:367-400.
+__global__ void __launch_bounds__(kBlock)
+tri_interp_kernel(long long npix, const float* __restrict__ rast, int C,
+                  const float* __restrict__ attr, const int* __restrict__ tri,
+                  float* __restrict__ image) {
+    const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= npix) return;
+    const float4 r = reinterpret_cast<const float4*>(rast)[i];
+    const float b0 = r.x, b1 = r.y;
+    const float b2 = 1.f - b0 - b1;
+    const int t = f2i(r.w) - 1;
+    if (t < 0) return;
+    const float* a0 = attr + static_cast<long long>(tri[t * 3 + 0]) * C;
+    const float* a1 = attr + static_cast<long long>(tri[t * 3 + 1]) * C;
+    const float* a2 = attr + static_cast<long long>(tri[t * 3 + 2]) * C;
+    for (int c = 0; c < C; ++c) image[i * C + c] = b0 * a0[c] + b1 * a1[c] + b2 * a2[c];
+}
+
+// rasterize.cu:402-438.
+__global__ void __launch_bounds__(kBlock)
+tri_interp_bwd_kernel(long long npix, const float* __restrict__ rast, int C,
+                      const float* __restrict__ d_image, const int* __restrict__ tri,
+                      float* __restrict__ d_attr) {
+    const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= npix) return;
+    const float4 r = reinterpret_cast<const float4*>(rast)[i];
+    const float b0 = r.x, b1 = r.y;
+    const float b2 = 1.f - b0 - b1;
+    const int t = f2i(r.w) - 1;
+    if (t < 0) return;
+    float* g0 = d_attr + static_cast<long long>(tri[t * 3 + 0]) * C;
+    float* g1 = d_attr + static_cast<long long>(tri[t * 3 + 1]) * C;
+    float* g2 = d_attr + static_cast<long long>(tri[t * 3 + 2]) * C;
+    for (int c = 0; c < C; ++c) {
+        const float g = d_image[i * C + c];
+        atomicAdd(&g0[c], g * b0);
+        atomicAdd(&g1[c], g * b1);
+        atomicAdd(&g2[c], g * b2);
+    }
+}
+
+// ------------------------------------------------- cluster mask (list)
+
+// rasterize.cu:474-498.
+__global__ void __launch_bounds__(kBlock)
+cluster_mask_kernel(int N, const int* __restrict__ frag_pix, const float* __restrict__ frag_attrs,
+                    const int* __restrict__ prim2cluster, int H, int W, int S,
+                    int* __restrict__ bitset) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const int prim = f2i(frag_attrs[static_cast<long long>(i) * 4 + 3]) - 1;
+    if (prim < 0) return;
+    const int k = prim2cluster[prim];
+    const long long idx =
+        ((static_cast<long long>(frag_pix[i * 3 + 0]) * H + frag_pix[i * 3 + 1]) * W +
+         frag_pix[i * 3 + 2]) * S + k / 32;
+    atomicOr(&bitset[idx], static_cast<int>(1u << (k % 32)));
+}
+
+// --------------------------------------------- fused triangle -> bitset
+// Workgroup = kBlock consecutive (b, t) pairs; their bounding rectangles are
+// block-scanned in LDS and every thread walks the block's candidate pixels.
+
+struct TriPrim {
+    float4 P0, P1, P2;
+    int4 r;
+    int b, cluster;
+};
+
+__global__ void __launch_bounds__(kBlock)
+tri_mask_kernel(long long BT, int H, int W, int V, const float* __restrict__ pos, int T,
+                const int* __restrict__ tri, const int* __restrict__ tri2cluster, int S,
+                int* __restrict__ bitset) {
+    __shared__ TriPrim prim[kBlock];
+    __shared__ int excl[kBlock];
+    __shared__ int sh[kBlock / kWave + 1];
+    const long long q = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    int cnt = 0;
+    if (q < BT) {
+        const int b = static_cast<int>(q / T);
+        const int t = static_cast<int>(q - static_cast<long long>(b) * T);
+        const long long vb = static_cast<long long>(b) * V;
+        TriPrim m;
+        m.P0 = ld4(pos, vb + tri[t * 3 + 0]);
+        m.P1 = ld4(pos, vb + tri[t * 3 + 1]);
+        m.P2 = ld4(pos, vb + tri[t * 3 + 2]);
+        cnt = tri_rect(H, W, m.P0, m.P1, m.P2, m.r);
+        m.b = b;
+        m.cluster = cnt > 0 ? tri2cluster[t] : 0;
+        prim[threadIdx.x] = m;
+    }
+    int total;
+    excl[threadIdx.x] = block_excl_scan(cnt, sh, &total);
+    __syncthreads();
+#pragma unroll 1
+    for (int j = threadIdx.x; j < total; j += kBlock) {
+        int lo = 0, hi = kBlock;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (excl[mid] <= j) lo = mid;
+            else hi = mid;
+        }
+        const TriPrim& m = prim[lo];
+        const int l = j - excl[lo];
+        const int y = m.r.x + l / m.r.w;
+        const int x = m.r.z + l % m.r.w;
+        float b0, b1, z;
+        if (!tri_cover(H, W, x, y, m.P0, m.P1, m.P2, b0, b1, z)) continue;
+        const long long idx =
+            ((static_cast<long long>(m.b) * H + y) * W + x) * S + (m.cluster >> 5);
+        atomicOr(&bitset[idx], static_cast<int>(1u << (m.cluster & 31)));
+    }
+}
+
+}  // namespace s2c
+
+using namespace s2c;
+
+extern "C" int s2c_compute_triangle_rects(int H, int W, int B, int V, const float* pos, int T,
+                                          const int* tri, int* triangle_rects,
+                                          int* frag_prefix_sum, int* num_frags_out,
+                                          void* stream) {
+    S2C_CHECK(num_frags_out != nullptr, "compute_triangle_rects: null count pointer");
+    *num_frags_out = 0;
+    S2C_CHECK(H >= 0 && W >= 0 && B >= 0 && V >= 0 && T >= 0,
+              "compute_triangle_rects: negative size");
+    const long long n = static_cast<long long>(B) * T;
+    S2C_CHECK(n <= INT_MAX, "compute_triangle_rects: B*T exceeds INT32_MAX");
+    if (n == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    tri_rects_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(H, W, B, V, pos, T, tri,
+                                                            triangle_rects, frag_prefix_sum);
+    S2C_LAUNCH_CHECK();
+    Temp scratch;
+    if (int r = scratch.alloc(sizeof(int) * scan_scratch_ints(n), s)) return r;
+    if (int r = inclusive_scan_inplace(frag_prefix_sum, n, scratch.as<int>(), s)) return r;
+    return read_scan_total(scratch.as<int>(), n, s, num_frags_out, "compute_triangle_rects");
+}
+
+extern "C" int s2c_compute_fragments(int H, int W, int V, const float* pos, int T,
+                                     const int* tri, int num_tris, int num_frags,
+                                     const int* frag_prefix_sum, const int* triangle_rects,
+                                     int* frag_pix, float* frag_attrs, void* stream) {
+    S2C_CHECK(num_tris >= 0 && num_frags >= 0 && T >= 0, "compute_fragments: negative size");
+    if (num_frags == 0 || num_tris == 0) return S2C_OK;
+    S2C_CHECK(T > 0, "compute_fragments: T must be positive");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    tri_fragments_kernel<<<grid_for(num_frags, kExpand), kBlock, 0, s>>>(
+        H, W, V, pos, T, tri, num_tris, num_frags, frag_prefix_sum, triangle_rects, frag_pix,
+        frag_attrs);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
+
+extern "C" int s2c_depth_test(int B, int H, int W, int num_fragments, const int* frag_pix,
+                              const float* frag_attrs, float* rast_out, void* stream) {
+    S2C_CHECK(B >= 0 && H >= 0 && W >= 0 && num_fragments >= 0, "depth_test: negative size");
+    const long long npix = static_cast<long long>(B) * H * W;
+    if (npix == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Temp keys;
+    if (int r = keys.alloc(sizeof(long long) * npix, s)) return r;
+    fill_i64_kernel<<<grid_for(dmin<long long>(npix, 1ll << 22), kBlock), kBlock, 0, s>>>(
+        keys.as<long long>(), LLONG_MAX, npix);
+    S2C_LAUNCH_CHECK();
+    if (num_fragments > 0) {
+        depth_key_kernel<<<grid_for(num_fragments, kBlock), kBlock, 0, s>>>(
+            num_fragments, frag_pix, frag_attrs, H, W, keys.as<long long>());
+        S2C_LAUNCH_CHECK();
+    }
+    depth_gather_kernel<<<grid_for(npix, kBlock), kBlock, 0, s>>>(npix, keys.as<long long>(),
+                                                                   frag_attrs, rast_out);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
+
+extern "C" int s2c_interpolate_triangle_attributes(int B, int H, int W, int attr_dim,
+                                                   const float* rast, const float* attr,
+                                                   const int* tri, float* image, void* stream) {
+    S2C_CHECK(B >= 0 && H >= 0 && W >= 0 && attr_dim >= 0,
+              "interpolate_triangle_attributes: negative size");
+    const long long n = static_cast<long long>(B) * H * W;
+    if (n == 0 || attr_dim == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    tri_interp_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, rast, attr_dim, attr, tri, image);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
+
+extern "C" int s2c_backward_interpolate_triangle_attributes(int B, int H, int W, int attr_dim,
+                                                            const float* rast,
+                                                            const float* d_image,
+                                                            const int* tri, float* d_attr,
+                                                            void* stream) {
+    S2C_CHECK(B >= 0 && H >= 0 && W >= 0 && attr_dim >= 0,
+              "backward_interpolate_triangle_attributes: negative size");
+    const long long n = static_cast<long long>(B) * H * W;
+    if (n == 0 || attr_dim == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    tri_interp_bwd_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, rast, attr_dim, d_image, tri,
+                                                                 d_attr);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
+
+extern "C" int s2c_cluster_mask_from_fragments(int num_frags, const int* frag_pix,
+                                               const float* frag_attrs,
+                                               const int* prim2cluster, int H, int W,
+                                               int num_slots, int* bitset, void* stream) {
+    S2C_CHECK(num_frags >= 0 && H >= 0 && W >= 0 && num_slots >= 0,
+              "cluster_mask_from_fragments: negative size");
+    if (num_frags == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    cluster_mask_kernel<<<grid_for(num_frags, kBlock), kBlock, 0, s>>>(
+        num_frags, frag_pix, frag_attrs, prim2cluster, H, W, num_slots, bitset);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
+
+extern "C" int s2c_triangle_cluster_mask(int B, int H, int W, int V, const float* pos, int T,
+                                         const int* tri, const int* tri2cluster, int num_slots,
+                                         int* bitset, void* stream) {
+    S2C_CHECK(B >= 0 && H >= 0 && W >= 0 && V >= 0 && T >= 0 && num_slots >= 0,
+              "triangle_cluster_mask: negative size");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long words = static_cast<long long>(B) * H * W * num_slots;
+    if (int r = fill_u32(bitset, 0u, words, s)) return r;
+    const long long BT = static_cast<long long>(B) * T;
+    if (BT == 0 || words == 0) return S2C_OK;
+    tri_mask_kernel<<<grid_for(BT, kBlock), kBlock, 0, s>>>(BT, H, W, V, pos, T, tri, tri2cluster,
+                                                            num_slots, bitset);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}

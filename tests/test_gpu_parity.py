@@ -1,0 +1,312 @@
+"""GPU parity: the C ABI (libstrns2cards_hip.so) against the CPU oracle.
+
+Bar: bit-exact for every integer / index / bitset output and for float
+outputs computed with the same operation sequence (fragment attributes,
+barycentrics, interpolated images); accumulated float outputs whose
+summation order differs (atomic scatters) within 1e-5 of the largest
+magnitude (normwise relative).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from capi import call, call_count
+from conftest import report
+from strns2cards_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+
+
+def T(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def normwise_close(got, want, rtol=RTOL):
+    got = np.asarray(got, np.float64)
+    want = np.asarray(want, np.float64)
+    scale = max(np.abs(want).max(initial=0.0), 1e-30)
+    err = np.abs(got - want).max(initial=0.0)
+    assert err <= rtol * scale, f"max abs err {err:.3e} > {rtol} * {scale:.3e}"
+
+
+# ------------------------------------------------------------- inputs
+
+
+def random_lines(rng, B, V, E, H, W, persp=False):
+    """Clip-space vertices spread over and beyond the screen, with
+    degenerate, axis-aligned and off-screen edges mixed in."""
+    xy = rng.uniform(-1.3, 1.3, size=(B, V, 2))
+    z = rng.uniform(-1.2, 1.2, size=(B, V, 1))
+    w = rng.uniform(0.5, 2.0, size=(B, V, 1)) if persp else np.ones((B, V, 1))
+    pos = np.concatenate([xy * w, z * w, w], -1).astype(np.float32)
+    edges = rng.integers(0, V, size=(E, 2)).astype(np.int32)
+    edges[: E // 10, 1] = edges[: E // 10, 0]  # zero-length
+    k = E // 10
+    pos[:, edges[k:2 * k, 1], 1] = pos[:, edges[k:2 * k, 0], 1]  # horizontal in NDC (w = 1)
+    return pos, edges
+
+
+def dda_gpu(dev, res, pos, edges, edge_ids=None):
+    H, W = res
+    B, V, _ = pos.shape
+    E = edges.shape[0]
+    ids = np.arange(B * E, dtype=np.int32) if edge_ids is None else edge_ids
+    tp, te, ti = T(pos, dev), T(edges, dev), T(ids, dev)
+    P = ids.shape[0]
+    prefix = torch.empty(P, dtype=torch.int32, device=dev)
+    slopes = torch.full((P, 2), float("nan"), device=dev)
+    spans = torch.full((P, 4), float("nan"), device=dev)
+    n = call_count("s2c_dda_compute_span", P, ti, H, W, V, tp, E, te, prefix, slopes, spans)
+    fp = torch.full((n, 3), -1, dtype=torch.int32, device=dev)
+    fa = torch.zeros((n, 4), device=dev)
+    call("s2c_dda_compute_fragments", P, n, prefix, ti, slopes, spans, H, W, V, tp, E, te, fp, fa)
+    fpo, fao = torch.empty_like(fp), torch.empty_like(fa)
+    m = call_count("s2c_filter_valid_fragments", n, fp, fa, fpo, fao)
+    torch.cuda.synchronize()
+    return (prefix.cpu().numpy(), slopes.cpu().numpy(), spans.cpu().numpy(), n,
+            fpo[:m].cpu().numpy(), fao[:m].cpu().numpy())
+
+
+def check_dda(dev, res, pos, edges, edge_ids=None):
+    prefix, slopes, spans, n, fp, fa = dda_gpu(dev, res, pos, edges, edge_ids)
+    o_prefix, o_slopes, o_spans, o_n = oracle.dda_compute_span(res, pos, edges, edge_ids)
+    assert n == o_n
+    np.testing.assert_array_equal(prefix, o_prefix)
+    np.testing.assert_array_equal(slopes, o_slopes)  # NaN == NaN where unwritten
+    np.testing.assert_array_equal(spans, o_spans)
+    o_fp, o_fa = oracle.dda_compute_fragments(res, pos, edges, edge_ids)
+    np.testing.assert_array_equal(fp, o_fp)
+    np.testing.assert_array_equal(fa, o_fa)
+    return fp, fa
+
+
+# ------------------------------------------------------------------ DDA
+
+
+@pytest.mark.parametrize("persp", [False, True])
+@pytest.mark.parametrize("res", [(37, 53), (128, 96)])
+def test_dda_random_lines(dev, res, persp):
+    report(f"dda random lines res={res} persp={persp}")
+    rng = np.random.default_rng(11 + res[0] + persp)
+    pos, edges = random_lines(rng, 3, 400, 3000, *res, persp=persp)
+    check_dda(dev, res, pos, edges)
+    # subset of primitives in arbitrary order, as the antialias path uses
+    ids = rng.permutation(3 * 3000)[:1000].astype(np.int32)
+    check_dda(dev, res, pos, edges, ids)
+
+
+def test_dda_small_workload(dev):
+    report("dda small workload (2k strands, 256^2)")
+    w = synth.small_workload()
+    fp, fa = check_dda(dev, (w.H, w.W), w.pos, w.edges)
+    assert fp.shape[0] == 133424
+
+
+def test_dda_empty(dev):
+    report("dda empty inputs")
+    pos = np.zeros((1, 4, 4), np.float32)
+    pos[..., 3] = 1
+    edges = np.zeros((0, 2), np.int32)
+    prefix, slopes, spans, n, fp, fa = dda_gpu(dev, (8, 8), pos, edges)
+    assert n == 0 and fp.shape == (0, 3)
+
+
+# --------------------------------------------------------- DDA -> mask
+
+
+def fused_dda_mask(dev, res, pos, edges, e2c, K):
+    S = (K + 31) // 32
+    B, V, _ = pos.shape
+    bits = torch.full((B, res[0], res[1], S), 0x5A5A5A5A, dtype=torch.int32, device=dev)
+    call("s2c_dda_cluster_mask", B, res[0], res[1], V, T(pos, dev), edges.shape[0], T(edges, dev),
+         T(e2c, dev), S, bits)
+    return bits.cpu().numpy()
+
+
+def test_dda_mask_small_workload(dev):
+    report("dda->mask small workload")
+    w = synth.small_workload()
+    want = oracle.dda_cluster_mask((w.H, w.W), w.pos, w.edges, w.edge2cluster, 64)
+    got = fused_dda_mask(dev, (w.H, w.W), w.pos, w.edges, w.edge2cluster, 64)
+    np.testing.assert_array_equal(got, want)
+    # and the list form: fragments -> cluster_mask_from_fragments
+    _, _, _, _, fp, fa = dda_gpu(dev, (w.H, w.W), w.pos, w.edges)
+    bits = torch.zeros((1, w.H, w.W, 2), dtype=torch.int32, device=dev)
+    call("s2c_cluster_mask_from_fragments", fp.shape[0], T(fp, dev), T(fa, dev),
+         T(w.edge2cluster, dev), w.H, w.W, 2, bits)
+    np.testing.assert_array_equal(bits.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("K", [1, 33, 1024])
+def test_dda_mask_random(dev, K):
+    report(f"dda->mask random K={K}")
+    rng = np.random.default_rng(K)
+    pos, edges = random_lines(rng, 2, 300, 2500, 61, 70, persp=True)
+    e2c = rng.integers(0, K, size=edges.shape[0]).astype(np.int32)
+    want = oracle.dda_cluster_mask((61, 70), pos, edges, e2c, K)
+    got = fused_dda_mask(dev, (61, 70), pos, edges, e2c, K)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.slow
+def test_dda_mask_full_workload(dev):
+    report("dda->mask full workload (50k strands x 100, 4 views 1024^2, K=1024)")
+    w = synth.full_workload()
+    want = oracle.dda_cluster_mask((w.H, w.W), w.pos, w.edges, w.edge2cluster, 1024, threads=16)
+    got = fused_dda_mask(dev, (w.H, w.W), w.pos, w.edges, w.edge2cluster, 1024)
+    np.testing.assert_array_equal(got, want)
+
+
+# ------------------------------------------------------------ triangles
+
+
+def tri_gpu(dev, res, pos, tri):
+    H, W = res
+    B, V, _ = pos.shape
+    Tn = tri.shape[0]
+    tp, tt = T(pos, dev), T(tri, dev)
+    rects = torch.empty((B * Tn, 4), dtype=torch.int32, device=dev)
+    prefix = torch.empty(B * Tn, dtype=torch.int32, device=dev)
+    n = call_count("s2c_compute_triangle_rects", H, W, B, V, tp, Tn, tt, rects, prefix)
+    fp = torch.full((n, 3), -1, dtype=torch.int32, device=dev)
+    fa = torch.zeros((n, 4), device=dev)
+    call("s2c_compute_fragments", H, W, V, tp, Tn, tt, B * Tn, n, prefix, rects, fp, fa)
+    fpo, fao = torch.empty_like(fp), torch.empty_like(fa)
+    m = call_count("s2c_filter_valid_fragments", n, fp, fa, fpo, fao)
+    return rects.cpu().numpy(), prefix.cpu().numpy(), n, fpo[:m], fao[:m]
+
+
+def random_tris(rng, B, V, Tn, persp):
+    xy = rng.uniform(-1.2, 1.2, size=(B, V, 2))
+    z = rng.uniform(-1.1, 1.1, size=(B, V, 1))
+    w = rng.uniform(0.6, 1.8, size=(B, V, 1)) if persp else np.ones((B, V, 1))
+    pos = np.concatenate([xy * w, z * w, w], -1).astype(np.float32)
+    tri = rng.integers(0, V, size=(Tn, 3)).astype(np.int32)
+    return pos, tri
+
+
+@pytest.mark.parametrize("persp", [False, True])
+def test_triangles_random(dev, persp):
+    report(f"triangles random persp={persp}")
+    rng = np.random.default_rng(5 + persp)
+    res = (45, 64)
+    pos, tri = random_tris(rng, 2, 60, 150, persp)
+    rects, prefix, n, fp, fa = tri_gpu(dev, res, pos, tri)
+    o_rects, o_prefix, o_n = oracle.compute_triangle_rects(res, pos, tri)
+    assert n == o_n
+    np.testing.assert_array_equal(rects, o_rects)
+    np.testing.assert_array_equal(prefix, o_prefix)
+    o_fp, o_fa = oracle.compute_fragments(res, pos, tri)
+    np.testing.assert_array_equal(fp.cpu().numpy(), o_fp)
+    np.testing.assert_array_equal(fa.cpu().numpy(), o_fa)
+    # depth test
+    rast = torch.zeros((2, *res, 4), device=dev)
+    call("s2c_depth_test", 2, res[0], res[1], fp.shape[0], fp, fa, rast)
+    o_rast = oracle.depth_test(2, res, o_fp, o_fa)
+    np.testing.assert_array_equal(rast.cpu().numpy(), o_rast)
+    # interpolation forward (same op sequence -> exact) and backward (atomics)
+    C = 5
+    attr = rng.normal(size=(60, C)).astype(np.float32)
+    img = torch.zeros((2, *res, C), device=dev)
+    call("s2c_interpolate_triangle_attributes", 2, res[0], res[1], C, rast, T(attr, dev), T(tri, dev),
+         img)
+    np.testing.assert_array_equal(img.cpu().numpy(),
+                                  oracle.interpolate_triangle_attributes(o_rast, attr, tri))
+    d_img = rng.normal(size=(2, *res, C)).astype(np.float32)
+    d_attr = torch.zeros((60, C), device=dev)
+    call("s2c_backward_interpolate_triangle_attributes", 2, res[0], res[1], C, rast, T(d_img, dev),
+         T(tri, dev), d_attr)
+    normwise_close(d_attr.cpu().numpy(),
+                   oracle.backward_interpolate_triangle_attributes(o_rast, d_img, tri, 60))
+
+
+def test_triangles_card_workload(dev):
+    report("triangles: small workload cards -> fragments, mask, fused mask")
+    w = synth.small_workload()
+    c = w.cards
+    res = (w.H, w.W)
+    _, _, _, fp, fa = tri_gpu(dev, res, w.card_pos, c.tri)
+    o_fp, o_fa = oracle.compute_fragments(res, w.card_pos, c.tri)
+    np.testing.assert_array_equal(fp.cpu().numpy(), o_fp)
+    np.testing.assert_array_equal(fa.cpu().numpy(), o_fa)
+    want = oracle.triangle_cluster_mask(res, w.card_pos, c.tri, c.tri2cluster, 64)
+    bits = torch.full((1, *res, 2), -1, dtype=torch.int32, device=dev)
+    call("s2c_triangle_cluster_mask", 1, res[0], res[1], w.card_pos.shape[1], T(w.card_pos, dev),
+         c.tri.shape[0], T(c.tri, dev), T(c.tri2cluster, dev), 2, bits)
+    np.testing.assert_array_equal(bits.cpu().numpy(), want)
+
+
+# ---------------------------------------------------------------- bitset
+
+
+@pytest.mark.parametrize("S", [1, 3, 4, 32])
+def test_bitset_reductions(dev, S):
+    report(f"bitset rgb/popcount S={S}")
+    rng = np.random.default_rng(S)
+    bits = rng.integers(-2**31, 2**31, size=(2, 19, 23, S), dtype=np.int64).astype(np.int32)
+    bits[bits % 3 == 0] = 0
+    rgb = rng.uniform(size=(32 * S, 3)).astype(np.float32)
+    out = torch.zeros((2, 19, 23, 3), device=dev)
+    call("s2c_accumulate_bitset_rgb", 2, 19, 23, S, T(bits, dev), T(rgb, dev), out)
+    np.testing.assert_array_equal(out.cpu().numpy(), oracle.accumulate_bitset_rgb(bits, rgb))
+    cnt = torch.zeros((2, 19, 23), device=dev)
+    call("s2c_popcount_bitset", 2, 19, 23, S, T(bits, dev), cnt)
+    np.testing.assert_array_equal(cnt.cpu().numpy(), oracle.popcount_bitset(bits))
+
+
+# ------------------------------------------------------------- antialias
+
+
+def test_antialias_small_workload(dev):
+    report("antialias: mark, compact, silhouette DDA, backward (small workload)")
+    w = synth.small_workload()
+    c = w.cards
+    res = (w.H, w.W)
+    B, V, _ = w.card_pos.shape
+    E = c.edges.shape[0]
+    target = oracle.dda_cluster_mask(res, w.pos, w.edges, w.edge2cluster, 64)
+    cards = oracle.triangle_cluster_mask(res, w.card_pos, c.tri, c.tri2cluster, 64)
+    tp = T(w.card_pos, dev)
+    prim = torch.full((B * E,), -1, dtype=torch.int32, device=dev)
+    normals = torch.zeros((B * E, 2), device=dev)
+    call("s2c_mark_discontinuity_edges", res[0], res[1], B, V, tp, T(c.tri, dev), E, T(c.edges, dev),
+         T(c.edge2tri, dev), prim, normals)
+    out = torch.empty_like(prim)
+    m = call_count("s2c_compact_valid_ints", B * E, prim, out)
+    o_prim, o_normals = oracle.mark_discontinuity_edges(res, w.card_pos, c.tri, c.edges, c.edge2tri)
+    np.testing.assert_array_equal(out[:m].cpu().numpy(), o_prim)
+    np.testing.assert_array_equal(normals.cpu().numpy(), o_normals)
+    _, _, _, _, fp, fa = dda_gpu(dev, res, w.card_pos, c.edges, o_prim)
+    for rho in (1.0, 0.5):
+        d_pos = torch.zeros((B, V, 4), device=dev)
+        call("s2c_backward_antialiased_cluster_bitset", fp.shape[0], T(fp, dev), T(fa, dev), res[0],
+             res[1], 2, T(cards, dev), T(target, dev), V, tp, E, T(c.edges, dev), normals,
+             T(c.edge2cluster, dev), d_pos, 7.0, rho)
+        want = oracle.antialiased_cluster_bitset_grad(cards, target, w.card_pos, c.tri, c.edges,
+                                                      c.edge2tri, c.edge2cluster, 7.0, rho)
+        assert np.abs(want).max() > 0
+        normwise_close(d_pos.cpu().numpy(), want)
+
+
+# ---------------------------------------------------------------- curves
+
+
+def test_smooth_and_skinning(dev):
+    report("smooth_strands / skinning")
+    rng = np.random.default_rng(3)
+    x = synth.generate_strands(300, 40, 0.004, 3)
+    s = oracle.compute_arclength(x)
+    for beta in (0.002, 0.01, 0.05):
+        out = torch.zeros(x.shape, device=dev)
+        call("s2c_smooth_strands", 300, 40, T(x, dev), T(s, dev), out, beta)
+        normwise_close(out.cpu().numpy(), oracle.smooth_strands(x, s, beta), 2e-6)
+    h = x + rng.normal(scale=1e-3, size=x.shape).astype(np.float32)
+    cn = x + rng.normal(scale=1e-3, size=x.shape).astype(np.float32)
+    q, _ = np.linalg.qr(rng.normal(size=(300, 40, 3, 3)))
+    R = q.astype(np.float32)
+    out = torch.zeros(x.shape, device=dev)
+    call("s2c_skinning", 300, 40, T(x, dev), T(s, dev), T(h, dev), T(cn, dev), T(R, dev), out, 0.01)
+    normwise_close(out.cpu().numpy(), oracle.skinning(x, s, h, cn, R, 0.01), 2e-6)
