@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Benchmark of the strns2cards hot path on one MI355X.
+
+Workload (synthetic, seeded; no dataset is available): 50,000 strands x 100
+vertices fp32 (step 0.003, seed 1), 1,024 k-means clusters, 4 orthographic
+views at 1024 x 1024, 1,024-bit cluster bitset per pixel (32 words), plus
+1,024 cards x 20 quads for the triangle / antialias stages.
+
+Timed step: the strand DDA -> cluster-bitset pass over all 4 views
+(``_core.dda_cluster_mask``: clear + fused DDA/OR kernel), inputs resident in
+HBM.  ``value`` is DDA GB/s with algorithmic bytes
+B*V*16 + E*8 + E*4 + B*H*W*S*4 (vertex reads per view, edges and
+edge->cluster once, bitset written once) per step.  The card raster -> mask
+pass and the antialias edge gradient at the same size are timed in the same
+run and reported as extra fields.  ``cpu_baseline`` times the CPU oracle
+(a C restatement of the reference kernels, OpenMP) on one of the 4 views.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "card-fit iters/s served vs plain loop at equal loss; p50/p99 latency; DDA GB/s"
+
+
+def dda_mask_bytes(B, V, E, H, W, S):
+    return B * V * 16 + E * 8 + E * 4 + B * H * W * S * 4
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the DDA->mask entry point, from the committed
+    rocprofv3 PMC summary (None when absent)."""
+    p = os.path.join(ROOT, "profiles", "dda_mask_full_traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(w, seconds):
+    """Oracle DDA->mask on view 0 of the workload, repeated for ~`seconds`."""
+    import numpy as np
+
+    import oracle
+
+    oracle.lib()
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    pos0 = np.ascontiguousarray(w.pos[:1])
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        oracle.dda_cluster_mask((w.H, w.W), pos0, w.edges, w.edge2cluster, w.num_clusters,
+                                threads=threads)
+        reps += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = (time.perf_counter() - t0) / reps
+    S = (w.num_clusters + 31) // 32
+    nbytes = dda_mask_bytes(1, pos0.shape[1], w.edges.shape[0], w.H, w.W, S)
+    return {"value": round(nbytes / dt / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"view 0 of 4 (all 50k strands, 1024^2, S=32), {reps} rep(s), "
+                      f"{dt * 1e3:.1f} ms/rep"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
+              "the path's multi-GPU sharding is design-only.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import strns2cards_amd as s2c
+    from strns2cards_amd import _core, synth
+
+    dev = torch.device("cuda", 0)
+    t_gen = time.perf_counter()
+    w = synth.full_workload()
+    gen_s = time.perf_counter() - t_gen
+    print(f"[bench] workload ready in {gen_s:.1f}s: {w.name}", file=sys.stderr, flush=True)
+
+    B, V, _ = w.pos.shape
+    E = w.edges.shape[0]
+    S = (w.num_clusters + 31) // 32
+    pos = torch.from_numpy(w.pos).to(dev)
+    edges = torch.from_numpy(w.edges).to(dev)
+    e2c = torch.from_numpy(w.edge2cluster).to(dev)
+    bitset = torch.empty((B, w.H, w.W, S), dtype=torch.int32, device=dev)
+
+    def step():
+        _core.dda_cluster_mask(pos, edges, e2c, bitset)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    t_all0 = torch.cuda.Event(enable_timing=True)
+    t_all1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_all0.record(stream)
+    for a, b in evs:
+        a.record(stream)
+        step()
+        b.record(stream)
+    t_all1.record(stream)
+    torch.cuda.synchronize()
+    ms_total = t_all0.elapsed_time(t_all1)
+    ms_step = ms_total / args.steps
+    per_launch = sorted(a.elapsed_time(b) for a, b in evs)
+    ms_launch = sum(per_launch) / len(per_launch)
+    nbytes = dda_mask_bytes(B, V, E, w.H, w.W, S)
+    value = nbytes / (ms_step / 1e3) / 1e9
+    achieved = nbytes / (ms_launch / 1e3) / 1e9
+    print(f"[bench] dda->mask {ms_step:.3f} ms/step, {value:.1f} GB/s", file=sys.stderr, flush=True)
+
+    # Card stages at the same size (reported beside the headline).
+    c = w.cards
+    cpos = torch.from_numpy(w.card_pos).to(dev)
+    tri = torch.from_numpy(c.tri).to(dev)
+    t2c = torch.from_numpy(c.tri2cluster).to(dev)
+    cedges = torch.from_numpy(c.edges).to(dev)
+    e2t = torch.from_numpy(c.edge2tri).to(dev)
+    ce2c = torch.from_numpy(c.edge2cluster).to(dev)
+    cbits = torch.empty_like(bitset)
+
+    def timed(fn, reps=10):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    ms_tri = timed(lambda: _core.triangle_cluster_mask(cpos, tri, t2c, cbits))
+    ms_aa = timed(lambda: s2c.antialiased_cluster_bitset_loss(
+        cbits, bitset, cpos, tri, cedges, e2t, ce2c, 7.0, 0.5, False))
+    print(f"[bench] card raster->mask {ms_tri:.3f} ms, antialias grad {ms_aa:.3f} ms",
+          file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(w, args.cpu_seconds)
+        print(f"[bench] cpu baseline {cpu}", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GB/s (DDA->mask algorithmic bytes)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded strands/cards; no dataset)",
+        "config": {"workload": w.name, "views": B, "resolution": [w.H, w.W],
+                   "strands": int(w.strands.shape[0]), "vertices_per_strand": int(w.strands.shape[1]),
+                   "clusters": w.num_clusters, "bitset_words": S, "edges": E,
+                   "algorithmic_bytes_per_step": nbytes},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": traffic_from_profiles(),
+                     "kernel": "s2c_dda_cluster_mask (fill_u32_kernel + dda_mask_kernel)",
+                     "avg_launch_ms": round(ms_launch, 4)},
+        "cpu_baseline": cpu,
+        "extra": {"card_raster_mask_ms": round(ms_tri, 4), "antialias_grad_ms": round(ms_aa, 4),
+                  "cards": int(c.tri.shape[0] // 40), "card_triangles": int(c.tri.shape[0])},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -8,8 +8,8 @@
 // checks do), then calls the C ABI on the current HIP stream of the
 // tensor's device.  HIP failures raise RuntimeError instead of exit().
 #include <torch/extension.h>
-#include <c10/hip/HIPStream.h>
-#include <c10/hip/HIPGuard.h>
+#include <ATen/hip/impl/HIPGuardImplMasqueradingAsCUDA.h>
+#include <ATen/hip/impl/HIPStreamMasqueradingAsCUDA.h>
 
 #include <string>
 #include <vector>
@@ -18,6 +18,9 @@
 
 namespace py = pybind11;
 using at::Tensor;
+// On PyTorch-ROCm, GPU tensors carry the CUDA device type; this guard
+// accepts that and switches the current HIP device.
+using DeviceGuard = c10::hip::HIPGuardMasqueradingAsCUDA;
 
 namespace {
 
@@ -58,7 +61,8 @@ void check(int rc, const char* fn) {
 }
 
 void* stream_of(const Tensor& t) {
-    return static_cast<void*>(c10::hip::getCurrentHIPStream(t.device().index()).stream());
+    return static_cast<void*>(
+        c10::hip::getCurrentHIPStreamMasqueradingAsCUDA(t.device().index()).stream());
 }
 
 int I(int64_t v, const char* fn) {
@@ -94,7 +98,7 @@ PYBIND11_MODULE(_core, m) {
             throw std::runtime_error("`triangle_rects` must have shape [B * T, 4].");
         if (prefix.size(0) != static_cast<int64_t>(B) * T)
             throw std::runtime_error("`frag_prefix_sum` must have shape [B * T].");
-        c10::hip::HIPGuard g(pos.device());
+        DeviceGuard g(pos.device());
         int n = 0;
         {
             py::gil_scoped_release nogil;
@@ -123,7 +127,7 @@ PYBIND11_MODULE(_core, m) {
             throw std::runtime_error("Mismatch in fragment count or prefix sum length.");
         if (rects.size(0) != num_tris)
             throw std::runtime_error("`triangle_rects` must have shape [B * T, 4].");
-        c10::hip::HIPGuard g(pos.device());
+        DeviceGuard g(pos.device());
         check(s2c_compute_fragments(H, W, V, P<float>(pos), T, P<int>(tri), num_tris, N,
                                     P<int>(prefix), P<int>(rects), P<int>(frag_pix),
                                     P<float>(frag_attrs), stream_of(pos)),
@@ -142,7 +146,7 @@ PYBIND11_MODULE(_core, m) {
             throw std::runtime_error("frag_attrs must match frag_pix in size.");
         if (rast_out.size(1) != H || rast_out.size(2) != W)
             throw std::runtime_error("rast_out must have shape (B, H, W, 4).");
-        c10::hip::HIPGuard g(rast_out.device());
+        DeviceGuard g(rast_out.device());
         check(s2c_depth_test(I(rast_out.size(0), fn), H, W, N, P<int>(frag_pix),
                              P<float>(frag_attrs), P<float>(rast_out), stream_of(rast_out)),
               fn);
@@ -159,7 +163,7 @@ PYBIND11_MODULE(_core, m) {
         const int N = I(fp.size(0), fn);
         if (fa.size(0) != N || fp_out.size(0) != N || fa_out.size(0) != N)
             throw std::runtime_error("Mismatch in fragment counts between input/output tensors.");
-        c10::hip::HIPGuard g(fp.device());
+        DeviceGuard g(fp.device());
         int n = 0;
         {
             py::gil_scoped_release nogil;
@@ -182,7 +186,7 @@ PYBIND11_MODULE(_core, m) {
         const int C = I(attr.size(1), fn);
         if (image.size(0) != B || image.size(1) != H || image.size(2) != W || image.size(3) != C)
             throw std::runtime_error("image must have shape [B, H, W, C]");
-        c10::hip::HIPGuard g(rast.device());
+        DeviceGuard g(rast.device());
         check(s2c_interpolate_triangle_attributes(B, H, W, C, P<float>(rast), P<float>(attr),
                                                   P<int>(tri), P<float>(image), stream_of(rast)),
               fn);
@@ -202,7 +206,7 @@ PYBIND11_MODULE(_core, m) {
               if (d_image.size(0) != B || d_image.size(1) != H || d_image.size(2) != W ||
                   d_image.size(3) != C)
                   throw std::runtime_error("`d_image` must have shape [B, H, W, C] matching rast");
-              c10::hip::HIPGuard g(rast.device());
+              DeviceGuard g(rast.device());
               check(s2c_backward_interpolate_triangle_attributes(
                         B, H, W, C, P<float>(rast), P<float>(d_image), P<int>(tri),
                         P<float>(d_attr), stream_of(rast)),
@@ -222,7 +226,7 @@ PYBIND11_MODULE(_core, m) {
               if (fa.size(0) != N)
                   throw std::runtime_error(
                       "frag_pix and frag_attrs must have the same number of fragments.");
-              c10::hip::HIPGuard g(bitset.device());
+              DeviceGuard g(bitset.device());
               check(s2c_cluster_mask_from_fragments(N, P<int>(fp), P<float>(fa),
                                                     P<int>(prim2cluster), I(bitset.size(1), fn),
                                                     I(bitset.size(2), fn), I(bitset.size(3), fn),
@@ -243,7 +247,7 @@ PYBIND11_MODULE(_core, m) {
             throw std::runtime_error("Too few slots for the number of clusters.");
         if (out.size(0) != B || out.size(1) != H || out.size(2) != W)
             throw std::runtime_error("accum_color must have shape [B, H, W, 3]");
-        c10::hip::HIPGuard g(bitset.device());
+        DeviceGuard g(bitset.device());
         check(s2c_accumulate_bitset_rgb(B, H, W, S, P<int>(bitset), P<float>(rgb), P<float>(out),
                                         stream_of(bitset)),
               fn);
@@ -258,7 +262,7 @@ PYBIND11_MODULE(_core, m) {
         const int B = I(bitset.size(0), fn), H = I(bitset.size(1), fn), W = I(bitset.size(2), fn);
         if (out.size(0) != B || out.size(1) != H || out.size(2) != W)
             throw std::runtime_error("count_image must have shape [B, H, W]");
-        c10::hip::HIPGuard g(bitset.device());
+        DeviceGuard g(bitset.device());
         check(s2c_popcount_bitset(B, H, W, I(bitset.size(3), fn), P<int>(bitset), P<float>(out),
                                   stream_of(bitset)),
               fn);
@@ -279,7 +283,7 @@ PYBIND11_MODULE(_core, m) {
         if (prefix.size(0) != P_ || slopes.size(0) != P_ || spans.size(0) != P_)
             throw std::runtime_error(
                 "Invalid output tensor shapes for frag_prefix_sum, frag_slopes, or frag_spans.");
-        c10::hip::HIPGuard g(pos.device());
+        DeviceGuard g(pos.device());
         int n = 0;
         {
             py::gil_scoped_release nogil;
@@ -310,7 +314,7 @@ PYBIND11_MODULE(_core, m) {
         if (prefix.size(0) != P_ || slopes.size(0) != P_ || spans.size(0) != P_ ||
             frag_attrs.size(0) != N)
             throw std::runtime_error("Invalid shape for input or output arrays.");
-        c10::hip::HIPGuard g(pos.device());
+        DeviceGuard g(pos.device());
         check(s2c_dda_compute_fragments(P_, N, P<int>(prefix), P<int>(edge_ids), P<float>(slopes),
                                         P<float>(spans), H, W, I(pos.size(1), fn), P<float>(pos),
                                         I(edges.size(0), fn), P<int>(edges), P<int>(frag_pix),
@@ -330,7 +334,7 @@ PYBIND11_MODULE(_core, m) {
         const int C = I(attr.size(1), fn);
         if (image.size(0) != B || image.size(1) != H || image.size(2) != W || image.size(3) != C)
             throw std::runtime_error("image must have shape [B, H, W, C]");
-        c10::hip::HIPGuard g(rast.device());
+        DeviceGuard g(rast.device());
         check(s2c_dda_interpolate_attributes(B, H, W, C, P<float>(rast), P<float>(attr),
                                              P<int>(edges), P<float>(image), stream_of(rast)),
               fn);
@@ -353,7 +357,7 @@ PYBIND11_MODULE(_core, m) {
               if (d_attr.size(1) != C)
                   throw std::runtime_error(
                       "`d_attr` must have shape [V, C] where C matches d_image.shape(3)");
-              c10::hip::HIPGuard g(rast.device());
+              DeviceGuard g(rast.device());
               check(s2c_backward_dda_interpolate_attributes(B, H, W, P<float>(rast), C,
                                                             P<float>(d_image), P<int>(edges),
                                                             P<float>(d_attr), stream_of(rast)),
@@ -380,7 +384,7 @@ PYBIND11_MODULE(_core, m) {
         if (normals.size(0) != static_cast<int64_t>(B) * E)
             throw std::runtime_error(
                 "mark_discontinuity_edges: normals must have shape (B * E, 2).");
-        c10::hip::HIPGuard g(pos.device());
+        DeviceGuard g(pos.device());
         check(s2c_mark_discontinuity_edges(H, W, B, V, P<float>(pos), P<int>(tri), E,
                                            P<int>(edges), P<int>(edge2tri), P<int>(prim_ids),
                                            P<float>(normals), stream_of(pos)),
@@ -418,7 +422,7 @@ PYBIND11_MODULE(_core, m) {
               if (d_pos.size(0) != B || d_pos.size(1) != V)
                   throw std::runtime_error(
                       "backward_antialiased_bitset: d_pos must match pos shape.");
-              c10::hip::HIPGuard g(pos.device());
+              DeviceGuard g(pos.device());
               check(s2c_backward_antialiased_cluster_bitset(
                         N, P<int>(fp), P<float>(fa), H, W, S, P<int>(bitset), P<int>(target), V,
                         P<float>(pos), E, P<int>(edges), P<float>(normals), P<int>(edge2cluster),
@@ -435,7 +439,7 @@ PYBIND11_MODULE(_core, m) {
         if (in.size(0) != out.size(0))
             throw std::runtime_error(
                 "compact_valid_ints: input and output must have the same size.");
-        c10::hip::HIPGuard g(in.device());
+        DeviceGuard g(in.device());
         int n = 0;
         {
             py::gil_scoped_release nogil;
@@ -460,7 +464,7 @@ PYBIND11_MODULE(_core, m) {
         if (out.size(0) != N || out.size(1) != Pn)
             throw std::runtime_error(
                 "smooth_strands: smoothed_strands must match (num_strands, num_points, 3).");
-        c10::hip::HIPGuard g(strands.device());
+        DeviceGuard g(strands.device());
         check(s2c_smooth_strands(N, Pn, P<float>(strands), P<float>(arclength), P<float>(out),
                                  beta, stream_of(strands)),
               fn);
@@ -492,7 +496,7 @@ PYBIND11_MODULE(_core, m) {
         if (!match(out))
             throw std::runtime_error(
                 "skinning: skinned_strands must match (num_strands, num_points, 3).");
-        c10::hip::HIPGuard g(strands.device());
+        DeviceGuard g(strands.device());
         check(s2c_skinning(N, Pn, P<float>(strands), P<float>(arclength), P<float>(handle),
                            P<float>(canonical), P<float>(R), P<float>(out), beta,
                            stream_of(strands)),
@@ -511,7 +515,7 @@ PYBIND11_MODULE(_core, m) {
             throw std::runtime_error("dda_cluster_mask: bitset must have B = pos.shape[0] views");
         if (edge2cluster.size(0) != edges.size(0))
             throw std::runtime_error("dda_cluster_mask: edge2cluster must have E entries");
-        c10::hip::HIPGuard g(pos.device());
+        DeviceGuard g(pos.device());
         check(s2c_dda_cluster_mask(I(pos.size(0), fn), I(bitset.size(1), fn), I(bitset.size(2), fn),
                                    I(pos.size(1), fn), P<float>(pos), I(edges.size(0), fn),
                                    P<int>(edges), P<int>(edge2cluster), I(bitset.size(3), fn),
@@ -531,7 +535,7 @@ PYBIND11_MODULE(_core, m) {
                 "triangle_cluster_mask: bitset must have B = pos.shape[0] views");
         if (tri2cluster.size(0) != tri.size(0))
             throw std::runtime_error("triangle_cluster_mask: tri2cluster must have T entries");
-        c10::hip::HIPGuard g(pos.device());
+        DeviceGuard g(pos.device());
         check(s2c_triangle_cluster_mask(I(pos.size(0), fn), I(bitset.size(1), fn),
                                         I(bitset.size(2), fn), I(pos.size(1), fn), P<float>(pos),
                                         I(tri.size(0), fn), P<int>(tri), P<int>(tri2cluster),
