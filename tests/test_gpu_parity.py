@@ -140,7 +140,7 @@ def test_dda_mask_small_workload(dev):
     np.testing.assert_array_equal(bits.cpu().numpy(), want)
 
 
-@pytest.mark.parametrize("K", [1, 33, 1024])
+@pytest.mark.parametrize("K", [1, 33, 1024, 17000])  # 17000: tile bitset exceeds LDS -> direct ORs
 def test_dda_mask_random(dev, K):
     report(f"dda->mask random K={K}")
     rng = np.random.default_rng(K)
