@@ -59,45 +59,66 @@ mark_edges_kernel(int H, int W, long long BE, int V, const float* __restrict__ p
     reinterpret_cast<float2*>(normals)[idx] = make_float2(sign * nx, sign * ny);
 }
 
-// antialias.cu:121-132 with HIP's accurate expf.
-__device__ __forceinline__ float gauss_w(float x, float y, int w, int h, float R) {
+__device__ __forceinline__ unsigned bit_u(const int* bs, long long pix, int S, int k) {
+    return (static_cast<unsigned>(bs[pix * S + (k >> 5)]) >> (k & 31)) & 1u;
+}
+
+// Residual bits of a window pixel: bit 0 = rendered (c0), bit 1 = target (c1).
+struct ResidualSrc {
+    const unsigned char* box;  // staged bytes, or nullptr for global reads
+    int h0, w0, width;
+    const int* bitset;
+    const int* target;
+    long long plane;
+    int W, S, k;
+    __device__ __forceinline__ unsigned at(int h, int w) const {
+        if (box) return box[(h - h0) * width + (w - w0)];
+        const long long pix = plane + w + static_cast<long long>(h) * W;
+        return bit_u(bitset, pix, S, k) | (bit_u(target, pix, S, k) << 1);
+    }
+};
+
+// antialias.cu:147-176 (compute_adjoint_sum with gaussian_weight,
+// antialias.cu:121-132).  Window pixels outside the radius (weight 0) or
+// with c0 == c1 (residual 0) add an exact zero in the reference and are
+// skipped here, so expf runs only where card and target differ; the
+// remaining terms are summed in the reference's order (h, then w).
+__device__ __forceinline__ float adjoint_sum(float x, float y, int H, int W, float R, float rho,
+                                             const ResidualSrc& src) {
     constexpr float pi = 3.14159265358979323846f;
     const float sigma = R / 4.f;
     const float c = 1.f / (sigma * sigma * 2.f * pi);
     const float alpha = 0.5f / (sigma * sigma);
-    const float dx = static_cast<float>(w) + 0.5f - x;
-    const float dy = static_cast<float>(h) + 0.5f - y;
-    const float n2 = dx * dx + dy * dy;
-    return c * expf(-alpha * n2) * static_cast<float>(n2 < R * R);
-}
-
-__device__ __forceinline__ float bit_at(const int* bs, long long pix, int S, int k) {
-    return static_cast<float>((static_cast<unsigned>(bs[pix * S + (k >> 5)]) >> (k & 31)) & 1u);
-}
-
-// antialias.cu:147-176.
-__device__ float adjoint_sum(int b, float x, float y, int k, int H, int W, int S,
-                             const int* __restrict__ bs, const int* __restrict__ tg, float R,
-                             float rho) {
+    const float R2 = R * R;
     const int Ri = f2i(floorf(R));
     const int cw = f2i(floorf(x)), ch = f2i(floorf(y));
     const int h0 = dmax(ch - Ri, 0), h1 = dmin(ch + Ri, H - 1) + 1;
     const int w0 = dmax(cw - Ri, 0), w1 = dmin(cw + Ri, W - 1) + 1;
-    const long long plane = static_cast<long long>(H) * W * b;
     float dc = 0.f;
     for (int h = h0; h < h1; ++h) {
+        const float dy = static_cast<float>(h) + 0.5f - y;
         for (int w = w0; w < w1; ++w) {
-            const long long pix = plane + w + static_cast<long long>(h) * W;
-            const float c0 = bit_at(bs, pix, S, k);
-            const float c1 = bit_at(tg, pix, S, k);
-            const float wt = gauss_w(x, y, w, h, R);
-            dc += wt * (c0 - c1) * (c1 == 0.f ? rho : 1.f);
+            const float dx = static_cast<float>(w) + 0.5f - x;
+            const float n2 = dx * dx + dy * dy;
+            if (!(n2 < R2)) continue;
+            const unsigned code = src.at(h, w);
+            const unsigned c0 = code & 1u, c1 = code >> 1;
+            if (c0 == c1) continue;
+            const float wt = c * expf(-alpha * n2) * 1.f;
+            dc += wt * (static_cast<float>(c0) - static_cast<float>(c1)) * (c1 == 0u ? rho : 1.f);
         }
     }
     return dc;
 }
 
-// antialias.cu:179-279.
+// antialias.cu:179-279.  A wave takes 64 consecutive fragments.  When all of
+// its live samples share one view and cluster (consecutive silhouette
+// fragments of one card) and the union of their windows fits kAaBox bytes,
+// the wave stages that box's residual bits in LDS once; otherwise every
+// window pixel is read from the bitsets directly.
+constexpr int kAaWaves = kBlock / kWave;
+constexpr int kAaBox = 8192;
+
 __global__ void __launch_bounds__(kBlock)
 antialias_bwd_kernel(int N, const int* __restrict__ frag_pix, const float* __restrict__ fa,
                      int H, int W, int S, const int* __restrict__ bitset,
@@ -105,53 +126,105 @@ antialias_bwd_kernel(int N, const int* __restrict__ frag_pix, const float* __res
                      const int* __restrict__ edges, const float* __restrict__ normals,
                      const int* __restrict__ edge2cluster, float* __restrict__ d_pos, float R,
                      float rho) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= N) return;
-    const int b = frag_pix[i * 3 + 0];
-    const int h = frag_pix[i * 3 + 1];
-    const int w = frag_pix[i * 3 + 2];
-    const float4 a = reinterpret_cast<const float4*>(fa)[i];
-    const float axis = a.x;
-    const float t1 = a.y;
-    const float t0 = 1.f - t1;
-    const int e = f2i(a.w) - 1;
-    if (e < 0) return;
-    const float* p = pos + static_cast<long long>(b) * V * 4;
-    const float* nb = normals + static_cast<long long>(b) * E * 2;
-    float* dp = d_pos + static_cast<long long>(b) * V * 4;
+    __shared__ unsigned char box[kAaWaves][kAaBox];
+    const int wv = threadIdx.x / kWave;
+    const long long i = (static_cast<long long>(blockIdx.x) * kAaWaves + wv) * kWave + lane_id();
     const float fH = static_cast<float>(H), fW = static_cast<float>(W);
-    const int v0 = edges[e * 2 + 0], v1 = edges[e * 2 + 1];
-    const float4 P0 = reinterpret_cast<const float4*>(p)[v0];
-    const float4 P1 = reinterpret_cast<const float4*>(p)[v1];
-    const float px = t0 * P0.x + t1 * P1.x;
-    const float py = t0 * P0.y + t1 * P1.y;
-    const float pw = t0 * P0.w + t1 * P1.w;
-    const float fx = 0.5f * (1.f + px / pw) * fW;
-    const float fy = 0.5f * (1.f - py / pw) * fH;
-    const float2 n = reinterpret_cast<const float2*>(nb)[e];
-    int nh = h, nw = w;
-    const float fxr = fx - floorf(fx);
-    const float fyr = fy - floorf(fy);
-    constexpr float eps = 1e-4f;
-    float dot = 0.f;
-    if (axis == 0.f) {
-        if (!isfinite(fxr) || fabsf(fxr - 0.5f) >= eps) return;
-        const int d = 2 * static_cast<int>(fyr > 0.5f) - 1;
-        nh += d;
-        dot += static_cast<float>(d) * n.y;
-    } else {
-        if (!isfinite(fyr) || fabsf(fyr - 0.5f) >= eps) return;
-        const int d = 2 * static_cast<int>(fxr > 0.5f) - 1;
-        nw += d;
-        dot += static_cast<float>(d) * n.x;
+
+    // ---- per-sample setup, as the reference (antialias.cu:198-256)
+    bool live = false;
+    int b = 0, h = 0, w = 0, nh = 0, nw = 0, k = 0, v0 = 0, v1 = 0;
+    float t0 = 0.f, t1 = 0.f, px = 0.f, py = 0.f, pw = 1.f, fx = 0.f, fy = 0.f, dot = 0.f;
+    float2 n = make_float2(0.f, 0.f);
+    if (i < N) {
+        b = frag_pix[i * 3 + 0];
+        h = frag_pix[i * 3 + 1];
+        w = frag_pix[i * 3 + 2];
+        const float4 a = reinterpret_cast<const float4*>(fa)[i];
+        const float axis = a.x;
+        t1 = a.y;
+        t0 = 1.f - t1;
+        const int e = f2i(a.w) - 1;
+        if (e >= 0) {
+            const float* p = pos + static_cast<long long>(b) * V * 4;
+            v0 = edges[e * 2 + 0];
+            v1 = edges[e * 2 + 1];
+            const float4 P0 = reinterpret_cast<const float4*>(p)[v0];
+            const float4 P1 = reinterpret_cast<const float4*>(p)[v1];
+            px = t0 * P0.x + t1 * P1.x;
+            py = t0 * P0.y + t1 * P1.y;
+            pw = t0 * P0.w + t1 * P1.w;
+            fx = 0.5f * (1.f + px / pw) * fW;
+            fy = 0.5f * (1.f - py / pw) * fH;
+            n = reinterpret_cast<const float2*>(normals + static_cast<long long>(b) * E * 2)[e];
+            nh = h;
+            nw = w;
+            const float fxr = fx - floorf(fx);
+            const float fyr = fy - floorf(fy);
+            constexpr float eps = 1e-4f;
+            bool ok;
+            if (axis == 0.f) {
+                ok = isfinite(fxr) && !(fabsf(fxr - 0.5f) >= eps);
+                const int d = 2 * static_cast<int>(fyr > 0.5f) - 1;
+                nh += d;
+                dot += static_cast<float>(d) * n.y;
+            } else {
+                ok = isfinite(fyr) && !(fabsf(fyr - 0.5f) >= eps);
+                const int d = 2 * static_cast<int>(fxr > 0.5f) - 1;
+                nw += d;
+                dot += static_cast<float>(d) * n.x;
+            }
+            // (h, w) outside the image only arises from a malformed fragment
+            // list; such samples are dropped instead of read out of bounds.
+            live = ok && !(nw < 0 || nw >= W || nh < 0 || nh >= H || dot == 0.f) &&
+                   h >= 0 && h < H && w >= 0 && w < W && b >= 0;
+            if (live) k = edge2cluster[e];
+        }
     }
-    if (nw < 0 || nw >= W || nh < 0 || nh >= H || dot == 0.f) return;
-    const int k = edge2cluster[e];
-    const long long plane = static_cast<long long>(H) * W * b;
-    const float c01 = (dot > 0.f)
-        ? 1.f - bit_at(bitset, plane + nw + static_cast<long long>(nh) * W, S, k)
-        : 1.f - bit_at(bitset, plane + w + static_cast<long long>(h) * W, S, k);
-    const float dc = c01 * adjoint_sum(b, fx, fy, k, H, W, S, bitset, target, R, rho);
+    const int ch = live ? f2i(floorf(fy)) : 0, cw = live ? f2i(floorf(fx)) : 0;
+    const int Ri = f2i(floorf(R));
+    const int ph = (dot > 0.f) ? nh : h;  // pixel whose coverage gates the sample (c01)
+    const int pwx = (dot > 0.f) ? nw : w;
+
+    // ---- wave-uniform staging decision
+    const unsigned long long lm = __ballot(live);
+    int bh0 = 0, bw0 = 0, bwid = 0, area = 0;
+    bool staged = false;
+    if (lm) {
+        const int first = __ffsll(static_cast<long long>(lm)) - 1;
+        const int b_ref = __shfl(b, first, kWave), k_ref = __shfl(k, first, kWave);
+        const bool same = !live || (b == b_ref && k == k_ref);
+        if (__ballot(!same) == 0) {
+            const int lo_h = live ? dmin(dmax(ch - Ri, 0), ph) : INT_MAX;
+            const int hi_h = live ? dmax(dmin(ch + Ri, H - 1), ph) : INT_MIN;
+            const int lo_w = live ? dmin(dmax(cw - Ri, 0), pwx) : INT_MAX;
+            const int hi_w = live ? dmax(dmin(cw + Ri, W - 1), pwx) : INT_MIN;
+            bh0 = wave_min(lo_h);
+            bw0 = wave_min(lo_w);
+            const long long hh = static_cast<long long>(wave_max(hi_h)) - bh0 + 1;
+            const long long ww = static_cast<long long>(wave_max(hi_w)) - bw0 + 1;
+            if (hh > 0 && ww > 0 && hh * ww <= kAaBox) {
+                staged = true;
+                bwid = static_cast<int>(ww);
+                area = static_cast<int>(hh * ww);
+                const long long plane = static_cast<long long>(H) * W * b_ref;
+                for (int q = lane_id(); q < area; q += kWave) {
+                    const int y = bh0 + q / bwid, x = bw0 + q % bwid;  // in [0,H) x [0,W)
+                    const long long pix = plane + x + static_cast<long long>(y) * W;
+                    box[wv][q] = static_cast<unsigned char>(bit_u(bitset, pix, S, k_ref) |
+                                                            (bit_u(target, pix, S, k_ref) << 1));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+
+    ResidualSrc src{staged ? box[wv] : nullptr, bh0, bw0, bwid, bitset, target,
+                    static_cast<long long>(H) * W * b, W, S, k};
+    const float c01 = 1.f - static_cast<float>(src.at(ph, pwx) & 1u);
+    if (c01 == 0.f) return;  // dc = 0 * sum: nothing to add (antialias.cu:261-262)
+    const float dc = c01 * adjoint_sum(fx, fy, H, W, R, rho, src);
     if (dc == 0.f) return;
     const float dx = dc * 0.5f * fW * n.x;
     const float dy = dc * -0.5f * fH * n.y;
@@ -159,6 +232,7 @@ antialias_bwd_kernel(int N, const int* __restrict__ frag_pix, const float* __res
     const float dpy = dy / pw;
     const float dpw = (dx * px + dy * py) / (pw * pw);
     if (isfinite(dpx) && isfinite(dpy) && isfinite(dpw)) {
+        float* dp = d_pos + static_cast<long long>(b) * V * 4;
         atomicAdd(&dp[v0 * 4 + 0], dpx * t0);
         atomicAdd(&dp[v0 * 4 + 1], dpy * t0);
         atomicAdd(&dp[v0 * 4 + 3], dpw * t0);

@@ -1,12 +1,21 @@
 // Screen-tile binning shared by the fused primitive -> cluster-bitset passes.
 //
-// Fragments are counted and scattered per (view, tile) as packed 32-bit
-// entries (cluster << local_bits | local pixel); one workgroup per tile then
-// ORs its entries into an LDS copy of the tile's bitset and writes the tile
-// once with coalesced stores.  Within a wave, consecutive fragment slots
-// mostly fall in the same tile (they walk along a primitive), so counts and
-// cursors are advanced with one atomic per run of equal tiles instead of one
-// per fragment.
+// Fragments are scattered per (view, tile) as packed 32-bit entries
+// (cluster << local_bits | local pixel); one workgroup per tile then ORs its
+// entries into an LDS copy of the tile's bitset and writes the tile once
+// with coalesced stores (no clear of the bitset, no global OR atomics).
+//
+// Device-scope atomics are the scarce resource (scattered ones run at a few
+// tens of G/s), so the count and fill passes aggregate in LDS: a workgroup
+// counts its fragments per tile in an LDS hash table, reserves each tile's
+// share with ONE global atomic, and records the table (tile, offset) as an
+// "image"; the fill pass reloads the image and places fragments with LDS
+// atomics only.  A workgroup whose table overflows takes a global-atomic
+// path in both passes (flagged per workgroup, so the passes agree).
+//
+// Per tile the entry segment is laid out as kBinGroups group slots
+// (workgroups privatized by blockIdx % kBinGroups) followed by one overflow
+// slot; the whole segment is contiguous.
 #pragma once
 
 #include "common.hpp"
@@ -14,6 +23,13 @@
 namespace s2c {
 
 constexpr int kTileLdsBytes = 32768;  // LDS bitset per tile workgroup
+constexpr int kBinGroups = 8;
+constexpr int kBinSlots = kBinGroups + 1;  // per tile: groups + overflow
+constexpr int kHashLog2 = 10;
+constexpr int kHash = 1 << kHashLog2;      // LDS table slots per workgroup
+// Per-workgroup image of the count pass: slot keys, offsets within the
+// tile's group slot, and fragment counts.
+constexpr int kImageInts = 3 * kHash;
 
 struct TileGrid {
     int ts_log2;   // tile side = 1 << ts_log2
@@ -41,7 +57,7 @@ inline bool choose_tiles(int B, int H, int W, int S, long long min_tiles, TileGr
     while (g.nt < min_tiles && lg > 3) grid(--lg);
     const long long max_cluster = static_cast<long long>(S) * 32;
     if (max_cluster > (1ll << (32 - g.local_bits))) return false;
-    return g.nt <= INT_MAX;
+    return g.nt * kBinSlots < INT_MAX;
 }
 
 __device__ __forceinline__ int tile_of(const TileGrid& g, int b, int h, int w) {
@@ -53,52 +69,59 @@ __device__ __forceinline__ unsigned local_of(const TileGrid& g, int h, int w) {
     return static_cast<unsigned>(((h & m) << g.ts_log2) | (w & m));
 }
 
-// Run structure of `key` across the wave (key < 0 = inactive lane).
-// `start` = first lane of this lane's run; at a run's first lane, `len` is
-// the run length.
-__device__ __forceinline__ void wave_runs(int key, int& start, int& len) {
-    const int l = lane_id();
-    const bool valid = key >= 0;
-    const int prev = __shfl_up(key, 1, kWave);
-    const bool first = valid && (l == 0 || prev != key);
-    const unsigned long long starts = __ballot(first);
-    const unsigned long long vmask = __ballot(valid);
-    const unsigned long long le = (2ull << l) - 1;  // lanes <= l (wraps to all at l = 63)
-    const unsigned long long mine = starts & le;
-    start = mine ? 63 - __clzll(mine) : 0;
-    const unsigned long long after = (starts | ~vmask) & ~le;
-    const int next = after ? __ffsll(static_cast<long long>(after)) - 1 : kWave;
-    len = next - l;
+__device__ __forceinline__ int group_slot(int tile) {
+    return tile * kBinSlots + static_cast<int>(blockIdx.x % kBinGroups);
 }
 
-// Pass 1 visitor: counts per tile.
-__device__ __forceinline__ void bin_count_visit(int key, int* __restrict__ counts) {
-    int start, len;
-    wave_runs(key, start, len);
-    if (key >= 0 && lane_id() == start) atomicAdd(&counts[key], len);
+__device__ __forceinline__ int overflow_slot(int tile) { return tile * kBinSlots + kBinGroups; }
+
+// ------------------------------------------------------------ LDS hash
+
+__device__ __forceinline__ int hash_home(int key) {
+    return static_cast<int>((static_cast<unsigned>(key) * 2654435761u) >> (32 - kHashLog2));
 }
 
-// Pass 2 visitor: scatter packed entries.
-__device__ __forceinline__ void bin_fill_visit(int key, unsigned value, int* __restrict__ cursor,
-                                               unsigned* __restrict__ entries) {
-    int start, len;
-    wave_runs(key, start, len);
-    int base = 0;
-    if (key >= 0 && lane_id() == start) base = atomicAdd(&cursor[key], len);
-    base = __shfl(base, start, kWave);
-    if (key >= 0) entries[base + (lane_id() - start)] = value;
+// Slot holding `key` (inserted if absent), or -1 when the table is full.
+__device__ __forceinline__ int hash_insert(int* keys, int key) {
+    int s = hash_home(key);
+    for (int p = 0; p < kHash; ++p) {
+        const int prev = atomicCAS(&keys[s], -1, key);
+        if (prev == -1 || prev == key) return s;
+        s = (s + 1) & (kHash - 1);
+    }
+    return -1;
 }
 
-// Host side of the three-pass pipeline (binning.hip).
+// Slot holding `key`, or -1 if absent.
+__device__ __forceinline__ int hash_find(const int* keys, int key) {
+    int s = hash_home(key);
+    for (int p = 0; p < kHash; ++p) {
+        const int k = keys[s];
+        if (k == key) return s;
+        if (k == -1) return -1;
+        s = (s + 1) & (kHash - 1);
+    }
+    return -1;
+}
+
+// ------------------------------------------------------------ host side
+
 struct BinPlan {
     TileGrid g;
-    int* counts = nullptr;  // nt + 1 ints (exclusive offsets after the scan)
-    int* cursor = nullptr;  // nt ints
+    long long blocks = 0;    // workgroups of the count / fill passes
+    int* counts = nullptr;   // nt*kBinSlots + 1 ints, shifted by one; offsets after the scan
+    int* cursor = nullptr;   // nt*kBinSlots ints (overflow placement)
+    int* scratch = nullptr;  // scan scratch
+    int* image = nullptr;    // blocks * kImageInts ints
+    int* flags = nullptr;    // blocks ints (1 = overflow path)
+    long long slots() const { return g.nt * kBinSlots; }
 };
 
-// Zeroes counts; after the caller's count pass, turns counts into offsets,
-// seeds the cursors and returns the entry total (one host sync).
-int bin_prepare(BinPlan& p, hipStream_t s);
+// Device ints a plan needs.
+size_t bin_plan_ints(const TileGrid& g, long long blocks);
+// Carves the plan's buffers out of `ints` and zeroes the counts.
+int bin_prepare(BinPlan& p, int* ints, hipStream_t s);
+// After the count pass: offsets, cursors and the entry total (host sync).
 int bin_offsets(BinPlan& p, hipStream_t s, int* total);
 // One workgroup per tile: OR entries into LDS, write the tile bitset.
 int bin_resolve(const BinPlan& p, const unsigned* entries, int H, int W, int S, int* bitset,

@@ -121,6 +121,18 @@ __device__ __forceinline__ long long wave_incl_scan64(long long v) {
     return v;
 }
 
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) v = dmin(v, __shfl_xor(v, d, kWave));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) v = dmax(v, __shfl_xor(v, d, kWave));
+    return v;
+}
+
 // Exclusive block scan over kBlock threads; returns the exclusive prefix and
 // writes the block total.  `sh` needs kBlock/kWave + 1 ints of LDS.
 __device__ __forceinline__ int block_excl_scan(int v, int* sh, int* total) {

@@ -1,6 +1,4 @@
-// DDA strand-line rasterization (reference: pystrns2cards/src/dda.cu) and
-// the fused DDA -> cluster-bitset kernel.
-#include "binning.hpp"
+// DDA strand-line rasterization (reference: pystrns2cards/src/dda.cu).
 #include "dda_math.hpp"
 
 namespace s2c {
@@ -136,106 +134,6 @@ dda_interp_bwd_kernel(long long npix, const float* __restrict__ rast, int C,
     }
 }
 
-// -------------------------------------------------- fused DDA -> bitset
-// One workgroup takes kBlock consecutive primitives (b*E + e order), keeps
-// their spans and vertices in LDS, block-scans their candidate counts and
-// then walks the block's candidates with every thread, so the per-thread
-// work is balanced no matter how long individual edges are.  The walk runs
-// a uniform number of iterations (every lane calls the visitor each time,
-// valid or not), so visitors may use wave ballots and shuffles.
-
-struct MaskPrim {
-    DdaSpan sp;
-    float4 p0, p1;
-    int b, cluster;
-};
-
-// visit(valid, b, h, w, cluster) for every candidate fragment of the block.
-template <class Visit>
-__device__ __forceinline__ void walk_dda_block(long long P, int H, int W, int V,
-                                               const float* __restrict__ pos, int E,
-                                               const int* __restrict__ edges,
-                                               const int* __restrict__ edge2cluster,
-                                               Visit&& visit) {
-    __shared__ MaskPrim prim[kBlock];
-    __shared__ int excl[kBlock];
-    __shared__ int sh[kBlock / kWave + 1];
-    const long long q = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
-    int cnt = 0;
-    if (q < P) {
-        const int b = static_cast<int>(q / E);
-        const int e = static_cast<int>(q - static_cast<long long>(b) * E);
-        const int2 ev = reinterpret_cast<const int2*>(edges)[e];
-        const long long vb = static_cast<long long>(b) * V;
-        MaskPrim m;
-        m.p0 = load_pos(pos, vb + ev.x);
-        m.p1 = load_pos(pos, vb + ev.y);
-        bool wrote;
-        cnt = dda_span(H, W, m.p0, m.p1, m.sp, wrote);
-        m.b = b;
-        m.cluster = cnt > 0 ? edge2cluster[e] : 0;
-        prim[threadIdx.x] = m;
-    }
-    int total;
-    excl[threadIdx.x] = block_excl_scan(cnt, sh, &total);
-    __syncthreads();
-#pragma unroll 1
-    for (int base = 0; base < total; base += kBlock) {
-        const int j = base + threadIdx.x;
-        bool ok = false;
-        int b = 0, c = 0;
-        DdaFrag fr{0, 0, 0.f, 0.f};
-        if (j < total) {
-            // last p with excl[p] <= j
-            int lo = 0, hi = kBlock;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (excl[mid] <= j) lo = mid;
-                else hi = mid;
-            }
-            const MaskPrim& m = prim[lo];
-            ok = dda_frag(H, W, m.sp, j - excl[lo], m.p0, m.p1, fr);
-            b = m.b;
-            c = m.cluster;
-        }
-        visit(ok, b, fr.h, fr.w, c);
-    }
-}
-
-// Direct form: one global atomicOr per fragment into a zeroed bitset (used
-// when a tile's bitset does not fit LDS).
-__global__ void __launch_bounds__(kBlock)
-dda_mask_kernel(long long P, int H, int W, int V, const float* __restrict__ pos, int E,
-                const int* __restrict__ edges, const int* __restrict__ edge2cluster, int S,
-                int* __restrict__ bitset) {
-    walk_dda_block(P, H, W, V, pos, E, edges, edge2cluster, [&](bool ok, int b, int h, int w, int c) {
-        if (!ok) return;
-        const long long idx = ((static_cast<long long>(b) * H + h) * W + w) * S + (c >> 5);
-        atomicOr(&bitset[idx], static_cast<int>(1u << (c & 31)));
-    });
-}
-
-// Binned form, pass 1: fragments per (view, tile).
-__global__ void __launch_bounds__(kBlock)
-dda_bin_count_kernel(long long P, int H, int W, int V, const float* __restrict__ pos, int E,
-                     const int* __restrict__ edges, const int* __restrict__ edge2cluster,
-                     TileGrid g, int* __restrict__ counts) {
-    walk_dda_block(P, H, W, V, pos, E, edges, edge2cluster, [&](bool ok, int b, int h, int w, int) {
-        bin_count_visit(ok ? tile_of(g, b, h, w) : -1, counts);
-    });
-}
-
-// Binned form, pass 2: packed (cluster, local pixel) entries per tile.
-__global__ void __launch_bounds__(kBlock)
-dda_bin_fill_kernel(long long P, int H, int W, int V, const float* __restrict__ pos, int E,
-                    const int* __restrict__ edges, const int* __restrict__ edge2cluster,
-                    TileGrid g, int* __restrict__ cursor, unsigned* __restrict__ entries) {
-    walk_dda_block(P, H, W, V, pos, E, edges, edge2cluster, [&](bool ok, int b, int h, int w, int c) {
-        const unsigned v = (static_cast<unsigned>(c) << g.local_bits) | local_of(g, h, w);
-        bin_fill_visit(ok ? tile_of(g, b, h, w) : -1, v, cursor, entries);
-    });
-}
-
 }  // namespace s2c
 
 using namespace s2c;
@@ -302,42 +200,4 @@ extern "C" int s2c_backward_dda_interpolate_attributes(int B, int H, int W,
                                                                  d_attr);
     S2C_LAUNCH_CHECK();
     return S2C_OK;
-}
-
-extern "C" int s2c_dda_cluster_mask(int B, int H, int W, int V, const float* pos, int E,
-                                    const int* edges, const int* edge2cluster, int num_slots,
-                                    int* bitset, void* stream) {
-    S2C_CHECK(B >= 0 && H >= 0 && W >= 0 && V >= 0 && E >= 0 && num_slots >= 0,
-              "dda_cluster_mask: negative size");
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const long long words = static_cast<long long>(B) * H * W * num_slots;
-    const long long P = static_cast<long long>(B) * E;
-    if (words == 0) return S2C_OK;
-    BinPlan plan;
-    if (P == 0 || !choose_tiles(B, H, W, num_slots, 4096, plan.g)) {
-        // Empty input, or a tile bitset too large for LDS: clear + direct ORs.
-        if (int r = fill_u32(bitset, 0u, words, s)) return r;
-        if (P == 0) return S2C_OK;
-        dda_mask_kernel<<<grid_for(P, kBlock), kBlock, 0, s>>>(P, H, W, V, pos, E, edges,
-                                                               edge2cluster, num_slots, bitset);
-        S2C_LAUNCH_CHECK();
-        return S2C_OK;
-    }
-    Temp tiles;
-    if (int r = tiles.alloc(sizeof(int) * (2 * plan.g.nt + 1), s)) return r;
-    plan.counts = tiles.as<int>();
-    plan.cursor = plan.counts + plan.g.nt + 1;
-    if (int r = bin_prepare(plan, s)) return r;
-    dda_bin_count_kernel<<<grid_for(P, kBlock), kBlock, 0, s>>>(P, H, W, V, pos, E, edges,
-                                                                edge2cluster, plan.g, plan.counts);
-    S2C_LAUNCH_CHECK();
-    int total = 0;
-    if (int r = bin_offsets(plan, s, &total)) return r;
-    Temp entries;
-    if (int r = entries.alloc(sizeof(unsigned) * static_cast<size_t>(total), s)) return r;
-    dda_bin_fill_kernel<<<grid_for(P, kBlock), kBlock, 0, s>>>(P, H, W, V, pos, E, edges,
-                                                               edge2cluster, plan.g, plan.cursor,
-                                                               entries.as<unsigned>());
-    S2C_LAUNCH_CHECK();
-    return bin_resolve(plan, entries.as<unsigned>(), H, W, num_slots, bitset, s);
 }

@@ -117,4 +117,46 @@ __device__ __forceinline__ float4 load_pos(const float* pos, long long row) {
     return reinterpret_cast<const float4*>(pos)[row];
 }
 
+// True when every fragment of the edge passes the depth test of
+// dda.cu:210-216 whatever its t1, so a mask pass may skip t1 and z.
+// For t1 in [0, 1] and t0 = fl(1 - t1), the exact (t0 z0 + t1 z1) /
+// (t0 w0 + t1 w1) is a weighted mean of the endpoints' NDC depths (weights
+// t0*w0, t1*w1 >= 0).  The fp32 evaluation of dda.cu:211-214 adds at most
+// 2u(Z/w_min + 1) + u with u = 2^-24 and Z = max|z_clip| <= w_max, i.e.
+// below 35u ~= 2.1e-6 when w_max <= 16 w_min.  So with both NDC depths in
+// [-1 + 1e-5, 1 - 1e-5] and w in [1e-30, 1e30] (no under/overflow in the
+// products), zw stays finite and inside [-1, 1].
+__device__ __forceinline__ bool dda_depth_always_passes(float4 p0, float4 p1) {
+    const float lo = fminf(p0.w, p1.w), hi = fmaxf(p0.w, p1.w);
+    if (!(lo >= 1e-30f && hi <= 1e30f && hi <= 16.f * lo)) return false;
+    const float z0 = p0.z / p0.w, z1 = p1.z / p1.w;
+    constexpr float lim = 1.f - 1e-5f;
+    return fabsf(z0) <= lim && fabsf(z1) <= lim;  // false for NaN
+}
+
+// dda.cu:185-197: pixel of candidate k, without t1 / depth.  Returns false
+// when it falls off screen.
+__device__ __forceinline__ bool dda_pixel(int H, int W, const DdaSpan& sp, int k, int& h, int& w) {
+    const float d = fminf(fmaxf(static_cast<float>(k) + 0.5f, sp.off0), sp.off1);
+    const float ddx = (sp.axis == 0.f) ? d : d * sp.slope;
+    const float ddy = (sp.axis == 0.f) ? d * sp.slope : d;
+    const float x = sp.sx + ddx;
+    const float y = sp.sy + ddy;
+    h = f2i(floorf(y));
+    w = f2i(floorf(x));
+    return !(h < 0 || h >= H || w < 0 || w >= W);
+}
+
+// Survival of candidate k for a cluster mask: the pixel test, then the
+// depth test unless it is known to pass for the whole edge.
+__device__ __forceinline__ bool dda_mask_frag(int H, int W, const DdaSpan& sp, int k, float4 p0,
+                                              float4 p1, bool depth_ok, int& h, int& w) {
+    if (depth_ok) return dda_pixel(H, W, sp, k, h, w);
+    DdaFrag f;
+    if (!dda_frag(H, W, sp, k, p0, p1, f)) return false;
+    h = f.h;
+    w = f.w;
+    return true;
+}
+
 }  // namespace s2c

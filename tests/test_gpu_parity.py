@@ -151,6 +151,40 @@ def test_dda_mask_random(dev, K):
     np.testing.assert_array_equal(got, want)
 
 
+def test_dda_mask_long_scattered_edges(dev):
+    """Long random edges over 1024^2: deferred (workgroup-walked) remainders,
+    and workgroups touching more distinct tiles than their LDS table holds
+    (the global overflow path)."""
+    report("dda->mask long scattered edges (overflow + deferred paths)")
+    rng = np.random.default_rng(31)
+    pos, edges = random_lines(rng, 2, 6000, 5000, 1024, 1024, persp=False)
+    e2c = rng.integers(0, 100, size=edges.shape[0]).astype(np.int32)
+    want = oracle.dda_cluster_mask((1024, 1024), pos, edges, e2c, 100, threads=8)
+    got = fused_dda_mask(dev, (1024, 1024), pos, edges, e2c, 100)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_dda_mask_depth_boundaries(dev):
+    """Edges whose endpoint depths sit at / around the fast-path threshold
+    (|z_ndc| = 1 - 1e-5) and at the clip limit, with w = 1 and with w
+    ratios beyond 16 (which must take the exact per-fragment test)."""
+    report("dda->mask depth boundaries")
+    rng = np.random.default_rng(21)
+    n = 4000
+    pos = np.zeros((2, 2 * n, 4), np.float32)
+    zs = np.array([1 - 1e-5, 1 - 2e-5, 1 - 5e-6, 1.0, 1 + 1e-7, 0.0, 0.999999], np.float32)
+    xy = rng.uniform(-1.1, 1.1, size=(2, 2 * n, 2)).astype(np.float32)
+    z = rng.choice(np.concatenate([zs, -zs]), size=(2, 2 * n)).astype(np.float32)
+    w = np.where(rng.uniform(size=(2, 2 * n)) < 0.5, 1.0,
+                 rng.uniform(0.05, 3.0, size=(2, 2 * n))).astype(np.float32)
+    pos[..., 0], pos[..., 1], pos[..., 2], pos[..., 3] = xy[..., 0] * w, xy[..., 1] * w, z * w, w
+    edges = np.stack([np.arange(0, 2 * n, 2), np.arange(1, 2 * n, 2)], 1).astype(np.int32)
+    e2c = rng.integers(0, 40, size=n).astype(np.int32)
+    want = oracle.dda_cluster_mask((90, 70), pos, edges, e2c, 40)
+    got = fused_dda_mask(dev, (90, 70), pos, edges, e2c, 40)
+    np.testing.assert_array_equal(got, want)
+
+
 @pytest.mark.slow
 def test_dda_mask_full_workload(dev):
     report("dda->mask full workload (50k strands x 100, 4 views 1024^2, K=1024)")
@@ -280,15 +314,43 @@ def test_antialias_small_workload(dev):
     np.testing.assert_array_equal(out[:m].cpu().numpy(), o_prim)
     np.testing.assert_array_equal(normals.cpu().numpy(), o_normals)
     _, _, _, _, fp, fa = dda_gpu(dev, res, w.card_pos, c.edges, o_prim)
+    check_antialias_backward(dev, w, cards, target, fp, fa, normals)
+
+
+def check_antialias_backward(dev, w, cards, target, fp, fa, normals, edges=None, e2t=None, e2c=None):
+    c = w.cards
+    edges = c.edges if edges is None else edges
+    e2t = c.edge2tri if e2t is None else e2t
+    e2c = c.edge2cluster if e2c is None else e2c
+    res = (w.H, w.W)
+    B, V, _ = w.card_pos.shape
+    E = edges.shape[0]
+    tp = T(w.card_pos, dev)
     for rho in (1.0, 0.5):
         d_pos = torch.zeros((B, V, 4), device=dev)
         call("s2c_backward_antialiased_cluster_bitset", fp.shape[0], T(fp, dev), T(fa, dev), res[0],
-             res[1], 2, T(cards, dev), T(target, dev), V, tp, E, T(c.edges, dev), normals,
-             T(c.edge2cluster, dev), d_pos, 7.0, rho)
-        want = oracle.antialiased_cluster_bitset_grad(cards, target, w.card_pos, c.tri, c.edges,
-                                                      c.edge2tri, c.edge2cluster, 7.0, rho)
+             res[1], 2, T(cards, dev), T(target, dev), V, tp, E, T(edges, dev), normals,
+             T(e2c, dev), d_pos, 7.0, rho)
+        want = oracle.antialiased_cluster_bitset_grad(cards, target, w.card_pos, c.tri, edges,
+                                                      e2t, e2c, 7.0, rho)
         assert np.abs(want).max() > 0
         normwise_close(d_pos.cpu().numpy(), want)
+
+
+def test_antialias_shuffled_edges(dev):
+    """Edges in random order (as an unordered edge map produces): waves mix
+    clusters, so the unstaged window path runs."""
+    report("antialias backward with shuffled card edges")
+    w = synth.small_workload()
+    c = w.cards
+    res = (w.H, w.W)
+    perm = np.random.default_rng(4).permutation(c.edges.shape[0])
+    edges, e2t, e2c = c.edges[perm].copy(), c.edge2tri[perm].copy(), c.edge2cluster[perm].copy()
+    target = oracle.dda_cluster_mask(res, w.pos, w.edges, w.edge2cluster, 64)
+    cards = oracle.triangle_cluster_mask(res, w.card_pos, c.tri, c.tri2cluster, 64)
+    prim, normals = oracle.mark_discontinuity_edges(res, w.card_pos, c.tri, edges, e2t)
+    fp, fa = oracle.dda_compute_fragments(res, w.card_pos, edges, prim)
+    check_antialias_backward(dev, w, cards, target, fp, fa, T(normals, dev), edges, e2t, e2c)
 
 
 # ---------------------------------------------------------------- curves
