@@ -179,6 +179,41 @@ int s2c_triangle_cluster_mask(int B, int H, int W, int V, const float* pos,
                               int T, const int* tri, const int* tri2cluster,
                               int num_slots, int* bitset, void* stream);
 
+/* ---- batched / served card fitting -----------------------------------------
+ * No reference launcher corresponds: these run, for many independent
+ * per-cluster fitting jobs at once, the iteration of 3_fitting.py:381-407
+ * (triangle mask, antialias edge gradient, planarity and uniform-Laplacian
+ * terms, UniformAdam of optimizer.py:33-68 with ExponentialLR decay), one
+ * workgroup per job, without the largesteps reparameterization.
+ *
+ * Job j: a res[j] x res[j] view (res <= 256), a Q = quads[j] <= 32 strip of
+ * 2(Q+1) vertices (row r, column c -> r*(Q+1) + c) stored from
+ * verts[vert_off[j]] (x,y,z fp32), a row-major 4x4 clip matrix mvp[16*j],
+ * and a packed target mask (bit p = pixel h*res + w of word p/32) starting
+ * at target_bits[bits_off[j]].  adam_m (3 per vertex), adam_v (1 per vertex)
+ * and steps[j] hold the optimizer state.  hyper is a HOST array of 8 floats:
+ * lr0, beta1, beta2, gamma (lr decay per step), w_pln, w_lap, rho, radius. */
+
+/* Target masks: for job j, num_strands[j] strands of verts_per_strand
+ * clip-space float4 vertices from strand_clip[strand_off[j]]; every edge is
+ * DDA-rasterized (dda.cu:9-225) into the job's packed mask. */
+int s2c_fit_targets(int num_jobs, const int* res, const int* bits_off,
+                    const int* strand_off, const int* num_strands,
+                    int verts_per_strand, const float* strand_clip,
+                    unsigned* target_bits, void* stream);
+
+/* Runs `iters` fitting iterations for the jobs slots[0..num_slots) (a
+ * negative entry is an idle slot and is skipped), then
+ * renders the updated card and writes popcount(render XOR target) to
+ * loss_out[job]; verts, Adam state and steps are updated in place.  Jobs
+ * never share state, and each job's arithmetic order is fixed, so a job's
+ * result does not depend on which other jobs run in the same call. */
+int s2c_fit_cards(int num_slots, const int* slots, int iters, const int* res,
+                  const int* quads, const int* vert_off, const int* bits_off,
+                  const float* mvp, float* verts, float* adam_m, float* adam_v,
+                  int* steps, const unsigned* target_bits, const float* hyper,
+                  float* loss_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

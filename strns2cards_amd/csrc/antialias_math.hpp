@@ -1,0 +1,147 @@
+// Per-edge and per-sample arithmetic of the antialiased cluster-bitset
+// gradient (reference: pystrns2cards/src/antialias.cu), shared by the
+// drop-in kernels and the served card-fitting kernel.  Every expression is
+// evaluated in the reference's order.
+#pragma once
+
+#include "common.hpp"
+
+namespace s2c {
+
+// antialias.cu:18-35: homogeneous vertex -> pixel coordinates.
+__device__ __forceinline__ void aa_project(float4 q, float fW, float fH, float& x, float& y) {
+    const float nx = q.x / q.w;
+    const float ny = q.y / q.w;
+    x = fW * 0.5f * (1.f + nx);
+    y = fH * 0.5f * (1.f - ny);
+}
+
+// antialias.cu:56-93 for one edge with its (up to two) opposite vertices.
+// Returns false for a zero-length edge (nothing written).  `silhouette` is
+// the reference's dot0 * dot1 >= 0 test; `n` the outward unit normal.
+__device__ __forceinline__ bool aa_edge(float fW, float fH, float4 P0, float4 P1, bool has2,
+                                        float4 Q2, bool has3, float4 Q3, bool& silhouette,
+                                        float2& n) {
+    float x0, y0, x1, y1;
+    aa_project(P0, fW, fH, x0, y0);
+    aa_project(P1, fW, fH, x1, y1);
+    const float ex = x1 - x0, ey = y1 - y0;
+    const float len = sqrtf(ex * ex + ey * ey);
+    if (len == 0.f) return false;
+    const float nx = -ey / len;
+    const float ny = ex / len;
+    float d0 = 0.f, d1 = 0.f;
+    if (has2) {
+        float x2, y2;
+        aa_project(Q2, fW, fH, x2, y2);
+        d0 = nx * (x2 - x0) + ny * (y2 - y0);
+    }
+    if (has3) {
+        float x2, y2;
+        aa_project(Q3, fW, fH, x2, y2);
+        d1 = nx * (x2 - x0) + ny * (y2 - y0);
+    }
+    silhouette = d0 * d1 >= 0.f;
+    const float sign = 2.f * (.5f - static_cast<float>(d0 > 0.f || d1 > 0.f));
+    n = make_float2(sign * nx, sign * ny);
+    return true;
+}
+
+// One DDA sample of a silhouette edge (antialias.cu:198-254).
+struct AaSample {
+    bool live;              // passes the interior-sample and neighbour tests
+    int nh, nw;             // neighbour pixel across the edge
+    int ph, pw_;            // pixel whose coverage gates the sample (c01)
+    float t0, t1, px, py, pw, fx, fy;
+};
+
+__device__ __forceinline__ AaSample aa_sample(int H, int W, int h, int w, float axis, float t1,
+                                              float4 P0, float4 P1, float2 n) {
+    AaSample s;
+    const float fH = static_cast<float>(H), fW = static_cast<float>(W);
+    s.t1 = t1;
+    s.t0 = 1.f - t1;
+    s.px = s.t0 * P0.x + s.t1 * P1.x;
+    s.py = s.t0 * P0.y + s.t1 * P1.y;
+    s.pw = s.t0 * P0.w + s.t1 * P1.w;
+    s.fx = 0.5f * (1.f + s.px / s.pw) * fW;
+    s.fy = 0.5f * (1.f - s.py / s.pw) * fH;
+    s.nh = h;
+    s.nw = w;
+    const float fxr = s.fx - floorf(s.fx);
+    const float fyr = s.fy - floorf(s.fy);
+    constexpr float eps = 1e-4f;
+    float dot = 0.f;
+    bool ok;
+    if (axis == 0.f) {
+        ok = isfinite(fxr) && !(fabsf(fxr - 0.5f) >= eps);
+        const int d = 2 * static_cast<int>(fyr > 0.5f) - 1;
+        s.nh += d;
+        dot += static_cast<float>(d) * n.y;
+    } else {
+        ok = isfinite(fyr) && !(fabsf(fyr - 0.5f) >= eps);
+        const int d = 2 * static_cast<int>(fxr > 0.5f) - 1;
+        s.nw += d;
+        dot += static_cast<float>(d) * n.x;
+    }
+    s.live = ok && !(s.nw < 0 || s.nw >= W || s.nh < 0 || s.nh >= H || dot == 0.f);
+    s.ph = (dot > 0.f) ? s.nh : h;
+    s.pw_ = (dot > 0.f) ? s.nw : w;
+    return s;
+}
+
+// antialias.cu:147-176 (compute_adjoint_sum with gaussian_weight,
+// antialias.cu:121-132).  `code(h, w)` returns the pixel's residual bits:
+// bit 0 = rendered (c0), bit 1 = target (c1).  Window pixels outside the
+// radius (weight 0) or with c0 == c1 (residual 0) add an exact zero in the
+// reference and are skipped, so expf runs only where card and target
+// differ; the remaining terms are summed in the reference's order.
+template <class Code>
+__device__ __forceinline__ float aa_adjoint_sum(float x, float y, int H, int W, float R, float rho,
+                                                const Code& code) {
+    constexpr float pi = 3.14159265358979323846f;
+    const float sigma = R / 4.f;
+    const float c = 1.f / (sigma * sigma * 2.f * pi);
+    const float alpha = 0.5f / (sigma * sigma);
+    const float R2 = R * R;
+    const int Ri = f2i(floorf(R));
+    const int cw = f2i(floorf(x)), ch = f2i(floorf(y));
+    const int h0 = dmax(ch - Ri, 0), h1 = dmin(ch + Ri, H - 1) + 1;
+    const int w0 = dmax(cw - Ri, 0), w1 = dmin(cw + Ri, W - 1) + 1;
+    float dc = 0.f;
+    for (int h = h0; h < h1; ++h) {
+        const float dy = static_cast<float>(h) + 0.5f - y;
+        for (int w = w0; w < w1; ++w) {
+            const float dx = static_cast<float>(w) + 0.5f - x;
+            const float n2 = dx * dx + dy * dy;
+            if (!(n2 < R2)) continue;
+            const unsigned cc = code(h, w);
+            const unsigned c0 = cc & 1u, c1 = cc >> 1;
+            if (c0 == c1) continue;
+            const float wt = c * expf(-alpha * n2) * 1.f;
+            dc += wt * (static_cast<float>(c0) - static_cast<float>(c1)) * (c1 == 0u ? rho : 1.f);
+        }
+    }
+    return dc;
+}
+
+// antialias.cu:256-270: from the gated sum to the clip-space gradient of
+// the sample point.  Returns false when nothing is to be added.
+template <class Code>
+__device__ __forceinline__ bool aa_sample_grad(const AaSample& s, int H, int W, float R, float rho,
+                                               float2 n, const Code& code, float& dpx, float& dpy,
+                                               float& dpw) {
+    const float c01 = 1.f - static_cast<float>(code(s.ph, s.pw_) & 1u);
+    if (c01 == 0.f) return false;  // dc = 0 * sum (antialias.cu:261-262)
+    const float dc = c01 * aa_adjoint_sum(s.fx, s.fy, H, W, R, rho, code);
+    if (dc == 0.f) return false;
+    const float fH = static_cast<float>(H), fW = static_cast<float>(W);
+    const float dx = dc * 0.5f * fW * n.x;
+    const float dy = dc * -0.5f * fH * n.y;
+    dpx = dx / s.pw;
+    dpy = dy / s.pw;
+    dpw = (dx * s.px + dy * s.py) / (s.pw * s.pw);
+    return isfinite(dpx) && isfinite(dpy) && isfinite(dpw);
+}
+
+}  // namespace s2c

@@ -523,6 +523,57 @@ PYBIND11_MODULE(_core, m) {
               fn);
     });
 
+    // Batched card fitting (see the C ABI header).
+    m.def("fit_targets", [](Tensor res, Tensor bits_off, Tensor strand_off, Tensor num_strands,
+                            Tensor strand_clip, Tensor target_bits) {
+        const char* fn = "fit_targets";
+        need(res, fn, "res", at::kInt, {-1});
+        const int64_t J = res.size(0);
+        need(bits_off, fn, "bits_off", at::kInt, {J});
+        need(strand_off, fn, "strand_off", at::kInt, {J});
+        need(num_strands, fn, "num_strands", at::kInt, {J});
+        need(strand_clip, fn, "strand_clip", at::kFloat, {-1, -1, 4});
+        need(target_bits, fn, "target_bits", at::kInt, {-1});
+        same_device(fn, {&res, &bits_off, &strand_off, &num_strands, &strand_clip, &target_bits});
+        DeviceGuard g(res.device());
+        check(s2c_fit_targets(I(J, fn), P<int>(res), P<int>(bits_off), P<int>(strand_off),
+                              P<int>(num_strands), I(strand_clip.size(1), fn),
+                              P<float>(strand_clip), P<unsigned>(target_bits), stream_of(res)),
+              fn);
+    });
+
+    m.def("fit_cards", [](Tensor slots, int iters, Tensor res, Tensor quads, Tensor vert_off,
+                          Tensor bits_off, Tensor mvp, Tensor verts, Tensor adam_m, Tensor adam_v,
+                          Tensor steps, Tensor target_bits, std::vector<float> hyper,
+                          Tensor loss_out) {
+        const char* fn = "fit_cards";
+        need(slots, fn, "slots", at::kInt, {-1});
+        need(res, fn, "res", at::kInt, {-1});
+        const int64_t J = res.size(0);
+        need(quads, fn, "quads", at::kInt, {J});
+        need(vert_off, fn, "vert_off", at::kInt, {J});
+        need(bits_off, fn, "bits_off", at::kInt, {J});
+        need(mvp, fn, "mvp", at::kFloat, {J, 4, 4});
+        need(verts, fn, "verts", at::kFloat, {-1, 3});
+        need(adam_m, fn, "adam_m", at::kFloat, {verts.size(0), 3});
+        need(adam_v, fn, "adam_v", at::kFloat, {verts.size(0)});
+        need(steps, fn, "steps", at::kInt, {J});
+        need(target_bits, fn, "target_bits", at::kInt, {-1});
+        need(loss_out, fn, "loss_out", at::kFloat, {J});
+        same_device(fn, {&slots, &res, &quads, &vert_off, &bits_off, &mvp, &verts, &adam_m, &adam_v,
+                         &steps, &target_bits, &loss_out});
+        if (hyper.size() != 8)
+            throw std::runtime_error("fit_cards: hyper must hold 8 values");
+        if (iters < 0) throw std::runtime_error("fit_cards: iters must be >= 0");
+        DeviceGuard g(res.device());
+        check(s2c_fit_cards(I(slots.size(0), fn), P<int>(slots), iters, P<int>(res), P<int>(quads),
+                            P<int>(vert_off), P<int>(bits_off), P<float>(mvp), P<float>(verts),
+                            P<float>(adam_m), P<float>(adam_v), P<int>(steps),
+                            P<unsigned>(target_bits), hyper.data(), P<float>(loss_out),
+                            stream_of(res)),
+              fn);
+    });
+
     m.def("triangle_cluster_mask", [](Tensor pos, Tensor tri, Tensor tri2cluster, Tensor bitset) {
         const char* fn = "triangle_cluster_mask";
         need(pos, fn, "pos", at::kFloat, {-1, -1, 4});

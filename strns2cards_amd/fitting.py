@@ -1,0 +1,401 @@
+"""Per-cluster card fitting: the plain per-job loop and the batched / served
+form.
+
+A fitting job is one cluster (wisp) of strands and one card strip fitted to
+it in one orthographic view, with the iteration of 3_fitting.py:381-407:
+triangle cluster mask, antialiased silhouette gradient (rho, radius 7),
+planarity and uniform-Laplacian regularizers, UniformAdam and an
+exponential learning-rate decay.  The largesteps reparameterization of the
+reference (3_fitting.py:362-384) is not available offline, so vertex
+positions are optimized directly, in both forms.
+
+* ``plain_fit`` runs one job at a time through the drop-in API
+  (``compute_fragments`` -> ``cluster_mask_from_fragments`` ->
+  ``antialiased_cluster_bitset_loss`` -> autograd -> ``UniformAdam``), with
+  the host synchronizations that API implies.
+* ``CardFitServer`` keeps every admitted job's state resident in HBM (slots)
+  and advances all active slots with one ``fit_cards`` launch (one
+  workgroup per job, masks in LDS); ``serve`` feeds it with arriving jobs
+  (continuous batching) and replays a captured HIP graph per chunk.
+"""
+from __future__ import annotations
+
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _core
+from . import synth
+from .antialias import antialiased_cluster_bitset_loss
+from .bitset_utils import popcount_bitset
+from .optimizer import UniformAdam, laplacian_edges
+from .rasterize import cluster_mask_from_fragments, compute_fragments
+
+MAX_RES = 256
+MAX_QUADS = 32
+MAX_VERTS = 2 * (MAX_QUADS + 1)
+MAX_WORDS = MAX_RES * MAX_RES // 32
+
+
+@dataclass
+class FitHyper:
+    """3_fitting.py:506-525 defaults (lr 0.1, alpha 0.5, w_pln 10,
+    w_lap 1e-2, kernel radius 7) with the decay to 1e-2 of the initial lr
+    over the job's iterations (3_fitting.py:376)."""
+    lr: float = 0.1
+    beta1: float = 0.9
+    beta2: float = 0.999
+    iters: int = 100
+    w_pln: float = 10.0
+    w_lap: float = 1e-2
+    rho: float = 0.5
+    radius: float = 7.0
+
+    @property
+    def gamma(self) -> float:
+        return 1e-2 ** (1.0 / self.iters)
+
+    def vector(self) -> list[float]:
+        return [self.lr, self.beta1, self.beta2, self.gamma, self.w_pln, self.w_lap, self.rho,
+                self.radius]
+
+
+@dataclass
+class FitJob:
+    jid: int
+    res: int
+    quads: int
+    strands: np.ndarray        # (n, nv, 3) cm
+    strand_clip: np.ndarray    # (n * nv, 4) float32, clip space of the job's view
+    mvp: np.ndarray            # (4, 4) float32
+    verts: np.ndarray          # (2 (Q + 1), 3) float32 initial strip
+    arrival: float = 0.0
+
+    @property
+    def bytes(self) -> int:
+        """HBM footprint while resident: target mask, vertices, optimizer
+        state and strand vertices (for the target)."""
+        nv = self.verts.shape[0]
+        return (self.res * self.res // 8) + nv * 7 * 4 + self.strand_clip.nbytes + 64 + 16 * 4
+
+
+def make_jobs(count: int, seed: int = 0, nv: int = 64, cap_angle: float = 0.15) -> list[FitJob]:
+    """Seeded per-cluster jobs: strand count log-uniform in [8, 512], nv
+    vertices each (step 0.004, strand model of synth.generate_strands, roots
+    in a cap of `cap_angle` rad around a random direction on the upper
+    hemisphere, scaled to cm as 3_fitting.py:289 does), quads uniform in
+    {8, 12, 16, 24, 32}, resolution uniform in {64, 128, 256}.  The card is
+    the mean strand resampled to Q + 1 points, swept 1 cm wide."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for j in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(8), math.log(512)))))
+        Q = int(rng.choice([8, 12, 16, 24, 32]))
+        res = int(rng.choice([64, 128, 256]))
+        c = rng.normal(size=3)
+        c[1] = abs(c[1])
+        c /= np.linalg.norm(c)
+        d = c[None] + cap_angle * rng.normal(size=(n, 3))
+        d[:, 1] = np.abs(d[:, 1])
+        strands = synth.generate_strands(n, nv, 0.004, seed, rng=rng, root_dirs=d) * 100.0
+        pts = strands.reshape(-1, 3)
+        mvp = synth.ortho_cameras(pts, 1, 0.3)[0]
+        centre = synth.resample_by_arclength(strands.mean(0, keepdims=True), Q + 1)
+        verts = synth.strip_rows(centre, 1.0)[0].reshape(-1, 3).astype(np.float32)
+        clip = synth.project(pts, mvp[None])[0]
+        jobs.append(FitJob(j, res, Q, strands.astype(np.float32), clip, mvp.astype(np.float32),
+                           verts))
+    return jobs
+
+
+# ----------------------------------------------------------------- topology
+
+
+def strip_topology(Q: int):
+    """Triangles, quads, unique edges (+ adjacency) and Laplacian edges of a
+    two-row strip with Q quads (wisp_to_grid_mesh, texture_utils.py:525-599)."""
+    N = Q + 1
+    tri = synth.grid_triangles(2, N)
+    quads = np.array([[c, N + c, N + c + 1, c + 1] for c in range(Q)], np.int32)
+    edges, e2t = synth.edges_from_triangles(tri)
+    h_edges = np.array([[c, c + 1] for c in range(Q)] + [[N + c, N + c + 1] for c in range(Q)],
+                       np.int32)
+    w_edges = np.array([[c, N + c] for c in range(N)], np.int32)
+    return tri, quads, edges, e2t, np.concatenate([h_edges, w_edges])
+
+
+def planarity_loss(pos: torch.Tensor, quads: torch.Tensor) -> torch.Tensor:
+    """sum |(q0 + q2) - (q1 + q3)|^2 over quads (3_fitting.py:181-199)."""
+    d = (pos[quads[:, 0]] + pos[quads[:, 2]]) - (pos[quads[:, 1]] + pos[quads[:, 3]])
+    return (d ** 2).sum()
+
+
+# --------------------------------------------------------------- plain loop
+
+
+def render_mask(res: int, pos_ndc: torch.Tensor, tri: torch.Tensor) -> torch.Tensor:
+    """render_bitset_from_triangles (3_fitting.py:154-163) for one cluster;
+    an empty render (card out of view) is an all-zero mask instead of the
+    reference's empty-max error."""
+    fp, fa = compute_fragments((res, res), pos_ndc, tri)
+    if fp.shape[0] == 0:
+        return torch.zeros((pos_ndc.shape[0], res, res, 1), dtype=torch.int32, device=pos_ndc.device)
+    t2c = torch.zeros(tri.shape[0], dtype=torch.int32, device=tri.device)
+    bits = cluster_mask_from_fragments((res, res), fp, fa, t2c, cluster_count=1)
+    if bits.shape[0] < pos_ndc.shape[0]:
+        pad = torch.zeros((pos_ndc.shape[0] - bits.shape[0], res, res, 1), dtype=bits.dtype,
+                          device=bits.device)
+        bits = torch.cat([bits, pad])
+    return bits
+
+
+def plain_target(job: FitJob, dev) -> torch.Tensor:
+    """Target mask through the drop-in API: dda_compute_fragments +
+    cluster_mask_from_fragments with every edge in cluster 0."""
+    from .dda import dda_compute_fragments
+
+    n, nv = job.strands.shape[:2]
+    pos = torch.from_numpy(job.strand_clip[None]).to(dev)
+    edges = torch.from_numpy(synth.strand_edges(n, nv)).to(dev)
+    fp, fa = dda_compute_fragments((job.res, job.res), pos, edges)
+    if fp.shape[0] == 0:
+        return torch.zeros((1, job.res, job.res, 1), dtype=torch.int32, device=dev)
+    e2c = torch.zeros(edges.shape[0], dtype=torch.int32, device=dev)
+    return cluster_mask_from_fragments((job.res, job.res), fp, fa, e2c, cluster_count=1)
+
+
+def plain_fit(job: FitJob, hyper: FitHyper, dev, iters: int | None = None):
+    """One job through the drop-in API, as 3_fitting.py:381-407 without
+    largesteps.  Returns (final vertices, final XOR pixel count, target
+    pixel count)."""
+    iters = hyper.iters if iters is None else iters
+    tri_np, quads_np, edges_np, e2t_np, lap_np = strip_topology(job.quads)
+    tri = torch.from_numpy(tri_np).to(dev)
+    quads = torch.from_numpy(quads_np).to(dev)
+    edges = torch.from_numpy(edges_np).to(dev)
+    e2t = torch.from_numpy(e2t_np).to(dev)
+    e2c = torch.zeros(edges.shape[0], dtype=torch.int32, device=dev)
+    mvp = torch.from_numpy(job.mvp).to(dev)
+    target = plain_target(job, dev)
+    pos = torch.from_numpy(job.verts.copy()).to(dev).requires_grad_(True)
+    L = laplacian_edges(pos.detach(), torch.from_numpy(lap_np).to(dev))
+    opt = UniformAdam([pos], lr=hyper.lr, betas=(hyper.beta1, hyper.beta2))
+    sched = torch.optim.lr_scheduler.ExponentialLR(opt, gamma=hyper.gamma)
+
+    def project(p):
+        ph = torch.cat([p, torch.ones_like(p[..., :1])], -1)
+        return torch.matmul(ph, mvp.t())[None].contiguous()
+
+    for _ in range(iters):
+        opt.zero_grad()
+        pos_ndc = project(pos)
+        bits = render_mask(job.res, pos_ndc.detach(), tri)
+        loss = antialiased_cluster_bitset_loss(bits, target, pos_ndc, tri, edges, e2t, e2c,
+                                               kernel_radius=hyper.radius, rho=hyper.rho)
+        loss = loss + hyper.w_pln * planarity_loss(pos, quads)
+        loss = loss + hyper.w_lap * torch.sum(torch.sparse.mm(L, pos) ** 2)
+        loss.backward()
+        opt.step()
+        sched.step()
+    with torch.no_grad():
+        bits = render_mask(job.res, project(pos), tri)
+        xor = int(popcount_bitset(torch.bitwise_xor(bits, target)).sum().item())
+        tpix = int(popcount_bitset(target).sum().item())
+    return pos.detach().cpu().numpy(), xor, tpix
+
+
+# ------------------------------------------------------------- batched form
+
+
+@dataclass
+class SlotState:
+    job: FitJob
+    admitted: float
+    done_iters: int = 0
+
+
+class CardFitServer:
+    """Device-resident slot table for fitting jobs.  Slot s owns vertices
+    [s*MAX_VERTS, ...) and target words [s*MAX_WORDS, ...); ``run(iters)``
+    advances every active slot with one ``fit_cards`` launch whose grid is
+    the whole table (idle slots exit at once), so the launch can be captured
+    once and replayed while jobs come and go."""
+
+    def __init__(self, capacity: int, hyper: FitHyper, dev, budget_bytes: float = 200e9):
+        self.cap = capacity
+        self.hyper = hyper
+        self.dev = dev
+        self.budget = budget_bytes
+        self.used_bytes = 0
+        i32 = dict(dtype=torch.int32, device=dev)
+        f32 = dict(dtype=torch.float32, device=dev)
+        self.res = torch.zeros(capacity, **i32)
+        self.quads = torch.ones(capacity, **i32)
+        self.vert_off = torch.arange(capacity, **i32) * MAX_VERTS
+        self.bits_off = torch.arange(capacity, **i32) * MAX_WORDS
+        self.mvp = torch.zeros((capacity, 4, 4), **f32)
+        self.steps = torch.zeros(capacity, **i32)
+        self.loss = torch.zeros(capacity, **f32)
+        self.verts = torch.zeros((capacity * MAX_VERTS, 3), **f32)
+        self.adam_m = torch.zeros((capacity * MAX_VERTS, 3), **f32)
+        self.adam_v = torch.zeros(capacity * MAX_VERTS, **f32)
+        self.target = torch.zeros(capacity * MAX_WORDS, **i32)
+        self.slots = torch.full((capacity,), -1, **i32)
+        self.free = deque(range(capacity))
+        self.state: dict[int, SlotState] = {}
+        self.graph = None
+        self.graph_iters = None
+
+    # -- admission -----------------------------------------------------
+    def can_admit(self, job: FitJob) -> bool:
+        return bool(self.free) and self.used_bytes + job.bytes <= self.budget
+
+    def admit(self, jobs: list[FitJob], now: float = 0.0) -> list[int]:
+        """Uploads the jobs into free slots, computes their target masks on
+        the GPU and activates the slots (all asynchronous on the current
+        stream)."""
+        if not jobs:
+            return []
+        slots = [self.free.popleft() for _ in jobs]
+        dev = self.dev
+        idx = torch.tensor(slots, dtype=torch.long, device=dev)
+        self.res[idx] = torch.tensor([j.res for j in jobs], dtype=torch.int32, device=dev)
+        self.quads[idx] = torch.tensor([j.quads for j in jobs], dtype=torch.int32, device=dev)
+        self.mvp[idx] = torch.from_numpy(np.stack([j.mvp for j in jobs])).to(dev)
+        self.steps[idx] = 0
+        vrows = np.concatenate([s * MAX_VERTS + np.arange(j.verts.shape[0]) for s, j in zip(slots, jobs)])
+        vidx = torch.from_numpy(vrows).to(dev)
+        self.verts[vidx] = torch.from_numpy(np.concatenate([j.verts for j in jobs])).to(dev)
+        self.adam_m[vidx] = 0.0
+        self.adam_v[vidx] = 0.0
+        # target masks
+        nv = jobs[0].strands.shape[1]
+        clip = np.concatenate([j.strand_clip for j in jobs]).reshape(-1, nv, 4)
+        counts = [j.strands.shape[0] for j in jobs]
+        offs = np.concatenate([[0], np.cumsum(counts)[:-1]]) * nv
+        _core.fit_targets(torch.tensor([j.res for j in jobs], dtype=torch.int32, device=dev),
+                          torch.tensor([s * MAX_WORDS for s in slots], dtype=torch.int32, device=dev),
+                          torch.from_numpy(offs.astype(np.int32)).to(dev),
+                          torch.tensor(counts, dtype=torch.int32, device=dev),
+                          torch.from_numpy(clip).to(dev), self.target)
+        self.slots[idx] = idx.to(torch.int32)
+        for s, j in zip(slots, jobs):
+            self.state[s] = SlotState(j, now)
+            self.used_bytes += j.bytes
+        return slots
+
+    def release(self, slots: list[int]) -> None:
+        if not slots:
+            return
+        self.slots[torch.tensor(slots, dtype=torch.long, device=self.dev)] = -1
+        for s in slots:
+            self.used_bytes -= self.state.pop(s).job.bytes
+            self.free.append(s)
+
+    # -- iterations ----------------------------------------------------
+    def _launch(self, iters: int) -> None:
+        _core.fit_cards(self.slots, iters, self.res, self.quads, self.vert_off, self.bits_off, self.mvp,
+                        self.verts, self.adam_m, self.adam_v, self.steps, self.target,
+                        self.hyper.vector(), self.loss)
+
+    def run(self, iters: int, graph: bool = False) -> None:
+        """Advances every active slot by `iters` iterations (and refreshes
+        its loss).  With graph=True the launch is captured once per
+        iteration count and replayed."""
+        if not graph:
+            self._launch(iters)
+        else:
+            if self.graph is None or self.graph_iters != iters:
+                s = torch.cuda.Stream()
+                s.wait_stream(torch.cuda.current_stream())
+                with torch.cuda.stream(s):
+                    self._launch(0)  # warm-up outside capture (no state change)
+                torch.cuda.current_stream().wait_stream(s)
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    self._launch(iters)
+                self.graph, self.graph_iters = g, iters
+            self.graph.replay()
+        for st in self.state.values():
+            st.done_iters += iters
+
+    def job_vertices(self, slot: int) -> np.ndarray:
+        n = self.state[slot].job.verts.shape[0]
+        return self.verts[slot * MAX_VERTS: slot * MAX_VERTS + n].cpu().numpy()
+
+    def target_mask(self, slot: int) -> np.ndarray:
+        """(res, res) bool target of a slot, unpacked from its bit words."""
+        res = self.state[slot].job.res
+        words = self.target[slot * MAX_WORDS: slot * MAX_WORDS + res * res // 32].cpu().numpy()
+        bits = np.unpackbits(words.view(np.uint8), bitorder="little")
+        return bits[: res * res].reshape(res, res).astype(bool)
+
+
+@dataclass
+class ServeReport:
+    completed: int
+    iterations: int
+    seconds: float
+    latencies_ms: list[float] = field(default_factory=list)
+    losses: dict[int, float] = field(default_factory=dict)
+    target_pixels: dict[int, int] = field(default_factory=dict)
+
+    @property
+    def iters_per_s(self) -> float:
+        return self.iterations / self.seconds if self.seconds > 0 else 0.0
+
+    def percentile(self, q: float) -> float:
+        return float(np.percentile(self.latencies_ms, q)) if self.latencies_ms else float("nan")
+
+
+def serve(jobs: list[FitJob], server: CardFitServer, chunk: int = 10, graph: bool = True,
+          clock=time.perf_counter) -> ServeReport:
+    """Continuous batching: jobs (with `arrival` seconds) are admitted as
+    slots and budget free up, all resident jobs advance together `chunk`
+    iterations per replay, finished jobs are retired and their latency
+    (arrival -> result on the host) recorded."""
+    total = server.hyper.iters
+    if total % chunk:
+        raise ValueError("chunk must divide the per-job iteration count")
+    pending = deque(sorted(jobs, key=lambda j: j.arrival))
+    rep = ServeReport(0, 0, 0.0)
+    t0 = clock()
+    while pending or server.state:
+        now = clock() - t0
+        batch = []
+        while pending and pending[0].arrival <= now and server.can_admit(pending[0]):
+            batch.append(pending.popleft())
+        if batch:
+            server.admit(batch, now)
+        if not server.state:
+            if pending:
+                time.sleep(max(0.0, pending[0].arrival - (clock() - t0)))
+            continue
+        server.run(chunk, graph=graph)
+        torch.cuda.current_stream().synchronize()
+        done = [s for s, st in server.state.items() if st.done_iters >= total]
+        if done:
+            t_done = clock() - t0
+            loss = server.loss.cpu().numpy()
+            for s in done:
+                st = server.state[s]
+                rep.latencies_ms.append((t_done - st.job.arrival) * 1e3)
+                rep.losses[st.job.jid] = float(loss[s])
+                rep.completed += 1
+                rep.iterations += total
+            server.release(done)
+    rep.seconds = clock() - t0
+    return rep
+
+
+def poisson_arrivals(jobs: list[FitJob], rate: float, seed: int = 0) -> list[FitJob]:
+    """Assigns seeded exponential inter-arrival times (mean 1/rate s)."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=len(jobs)))
+    for j, a in zip(jobs, t):
+        j.arrival = float(a)
+    return jobs
