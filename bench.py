@@ -69,6 +69,85 @@ def cpu_baseline(w, seconds):
                       f"{dt * 1e3:.1f} ms/rep"}
 
 
+def fit_section(dev, num_jobs=4096, plain_jobs=6, rates=(200.0, 1000.0, 5000.0)):
+    """Card fitting: batched persistent launch, plain per-job loop, served
+    continuous batching under Poisson arrivals (all 100 iterations per
+    job)."""
+    import numpy as np
+    import torch
+
+    from strns2cards_amd import fitting
+
+    t0 = time.perf_counter()
+    jobs = fitting.make_jobs(num_jobs, seed=0)
+    print(f"[bench] {num_jobs} fitting jobs generated in {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    hyper = fitting.FitHyper()
+    out = {}
+
+    # one persistent launch: every job runs all its iterations
+    srv = fitting.CardFitServer(num_jobs, hyper, dev)
+    slots = srv.admit(jobs)
+    srv.run(0)
+    torch.cuda.synchronize()
+    init_loss = srv.loss.cpu().numpy()[slots]
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    srv.run(hyper.iters)
+    b.record()
+    torch.cuda.synchronize()
+    batched_s = a.elapsed_time(b) / 1e3
+    final_loss = srv.loss.cpu().numpy()[slots]
+    tpix = {}
+    out["batched_jobs"] = num_jobs
+    out["batched_launch_ms"] = round(batched_s * 1e3, 3)
+    out["batched_iters_per_s"] = round(num_jobs * hyper.iters / batched_s, 1)
+    print(f"[bench] batched: {num_jobs} jobs x {hyper.iters} its in {batched_s * 1e3:.1f} ms",
+          file=sys.stderr, flush=True)
+
+    # plain per-job loop through the drop-in API, on the first jobs
+    plain_xor, served_xor, plain_tp = 0, 0.0, 0
+    fitting.plain_fit(jobs[0], hyper, dev, iters=2)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for j in jobs[:plain_jobs]:
+        _, xor, tp = fitting.plain_fit(j, hyper, dev)
+        plain_xor += xor
+        plain_tp += tp
+        served_xor += float(final_loss[j.jid])
+        tpix[j.jid] = tp
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    plain_ips = plain_jobs * hyper.iters / plain_s
+    out["plain_jobs"] = plain_jobs
+    out["plain_iters_per_s"] = round(plain_ips, 2)
+    out["plain_final_coverage_loss"] = round(plain_xor / max(plain_tp, 1), 5)
+    out["served_final_coverage_loss"] = round(served_xor / max(plain_tp, 1), 5)
+    out["final_loss_rel_diff"] = round(abs(served_xor - plain_xor) / max(plain_xor, 1), 5)
+    out["initial_coverage_loss_all_jobs"] = float(init_loss.sum())
+    out["final_coverage_loss_all_jobs"] = float(final_loss.sum())
+    print(f"[bench] plain loop {plain_ips:.1f} it/s, loss plain {plain_xor} served {served_xor}",
+          file=sys.stderr, flush=True)
+
+    # served: Poisson arrivals, continuous batching, graph replays
+    served = {}
+    for lam in rates:
+        n = min(num_jobs, max(64, int(lam * 1.0)))
+        js = fitting.poisson_arrivals(jobs[:n], lam, seed=int(lam))
+        srv = fitting.CardFitServer(num_jobs, hyper, dev)
+        rep = fitting.serve(js, srv, chunk=10)
+        served[str(int(lam))] = {
+            "jobs": n, "iters_per_s": round(rep.iters_per_s, 1),
+            "p50_ms": round(rep.percentile(50), 3), "p99_ms": round(rep.percentile(99), 3),
+            "makespan_s": round(rep.seconds, 3),
+            "vs_plain": round(rep.iters_per_s / plain_ips, 1)}
+        print(f"[bench] serve lambda={lam:.0f}: {served[str(int(lam))]}", file=sys.stderr, flush=True)
+    out["served"] = served
+    out["batched_vs_plain"] = round(out["batched_iters_per_s"] / plain_ips, 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +155,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fit", action="store_true")
+    ap.add_argument("--fit-jobs", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -154,6 +235,8 @@ def main():
     print(f"[bench] card raster->mask {ms_tri:.3f} ms, antialias grad {ms_aa:.3f} ms",
           file=sys.stderr, flush=True)
 
+    fit = None if args.no_fit else fit_section(dev, num_jobs=args.fit_jobs)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, args.cpu_seconds)
@@ -184,6 +267,7 @@ def main():
         "cpu_baseline": cpu,
         "extra": {"card_raster_mask_ms": round(ms_tri, 4), "antialias_grad_ms": round(ms_aa, 4),
                   "cards": int(c.tri.shape[0] // 40), "card_triangles": int(c.tri.shape[0])},
+        "card_fit": fit,
     }
     print(json.dumps(out), flush=True)
 

@@ -91,24 +91,33 @@ def make_jobs(count: int, seed: int = 0, nv: int = 64, cap_angle: float = 0.15) 
     {8, 12, 16, 24, 32}, resolution uniform in {64, 128, 256}.  The card is
     the mean strand resampled to Q + 1 points, swept 1 cm wide."""
     rng = np.random.default_rng(seed)
+    ns = np.rint(np.exp(rng.uniform(math.log(8), math.log(512), size=count))).astype(np.int64)
+    Qs = rng.choice([8, 12, 16, 24, 32], size=count)
+    ress = rng.choice([64, 128, 256], size=count)
+    centres = rng.normal(size=(count, 3))
+    centres[:, 1] = np.abs(centres[:, 1])
+    centres /= np.linalg.norm(centres, axis=1, keepdims=True)
+    owner = np.repeat(np.arange(count), ns)
+    d = centres[owner] + cap_angle * rng.normal(size=(owner.size, 3))
+    d[:, 1] = np.abs(d[:, 1])
+    # all jobs' strands in one vectorized pass, then split per job
+    strands_all = synth.generate_strands(owner.size, nv, 0.004, seed, rng=rng, root_dirs=d) * 100.0
+    starts = np.concatenate([[0], np.cumsum(ns)])
     jobs = []
     for j in range(count):
-        n = int(round(math.exp(rng.uniform(math.log(8), math.log(512)))))
-        Q = int(rng.choice([8, 12, 16, 24, 32]))
-        res = int(rng.choice([64, 128, 256]))
-        c = rng.normal(size=3)
-        c[1] = abs(c[1])
-        c /= np.linalg.norm(c)
-        d = c[None] + cap_angle * rng.normal(size=(n, 3))
-        d[:, 1] = np.abs(d[:, 1])
-        strands = synth.generate_strands(n, nv, 0.004, seed, rng=rng, root_dirs=d) * 100.0
+        strands = strands_all[starts[j]:starts[j + 1]]
         pts = strands.reshape(-1, 3)
         mvp = synth.ortho_cameras(pts, 1, 0.3)[0]
-        centre = synth.resample_by_arclength(strands.mean(0, keepdims=True), Q + 1)
-        verts = synth.strip_rows(centre, 1.0)[0].reshape(-1, 3).astype(np.float32)
         clip = synth.project(pts, mvp[None])[0]
-        jobs.append(FitJob(j, res, Q, strands.astype(np.float32), clip, mvp.astype(np.float32),
-                           verts))
+        jobs.append(FitJob(j, int(ress[j]), int(Qs[j]), strands.astype(np.float32), clip,
+                           mvp.astype(np.float32), np.zeros((2 * (int(Qs[j]) + 1), 3), np.float32)))
+    # cards: mean strand -> Q + 1 points -> 1 cm strip, batched per quad count
+    for Q in np.unique(Qs):
+        sel = [j for j in range(count) if Qs[j] == Q]
+        means = np.stack([jobs[j].strands.mean(0) for j in sel])
+        rows = synth.strip_rows(synth.resample_by_arclength(means, int(Q) + 1), 1.0)
+        for k, j in enumerate(sel):
+            jobs[j].verts = rows[k].reshape(-1, 3).astype(np.float32)
     return jobs
 
 
