@@ -30,6 +30,7 @@ constexpr int kFitMaxQuads = 32;
 constexpr int kFitMaxVerts = 2 * (kFitMaxQuads + 1);
 constexpr int kFitMaxTris = 2 * kFitMaxQuads;
 constexpr int kFitMaxEdges = 4 * kFitMaxQuads + 1;
+constexpr int kFitChunk = 512;  // silhouette samples evaluated per workgroup pass
 
 struct FitHyper {
     float lr0, b1, b2, gamma, w_pln, w_lap, rho, R;
@@ -140,6 +141,10 @@ fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict
     __shared__ float P[kFitMaxVerts][3];
     __shared__ float lp[kFitMaxVerts][3];
     __shared__ float acc[kFitMaxEdges][6];
+    __shared__ DdaSpan espan[kFitMaxEdges];
+    __shared__ float2 enorm[kFitMaxEdges];
+    __shared__ int eexcl[kFitMaxEdges];
+    __shared__ float contrib[kFitChunk][6];
     __shared__ int4 rect[kFitMaxTris];
     __shared__ int excl[kBlock];
     __shared__ int sh[kBlock / kWave + 1];
@@ -189,7 +194,12 @@ fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict
         __syncthreads();
         if (it == iters) break;  // final render of the updated card
 
-        // ---- silhouette edges and their antialiased gradient
+        // ---- silhouette edges and their antialiased gradient.  Each edge's
+        // DDA samples are spread over the whole workgroup (kFitChunk at a
+        // time); a sample's six contributions go to LDS and the edge's
+        // thread then adds them in sample order, so the sums are exactly
+        // those of a serial walk of the edge.
+        int cnt = 0;
         if (tid < E) {
             const StripEdge se = strip_edge(tid, Q);
             const float4 P0 = clip[se.v0], P1 = clip[se.v1];
@@ -198,27 +208,56 @@ fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict
             const float4 Q3 = se.tb >= 0 ? clip[third_of(strip_tri(se.tb, N), se.v0, se.v1)] : none;
             const float fres = static_cast<float>(res);
             bool sil = false;
-            float2 n;
-            float a[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            float2 n = make_float2(0.f, 0.f);
             if (aa_edge(fres, fres, P0, P1, se.ta >= 0, Q2, se.tb >= 0, Q3, sil, n) && sil) {
-                DdaSpan sp;
                 bool wrote;
-                const int cnt = dda_span(res, res, P0, P1, sp, wrote);
-                for (int k = 0; k < cnt; ++k) {
-                    DdaFrag fr;
-                    if (!dda_frag(res, res, sp, k, P0, P1, fr)) continue;
+                cnt = dda_span(res, res, P0, P1, espan[tid], wrote);
+            }
+            enorm[tid] = n;
+        }
+        int nsamp;
+        const int first = block_excl_scan(cnt, sh, &nsamp);
+        if (tid < kFitMaxEdges) eexcl[tid] = first;
+        float a[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        __syncthreads();
+        for (int base = 0; base < nsamp; base += kFitChunk) {
+            const int lim = dmin(nsamp - base, kFitChunk);
+            for (int q = tid; q < lim; q += kBlock) {
+                const int j = base + q;
+                int lo = 0, hi = E;  // last edge with eexcl <= j
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (eexcl[mid] <= j) lo = mid;
+                    else hi = mid;
+                }
+                const StripEdge se = strip_edge(lo, Q);
+                const float4 P0 = clip[se.v0], P1 = clip[se.v1];
+                const DdaSpan sp = espan[lo];
+                const float2 n = enorm[lo];
+                float c[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                DdaFrag fr;
+                if (dda_frag(res, res, sp, j - eexcl[lo], P0, P1, fr)) {
                     const AaSample s = aa_sample(res, res, fr.h, fr.w, sp.axis, fr.t1, P0, P1, n);
-                    if (!s.live) continue;
                     float dpx, dpy, dpw;
-                    if (!aa_sample_grad(s, res, res, hp.R, hp.rho, n, code, dpx, dpy, dpw)) continue;
-                    a[0] += dpx * s.t0;
-                    a[1] += dpy * s.t0;
-                    a[2] += dpw * s.t0;
-                    a[3] += dpx * s.t1;
-                    a[4] += dpy * s.t1;
-                    a[5] += dpw * s.t1;
+                    if (s.live && aa_sample_grad(s, res, res, hp.R, hp.rho, n, code, dpx, dpy, dpw)) {
+                        c[0] = dpx * s.t0; c[1] = dpy * s.t0; c[2] = dpw * s.t0;
+                        c[3] = dpx * s.t1; c[4] = dpy * s.t1; c[5] = dpw * s.t1;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 6; ++k) contrib[q][k] = c[k];
+            }
+            __syncthreads();
+            if (tid < E && cnt > 0) {
+                const int s0 = dmax(first - base, 0), s1 = dmin(first + cnt - base, lim);
+                for (int q = s0; q < s1; ++q) {
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) a[k] += contrib[q][k];
                 }
             }
+            __syncthreads();
+        }
+        if (tid < E) {
 #pragma unroll
             for (int k = 0; k < 6; ++k) acc[tid][k] = a[k];
         }
