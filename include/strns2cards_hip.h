@@ -214,6 +214,27 @@ int s2c_fit_cards(int num_slots, const int* slots, int iters, const int* res,
                   int* steps, const unsigned* target_bits, const float* hyper,
                   float* loss_out, void* stream);
 
+/* ---------------------------------------------------------------------- */
+/* Differentiable card coverage (hand-written adjoint)                     */
+/* ---------------------------------------------------------------------- */
+/* No single reference entry point: the differentiable counterpart of the
+ * rasterize (rasterize.py:6-95) + antialias (antialias.py:66-121) chain
+ * that 3_fitting.py:380-400 differentiates.  Coverage is area-sampled:
+ * coverage[b, h, w] += area(triangle ∩ [w, w+1] x [h, h+1]) for every
+ * triangle of view b, in the screen coordinates of dda.cu:33-36.  It is
+ * continuous in the vertices, so its gradient is checkable by finite
+ * differences.  pos: (B, V, 4) clip space (w > 0; other triangles are
+ * skipped), tri: (T, 3) indices in [0, V), H * W <= 8,388,607.  Both
+ * calls ACCUMULATE (coverage and d_pos must be zero-filled by the caller
+ * for a fresh result).  The backward recomputes all geometry from pos and
+ * writes d loss / d pos (x, y, w; z gets 0) given d_coverage = d loss /
+ * d coverage. */
+int s2c_card_coverage_forward(int B, int H, int W, int V, const float* pos, int T,
+                              const int* tri, float* coverage, void* stream);
+int s2c_card_coverage_backward(int B, int H, int W, int V, const float* pos, int T,
+                               const int* tri, const float* d_coverage, float* d_pos,
+                               void* stream);
+
 #ifdef __cplusplus
 }
 #endif

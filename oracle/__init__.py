@@ -31,6 +31,8 @@ _lib = None
 
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_opt_i32p = ctypes.c_void_p  # an int32 array's .ctypes.data, or None
 _I = ctypes.c_int
 _F = ctypes.c_float
 
@@ -61,6 +63,9 @@ _SIGS = {
     "orc_skinning": (None, [_I, _I, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _F]),
     "orc_dda_cluster_mask": (None, [_I, _I, _I, _I, _f32p, _I, _i32p, _i32p, _I, _i32p, _I]),
     "orc_triangle_cluster_mask": (None, [_I, _I, _I, _I, _f32p, _I, _i32p, _i32p, _I, _i32p]),
+    "orc_card_coverage": (None, [_I, _I, _I, _I, _f64p, _I, _i32p, _I, _opt_i32p, _opt_i32p,
+                                 _opt_i32p, _I, _I, _f64p]),
+    "orc_card_coverage_grad": (None, [_I, _I, _I, _I, _f64p, _I, _i32p, _f64p, _f64p]),
 }
 
 
@@ -334,3 +339,90 @@ def skinning(strands, arclength, handle, canonical, R, beta):
     out = np.zeros_like(x)
     lib().orc_skinning(N, P, x, s, h, c, r, out, float(beta))
     return out
+
+
+# ------------------------------------------------------ card coverage (adjoint)
+
+
+def _d(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def card_coverage(resolution, pos, tri, sel=None, window=None) -> np.ndarray:
+    """Area-sampled coverage (B, H, W) float64 of the triangles (all, or the
+    indices ``sel``) of each view; ``window`` = (y0[B], x0[B], h, w) restricts
+    the output to an h x w window per view."""
+    H, W = resolution
+    pos = _d(pos)
+    tri = _i(tri)
+    B, V = pos.shape[:2]
+    if window is None:
+        y0 = x0 = None
+        h, w = H, W
+    else:
+        y0, x0, h, w = window
+        y0, x0 = _i(y0), _i(x0)
+    out = np.zeros((B, h, w), np.float64)
+    sel_a = None if sel is None else _i(sel)
+    lib().orc_card_coverage(B, H, W, V, pos, tri.shape[0],
+                            tri, 0 if sel_a is None else sel_a.size,
+                            None if sel_a is None else sel_a.ctypes.data,
+                            None if y0 is None else y0.ctypes.data,
+                            None if x0 is None else x0.ctypes.data, h, w, out)
+    return out
+
+
+def card_coverage_grad(resolution, pos, tri, d_cov) -> np.ndarray:
+    """d loss / d pos (B, V, 4) float64 given d loss / d coverage."""
+    H, W = resolution
+    pos = _d(pos)
+    tri = _i(tri)
+    B, V = pos.shape[:2]
+    out = np.zeros((B, V, 4), np.float64)
+    lib().orc_card_coverage_grad(B, H, W, V, pos, tri.shape[0], tri, _d(d_cov), out)
+    return out
+
+
+def _screen(pos, H, W):
+    x = 0.5 * (1.0 + pos[..., 0] / pos[..., 3]) * W
+    y = 0.5 * (1.0 - pos[..., 1] / pos[..., 3]) * H
+    return x, y
+
+
+def coverage_loss_difference(resolution, pos, tri, target, cov, delta) -> float:
+    """(L(pos + delta) - L(pos - delta)) / 2 for L = sum (C - target)^2, in
+    float64, where ``cov`` = card_coverage(pos) and ``delta`` (B, V, 4) is
+    non-zero on a few vertices: only the triangles touching them are
+    rendered again, inside a per-view window around them."""
+    H, W = resolution
+    pos = _d(pos)
+    delta = _d(delta)
+    tri = _i(tri)
+    verts = np.flatnonzero(np.any(delta != 0, axis=(0, 2)))
+    sel = np.flatnonzero(np.isin(tri, verts).any(1)).astype(np.int32)
+    if sel.size == 0:
+        return 0.0
+    B = pos.shape[0]
+    vs = np.unique(tri[sel])
+    xs, ys = [], []
+    for p in (pos - delta, pos, pos + delta):
+        x, y = _screen(p[:, vs], H, W)
+        xs.append(x)
+        ys.append(y)
+    xs, ys = np.concatenate(xs, 1), np.concatenate(ys, 1)
+    x0 = np.clip(np.floor(xs.min(1)) - 1, 0, W - 1).astype(np.int32)
+    y0 = np.clip(np.floor(ys.min(1)) - 1, 0, H - 1).astype(np.int32)
+    x1 = np.clip(np.floor(xs.max(1)) + 2, 1, W).astype(np.int32)
+    y1 = np.clip(np.floor(ys.max(1)) + 2, 1, H).astype(np.int32)
+    h, w = int((y1 - y0).max()), int((x1 - x0).max())
+    win = (y0, x0, h, w)
+    old = card_coverage(resolution, pos, tri, sel, win)
+    e = np.zeros((B, h, w))
+    for b in range(B):
+        hb = min(h, H - y0[b])
+        wb = min(w, W - x0[b])
+        e[b, :hb, :wb] = (np.asarray(cov, np.float64)[b, y0[b]:y0[b] + hb, x0[b]:x0[b] + wb] -
+                          np.asarray(target, np.float64)[b, y0[b]:y0[b] + hb, x0[b]:x0[b] + wb])
+    lp = ((e + card_coverage(resolution, pos + delta, tri, sel, win) - old) ** 2).sum()
+    lm = ((e + card_coverage(resolution, pos - delta, tri, sel, win) - old) ** 2).sum()
+    return 0.5 * (lp - lm)

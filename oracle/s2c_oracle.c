@@ -712,3 +712,186 @@ void orc_triangle_cluster_mask(int B, int H, int W, int V, const float *pos,
         }
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Area-sampled card coverage (the differentiable card-fitting form)         */
+/* ------------------------------------------------------------------------ */
+/* The reference differentiates its rasterize + antialias chain
+ * (3_fitting.py:380-400) through a surrogate edge gradient.  The adjoint
+ * form of this library renders area-sampled coverage instead,
+ *   C[b, h, w] = sum_t area(triangle t of view b ∩ [w, w+1] x [h, h+1]),
+ * in the screen coordinates of dda.cu:33-36, so the gradient is exact and
+ * checkable by finite differences.  This restatement works in double and
+ * uses a different algorithm from the HIP kernel: the pixel square is
+ * clipped by the triangle's three half-planes (Sutherland-Hodgman) and the
+ * polygon's area is taken with the shoelace formula.  The backward is the
+ * transport theorem: moving the vertices of edge a->b changes the covered
+ * area by the outward normal times |ab| integrated against (1 - t) (for a)
+ * and t (for b) over the part of the edge inside the pixel box. */
+
+typedef struct {
+    double p[3][2];
+    int map[3];
+    int valid;
+} CovTri;
+
+static CovTri cov_tri(const double *pos, long long vb, const int *t, int H, int W) {
+    CovTri c;
+    c.valid = 1;
+    for (int i = 0; i < 3; ++i) {
+        const double *P = pos + (vb + t[i]) * 4;
+        if (!(P[3] > 0.0)) c.valid = 0;
+        c.p[i][0] = 0.5 * (1.0 + P[0] / P[3]) * W;
+        c.p[i][1] = 0.5 * (1.0 - P[1] / P[3]) * H;
+        c.map[i] = i;
+    }
+    const double cr = (c.p[1][0] - c.p[0][0]) * (c.p[2][1] - c.p[0][1]) -
+                      (c.p[1][1] - c.p[0][1]) * (c.p[2][0] - c.p[0][0]);
+    if (!(cr != 0.0) || !isfinite(cr)) c.valid = 0;
+    if (cr < 0.0) {
+        double q0 = c.p[1][0], q1 = c.p[1][1];
+        c.p[1][0] = c.p[2][0];
+        c.p[1][1] = c.p[2][1];
+        c.p[2][0] = q0;
+        c.p[2][1] = q1;
+        c.map[1] = 2;
+        c.map[2] = 1;
+    }
+    return c;
+}
+
+/* Pixel-rectangle of the triangle, clamped to the image: r = {h0, rows, w0, cols}. */
+static void cov_rect(const CovTri *c, int H, int W, int r[4]) {
+    double xmin = c->p[0][0], xmax = xmin, ymin = c->p[0][1], ymax = ymin;
+    for (int i = 1; i < 3; ++i) {
+        xmin = fmin(xmin, c->p[i][0]);
+        xmax = fmax(xmax, c->p[i][0]);
+        ymin = fmin(ymin, c->p[i][1]);
+        ymax = fmax(ymax, c->p[i][1]);
+    }
+    const double w0 = fmax(floor(xmin), 0.0), w1 = fmin(floor(xmax), W - 1.0);
+    const double h0 = fmax(floor(ymin), 0.0), h1 = fmin(floor(ymax), H - 1.0);
+    r[0] = (int)h0;
+    r[1] = h1 >= h0 ? (int)(h1 - h0) + 1 : 0;
+    r[2] = (int)w0;
+    r[3] = w1 >= w0 ? (int)(w1 - w0) + 1 : 0;
+    if (!c->valid) r[1] = r[3] = 0;
+}
+
+/* Area of the unit pixel square at (bx, by) inside the positively oriented
+ * triangle. */
+static double square_in_tri(const CovTri *c, double bx, double by) {
+    double a[16][2], b[16][2];
+    int n = 4;
+    const double sq[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+    for (int i = 0; i < 4; ++i) {
+        a[i][0] = sq[i][0];
+        a[i][1] = sq[i][1];
+    }
+    for (int e = 0; e < 3 && n > 0; ++e) {
+        const double ox = c->p[e][0] - bx, oy = c->p[e][1] - by;
+        const double dx = c->p[(e + 1) % 3][0] - c->p[e][0];
+        const double dy = c->p[(e + 1) % 3][1] - c->p[e][1];
+        int m = 0;
+        for (int i = 0; i < n; ++i) {
+            const double *u = a[i], *v = a[(i + 1) % n];
+            const double fu = dx * (u[1] - oy) - dy * (u[0] - ox);
+            const double fv = dx * (v[1] - oy) - dy * (v[0] - ox);
+            if (fu >= 0) {
+                b[m][0] = u[0];
+                b[m][1] = u[1];
+                ++m;
+            }
+            if ((fu >= 0) != (fv >= 0)) {
+                const double s = fu / (fu - fv);
+                b[m][0] = u[0] + s * (v[0] - u[0]);
+                b[m][1] = u[1] + s * (v[1] - u[1]);
+                ++m;
+            }
+        }
+        n = m;
+        for (int i = 0; i < n; ++i) {
+            a[i][0] = b[i][0];
+            a[i][1] = b[i][1];
+        }
+    }
+    double s = 0;
+    for (int i = 0; i < n; ++i) s += a[i][0] * a[(i + 1) % n][1] - a[i][1] * a[(i + 1) % n][0];
+    return 0.5 * s;
+}
+
+/* out[(b*h + i)*w + j] += coverage of pixel (y0[b] + i, x0[b] + j) by the
+ * triangles sel[0..n_sel) (all T triangles when sel is NULL) of view b. */
+void orc_card_coverage(int B, int H, int W, int V, const double *pos, int T, const int *tri,
+                       int n_sel, const int *sel, const int *y0, const int *x0, int h, int w,
+                       double *out) {
+    const int n = sel ? n_sel : T;
+    for (int b = 0; b < B; ++b) {
+        const int oy = y0 ? y0[b] : 0, ox = x0 ? x0[b] : 0;
+        for (int k = 0; k < n; ++k) {
+            const int t = sel ? sel[k] : k;
+            const CovTri c = cov_tri(pos, (long long)b * V, tri + 3 * t, H, W);
+            int r[4];
+            cov_rect(&c, H, W, r);
+            for (int y = r[0]; y < r[0] + r[1]; ++y) {
+                if (y < oy || y >= oy + h) continue;
+                for (int x = r[2]; x < r[2] + r[3]; ++x) {
+                    if (x < ox || x >= ox + w) continue;
+                    out[((long long)b * h + (y - oy)) * w + (x - ox)] += square_in_tri(&c, x, y);
+                }
+            }
+        }
+    }
+}
+
+/* d_pos[b, v, (x, y, w)] += d loss / d pos given d_cov = d loss / d C. */
+void orc_card_coverage_grad(int B, int H, int W, int V, const double *pos, int T,
+                            const int *tri, const double *d_cov, double *d_pos) {
+    for (int b = 0; b < B; ++b) {
+        for (int t = 0; t < T; ++t) {
+            const CovTri c = cov_tri(pos, (long long)b * V, tri + 3 * t, H, W);
+            int r[4];
+            cov_rect(&c, H, W, r);
+            double gs[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+            for (int y = r[0]; y < r[0] + r[1]; ++y) {
+                for (int x = r[2]; x < r[2] + r[3]; ++x) {
+                    const double g = d_cov[((long long)b * H + y) * W + x];
+                    if (g == 0.0) continue;
+                    for (int e = 0; e < 3; ++e) {
+                        const int f = (e + 1) % 3;
+                        const double px = c.p[e][0], py = c.p[e][1];
+                        const double dx = c.p[f][0] - px, dy = c.p[f][1] - py;
+                        /* Liang-Barsky clip of p + s d to the box */
+                        const double pp[4] = {-dx, dx, -dy, dy};
+                        const double qq[4] = {px - x, x + 1.0 - px, py - y, y + 1.0 - py};
+                        double t0 = 0.0, t1 = 1.0;
+                        int empty = 0;
+                        for (int i = 0; i < 4; ++i) {
+                            if (pp[i] == 0.0) {
+                                if (qq[i] < 0.0) empty = 1;
+                            } else if (pp[i] < 0.0) {
+                                t0 = fmax(t0, qq[i] / pp[i]);
+                            } else {
+                                t1 = fmin(t1, qq[i] / pp[i]);
+                            }
+                        }
+                        if (empty || !(t1 > t0)) continue;
+                        const double i1 = 0.5 * (t1 * t1 - t0 * t0), i0 = (t1 - t0) - i1;
+                        gs[e][0] += g * dy * i0;
+                        gs[e][1] += -g * dx * i0;
+                        gs[f][0] += g * dy * i1;
+                        gs[f][1] += -g * dx * i1;
+                    }
+                }
+            }
+            for (int i = 0; i < 3; ++i) {
+                const long long row = (long long)b * V + tri[3 * t + c.map[i]];
+                const double *P = pos + row * 4;
+                const double sx = 0.5 * W, sy = 0.5 * H;
+                d_pos[row * 4 + 0] += gs[i][0] * sx / P[3];
+                d_pos[row * 4 + 1] += -gs[i][1] * sy / P[3];
+                d_pos[row * 4 + 3] += (-gs[i][0] * sx * P[0] + gs[i][1] * sy * P[1]) / (P[3] * P[3]);
+            }
+        }
+    }
+}

@@ -22,6 +22,7 @@ except ImportError as _e:  # pragma: no cover - exercised only on broken install
         "strns2cards_amd: native extension not built or not loadable "
         f"({_e}); run `python -m strns2cards_amd.build`") from _e
 
+from .adjoint import *  # noqa: E402,F401,F403
 from .antialias import *  # noqa: E402,F401,F403
 from .bitset_utils import *  # noqa: E402,F401,F403
 from .curve_utils import *  # noqa: E402,F401,F403

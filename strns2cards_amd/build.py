@@ -24,7 +24,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["scan.hip", "binning.hip", "dda.hip", "dda_mask.hip", "raster.hip", "bitset.hip",
-               "antialias.hip", "curve.hip", "fit.hip"]
+               "antialias.hip", "curve.hip", "fit.hip", "coverage.hip"]
 
 # Contraction off and correctly rounded divide/sqrt are the numerics contract
 # with the CPU oracle; packed fp32 is disabled for the device code.

@@ -574,6 +574,36 @@ PYBIND11_MODULE(_core, m) {
               fn);
     });
 
+    // Differentiable card coverage (see the C ABI header).
+    m.def("card_coverage_forward", [](Tensor pos, Tensor tri, Tensor coverage) {
+        const char* fn = "card_coverage_forward";
+        need(pos, fn, "pos", at::kFloat, {-1, -1, 4});
+        need(tri, fn, "tri", at::kInt, {-1, 3});
+        need(coverage, fn, "coverage", at::kFloat, {pos.size(0), -1, -1});
+        same_device(fn, {&pos, &tri, &coverage});
+        DeviceGuard g(pos.device());
+        check(s2c_card_coverage_forward(I(pos.size(0), fn), I(coverage.size(1), fn),
+                                        I(coverage.size(2), fn), I(pos.size(1), fn), P<float>(pos),
+                                        I(tri.size(0), fn), P<int>(tri), P<float>(coverage),
+                                        stream_of(pos)),
+              fn);
+    });
+
+    m.def("card_coverage_backward", [](Tensor pos, Tensor tri, Tensor d_coverage, Tensor d_pos) {
+        const char* fn = "card_coverage_backward";
+        need(pos, fn, "pos", at::kFloat, {-1, -1, 4});
+        need(tri, fn, "tri", at::kInt, {-1, 3});
+        need(d_coverage, fn, "d_coverage", at::kFloat, {pos.size(0), -1, -1});
+        need(d_pos, fn, "d_pos", at::kFloat, {pos.size(0), pos.size(1), 4});
+        same_device(fn, {&pos, &tri, &d_coverage, &d_pos});
+        DeviceGuard g(pos.device());
+        check(s2c_card_coverage_backward(I(pos.size(0), fn), I(d_coverage.size(1), fn),
+                                         I(d_coverage.size(2), fn), I(pos.size(1), fn),
+                                         P<float>(pos), I(tri.size(0), fn), P<int>(tri),
+                                         P<float>(d_coverage), P<float>(d_pos), stream_of(pos)),
+              fn);
+    });
+
     m.def("triangle_cluster_mask", [](Tensor pos, Tensor tri, Tensor tri2cluster, Tensor bitset) {
         const char* fn = "triangle_cluster_mask";
         need(pos, fn, "pos", at::kFloat, {-1, -1, 4});

@@ -321,3 +321,11 @@ def full_workload() -> Workload:
     clusters, 4 views at 1024 x 1024, 1,024 cards x 20 quads."""
     return make_workload("50k strands x 100 verts, 4 views 1024^2, K=1024, 1024 cards x 20 quads",
                          50000, 100, 0.003, 1, 4, 1024, "kmeans", 1024, 20)
+
+
+def adjoint_workload() -> Workload:
+    """The strands and cards of ``full_workload`` (50,000 x 100, 1,024
+    k-means clusters, 1,024 cards x 20 quads) seen from 8 views at 512 x 512:
+    the card-density fitting problem of the hand-written adjoint."""
+    return make_workload("50k strands x 100 verts, 8 views 512^2, K=1024, 1024 cards x 20 quads",
+                         50000, 100, 0.003, 1, 8, 512, "kmeans", 1024, 20)
