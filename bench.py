@@ -148,6 +148,69 @@ def fit_section(dev, num_jobs=4096, plain_jobs=6, rates=(200.0, 1000.0, 5000.0))
     return out
 
 
+def adjoint_section(dev, steps=200):
+    """Card-density fitting through the hand-written coverage adjoint: 1,024
+    cards x 20 quads (centre-lines and widths as parameters) over 8 views at
+    512 x 512, L2 against the 3x3 box-filtered DDA strand density, Adam at
+    lr 1e-3."""
+    import torch
+
+    from strns2cards_amd import _core, adjoint, synth
+
+    t0 = time.perf_counter()
+    wl = synth.adjoint_workload()
+    print(f"[bench] adjoint workload ready in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
+          flush=True)
+    pos = torch.from_numpy(wl.pos).to(dev)
+    target = adjoint.density_target(pos, torch.from_numpy(wl.edges).to(dev),
+                                    torch.from_numpy(wl.edge2cluster).to(dev), wl.num_clusters,
+                                    (wl.H, wl.W))
+    fit = adjoint.CardDensityFit.from_cards(wl.cards, wl.mvp, target, lr=1e-3)
+    clip0 = fit.clip(fit.verts()).detach()
+    fit.run(3)  # warm-up (allocator, Adam state); the timed run restarts from here
+    rep = fit.run(steps)
+    # forward / backward kernels alone at the fitted state, timed on their stream
+    clip = fit.clip(fit.verts()).detach()
+    stream = torch.cuda.current_stream()
+    cov = torch.zeros((clip.shape[0], wl.H, wl.W), dtype=torch.float32, device=dev)
+    d_pos = torch.zeros_like(clip)
+    d_cov = torch.ones_like(cov)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    reps = 20
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    for _ in range(reps):
+        _core.card_coverage_forward(clip, fit.tri, cov)
+    ev[1].record(stream)
+    for _ in range(reps):
+        _core.card_coverage_backward(clip, fit.tri, d_cov, d_pos)
+    ev[2].record(stream)
+    torch.cuda.synchronize()
+    out = {"cards": int(wl.cards.tri.shape[0] // 40), "quads_per_card": 20, "views": int(clip.shape[0]),
+           "resolution": [wl.H, wl.W], "steps": steps, "lr": 1e-3,
+           "ms_per_step": round(rep.ms_per_step, 4),
+           "steps_per_s": round(1e3 / rep.ms_per_step, 2),
+           "loss_first": round(rep.loss_first, 6), "loss_last": round(rep.loss_last, 6),
+           "forward_ms": round(ev[0].elapsed_time(ev[1]) / reps, 4),
+           "backward_ms": round(ev[1].elapsed_time(ev[2]) / reps, 4),
+           "initial_clip_vertices": int(clip0.shape[1])}
+    print(f"[bench] adjoint fit: {out}", file=sys.stderr, flush=True)
+    return out
+
+
+def copy_calibration(dev, nbytes=1 << 29, reps=3):
+    """A plain device-to-device copy of known size, so PMC byte counters can
+    be scaled against it."""
+    import torch
+
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"[bench] copy calibration: {reps} x {nbytes} bytes", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,6 +220,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fit", action="store_true")
     ap.add_argument("--fit-jobs", type=int, default=4096)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--dda-only", action="store_true",
+                    help="time the DDA->mask step only (for PMC runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 512 MiB device copy (PMC byte-counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -210,6 +278,15 @@ def main():
     achieved = nbytes / (ms_launch / 1e3) / 1e9
     print(f"[bench] dda->mask {ms_step:.3f} ms/step, {value:.1f} GB/s", file=sys.stderr, flush=True)
 
+    if args.copy_calibration:
+        copy_calibration(dev)
+    if args.dda_only:
+        print(json.dumps({"metric": METRIC, "value": round(value, 3),
+                          "unit": "GB/s (DDA->mask algorithmic bytes)", "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": round(ms_step, 4), "dda_only": True}), flush=True)
+        return
+
     # Card stages at the same size (reported beside the headline).
     c = w.cards
     cpos = torch.from_numpy(w.card_pos).to(dev)
@@ -236,6 +313,7 @@ def main():
           file=sys.stderr, flush=True)
 
     fit = None if args.no_fit else fit_section(dev, num_jobs=args.fit_jobs)
+    adj = None if args.no_adjoint else adjoint_section(dev)
 
     cpu = None
     if not args.no_cpu:
@@ -262,12 +340,15 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": traffic_from_profiles(),
-                     "kernel": "s2c_dda_cluster_mask (fill_u32_kernel + dda_mask_kernel)",
+                     "kernel": "s2c_dda_cluster_mask entry point (fill_u32_kernel, "
+                               "dda_bin_count_kernel, scan_*, copyBuffer, dda_bin_fill_kernel, "
+                               "tile_resolve_kernel)",
                      "avg_launch_ms": round(ms_launch, 4)},
         "cpu_baseline": cpu,
         "extra": {"card_raster_mask_ms": round(ms_tri, 4), "antialias_grad_ms": round(ms_aa, 4),
                   "cards": int(c.tri.shape[0] // 40), "card_triangles": int(c.tri.shape[0])},
         "card_fit": fit,
+        "adjoint_fit": adj,
     }
     print(json.dumps(out), flush=True)
 
