@@ -214,6 +214,13 @@ int s2c_fit_cards(int num_slots, const int* slots, int iters, const int* res,
                   int* steps, const unsigned* target_bits, const float* hyper,
                   float* loss_out, void* stream);
 
+/* compute_bishop_frame (curve_utils.py:104-165, a Python loop over the
+ * vertices in the reference): rotation-minimizing frames of num_strands
+ * polylines of num_points float3 points by double reflection.  Writes
+ * tangents, normals and binormals, each (num_strands, num_points, 3). */
+int s2c_bishop_frame(int num_strands, int num_points, const float* strands, float* tangents,
+                     float* normals, float* binormals, void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* Differentiable card coverage (hand-written adjoint)                     */
 /* ---------------------------------------------------------------------- */

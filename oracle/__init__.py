@@ -66,6 +66,7 @@ _SIGS = {
     "orc_card_coverage": (None, [_I, _I, _I, _I, _f64p, _I, _i32p, _I, _opt_i32p, _opt_i32p,
                                  _opt_i32p, _I, _I, _f64p]),
     "orc_card_coverage_grad": (None, [_I, _I, _I, _I, _f64p, _I, _i32p, _f64p, _f64p]),
+    "orc_bishop_frame": (None, [_I, _I, _f32p, _f32p, _f32p, _f32p]),
 }
 
 
@@ -331,6 +332,17 @@ def smooth_strands(strands, arclength, beta):
     out = np.zeros_like(x)
     lib().orc_smooth_strands(N, P, x, s, out, float(beta))
     return out
+
+
+def bishop_frame(strands):
+    """(T, N, B) rotation-minimizing frames, each (N, S, 3) float32."""
+    x = _f(strands)
+    N, S, _ = x.shape
+    T = np.zeros_like(x)
+    Nr = np.zeros_like(x)
+    Bn = np.zeros_like(x)
+    lib().orc_bishop_frame(N, S, x, T, Nr, Bn)
+    return T, Nr, Bn
 
 
 def skinning(strands, arclength, handle, canonical, R, beta):

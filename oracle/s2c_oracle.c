@@ -895,3 +895,73 @@ void orc_card_coverage_grad(int B, int H, int W, int V, const double *pos, int T
         }
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Bishop frames (curve_utils.py:76-165, torch ops restated in fp32)        */
+/* ------------------------------------------------------------------------ */
+/* Tangent of vertex i = normalized (x[i+1]-x[i]) + (x[i]-x[i-1]) (one-sided at
+ * the ends, curve_utils.py:120-125); root normal/binormal from the
+ * branchless orthonormal basis of tangent_frame (curve_utils.py:76-102);
+ * then the double-reflection recurrence (curve_utils.py:140-163).  Norms
+ * are clamped at 1e-8 and squared lengths at 1e-8, as there. */
+static void v3_unit(float *v) {
+    float n = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (n < 1e-8f) n = 1e-8f;
+    v[0] /= n;
+    v[1] /= n;
+    v[2] /= n;
+}
+
+void orc_bishop_frame(int N, int S, const float *x, float *T, float *Nr, float *Bn) {
+    for (int n = 0; n < N; ++n) {
+        const float *p = x + (long long)n * S * 3;
+        float *t = T + (long long)n * S * 3, *nr = Nr + (long long)n * S * 3,
+              *bn = Bn + (long long)n * S * 3;
+        for (int i = 0; i < S; ++i) {
+            float v[3] = {0, 0, 0};
+            for (int k = 0; k < 3; ++k) {
+                if (i + 1 < S) v[k] += p[3 * (i + 1) + k] - p[3 * i + k];
+                if (i > 0) v[k] += p[3 * i + k] - p[3 * (i - 1) + k];
+            }
+            v3_unit(v);
+            memcpy(t + 3 * i, v, sizeof v);
+        }
+        if (S == 0) continue;
+        float t0[3] = {t[0], t[1], t[2]};
+        v3_unit(t0);
+        const float sg = t0[2] < 0.f ? -1.f : 1.f;
+        const float a = -1.f / (sg + t0[2]);
+        const float b = t0[0] * t0[1] * a;
+        nr[0] = 1.f + sg * t0[0] * t0[0] * a;
+        nr[1] = sg * b;
+        nr[2] = -sg * t0[0];
+        bn[0] = b;
+        bn[1] = sg + t0[1] * t0[1] * a;
+        bn[2] = -t0[1];
+        for (int i = 1; i < S; ++i) {
+            float v1[3], nL[3], tL[3], v2[3], n1[3], b1[3];
+            for (int k = 0; k < 3; ++k) v1[k] = p[3 * i + k] - p[3 * (i - 1) + k];
+            float c1 = v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2];
+            if (c1 < 1e-8f) c1 = 1e-8f;
+            const float *np_ = nr + 3 * (i - 1), *tp = t + 3 * (i - 1), *tc = t + 3 * i;
+            const float dn = v1[0] * np_[0] + v1[1] * np_[1] + v1[2] * np_[2];
+            const float dt = v1[0] * tp[0] + v1[1] * tp[1] + v1[2] * tp[2];
+            for (int k = 0; k < 3; ++k) {
+                nL[k] = np_[k] - 2.f * (dn / c1) * v1[k];
+                tL[k] = tp[k] - 2.f * (dt / c1) * v1[k];
+                v2[k] = tc[k] - tL[k];
+            }
+            float c2 = v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2];
+            if (c2 < 1e-8f) c2 = 1e-8f;
+            const float d2 = v2[0] * nL[0] + v2[1] * nL[1] + v2[2] * nL[2];
+            for (int k = 0; k < 3; ++k) n1[k] = nL[k] - 2.f * (d2 / c2) * v2[k];
+            v3_unit(n1);
+            b1[0] = tc[1] * n1[2] - tc[2] * n1[1];
+            b1[1] = tc[2] * n1[0] - tc[0] * n1[2];
+            b1[2] = tc[0] * n1[1] - tc[1] * n1[0];
+            v3_unit(b1);
+            memcpy(nr + 3 * i, n1, sizeof n1);
+            memcpy(bn + 3 * i, b1, sizeof b1);
+        }
+    }
+}

@@ -574,6 +574,19 @@ PYBIND11_MODULE(_core, m) {
               fn);
     });
 
+    m.def("bishop_frame", [](Tensor strands, Tensor T, Tensor N, Tensor B) {
+        const char* fn = "bishop_frame";
+        need(strands, fn, "strands", at::kFloat, {-1, -1, 3});
+        need(T, fn, "T", at::kFloat, {strands.size(0), strands.size(1), 3});
+        need(N, fn, "N", at::kFloat, {strands.size(0), strands.size(1), 3});
+        need(B, fn, "B", at::kFloat, {strands.size(0), strands.size(1), 3});
+        same_device(fn, {&strands, &T, &N, &B});
+        DeviceGuard g(strands.device());
+        check(s2c_bishop_frame(I(strands.size(0), fn), I(strands.size(1), fn), P<float>(strands),
+                               P<float>(T), P<float>(N), P<float>(B), stream_of(strands)),
+              fn);
+    });
+
     // Differentiable card coverage (see the C ABI header).
     m.def("card_coverage_forward", [](Tensor pos, Tensor tri, Tensor coverage) {
         const char* fn = "card_coverage_forward";

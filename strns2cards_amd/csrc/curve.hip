@@ -114,9 +114,94 @@ skinning_kernel(int N, int P, const float* __restrict__ x, const float* __restri
     }
 }
 
+// ------------------------------------------------------------ Bishop frame
+//
+// Rotation-minimizing frames by double reflection (curve_utils.py:104-165):
+// tangents are the normalized sums of the adjacent segments, the root frame
+// comes from tangent_frame (curve_utils.py:76-102), and each next normal is
+// the previous one reflected across the bisector planes of the segment and
+// of the tangent change.  The recurrence is sequential along a strand, so
+// one thread owns one strand; lanes read their strands' points in lock-step
+// and each 128-byte line is reused for the next points from L1.
+
+__device__ __forceinline__ float3 f3(const float* p) { return make_float3(p[0], p[1], p[2]); }
+__device__ __forceinline__ float3 sub3(float3 a, float3 b) {
+    return make_float3(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 axpy3(float a, float3 x, float3 y) {  // y - a x
+    return make_float3(y.x - a * x.x, y.y - a * x.y, y.z - a * x.z);
+}
+__device__ __forceinline__ float3 unit3(float3 v) {
+    const float n = fmaxf(sqrtf(dot3(v, v)), 1e-8f);
+    return make_float3(v.x / n, v.y / n, v.z / n);
+}
+__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
+    return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ void put3(float* p, float3 v) {
+    p[0] = v.x;
+    p[1] = v.y;
+    p[2] = v.z;
+}
+
+__global__ void __launch_bounds__(kBlock)
+bishop_kernel(int N, int S, const float* __restrict__ x, float* __restrict__ T,
+              float* __restrict__ Nr, float* __restrict__ Bn) {
+    const int n = blockIdx.x * kBlock + threadIdx.x;
+    if (n >= N) return;
+    const long long o = static_cast<long long>(n) * S * 3;
+    const float* p = x + o;
+    // tangents
+    float3 prev_seg = make_float3(0.f, 0.f, 0.f);
+    for (int i = 0; i < S; ++i) {
+        const float3 seg = i + 1 < S ? sub3(f3(p + 3 * (i + 1)), f3(p + 3 * i))
+                                     : make_float3(0.f, 0.f, 0.f);
+        const float3 t = make_float3(seg.x + prev_seg.x, seg.y + prev_seg.y, seg.z + prev_seg.z);
+        put3(T + o + 3 * i, unit3(t));
+        prev_seg = seg;
+    }
+    if (S == 0) return;
+    // root frame (tangent_frame)
+    const float3 t0 = unit3(f3(T + o));
+    const float sg = t0.z < 0.f ? -1.f : 1.f;
+    const float a = -1.f / (sg + t0.z);
+    const float b = t0.x * t0.y * a;
+    float3 nprev = make_float3(1.f + sg * t0.x * t0.x * a, sg * b, -sg * t0.x);
+    put3(Nr + o, nprev);
+    put3(Bn + o, make_float3(b, sg + t0.y * t0.y * a, -t0.y));
+    float3 tprev = f3(T + o);
+    for (int i = 1; i < S; ++i) {
+        const float3 v1 = sub3(f3(p + 3 * i), f3(p + 3 * (i - 1)));
+        const float c1 = fmaxf(dot3(v1, v1), 1e-8f);
+        const float3 nL = axpy3(2.f * (dot3(v1, nprev) / c1), v1, nprev);
+        const float3 tL = axpy3(2.f * (dot3(v1, tprev) / c1), v1, tprev);
+        const float3 tcur = f3(T + o + 3 * i);
+        const float3 v2 = sub3(tcur, tL);
+        const float c2 = fmaxf(dot3(v2, v2), 1e-8f);
+        const float3 n1 = unit3(axpy3(2.f * (dot3(v2, nL) / c2), v2, nL));
+        put3(Nr + o + 3 * i, n1);
+        put3(Bn + o + 3 * i, unit3(cross3(tcur, n1)));
+        nprev = n1;
+        tprev = tcur;
+    }
+}
+
 }  // namespace s2c
 
 using namespace s2c;
+
+extern "C" int s2c_bishop_frame(int num_strands, int num_points, const float* strands,
+                                float* tangents, float* normals, float* binormals,
+                                void* stream) {
+    S2C_CHECK(num_strands >= 0 && num_points >= 0, "bishop_frame: negative size");
+    if (num_strands == 0 || num_points == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    bishop_kernel<<<grid_for(num_strands, kBlock), kBlock, 0, s>>>(num_strands, num_points, strands,
+                                                                  tangents, normals, binormals);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
 
 extern "C" int s2c_smooth_strands(int num_strands, int num_points, const float* strands,
                                   const float* arclength, float* smoothed_strands, float beta,
