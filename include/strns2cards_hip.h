@@ -221,6 +221,17 @@ int s2c_fit_cards(int num_slots, const int* slots, int iters, const int* res,
 int s2c_bishop_frame(int num_strands, int num_points, const float* strands, float* tangents,
                      float* normals, float* binormals, void* stream);
 
+/* resample_strands_by_arclength (hair_loader.cpp:261-307): num_strands
+ * strands of num_samples[i] >= 2 float3 points starting at point
+ * offsets[i] (exclusive prefix sum of num_samples) of `points`, resampled
+ * to target_num_samples points evenly spaced in arclength into
+ * out (num_strands, target_num_samples, 3).  `lengths` is caller-provided
+ * scratch of sum(num_samples) floats. */
+int s2c_resample_strands_by_arclength(int num_strands, const int* num_samples,
+                                      const long long* offsets, const float* points,
+                                      int target_num_samples, float* lengths, float* out,
+                                      void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* Differentiable card coverage (hand-written adjoint)                     */
 /* ---------------------------------------------------------------------- */

@@ -67,6 +67,7 @@ _SIGS = {
                                  _opt_i32p, _I, _I, _f64p]),
     "orc_card_coverage_grad": (None, [_I, _I, _I, _I, _f64p, _I, _i32p, _f64p, _f64p]),
     "orc_bishop_frame": (None, [_I, _I, _f32p, _f32p, _f32p, _f32p]),
+    "orc_resample_strands_by_arclength": (None, [_I, _i32p, _f32p, _I, _f32p]),
 }
 
 
@@ -438,3 +439,11 @@ def coverage_loss_difference(resolution, pos, tri, target, cov, delta) -> float:
     lp = ((e + card_coverage(resolution, pos + delta, tri, sel, win) - old) ** 2).sum()
     lm = ((e + card_coverage(resolution, pos - delta, tri, sel, win) - old) ** 2).sum()
     return 0.5 * (lp - lm)
+
+
+def resample_strands_by_arclength(num_samples, points, target):
+    """(N, target, 3) float32 arclength resampling (hair_loader.cpp:261-307)."""
+    ns = _i(np.asarray(num_samples).ravel())
+    out = np.zeros((ns.size, int(target), 3), np.float32)
+    lib().orc_resample_strands_by_arclength(ns.size, ns, _f(points), int(target), out)
+    return out

@@ -965,3 +965,54 @@ void orc_bishop_frame(int N, int S, const float *x, float *T, float *Nr, float *
         }
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Arclength resampling (hair_loader.cpp:261-307)                            */
+/* ------------------------------------------------------------------------ */
+/* Per strand: cumulative float lengths divided by the total; sample j < T-1
+ * at t = j/(T-1) interpolates points idx-1, idx around the upper bound of t
+ * (clamped to [1, N-1]); the last sample copies the last point. */
+static int upper_bound_f(const float *l, int n, float t) {
+    int first = 0, len = n;
+    while (len > 0) {
+        int half = len >> 1;
+        if (t < l[first + half]) {
+            len = half;
+        } else {
+            first += half + 1;
+            len -= half + 1;
+        }
+    }
+    return first;
+}
+
+void orc_resample_strands_by_arclength(int num_strands, const int *num_samples, const float *points,
+                                       int T, float *out) {
+    long long off = 0;
+    for (int i = 0; i < num_strands; ++i) {
+        const int N = num_samples[i];
+        const float *p = points + off * 3;
+        float *l = (float *)malloc(sizeof(float) * (size_t)N);
+        l[0] = 0.f;
+        for (int j = 1; j < N; ++j) {
+            const float dx = p[j * 3 + 0] - p[(j - 1) * 3 + 0];
+            const float dy = p[j * 3 + 1] - p[(j - 1) * 3 + 1];
+            const float dz = p[j * 3 + 2] - p[(j - 1) * 3 + 2];
+            l[j] = l[j - 1] + sqrtf(dx * dx + dy * dy + dz * dz);
+        }
+        const float total = l[N - 1];
+        for (int j = 0; j < N; ++j) l[j] /= total;
+        float *o = out + (long long)i * T * 3;
+        for (int j = 0; j < T - 1; ++j) {
+            const float t = (float)j / (float)(T - 1);
+            int idx = upper_bound_f(l, N, t);
+            idx = S2C_MIN(S2C_MAX(idx, 1), N - 1);
+            const float t0 = (l[idx] - t) / (l[idx] - l[idx - 1]);
+            const float t1 = (t - l[idx - 1]) / (l[idx] - l[idx - 1]);
+            for (int k = 0; k < 3; ++k) o[j * 3 + k] = t0 * p[(idx - 1) * 3 + k] + t1 * p[idx * 3 + k];
+        }
+        for (int k = 0; k < 3; ++k) o[(T - 1) * 3 + k] = p[(N - 1) * 3 + k];
+        free(l);
+        off += N;
+    }
+}

@@ -28,6 +28,8 @@ from .bitset_utils import *  # noqa: E402,F401,F403
 from .curve_utils import *  # noqa: E402,F401,F403
 from .dda import *  # noqa: E402,F401,F403
 from .rasterize import *  # noqa: E402,F401,F403
+from .strand_io import *  # noqa: E402,F401,F403
+from . import optimizer  # noqa: E402,F401
 
 
 def print_version() -> None:

@@ -587,6 +587,26 @@ PYBIND11_MODULE(_core, m) {
               fn);
     });
 
+    m.def("resample_strands_by_arclength_device", [](Tensor num_samples, Tensor offsets,
+                                                      Tensor points, int target, Tensor lengths,
+                                                      Tensor out) {
+        const char* fn = "resample_strands_by_arclength";
+        need(num_samples, fn, "num_samples", at::kInt, {-1});
+        if (offsets.scalar_type() != at::kLong || offsets.dim() != 1 ||
+            offsets.size(0) != num_samples.size(0) || !offsets.is_contiguous())
+            throw py::type_error("resample_strands_by_arclength: `offsets` must be int64 (N,)");
+        need(points, fn, "points", at::kFloat, {-1, 3});
+        need(lengths, fn, "lengths", at::kFloat, {points.size(0)});
+        need(out, fn, "out", at::kFloat, {num_samples.size(0), target, 3});
+        same_device(fn, {&num_samples, &offsets, &points, &lengths, &out});
+        DeviceGuard g(points.device());
+        check(s2c_resample_strands_by_arclength(I(num_samples.size(0), fn), P<int>(num_samples),
+                                                P<long long>(offsets), P<float>(points), target,
+                                                P<float>(lengths), P<float>(out),
+                                                stream_of(points)),
+              fn);
+    });
+
     // Differentiable card coverage (see the C ABI header).
     m.def("card_coverage_forward", [](Tensor pos, Tensor tri, Tensor coverage) {
         const char* fn = "card_coverage_forward";

@@ -187,9 +187,94 @@ bishop_kernel(int N, int S, const float* __restrict__ x, float* __restrict__ T,
     }
 }
 
+// ------------------------------------------------------ arclength resampling
+//
+// resample_strands_by_arclength (hair_loader.cpp:261-307): per strand the
+// cumulative segment lengths normalized by the total; output sample j of T
+// sits at t = j / (T - 1) and interpolates the two points around the first
+// normalized length greater than t (an upper bound, clamped to [1, N-1]);
+// the last sample copies the last point.  Pass 1 (one thread per strand)
+// writes the normalized lengths in the reference's sequential order; pass 2
+// (one thread per output sample) searches and interpolates.
+
+__global__ void __launch_bounds__(kBlock)
+resample_lengths_kernel(int num_strands, const int* __restrict__ num_samples,
+                        const long long* __restrict__ offsets, const float* __restrict__ points,
+                        float* __restrict__ lens) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= num_strands) return;
+    const int N = num_samples[i];
+    const long long o = offsets[i];
+    const float* p = points + o * 3;
+    float* l = lens + o;
+    float acc = 0.f;
+    l[0] = 0.f;
+    for (int j = 1; j < N; ++j) {
+        const float dx = p[j * 3 + 0] - p[(j - 1) * 3 + 0];
+        const float dy = p[j * 3 + 1] - p[(j - 1) * 3 + 1];
+        const float dz = p[j * 3 + 2] - p[(j - 1) * 3 + 2];
+        acc = acc + sqrtf(dx * dx + dy * dy + dz * dz);
+        l[j] = acc;
+    }
+    for (int j = 0; j < N; ++j) l[j] = l[j] / acc;
+}
+
+__global__ void __launch_bounds__(kBlock)
+resample_points_kernel(int num_strands, int T, const int* __restrict__ num_samples,
+                       const long long* __restrict__ offsets, const float* __restrict__ points,
+                       const float* __restrict__ lens, float* __restrict__ out) {
+    const long long q = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (q >= static_cast<long long>(num_strands) * T) return;
+    const int i = static_cast<int>(q / T), j = static_cast<int>(q - static_cast<long long>(i) * T);
+    const int N = num_samples[i];
+    const float* p = points + offsets[i] * 3;
+    float* o = out + q * 3;
+    if (j == T - 1) {
+        o[0] = p[(N - 1) * 3 + 0];
+        o[1] = p[(N - 1) * 3 + 1];
+        o[2] = p[(N - 1) * 3 + 2];
+        return;
+    }
+    const float* l = lens + offsets[i];
+    const float t = static_cast<float>(j) / static_cast<float>(T - 1);
+    int first = 0, len = N;  // first index with t < l[idx] (upper bound)
+    while (len > 0) {
+        const int half = len >> 1;
+        if (t < l[first + half]) {
+            len = half;
+        } else {
+            first += half + 1;
+            len -= half + 1;
+        }
+    }
+    const int idx = dmin(dmax(first, 1), N - 1);
+    const float den = l[idx] - l[idx - 1];
+    const float t0 = (l[idx] - t) / den;
+    const float t1 = (t - l[idx - 1]) / den;
+    for (int k = 0; k < 3; ++k) o[k] = t0 * p[(idx - 1) * 3 + k] + t1 * p[idx * 3 + k];
+}
+
 }  // namespace s2c
 
 using namespace s2c;
+
+extern "C" int s2c_resample_strands_by_arclength(int num_strands, const int* num_samples,
+                                                 const long long* offsets, const float* points,
+                                                 int target_num_samples, float* lengths,
+                                                 float* out, void* stream) {
+    S2C_CHECK(num_strands >= 0 && target_num_samples >= 1,
+              "resample_strands_by_arclength: bad size");
+    if (num_strands == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    resample_lengths_kernel<<<grid_for(num_strands, kBlock), kBlock, 0, s>>>(
+        num_strands, num_samples, offsets, points, lengths);
+    S2C_LAUNCH_CHECK();
+    const long long n = static_cast<long long>(num_strands) * target_num_samples;
+    resample_points_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(
+        num_strands, target_num_samples, num_samples, offsets, points, lengths, out);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
 
 extern "C" int s2c_bishop_frame(int num_strands, int num_points, const float* strands,
                                 float* tangents, float* normals, float* binormals,

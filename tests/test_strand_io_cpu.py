@@ -1,0 +1,170 @@
+"""CPU tests of the strand-file readers, topology helpers, OBJ writer and the
+oracle's arclength resampling.
+
+The reference ships no sample files, so each reader is checked on files
+written here byte by byte from the layouts the reference reads
+(hair_loader.cpp:15-256), including its filtering rules.  Resampling is
+pinned by closed forms: evenly spaced points resample to themselves, a
+two-point strand interpolates linearly, endpoints are kept.
+"""
+import struct
+
+import numpy as np
+import pytest
+
+import oracle
+from strns2cards_amd import strand_io as sio
+
+
+def _strands(rng, counts):
+    return [np.cumsum(rng.normal(size=(m, 3)).astype(np.float32) * 0.01, 0) for m in counts]
+
+
+def test_build_edges_from_triangles_brute_force():
+    rng = np.random.default_rng(0)
+    tri = rng.integers(0, 30, size=(60, 3)).astype(np.int32)
+    edges, e2t = sio.build_edges_from_triangles(tri)
+    want = {}
+    for t, (a, b, c) in enumerate(tri):
+        for u, v in ((a, b), (b, c), (c, a)):
+            want.setdefault((min(u, v), max(u, v)), []).append(t)
+    got = {tuple(e): tuple(x) for e, x in zip(edges.tolist(), e2t.tolist())}
+    assert len(got) == len(want) == edges.shape[0]
+    for k, ts in want.items():
+        assert got[k] == (ts[0], ts[1] if len(ts) > 1 else -1)
+    # first-appearance order
+    seen = []
+    for a, b, c in tri:
+        for u, v in ((a, b), (b, c), (c, a)):
+            k = (min(u, v), max(u, v))
+            if k not in seen:
+                seen.append(k)
+    assert [tuple(e) for e in edges.tolist()] == seen
+    assert edges.dtype == np.int32 and e2t.dtype == np.int32
+
+
+def test_build_edges_errors():
+    with pytest.raises(TypeError):
+        sio.build_edges_from_triangles([[0, 1, 2]])
+    with pytest.raises(TypeError):
+        sio.build_edges_from_triangles(np.zeros((1, 3), np.int64))
+    with pytest.raises(ValueError):
+        sio.build_edges_from_triangles(np.zeros((1, 4), np.int32))
+    e, t = sio.build_edges_from_triangles(np.zeros((0, 3), np.int32))
+    assert e.shape == (0, 2) and t.shape == (0, 2)
+
+
+def test_edge_indices_from_wisp():
+    e = sio.build_edge_indices_from_wisp(np.zeros((2, 4, 3)))
+    np.testing.assert_array_equal(e, [[0, 1], [1, 2], [2, 3], [4, 5], [5, 6], [6, 7]])
+
+
+def test_load_bin(tmp_path):
+    rng = np.random.default_rng(1)
+    ss = _strands(rng, [5, 1, 7, 3])
+    ss[2][3] = ss[2][2] + 5e-5  # a segment shorter than 1e-4: dropped
+    p = tmp_path / "x.bin"
+    with open(p, "wb") as f:
+        f.write(struct.pack("<i", len(ss)))
+        for s in ss:
+            f.write(struct.pack("<i", len(s)))
+            rec = np.concatenate([s, np.ones((len(s), 4), np.float32)], 1)
+            f.write(rec.astype(np.float32).tobytes())
+    arr, ns, pts = sio.load_strands(str(p), return_raw=True)
+    # one-point strand passes the loader but not the >= 2 filter
+    assert ns.tolist() == [5, 3]
+    np.testing.assert_array_equal(arr[0], ss[0])
+    np.testing.assert_array_equal(arr[1], ss[3])
+    np.testing.assert_array_equal(pts, np.concatenate([ss[0], ss[3]]))
+
+
+def test_load_usc_data(tmp_path):
+    rng = np.random.default_rng(2)
+    ss = _strands(rng, [4, 1, 6, 0, 2])
+    p = tmp_path / "x.data"
+    with open(p, "wb") as f:
+        f.write(struct.pack("<i", len(ss)))
+        for s in ss:
+            f.write(struct.pack("<i", len(s)))
+            f.write((s if len(s) != 1 else np.zeros((1, 3), np.float32)).tobytes())
+    arr = sio.load_strands(str(p))
+    assert [a.shape[0] for a in arr] == [4, 6, 2]
+    np.testing.assert_array_equal(arr[1], ss[2])
+
+
+def test_load_hair_variants(tmp_path):
+    rng = np.random.default_rng(3)
+    ss = _strands(rng, [3, 5, 4])
+    pts = np.concatenate(ss)
+    # HAIR header with a per-strand segment array (bit 0 set): data after
+    # 16 + 4*124 bytes, as the reference seeks
+    p = tmp_path / "a.hair"
+    with open(p, "wb") as f:
+        f.write(b"HAIR" + struct.pack("<III", len(ss), len(pts), 3))
+        f.write(b"\0" * (4 * 124))
+        f.write(np.array([len(s) - 1 for s in ss], np.uint16).tobytes())
+        f.write(pts.tobytes())
+    arr = sio.load_strands(str(p))
+    assert [a.shape[0] for a in arr] == [3, 5, 4]
+    np.testing.assert_array_equal(arr[2], ss[2])
+    # HAIR header with a default segment count (bit 0 clear)
+    same = _strands(rng, [4, 4])
+    q = tmp_path / "b.hair"
+    with open(q, "wb") as f:
+        f.write(b"HAIR" + struct.pack("<IIII", 2, 8, 2, 3))
+        f.write(b"\0" * (4 * 125))
+        f.write(np.concatenate(same).tobytes())
+    arr = sio.load_strands(str(q))
+    np.testing.assert_array_equal(arr[1], same[1])
+    # ZJU: no header; per-strand point counts
+    z = tmp_path / "c.hair"
+    with open(z, "wb") as f:
+        f.write(struct.pack("<II", len(ss), len(pts)))
+        f.write(np.array([len(s) for s in ss], np.uint16).tobytes())
+        f.write(pts.tobytes())
+    arr = sio.load_strands(str(z))
+    np.testing.assert_array_equal(arr[1], ss[1])
+    with pytest.raises(ValueError):
+        sio.load_strands(str(tmp_path / "x.obj"))
+
+
+def test_npz_and_obj(tmp_path):
+    rng = np.random.default_rng(4)
+    ss = _strands(rng, [3, 4])
+    p = tmp_path / "s.npz"
+    np.savez(p, num_samples=np.array([3, 4], np.int32), points=np.concatenate(ss))
+    got = sio.load_npz_as_strands(str(p))
+    np.testing.assert_array_equal(got[1], ss[1])
+    o = tmp_path / "m.obj"
+    sio.save_obj(str(o), np.eye(3), np.array([[0, 1, 2]]), np.array([[0, 0], [1, 0.25], [0, 1]]))
+    lines = o.read_text().splitlines()
+    assert lines[0] == "v 1.0 0.0 0.0" and lines[4] == "vt 1.0 0.75"
+    assert lines[-1] == "f 1/1 2/2 3/3"
+
+
+def test_resample_single_strand():
+    s = np.array([[0, 0, 0], [1, 0, 0], [1, 2, 0]], np.float32)
+    r = sio.resample_single_strand_by_arclength(s, 4)
+    np.testing.assert_allclose(r, [[0, 0, 0], [1, 0, 0], [1, 1, 0], [1, 2, 0]], atol=1e-6)
+    with pytest.raises(ValueError):
+        sio.resample_single_strand_by_arclength(s[:1], 3)
+
+
+def test_oracle_resample_closed_forms():
+    # evenly spaced line: resampling to the same count returns the points
+    line = (np.arange(11)[:, None] * np.array([[0.5, -0.25, 1.0]])).astype(np.float32)
+    r = oracle.resample_strands_by_arclength([11], line, 11)[0]
+    np.testing.assert_allclose(r, line, atol=1e-5)
+    # two points: linear interpolation
+    two = np.array([[0, 0, 0], [2, 4, -6]], np.float32)
+    r = oracle.resample_strands_by_arclength([2], two, 5)[0]
+    np.testing.assert_allclose(r, np.linspace(0, 1, 5)[:, None] * two[1], atol=1e-6)
+    # several strands: endpoints exact, samples evenly spaced in arclength
+    rng = np.random.default_rng(5)
+    ss = _strands(rng, [7, 12, 3])
+    r = oracle.resample_strands_by_arclength([7, 12, 3], np.concatenate(ss), 9)
+    for s, q in zip(ss, r):
+        np.testing.assert_array_equal(q[0], s[0])
+        np.testing.assert_array_equal(q[-1], s[-1])
+        ref = sio.resample_single_strand_by_arclength(s, 9)
+        np.testing.assert_allclose(q, ref, atol=1e-5)
