@@ -1,6 +1,8 @@
-// Tile offsets and the per-tile LDS resolve of the binned cluster-bitset
-// passes (see binning.hpp).
+// Run ordering and the per-tile LDS resolve of the binned cluster-bitset
+// pass (see binning.hpp).
 #include "binning.hpp"
+
+#include <atomic>
 
 namespace s2c {
 
@@ -9,54 +11,120 @@ namespace s2c {
 // keeps kResolveUnroll loads in flight per thread: the hottest tiles set the
 // kernel's duration.
 constexpr int kResolveThreads = 1024;
-constexpr int kResolveUnroll = 4;
+constexpr int kResolveUnroll = 8;
+
+// Runs into tile order.  One workgroup per stage workgroup copies the
+// records of its fixed region; the workgroups past `blocks` cover the
+// overflow area, kBlock records each.
+__global__ void __launch_bounds__(kBlock)
+bin_order_runs_kernel(long long blocks, const int* __restrict__ block_runs, int ovf_cap,
+                      const int* __restrict__ counters, const BinRecord* __restrict__ recs,
+                      const int* __restrict__ offs, BinRun* __restrict__ runs) {
+    const long long b = blockIdx.x;
+    long long first, count;
+    if (b < blocks) {
+        first = b * kBinRecs;
+        count = block_runs[b];
+    } else {
+        const long long o = (b - blocks) * kBlock;
+        const long long used = dmin(static_cast<long long>(counters[kRunsOvf]),
+                                    static_cast<long long>(ovf_cap));
+        first = blocks * kBinRecs + o;
+        count = dmax(dmin(used - o, static_cast<long long>(kBlock)), 0ll);
+    }
+    for (long long i = threadIdx.x; i < count; i += kBlock) {
+        const BinRecord rec = recs[first + i];
+        runs[offs[rec.slot] + rec.k] = BinRun{rec.src, rec.n};
+    }
+}
 
 template <bool VEC4>
 __global__ void __launch_bounds__(kResolveThreads)
-tile_resolve_kernel(TileGrid g, const int* __restrict__ offs, const unsigned* __restrict__ entries,
-                    int H, int W, int S, int* __restrict__ bitset) {
+tile_resolve_kernel(TileGrid g, const int* __restrict__ offs, const BinRun* __restrict__ runs,
+                    const unsigned* __restrict__ entries, int H, int W, int S,
+                    int* __restrict__ bitset) {
     __shared__ unsigned tile[kTileLdsBytes / 4];
+    __shared__ int pre[kResolveThreads];
+    __shared__ int src[kResolveThreads];
+    __shared__ int sh[kResolveThreads / kWave + 1];
     const int t = blockIdx.x;
     const int side = 1 << g.ts_log2;
     const int words = side * side * S;
     for (int i = threadIdx.x; i < words; i += kResolveThreads) tile[i] = 0u;
-    __syncthreads();
-    const int e0 = offs[t * kBinSlots], e1 = offs[(t + 1) * kBinSlots];
+    __syncthreads();  // a tile without runs goes straight to the write-out
+    const int r0 = offs[t * kBinGroups], r1 = offs[(t + 1) * kBinGroups];
     const unsigned lmask = (1u << g.local_bits) - 1u;
     const unsigned kmax = static_cast<unsigned>(S) * 32u;
-    constexpr int step = kResolveThreads * kResolveUnroll;
-    for (int e = e0 + threadIdx.x; e < e1; e += step) {
-        unsigned v[kResolveUnroll];
-#pragma unroll
-        for (int u = 0; u < kResolveUnroll; ++u) {
-            const int idx = e + u * kResolveThreads;
-            v[u] = idx < e1 ? entries[idx] : 0xFFFFFFFFu;
+    const int w = threadIdx.x / kWave;
+    // Runs are taken kResolveThreads at a time: their lengths are scanned
+    // into `pre`, then the chunk's entries are read as one flat range (each
+    // lane finds its run by binary search), so consecutive lanes read
+    // consecutive entries of a run and kResolveUnroll loads are in flight.
+    for (int rb = r0; rb < r1; rb += kResolveThreads) {
+        const int r = rb + static_cast<int>(threadIdx.x);
+        BinRun run{0, 0};
+        if (r < r1) run = runs[r];
+        const int incl = wave_incl_scan(run.n);
+        if (lane_id() == kWave - 1) sh[w] = incl;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int acc = 0;
+            for (int i = 0; i < kResolveThreads / kWave; ++i) {
+                const int v = sh[i];
+                sh[i] = acc;
+                acc += v;
+            }
+            sh[kResolveThreads / kWave] = acc;
         }
+        __syncthreads();
+        pre[threadIdx.x] = sh[w] + incl - run.n;
+        src[threadIdx.x] = run.src;
+        const int total = sh[kResolveThreads / kWave];
+        __syncthreads();
+        const int nr = dmin(r1 - rb, kResolveThreads);
+        constexpr int step = kResolveThreads * kResolveUnroll;
+        for (int e = threadIdx.x; e < total; e += step) {
+            unsigned v[kResolveUnroll];
 #pragma unroll
-        for (int u = 0; u < kResolveUnroll; ++u) {
-            const unsigned local = v[u] & lmask;
-            const unsigned k = v[u] >> g.local_bits;
-            if (k < kmax) atomicOr(&tile[local * S + (k >> 5)], 1u << (k & 31));
+            for (int u = 0; u < kResolveUnroll; ++u) {
+                const int j = e + u * kResolveThreads;
+                v[u] = 0xFFFFFFFFu;
+                if (j < total) {
+                    int lo = 0, hi = nr;  // last run with pre <= j
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (pre[mid] <= j) lo = mid;
+                        else hi = mid;
+                    }
+                    v[u] = entries[src[lo] + (j - pre[lo])];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kResolveUnroll; ++u) {
+                const unsigned local = v[u] & lmask;
+                const unsigned k = v[u] >> g.local_bits;
+                if (k < kmax) atomicOr(&tile[local * S + (k >> 5)], 1u << (k & 31));
+            }
         }
+        __syncthreads();  // pre / src / sh are rewritten by the next chunk
     }
-    __syncthreads();
     const int per_b = g.tx * g.ty;
     const int b = t / per_b;
-    const int r = t - b * per_b;
-    const int y0 = (r / g.tx) * side;
-    const int x0 = (r % g.tx) * side;
+    const int rr = t - b * per_b;
+    const int y0 = (rr / g.tx) * side;
+    const int x0 = (rr % g.tx) * side;
     const int row_w = dmin(side, W - x0);  // pixels of this tile inside the image
     const int rows = dmin(side, H - y0);
     if (VEC4) {
         const int q = S / 4;
         const int per_row = row_w * q;
-        const uint4* src = reinterpret_cast<const uint4*>(tile);
+        const uint4* srcv = reinterpret_cast<const uint4*>(tile);
         for (int i = threadIdx.x; i < rows * per_row; i += kResolveThreads) {
             const int ly = i / per_row;
             const int rem = i - ly * per_row;
             const long long dst =
                 ((static_cast<long long>(b) * H + y0 + ly) * W + x0) * q + rem;
-            reinterpret_cast<uint4*>(bitset)[dst] = src[ly * side * q + rem];
+            reinterpret_cast<uint4*>(bitset)[dst] = srcv[ly * side * q + rem];
         }
     } else {
         const int per_row = row_w * S;
@@ -70,44 +138,68 @@ tile_resolve_kernel(TileGrid g, const int* __restrict__ offs, const unsigned* __
 }
 
 size_t bin_plan_ints(const TileGrid& g, long long blocks) {
-    const long long n = g.nt * kBinSlots;
-    return static_cast<size_t>((n + 2) + n + scan_scratch_ints(n + 1) +
-                               kImageInts * blocks + blocks + 16);
+    const long long n = g.nt * kBinGroups;
+    return static_cast<size_t>(scan_scratch_ints(n + 1) + 4 + kBinCounters + (n + 2) + blocks +
+                               4);
 }
 
-int bin_prepare(BinPlan& p, int* ints, hipStream_t s) {
+void bin_carve(BinPlan& p, int* ints) {
     const long long n = p.slots();
     p.scratch = ints;  // int64 values first: 8-byte aligned
     long long off = static_cast<long long>(scan_scratch_ints(n + 1));
-    off = (off + 3) & ~3ll;  // counts 16-byte aligned for the vector fill
-    p.counts = ints + off;
-    off += n + 2;
-    p.cursor = ints + off;
-    off += n;
-    p.image = ints + off;
-    off += static_cast<long long>(kImageInts) * p.blocks;
-    p.flags = ints + off;
-    return fill_u32(p.counts, 0u, n + 1, s);
+    off = (off + 3) & ~3ll;
+    p.counters = ints + off;  // counters then run counts: one contiguous clear
+    p.runs_per_slot = ints + off + kBinCounters;
+    p.block_runs = ints + off + kBinCounters + n + 2;
 }
 
-int bin_offsets(BinPlan& p, hipStream_t s, int* total) {
-    const long long n = p.slots();
-    if (int r = inclusive_scan_inplace(p.counts, n + 1, p.scratch, s)) return r;
-    S2C_HIP(hipMemcpyAsync(p.cursor, p.counts, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
-    return read_scan_total(p.scratch, n + 1, s, total, "binning");
+int bin_clear(const BinPlan& p, hipStream_t s) {
+    return fill_u32(p.counters, 0u, kBinCounters + p.slots() + 1, s);
 }
 
-int bin_resolve(const BinPlan& p, const unsigned* entries, int H, int W, int S, int* bitset,
-                hipStream_t s) {
+int bin_read_overflow(const BinPlan& p, hipStream_t s, long long* entries, long long* runs) {
+    int h[kBinCounters];
+    S2C_HIP(hipMemcpyAsync(h, p.counters, sizeof(h), hipMemcpyDeviceToHost, s));
+    S2C_HIP(hipStreamSynchronize(s));
+    // int32 atomics: a wrapped (negative) value means more than 2^31
+    if (h[kEntriesOvf] < 0 || h[kRunsOvf] < 0) {
+        set_error("dda_cluster_mask: overflow area beyond 2^31 entries");
+        return S2C_ERR_OVERFLOW;
+    }
+    *entries = h[kEntriesOvf];
+    *runs = h[kRunsOvf];
+    return S2C_OK;
+}
+
+int bin_resolve(const BinPlan& p, int H, int W, int S, int* bitset, hipStream_t s) {
+    if (int r = inclusive_scan_inplace(p.runs_per_slot, p.slots() + 1, p.scratch, s)) return r;
+    const long long grid = p.blocks + (p.ovf_runs + kBlock - 1) / kBlock;
+    bin_order_runs_kernel<<<static_cast<unsigned>(grid), kBlock, 0, s>>>(
+        p.blocks, p.block_runs, p.ovf_runs, p.counters, p.records, p.runs_per_slot, p.runs);
+    S2C_LAUNCH_CHECK();
     const bool vec = (S % 4 == 0) && ((reinterpret_cast<uintptr_t>(bitset) & 15) == 0);
     if (vec)
         tile_resolve_kernel<true><<<static_cast<unsigned>(p.g.nt), kResolveThreads, 0, s>>>(
-            p.g, p.counts, entries, H, W, S, bitset);
+            p.g, p.runs_per_slot, p.runs, p.entries, H, W, S, bitset);
     else
         tile_resolve_kernel<false><<<static_cast<unsigned>(p.g.nt), kResolveThreads, 0, s>>>(
-            p.g, p.counts, entries, H, W, S, bitset);
+            p.g, p.runs_per_slot, p.runs, p.entries, H, W, S, bitset);
     S2C_LAUNCH_CHECK();
     return S2C_OK;
 }
+
+namespace {
+std::atomic<long long> g_entry_hint{0}, g_run_hint{0};
+
+long long raise_hint(std::atomic<long long>& h, long long want) {
+    long long cur = h.load(std::memory_order_relaxed);
+    while (want > cur && !h.compare_exchange_weak(cur, want, std::memory_order_relaxed)) {
+    }
+    return dmax(cur, want);
+}
+}  // namespace
+
+long long bin_entry_hint(long long want) { return raise_hint(g_entry_hint, want); }
+long long bin_run_hint(long long want) { return raise_hint(g_run_hint, want); }
 
 }  // namespace s2c

@@ -1,21 +1,22 @@
-// Screen-tile binning shared by the fused primitive -> cluster-bitset passes.
+// Screen-tile binning for the fused primitive -> cluster-bitset pass.
 //
-// Fragments are scattered per (view, tile) as packed 32-bit entries
-// (cluster << local_bits | local pixel); one workgroup per tile then ORs its
-// entries into an LDS copy of the tile's bitset and writes the tile once
-// with coalesced stores (no clear of the bitset, no global OR atomics).
+// One walk over the primitives.  A workgroup stages its fragments in LDS
+// as packed 32-bit entries (cluster << local_bits | local pixel) together
+// with the LDS hash slot of their tile, counts them per tile, and writes
+// them into its own fixed region of the entry buffer grouped by tile.  For
+// every (workgroup, tile) pair it writes a run record (tile counter slot,
+// index of the run within that slot, region offset, length) into its own
+// fixed record region; the index comes from one global atomic on the
+// tile's run counter, privatized kBinGroups ways.  A scan of the run
+// counters and a scatter put the runs in tile order, and one workgroup per
+// tile gathers its runs, ORs the entries into an LDS copy of the tile's
+// bitset and writes the tile once with coalesced stores (no clear of the
+// bitset, no global OR atomics).
 //
-// Device-scope atomics are the scarce resource (scattered ones run at a few
-// tens of G/s), so the count and fill passes aggregate in LDS: a workgroup
-// counts its fragments per tile in an LDS hash table, reserves each tile's
-// share with ONE global atomic, and records the table (tile, offset) as an
-// "image"; the fill pass reloads the image and places fragments with LDS
-// atomics only.  A workgroup whose table overflows takes a global-atomic
-// path in both passes (flagged per workgroup, so the passes agree).
-//
-// Per tile the entry segment is laid out as kBinGroups group slots
-// (workgroups privatized by blockIdx % kBinGroups) followed by one overflow
-// slot; the whole segment is contiguous.
+// A workgroup that needs more than its fixed regions takes space from
+// overflow areas; their sizes come from a running estimate, the use is
+// counted even past the capacity, and the host relaunches with exact sizes
+// when it did not fit.
 #pragma once
 
 #include "common.hpp"
@@ -23,13 +24,10 @@
 namespace s2c {
 
 constexpr int kTileLdsBytes = 32768;  // LDS bitset per tile workgroup
-constexpr int kBinGroups = 8;
-constexpr int kBinSlots = kBinGroups + 1;  // per tile: groups + overflow
 constexpr int kHashLog2 = 10;
-constexpr int kHash = 1 << kHashLog2;      // LDS table slots per workgroup
-// Per-workgroup image of the count pass: slot keys, offsets within the
-// tile's group slot, and fragment counts.
-constexpr int kImageInts = 3 * kHash;
+constexpr int kHash = 1 << kHashLog2;  // LDS table slots per workgroup
+constexpr int kStage = 4096;           // fragments staged in LDS per workgroup
+constexpr int kBinGroups = 8;          // run-counter privatization per tile
 
 struct TileGrid {
     int ts_log2;   // tile side = 1 << ts_log2
@@ -55,9 +53,12 @@ inline bool choose_tiles(int B, int H, int W, int S, long long min_tiles, TileGr
     };
     grid(lg);
     while (g.nt < min_tiles && lg > 3) grid(--lg);
+    // packed entries (cluster << local_bits | local pixel) stay below 2^22,
+    // leaving 10 bits for a hash slot in the stage (always true while the
+    // tile bitset fits kTileLdsBytes)
     const long long max_cluster = static_cast<long long>(S) * 32;
-    if (max_cluster > (1ll << (32 - g.local_bits))) return false;
-    return g.nt * kBinSlots < INT_MAX;
+    if (max_cluster > (1ll << (22 - g.local_bits))) return false;
+    return g.nt * kBinGroups + 2 < INT_MAX;
 }
 
 __device__ __forceinline__ int tile_of(const TileGrid& g, int b, int h, int w) {
@@ -68,12 +69,6 @@ __device__ __forceinline__ unsigned local_of(const TileGrid& g, int h, int w) {
     const int m = (1 << g.ts_log2) - 1;
     return static_cast<unsigned>(((h & m) << g.ts_log2) | (w & m));
 }
-
-__device__ __forceinline__ int group_slot(int tile) {
-    return tile * kBinSlots + static_cast<int>(blockIdx.x % kBinGroups);
-}
-
-__device__ __forceinline__ int overflow_slot(int tile) { return tile * kBinSlots + kBinGroups; }
 
 // ------------------------------------------------------------ LDS hash
 
@@ -104,27 +99,70 @@ __device__ __forceinline__ int hash_find(const int* keys, int key) {
     return -1;
 }
 
-// ------------------------------------------------------------ host side
+// One slot per active lane from an LDS or global counter, with one atomic
+// per wave: the active lanes get consecutive indices in lane order.
+__device__ __forceinline__ int wave_append(int* counter) {
+    const unsigned long long mask = __ballot(1);
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    int base = 0;
+    if (lane_id() == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader, kWave);
+    return base + __popcll(mask & lanemask_lt());
+}
+
+// ------------------------------------------------------------ buffers
+
+// Run record of one (workgroup, tile) pair, as the stage pass writes it.
+// `slot` is the tile's counter slot: tile * kBinGroups + group.
+struct BinRecord {
+    int slot, k, src, n;
+};
+// The same run in tile order: entries[src .. src + n).
+struct BinRun {
+    int src, n;
+};
+
+// Run counters are privatized per tile over kBinGroups workgroup groups
+// (blockIdx % kBinGroups), so a hot tile's counter is not one address.
+// Each workgroup owns a fixed region of kStage entries and kBinRecs run
+// records; the rare workgroup that needs more takes space from the
+// overflow areas behind the fixed regions.
+constexpr int kBinRecs = 512;
+
+// Global counters of the overflow areas.
+enum BinCounter { kEntriesOvf = 0, kRunsOvf = 1, kBinCounters = 4 };
 
 struct BinPlan {
     TileGrid g;
-    long long blocks = 0;    // workgroups of the count / fill passes
-    int* counts = nullptr;   // nt*kBinSlots + 1 ints, shifted by one; offsets after the scan
-    int* cursor = nullptr;   // nt*kBinSlots ints (overflow placement)
-    int* scratch = nullptr;  // scan scratch
-    int* image = nullptr;    // blocks * kImageInts ints
-    int* flags = nullptr;    // blocks ints (1 = overflow path)
-    long long slots() const { return g.nt * kBinSlots; }
+    long long blocks = 0;          // workgroups of the stage pass
+    int* runs_per_slot = nullptr;  // nt*kBinGroups + 1 ints, shifted by one; offsets after the scan
+    int* counters = nullptr;       // kBinCounters ints
+    int* block_runs = nullptr;     // runs in each workgroup's fixed record region
+    int* scratch = nullptr;        // scan scratch
+    unsigned* entries = nullptr;   // blocks*kStage fixed + ovf_entries
+    BinRecord* records = nullptr;  // blocks*kBinRecs fixed + ovf_runs
+    BinRun* runs = nullptr;        // same size as records
+    int ovf_entries = 0, ovf_runs = 0;
+    long long slots() const { return g.nt * kBinGroups; }
+    long long fixed_entries() const { return blocks * kStage; }
+    long long fixed_runs() const { return blocks * kBinRecs; }
 };
 
-// Device ints a plan needs.
+// Device ints of the fixed part of a plan (counters, run counters, scan
+// scratch, per-workgroup run counts).
 size_t bin_plan_ints(const TileGrid& g, long long blocks);
-// Carves the plan's buffers out of `ints` and zeroes the counts.
-int bin_prepare(BinPlan& p, int* ints, hipStream_t s);
-// After the count pass: offsets, cursors and the entry total (host sync).
-int bin_offsets(BinPlan& p, hipStream_t s, int* total);
-// One workgroup per tile: OR entries into LDS, write the tile bitset.
-int bin_resolve(const BinPlan& p, const unsigned* entries, int H, int W, int S, int* bitset,
-                hipStream_t s);
+// Carves the fixed part out of `ints`.
+void bin_carve(BinPlan& p, int* ints);
+// Zeroes the run counters and the global counters.
+int bin_clear(const BinPlan& p, hipStream_t s);
+// Reads the overflow-area use of the stage pass (a host sync).
+int bin_read_overflow(const BinPlan& p, hipStream_t s, long long* entries, long long* runs);
+// Run offsets per tile, runs scattered into tile order, then one workgroup
+// per tile ORs its entries into LDS and writes the tile bitset.
+int bin_resolve(const BinPlan& p, int H, int W, int S, int* bitset, hipStream_t s);
+
+// Running estimates of the overflow-area sizes, per process.
+long long bin_entry_hint(long long want);
+long long bin_run_hint(long long want);
 
 }  // namespace s2c

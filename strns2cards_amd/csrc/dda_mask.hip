@@ -15,6 +15,7 @@ namespace s2c {
 constexpr int kPrimsPerThread = 4;
 constexpr int kPrimsPerBlock = kBlock * kPrimsPerThread;
 constexpr int kSerialFrags = 64;
+constexpr int kDeferred = 256;  // long edges per workgroup walked by all threads
 
 struct EdgeView {
     DdaSpan sp;
@@ -42,7 +43,7 @@ __device__ __forceinline__ EdgeView load_edge(long long q, int H, int W, int V,
 }
 
 // emit(b, h, w, cluster) for every surviving fragment of the workgroup's
-// edges.  `deferred` / `ndeferred` are LDS scratch (kPrimsPerBlock ints + 1,
+// edges.  `deferred` / `ndeferred` are LDS scratch (kDeferred ints + 1,
 // ndeferred zeroed and synchronized by the caller).  Contains
 // __syncthreads(): every thread of the workgroup must call it.
 template <class Emit>
@@ -53,22 +54,65 @@ __device__ __forceinline__ void walk_edges(long long P, int H, int W, int V,
                                            int* ndeferred, Emit&& emit) {
     const long long q0 = static_cast<long long>(blockIdx.x) * kPrimsPerBlock +
                          static_cast<long long>(threadIdx.x) * kPrimsPerThread;
-#pragma unroll 1
+    // All loads of the thread's edges are issued before any is used (one
+    // memory round trip for kPrimsPerThread edges instead of one per edge).
+    int b0 = 0, e0 = 0;
+    if (q0 < P) {
+        b0 = static_cast<int>(q0 / E);
+        e0 = static_cast<int>(q0 - static_cast<long long>(b0) * E);
+    }
+    int bv[kPrimsPerThread], ev[kPrimsPerThread];
+    bool ok[kPrimsPerThread];
+    int2 vv[kPrimsPerThread];
+#pragma unroll
     for (int i = 0; i < kPrimsPerThread; ++i) {
-        const long long q = q0 + i;
-        if (q >= P) break;
-        const EdgeView v = load_edge(q, H, W, V, pos, E, edges, edge2cluster);
-        const int n = dmin(v.cnt, kSerialFrags);
+        int e = e0 + i, b = b0;
+        if (e >= E) {  // at most one view boundary inside a thread's edges
+            e -= E;
+            ++b;
+        }
+        ok[i] = q0 + i < P;
+        bv[i] = b;
+        ev[i] = e;
+        vv[i] = ok[i] ? reinterpret_cast<const int2*>(edges)[e] : make_int2(0, 0);
+    }
+    float4 pa[kPrimsPerThread], pb[kPrimsPerThread];
+    int cl[kPrimsPerThread];
+#pragma unroll
+    for (int i = 0; i < kPrimsPerThread; ++i) {
+        const long long vb = static_cast<long long>(bv[i]) * V;
+        pa[i] = ok[i] ? load_pos(pos, vb + vv[i].x) : make_float4(0.f, 0.f, 0.f, 1.f);
+        pb[i] = ok[i] ? load_pos(pos, vb + vv[i].y) : make_float4(0.f, 0.f, 0.f, 1.f);
+        cl[i] = ok[i] ? edge2cluster[ev[i]] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < kPrimsPerThread; ++i) {
+        if (!ok[i]) break;
+        DdaSpan sp;
+        bool wrote;
+        const int cnt = dda_span(H, W, pa[i], pb[i], sp, wrote);
+        const bool depth_ok = dda_depth_always_passes(pa[i], pb[i]);
+        const int n = dmin(cnt, kSerialFrags);
         for (int k = 0; k < n; ++k) {
             int h, w;
-            if (dda_mask_frag(H, W, v.sp, k, v.p0, v.p1, v.depth_ok, h, w)) emit(v.b, h, w, v.cluster);
+            if (dda_mask_frag(H, W, sp, k, pa[i], pb[i], depth_ok, h, w)) emit(bv[i], h, w, cl[i]);
         }
-        if (v.cnt > kSerialFrags)
-            deferred[atomicAdd(ndeferred, 1)] =
-                static_cast<int>(q - static_cast<long long>(blockIdx.x) * kPrimsPerBlock);
+        if (cnt > kSerialFrags) {
+            const int d = atomicAdd(ndeferred, 1);
+            if (d < kDeferred) {
+                deferred[d] =
+                    static_cast<int>(q0 + i - static_cast<long long>(blockIdx.x) * kPrimsPerBlock);
+            } else {  // list full: the thread walks the rest of its edge itself
+                for (int k = kSerialFrags; k < cnt; ++k) {
+                    int h, w;
+                    if (dda_mask_frag(H, W, sp, k, pa[i], pb[i], depth_ok, h, w))
+                        emit(bv[i], h, w, cl[i]);
+                }
+            }
+        }
     }
     __syncthreads();
-    const int nd = *ndeferred;
+    const int nd = dmin(*ndeferred, kDeferred);
     const long long qb = static_cast<long long>(blockIdx.x) * kPrimsPerBlock;
 #pragma unroll 1
     for (int d = 0; d < nd; ++d) {
@@ -86,7 +130,7 @@ __global__ void __launch_bounds__(kBlock)
 dda_mask_direct_kernel(long long P, int H, int W, int V, const float* __restrict__ pos, int E,
                        const int* __restrict__ edges, const int* __restrict__ edge2cluster, int S,
                        int* __restrict__ bitset) {
-    __shared__ int deferred[kPrimsPerBlock];
+    __shared__ int deferred[kDeferred];
     __shared__ int ndeferred;
     if (threadIdx.x == 0) ndeferred = 0;
     __syncthreads();
@@ -97,143 +141,172 @@ dda_mask_direct_kernel(long long P, int H, int W, int V, const float* __restrict
                });
 }
 
-// Binned form, pass 1: per-tile counts in an LDS table, one global atomic
-// per (workgroup, tile); the table is saved as the workgroup's image.
+// Binned form: one walk.  Each thread stages its fragments in its own
+// kStage / kBlock words of LDS as slot << 22 | entry (the tile's LDS hash
+// slot and the packed entry), and the workgroup counts fragments per slot
+// with one LDS atomic per run of same-tile fragments.  After the walk the
+// slots' counts are scanned into offsets within the workgroup's fixed
+// entry region, a run record per (workgroup, tile) pair goes to its fixed
+// record region, and each thread copies its runs out, ranked by one LDS
+// atomic per run.  A workgroup whose table overflows (> kHash distinct
+// tiles) makes one run per fragment in the overflow areas; one whose thread
+// has more fragments than its stage share takes its region from the
+// overflow area and walks again to place them.  Nothing is written past the
+// overflow capacities; their use is counted regardless, so the host can
+// size them and launch again.  (choose_tiles keeps entries below 2^22.)
+constexpr int kPerThread = kStage / kBlock;
+constexpr unsigned kEntryMask = (1u << 22) - 1u;
+
 __global__ void __launch_bounds__(kBlock)
-dda_bin_count_kernel(long long P, int H, int W, int V, const float* __restrict__ pos, int E,
+dda_bin_stage_kernel(long long P, int H, int W, int V, const float* __restrict__ pos, int E,
                      const int* __restrict__ edges, const int* __restrict__ edge2cluster,
-                     TileGrid g, int* __restrict__ counts, int* __restrict__ image,
-                     int* __restrict__ flags) {
-    __shared__ int keys[kHash];
-    __shared__ int vals[kHash];
-    __shared__ int deferred[kPrimsPerBlock];
-    __shared__ int ndeferred, overflow;
+                     TileGrid g, int* __restrict__ runs_per_slot, int* __restrict__ counters,
+                     int* __restrict__ block_runs, unsigned* __restrict__ entries, int ovf_entries,
+                     BinRecord* __restrict__ records, int ovf_runs) {
+    __shared__ int keys[kHash];   // tile of each slot, then per-slot cursors
+    __shared__ int vals[kHash];   // fragments per slot, then slot offsets in the region
+    __shared__ unsigned stage[kStage];
+    __shared__ int deferred[kDeferred];
+    __shared__ int ndeferred, spill, full, nused, base_e, base_r;
+    __shared__ int sh[kBlock / kWave + 1];
+    const long long fixed_e = static_cast<long long>(gridDim.x) * kStage;
+    const long long fixed_r = static_cast<long long>(gridDim.x) * kBinRecs;
+    const int group = static_cast<int>(blockIdx.x % kBinGroups);
     for (int s = threadIdx.x; s < kHash; s += kBlock) {
         keys[s] = -1;
         vals[s] = 0;
     }
-    if (threadIdx.x == 0) ndeferred = overflow = 0;
+    if (threadIdx.x == 0) ndeferred = spill = full = nused = 0;
     __syncthreads();
-    int cur = -1, len = 0;
+    // thread t's i-th staged fragment lives at i * kBlock + t: consecutive
+    // lanes touch consecutive LDS words
+    int cur = -1, slot = -1, len = 0, mine = 0;
     auto flush = [&]() {
-        if (cur < 0 || overflow) return;
-        const int s = hash_insert(keys, cur);
-        if (s < 0) overflow = 1;
-        else atomicAdd(&vals[s], len);
+        if (slot >= 0 && len > 0) atomicAdd(&vals[slot], len);
+        len = 0;
     };
-    walk_edges(P, H, W, V, pos, E, edges, edge2cluster, deferred, &ndeferred,
-               [&](int b, int h, int w, int) {
-                   const int key = tile_of(g, b, h, w);
-                   if (key == cur) {
-                       ++len;
-                   } else {
-                       flush();
-                       cur = key;
-                       len = 1;
-                   }
-               });
-    flush();
-    __syncthreads();
-    int* img = image + static_cast<long long>(blockIdx.x) * kImageInts;
-    if (overflow) {  // workgroup-uniform: recount into the tiles' overflow slots
-        if (threadIdx.x == 0) ndeferred = 0;
-        __syncthreads();
-        walk_edges(P, H, W, V, pos, E, edges, edge2cluster, deferred, &ndeferred,
-                   [&](int b, int h, int w, int) {
-                       atomicAdd(&counts[overflow_slot(tile_of(g, b, h, w)) + 1], 1);
-                   });
-        if (threadIdx.x == 0) flags[blockIdx.x] = 1;
-        return;
-    }
-    for (int s = threadIdx.x; s < kHash; s += kBlock) {
-        const int key = keys[s];
-        int rel = 0;
-        if (key >= 0) rel = atomicAdd(&counts[group_slot(key) + 1], vals[s]);
-        img[s] = key;
-        img[kHash + s] = rel;
-        img[2 * kHash + s] = key >= 0 ? vals[s] : 0;
-    }
-    if (threadIdx.x == 0) flags[blockIdx.x] = 0;
-}
-
-// Binned form, pass 2: place every fragment's packed entry.  The
-// workgroup's entries are staged in LDS grouped by tile (segment sizes come
-// from the count pass's image) and then copied out as contiguous runs, so
-// the global writes are coalesced; a workgroup with more entries than the
-// stage holds writes each entry directly.
-constexpr int kStage = 4096;
-
-__global__ void __launch_bounds__(kBlock)
-dda_bin_fill_kernel(long long P, int H, int W, int V, const float* __restrict__ pos, int E,
-                    const int* __restrict__ edges, const int* __restrict__ edge2cluster,
-                    TileGrid g, const int* __restrict__ offs, int* __restrict__ cursor,
-                    const int* __restrict__ image, const int* __restrict__ flags,
-                    unsigned* __restrict__ entries) {
-    __shared__ int keys[kHash];
-    __shared__ int base[kHash];
-    __shared__ int used[kHash];
-    __shared__ int soff[kHash + 1];
-    __shared__ unsigned stage[kStage];
-    __shared__ int deferred[kPrimsPerBlock];
-    __shared__ int ndeferred;
-    __shared__ int sh[kBlock / kWave + 1];
-    const bool ovf = flags[blockIdx.x] != 0;
-    const int* img = image + static_cast<long long>(blockIdx.x) * kImageInts;
-    constexpr int per_thread = kHash / kBlock;
-    int cnt[per_thread];
-    int mine = 0;
-#pragma unroll
-    for (int j = 0; j < per_thread; ++j) {
-        const int s = threadIdx.x * per_thread + j;
-        const int key = ovf ? -1 : img[s];
-        keys[s] = key;
-        base[s] = key >= 0 ? offs[group_slot(key)] + img[kHash + s] : 0;
-        used[s] = 0;
-        cnt[j] = key >= 0 ? img[2 * kHash + s] : 0;
-        mine += cnt[j];
-    }
-    int total;
-    int run = block_excl_scan(mine, sh, &total);
-#pragma unroll
-    for (int j = 0; j < per_thread; ++j) {
-        soff[threadIdx.x * per_thread + j] = run;
-        run += cnt[j];
-    }
-    if (threadIdx.x == 0) {
-        soff[kHash] = total;
-        ndeferred = 0;
-    }
-    const bool staged = !ovf && total <= kStage;  // workgroup-uniform
-    __syncthreads();
-    int cur = -1, slot = -1;
     walk_edges(P, H, W, V, pos, E, edges, edge2cluster, deferred, &ndeferred,
                [&](int b, int h, int w, int c) {
                    const int key = tile_of(g, b, h, w);
-                   const unsigned v = (static_cast<unsigned>(c) << g.local_bits) | local_of(g, h, w);
-                   if (ovf) {
-                       entries[atomicAdd(&cursor[overflow_slot(key)], 1)] = v;
-                       return;
+                   if (key != cur) {
+                       flush();
+                       cur = key;
+                       slot = full ? -1 : hash_insert(keys, key);
+                       if (slot < 0) full = 1;
                    }
+                   ++len;
+                   if (mine < kPerThread)
+                       stage[mine * kBlock + threadIdx.x] =
+                           (static_cast<unsigned>(slot) << 22) |
+                           (static_cast<unsigned>(c) << g.local_bits) | local_of(g, h, w);
+                   else
+                       spill = 1;
+                   ++mine;
+               });
+    flush();
+    __syncthreads();
+    if (full) {  // workgroup-uniform: one run per fragment, in the overflow areas
+        if (threadIdx.x == 0) {
+            ndeferred = 0;
+            block_runs[blockIdx.x] = 0;
+        }
+        __syncthreads();
+        walk_edges(P, H, W, V, pos, E, edges, edge2cluster, deferred, &ndeferred,
+                   [&](int b, int h, int w, int c) {
+                       const int ks = tile_of(g, b, h, w) * kBinGroups + group;
+                       const int j = atomicAdd(&counters[kEntriesOvf], 1);
+                       const int r = atomicAdd(&counters[kRunsOvf], 1);
+                       const int k = atomicAdd(&runs_per_slot[ks + 1], 1);
+                       if (j < ovf_entries)
+                           entries[fixed_e + j] =
+                               (static_cast<unsigned>(c) << g.local_bits) | local_of(g, h, w);
+                       if (r < ovf_runs)
+                           records[fixed_r + r] =
+                               BinRecord{ks, k, static_cast<int>(fixed_e + j), 1};
+                   });
+        return;
+    }
+    // slot offsets within the region (kHash / kBlock slots per thread)
+    constexpr int per_thread = kHash / kBlock;
+    int cnt[per_thread];
+    int here = 0, used = 0;
+#pragma unroll
+    for (int j = 0; j < per_thread; ++j) {
+        cnt[j] = vals[threadIdx.x * per_thread + j];
+        here += cnt[j];
+        used += keys[threadIdx.x * per_thread + j] >= 0;
+    }
+    int total;
+    int run = block_excl_scan(here, sh, &total);
+#pragma unroll
+    for (int j = 0; j < per_thread; ++j) {
+        vals[threadIdx.x * per_thread + j] = run;  // now the slot's offset
+        run += cnt[j];
+    }
+    if (used) atomicAdd(&nused, used);
+    __syncthreads();
+    const bool staged = spill == 0;  // workgroup-uniform
+    if (threadIdx.x == 0) {
+        base_e = staged ? static_cast<int>(blockIdx.x) * kStage
+                        : static_cast<int>(fixed_e) + atomicAdd(&counters[kEntriesOvf], total);
+        const bool in_fixed = nused <= kBinRecs;
+        base_r = in_fixed ? static_cast<int>(blockIdx.x) * kBinRecs
+                          : static_cast<int>(fixed_r) + atomicAdd(&counters[kRunsOvf], nused);
+        block_runs[blockIdx.x] = in_fixed ? nused : 0;
+        nused = 0;  // reused as the run cursor below
+    }
+    __syncthreads();
+    const int be = base_e, br = base_r;
+    const bool e_fits = staged || static_cast<long long>(be) + total <= fixed_e + ovf_entries;
+    const bool r_fits = br < fixed_r || br - fixed_r + kHash <= ovf_runs;
+    // run records; then the slots' keys become per-slot cursors
+#pragma unroll
+    for (int j = 0; j < per_thread; ++j) {
+        const int s = threadIdx.x * per_thread + j;
+        const int key = keys[s];
+        if (key < 0) continue;
+        const int ks = key * kBinGroups + group;
+        const int k = atomicAdd(&runs_per_slot[ks + 1], 1);
+        const int r = br + atomicAdd(&nused, 1);
+        if (r_fits || r < fixed_r + ovf_runs) records[r] = BinRecord{ks, k, be + vals[s], cnt[j]};
+    }
+    if (!e_fits) return;
+    if (staged) {
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < per_thread; ++j) keys[threadIdx.x * per_thread + j] = 0;
+        __syncthreads();
+        int i = 0;
+        while (i < mine) {
+            const unsigned s0 = stage[i * kBlock + threadIdx.x] >> 22;
+            int j = i + 1;
+            while (j < mine && (stage[j * kBlock + threadIdx.x] >> 22) == s0) ++j;
+            const int at = be + vals[s0] + atomicAdd(&keys[s0], j - i);
+            for (int k = i; k < j; ++k) entries[at + k - i] = stage[k * kBlock + threadIdx.x] & kEntryMask;
+            i = j;
+        }
+        return;
+    }
+    // more fragments than the stage holds: walk again and place directly;
+    // the stage words now serve as per-slot cursors
+    __syncthreads();
+    for (int s = threadIdx.x; s < kHash; s += kBlock) stage[s] = 0u;
+    if (threadIdx.x == 0) ndeferred = 0;
+    __syncthreads();
+    cur = -1;
+    slot = -1;
+    walk_edges(P, H, W, V, pos, E, edges, edge2cluster, deferred, &ndeferred,
+               [&](int b, int h, int w, int c) {
+                   const int key = tile_of(g, b, h, w);
                    if (key != cur) {
                        cur = key;
                        slot = hash_find(keys, key);
                    }
                    if (slot < 0) return;
-                   const int k = atomicAdd(&used[slot], 1);
-                   if (staged) stage[soff[slot] + k] = v;
-                   else entries[base[slot] + k] = v;
+                   const int p = static_cast<int>(atomicAdd(&stage[slot], 1u));
+                   entries[be + vals[slot] + p] =
+                       (static_cast<unsigned>(c) << g.local_bits) | local_of(g, h, w);
                });
-    if (!staged) return;
-    __syncthreads();
-    for (int p = threadIdx.x; p < total; p += kBlock) {
-        int lo = 0, hi = kHash;  // last slot with soff[slot] <= p
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (soff[mid] <= p) lo = mid;
-            else hi = mid;
-        }
-        entries[base[lo] + (p - soff[lo])] = stage[p];
-    }
 }
 
 }  // namespace s2c
@@ -260,20 +333,39 @@ extern "C" int s2c_dda_cluster_mask(int B, int H, int W, int V, const float* pos
         S2C_LAUNCH_CHECK();
         return S2C_OK;
     }
+    S2C_CHECK(plan.blocks * kStage < INT_MAX / 2 && plan.blocks * kBinRecs < INT_MAX / 2,
+              "dda_cluster_mask: too many primitives for 32-bit entry offsets");
     Temp work;
     if (int r = work.alloc(sizeof(int) * bin_plan_ints(plan.g, plan.blocks), s)) return r;
-    if (int r = bin_prepare(plan, work.as<int>(), s)) return r;
-    const unsigned grid = static_cast<unsigned>(plan.blocks);
-    dda_bin_count_kernel<<<grid, kBlock, 0, s>>>(P, H, W, V, pos, E, edges, edge2cluster, plan.g,
-                                                 plan.counts, plan.image, plan.flags);
-    S2C_LAUNCH_CHECK();
-    int total = 0;
-    if (int r = bin_offsets(plan, s, &total)) return r;
-    Temp entries;
-    if (int r = entries.alloc(sizeof(unsigned) * static_cast<size_t>(total), s)) return r;
-    dda_bin_fill_kernel<<<grid, kBlock, 0, s>>>(P, H, W, V, pos, E, edges, edge2cluster, plan.g,
-                                                plan.counts, plan.cursor, plan.image, plan.flags,
-                                                entries.as<unsigned>());
-    S2C_LAUNCH_CHECK();
-    return bin_resolve(plan, entries.as<unsigned>(), H, W, num_slots, bitset, s);
+    bin_carve(plan, work.as<int>());
+    long long want_e = bin_entry_hint(4 * kStage), want_r = bin_run_hint(4 * kHash);
+    Temp ebuf, rbuf, obuf;
+    for (int attempt = 0;; ++attempt) {
+        want_e = dmin(want_e, static_cast<long long>(INT_MAX / 2));
+        want_r = dmin(want_r, static_cast<long long>(INT_MAX / 2));
+        plan.ovf_entries = static_cast<int>(want_e);
+        plan.ovf_runs = static_cast<int>(want_r);
+        if (int r = ebuf.alloc(sizeof(unsigned) * static_cast<size_t>(plan.fixed_entries() + want_e), s))
+            return r;
+        if (int r = rbuf.alloc(sizeof(BinRecord) * static_cast<size_t>(plan.fixed_runs() + want_r), s))
+            return r;
+        plan.entries = ebuf.as<unsigned>();
+        plan.records = rbuf.as<BinRecord>();
+        if (int r = bin_clear(plan, s)) return r;
+        dda_bin_stage_kernel<<<static_cast<unsigned>(plan.blocks), kBlock, 0, s>>>(
+            P, H, W, V, pos, E, edges, edge2cluster, plan.g, plan.runs_per_slot, plan.counters,
+            plan.block_runs, plan.entries, plan.ovf_entries, plan.records, plan.ovf_runs);
+        S2C_LAUNCH_CHECK();
+        long long ne = 0, nr = 0;
+        if (int r = bin_read_overflow(plan, s, &ne, &nr)) return r;
+        // a workgroup writing records into the overflow area reserves up to kHash
+        if (ne <= want_e && nr + kHash <= want_r) break;
+        S2C_CHECK(attempt == 0, "dda_cluster_mask: buffer sizing did not converge");
+        want_e = bin_entry_hint(ne + ne / 8 + 4 * kStage);
+        want_r = bin_run_hint(nr + nr / 8 + 4 * kHash);
+    }
+    if (int r = obuf.alloc(sizeof(BinRun) * static_cast<size_t>(plan.fixed_runs() + plan.ovf_runs), s))
+        return r;
+    plan.runs = obuf.as<BinRun>();
+    return bin_resolve(plan, H, W, num_slots, bitset, s);
 }

@@ -18,10 +18,20 @@ struct DdaSpan {
 // zero-length edge).  `p0`/`p1` receive the two clip-space vertices.
 __device__ __forceinline__ int dda_span(int H, int W, float4 p0, float4 p1, DdaSpan& sp,
                                         bool& wrote) {
-    const float x0n = p0.x / p0.w;
-    const float y0n = p0.y / p0.w;
-    const float x1n = p1.x / p1.w;
-    const float y1n = p1.y / p1.w;
+    // x / 1 == x exactly in IEEE arithmetic, so orthographic vertices
+    // (w == 1) skip the four correctly rounded divisions bit-exactly
+    float x0n, y0n, x1n, y1n;
+    if (p0.w == 1.f && p1.w == 1.f) {
+        x0n = p0.x;
+        y0n = p0.y;
+        x1n = p1.x;
+        y1n = p1.y;
+    } else {
+        x0n = p0.x / p0.w;
+        y0n = p0.y / p0.w;
+        x1n = p1.x / p1.w;
+        y1n = p1.y / p1.w;
+    }
     const float fH = static_cast<float>(H);
     const float fW = static_cast<float>(W);
     float x0 = 0.5f * (x0n + 1.f) * fW;
@@ -126,12 +136,15 @@ __device__ __forceinline__ float4 load_pos(const float* pos, long long row) {
 // below 35u ~= 2.1e-6 when w_max <= 16 w_min.  So with both NDC depths in
 // [-1 + 1e-5, 1 - 1e-5] and w in [1e-30, 1e30] (no under/overflow in the
 // products), zw stays finite and inside [-1, 1].
+// The NDC depths are compared without dividing: |z| <= lim2 * w with
+// lim2 = 1 - 2e-5 implies the rounded quotient |fl(z / w)| <= 1 - 1e-5
+// (relative error of the product and quotient below 2^-23), so this test
+// only ever answers "passes" where the divided test above would.
 __device__ __forceinline__ bool dda_depth_always_passes(float4 p0, float4 p1) {
     const float lo = fminf(p0.w, p1.w), hi = fmaxf(p0.w, p1.w);
     if (!(lo >= 1e-30f && hi <= 1e30f && hi <= 16.f * lo)) return false;
-    const float z0 = p0.z / p0.w, z1 = p1.z / p1.w;
-    constexpr float lim = 1.f - 1e-5f;
-    return fabsf(z0) <= lim && fabsf(z1) <= lim;  // false for NaN
+    constexpr float lim2 = 1.f - 2e-5f;
+    return fabsf(p0.z) <= lim2 * p0.w && fabsf(p1.z) <= lim2 * p1.w;  // false for NaN
 }
 
 // dda.cu:185-197: pixel of candidate k, without t1 / depth.  Returns false

@@ -41,6 +41,10 @@ static void configure_pool_once() {
 
 int Temp::alloc(size_t bytes, hipStream_t s) {
     configure_pool_once();
+    if (ptr) {  // re-sizing: release the previous block on its stream
+        (void)hipFreeAsync(ptr, stream);
+        ptr = nullptr;
+    }
     stream = s;
     if (bytes < 256) bytes = 256;
     S2C_HIP(hipMallocAsync(&ptr, bytes, s));
