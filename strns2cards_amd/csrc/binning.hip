@@ -13,26 +13,113 @@ namespace s2c {
 constexpr int kResolveThreads = 1024;
 constexpr int kResolveUnroll = 8;
 
-// Runs into tile order.  One workgroup per stage workgroup copies the
-// records of its fixed region; the workgroups past `blocks` cover the
-// overflow area, kBlock records each.
-__global__ void __launch_bounds__(kBlock)
-bin_order_runs_kernel(long long blocks, const int* __restrict__ block_runs, int ovf_cap,
-                      const int* __restrict__ counters, const BinRecord* __restrict__ recs,
-                      const int* __restrict__ offs, BinRun* __restrict__ runs) {
-    const long long b = blockIdx.x;
+// Resolve order: tiles by decreasing run count in classes of powers of 4,
+// so the heaviest tiles start first and the light and empty ones fill the
+// tail.  One workgroup; each thread classifies kOrderPer consecutive tiles
+// from one round of loads, and counts / places them with wave scans and one
+// LDS atomic per wave and class.  The order within a class is arbitrary.
+constexpr int kOrderThreads = 1024;
+constexpr int kOrderClasses = 8;
+constexpr int kOrderPer = 16;
+
+__device__ __forceinline__ void order_classes(long long t0, long long nt, const int* offs,
+                                              int (&cls)[kOrderPer]) {
+    int v[kOrderPer + 1];
+#pragma unroll
+    for (int i = 0; i <= kOrderPer; ++i) v[i] = offs[dmin(t0 + i, nt) * kBinGroups];
+#pragma unroll
+    for (int i = 0; i < kOrderPer; ++i) {
+        const int w = v[i + 1] - v[i];
+        cls[i] = t0 + i >= nt ? -1
+                 : w <= 0     ? kOrderClasses - 1
+                              : dmax(0, kOrderClasses - 2 - (31 - __clz(w)) / 2);
+    }
+}
+
+__device__ __forceinline__ void tile_order(long long nt, const int* __restrict__ offs,
+                                           int* __restrict__ order) {
+    __shared__ int hist[kOrderClasses];
+    if (threadIdx.x < kOrderClasses) hist[threadIdx.x] = 0;
+    __syncthreads();
+    const int lane = lane_id();
+    constexpr long long step = static_cast<long long>(kOrderThreads) * kOrderPer;
+    for (long long c0 = 0; c0 < nt; c0 += step) {
+        int cls[kOrderPer];
+        order_classes(c0 + threadIdx.x * kOrderPer, nt, offs, cls);
+#pragma unroll
+        for (int k = 0; k < kOrderClasses; ++k) {
+            int n = 0;
+#pragma unroll
+            for (int i = 0; i < kOrderPer; ++i) n += cls[i] == k;
+            const int incl = wave_incl_scan(n);
+            if (lane == kWave - 1 && incl > 0) atomicAdd(&hist[k], incl);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < kOrderClasses; ++k) {
+            const int v = hist[k];
+            hist[k] = acc;
+            acc += v;
+        }
+    }
+    __syncthreads();
+    for (long long c0 = 0; c0 < nt; c0 += step) {
+        const long long t0 = c0 + threadIdx.x * kOrderPer;
+        int cls[kOrderPer];
+        order_classes(t0, nt, offs, cls);
+#pragma unroll
+        for (int k = 0; k < kOrderClasses; ++k) {
+            int n = 0;
+#pragma unroll
+            for (int i = 0; i < kOrderPer; ++i) n += cls[i] == k;
+            const int incl = wave_incl_scan(n);
+            int b = 0;
+            if (lane == kWave - 1 && incl > 0) b = atomicAdd(&hist[k], incl);
+            b = __shfl(b, kWave - 1, kWave) + incl - n;
+#pragma unroll
+            for (int i = 0; i < kOrderPer; ++i)
+                if (cls[i] == k) order[b++] = static_cast<int>(t0 + i);
+        }
+    }
+}
+
+// Runs into tile order, and the resolve order, in one launch of
+// kOrderThreads-thread workgroups: workgroup 0 orders the tiles while the
+// others scatter the run records, each covering kOrderThreads / kBlock
+// stage workgroups' fixed record regions (one kBlock-thread group each) or
+// kOrderThreads records of the overflow area.
+constexpr int kStagePerOrder = kOrderThreads / kBlock;
+
+__global__ void __launch_bounds__(kOrderThreads)
+bin_order_kernel(long long nt, long long blocks, const int* __restrict__ block_runs, int ovf_cap,
+                 const int* __restrict__ counters, const BinRecord* __restrict__ recs,
+                 const int* __restrict__ offs, BinRun* __restrict__ runs,
+                 int* __restrict__ order) {
+    if (blockIdx.x == 0) {
+        tile_order(nt, offs, order);
+        return;
+    }
+    const long long fixed_groups = (blocks + kStagePerOrder - 1) / kStagePerOrder;
+    const long long w = blockIdx.x - 1;
     long long first, count;
-    if (b < blocks) {
+    int tid = threadIdx.x, stride = kOrderThreads;
+    if (w < fixed_groups) {
+        const long long b = w * kStagePerOrder + threadIdx.x / kBlock;
+        if (b >= blocks) return;
         first = b * kBinRecs;
         count = block_runs[b];
+        tid = threadIdx.x % kBlock;
+        stride = kBlock;
     } else {
-        const long long o = (b - blocks) * kBlock;
+        const long long o = (w - fixed_groups) * kOrderThreads;
         const long long used = dmin(static_cast<long long>(counters[kRunsOvf]),
                                     static_cast<long long>(ovf_cap));
         first = blocks * kBinRecs + o;
-        count = dmax(dmin(used - o, static_cast<long long>(kBlock)), 0ll);
+        count = dmax(dmin(used - o, static_cast<long long>(kOrderThreads)), 0ll);
     }
-    for (long long i = threadIdx.x; i < count; i += kBlock) {
+    for (long long i = tid; i < count; i += stride) {
         const BinRecord rec = recs[first + i];
         runs[offs[rec.slot] + rec.k] = BinRun{rec.src, rec.n};
     }
@@ -40,14 +127,15 @@ bin_order_runs_kernel(long long blocks, const int* __restrict__ block_runs, int 
 
 template <bool VEC4>
 __global__ void __launch_bounds__(kResolveThreads)
-tile_resolve_kernel(TileGrid g, const int* __restrict__ offs, const BinRun* __restrict__ runs,
+tile_resolve_kernel(TileGrid g, const int* __restrict__ order, const int* __restrict__ offs,
+                    const BinRun* __restrict__ runs,
                     const unsigned* __restrict__ entries, int H, int W, int S,
                     int* __restrict__ bitset) {
     __shared__ unsigned tile[kTileLdsBytes / 4];
     __shared__ int pre[kResolveThreads];
     __shared__ int src[kResolveThreads];
     __shared__ int sh[kResolveThreads / kWave + 1];
-    const int t = blockIdx.x;
+    const int t = order[blockIdx.x];
     const int side = 1 << g.ts_log2;
     const int words = side * side * S;
     for (int i = threadIdx.x; i < words; i += kResolveThreads) tile[i] = 0u;
@@ -140,7 +228,7 @@ tile_resolve_kernel(TileGrid g, const int* __restrict__ offs, const BinRun* __re
 size_t bin_plan_ints(const TileGrid& g, long long blocks) {
     const long long n = g.nt * kBinGroups;
     return static_cast<size_t>(scan_scratch_ints(n + 1) + 4 + kBinCounters + (n + 2) + blocks +
-                               4);
+                               g.nt + 4);
 }
 
 void bin_carve(BinPlan& p, int* ints) {
@@ -151,6 +239,7 @@ void bin_carve(BinPlan& p, int* ints) {
     p.counters = ints + off;  // counters then run counts: one contiguous clear
     p.runs_per_slot = ints + off + kBinCounters;
     p.block_runs = ints + off + kBinCounters + n + 2;
+    p.order = p.block_runs + p.blocks;
 }
 
 int bin_clear(const BinPlan& p, hipStream_t s) {
@@ -173,17 +262,19 @@ int bin_read_overflow(const BinPlan& p, hipStream_t s, long long* entries, long 
 
 int bin_resolve(const BinPlan& p, int H, int W, int S, int* bitset, hipStream_t s) {
     if (int r = inclusive_scan_inplace(p.runs_per_slot, p.slots() + 1, p.scratch, s)) return r;
-    const long long grid = p.blocks + (p.ovf_runs + kBlock - 1) / kBlock;
-    bin_order_runs_kernel<<<static_cast<unsigned>(grid), kBlock, 0, s>>>(
-        p.blocks, p.block_runs, p.ovf_runs, p.counters, p.records, p.runs_per_slot, p.runs);
+    const long long grid = 1 + (p.blocks + kStagePerOrder - 1) / kStagePerOrder +
+                           (p.ovf_runs + kOrderThreads - 1) / kOrderThreads;
+    bin_order_kernel<<<static_cast<unsigned>(grid), kOrderThreads, 0, s>>>(
+        p.g.nt, p.blocks, p.block_runs, p.ovf_runs, p.counters, p.records, p.runs_per_slot, p.runs,
+        p.order);
     S2C_LAUNCH_CHECK();
     const bool vec = (S % 4 == 0) && ((reinterpret_cast<uintptr_t>(bitset) & 15) == 0);
     if (vec)
         tile_resolve_kernel<true><<<static_cast<unsigned>(p.g.nt), kResolveThreads, 0, s>>>(
-            p.g, p.runs_per_slot, p.runs, p.entries, H, W, S, bitset);
+            p.g, p.order, p.runs_per_slot, p.runs, p.entries, H, W, S, bitset);
     else
         tile_resolve_kernel<false><<<static_cast<unsigned>(p.g.nt), kResolveThreads, 0, s>>>(
-            p.g, p.runs_per_slot, p.runs, p.entries, H, W, S, bitset);
+            p.g, p.order, p.runs_per_slot, p.runs, p.entries, H, W, S, bitset);
     S2C_LAUNCH_CHECK();
     return S2C_OK;
 }
