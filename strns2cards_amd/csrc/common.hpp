@@ -77,11 +77,13 @@ inline unsigned grid_for(long long n, int per_block) {
 // Saturating fp32 -> int32 truncation with NaN -> 0 (the semantics of the
 // hardware conversion on both the reference's GPU and gfx950, written out so
 // the compiler cannot exploit the undefined C++ overflow case).
+// v_cvt_i32_f32 truncates toward zero, clamps to [INT_MIN, INT_MAX] and
+// maps NaN to 0: exactly this conversion, in one instruction (a C++ cast of
+// an out-of-range float would be undefined, hence the asm).
 __device__ __forceinline__ int f2i(float x) {
-    if (x != x) return 0;
-    if (x >= 2147483648.0f) return INT_MAX;
-    if (x <= -2147483648.0f) return INT_MIN;
-    return static_cast<int>(x);
+    int r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
 }
 
 // The reference's MIN / MAX macros (cuda_common.cuh:16-22), NaN behaviour
