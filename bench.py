@@ -7,8 +7,8 @@ views at 1024 x 1024, 1,024-bit cluster bitset per pixel (32 words), plus
 1,024 cards x 20 quads for the triangle / antialias stages.
 
 Timed step: the strand DDA -> cluster-bitset pass over all 4 views
-(``_core.dda_cluster_mask``: clear + fused DDA/OR kernel), inputs resident in
-HBM.  ``value`` is DDA GB/s with algorithmic bytes
+(``_core.dda_cluster_mask``: one staged walk over the edges, run ordering,
+per-tile resolve), inputs resident in HBM.  ``value`` is DDA GB/s with algorithmic bytes
 B*V*16 + E*8 + E*4 + B*H*W*S*4 (vertex reads per view, edges and
 edge->cluster once, bitset written once) per step.  The card raster -> mask
 pass and the antialias edge gradient at the same size are timed in the same
@@ -341,8 +341,8 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": traffic_from_profiles(),
                      "kernel": "s2c_dda_cluster_mask entry point (fill_u32_kernel, "
-                               "dda_bin_count_kernel, scan_*, copyBuffer, dda_bin_fill_kernel, "
-                               "tile_resolve_kernel)",
+                               "dda_bin_stage_kernel, counter readback, scan_*, "
+                               "bin_order_kernel, tile_resolve_kernel)",
                      "avg_launch_ms": round(ms_launch, 4)},
         "cpu_baseline": cpu,
         "extra": {"card_raster_mask_ms": round(ms_tri, 4), "antialias_grad_ms": round(ms_aa, 4),

@@ -143,6 +143,10 @@ tile_resolve_kernel(TileGrid g, const int* __restrict__ order, const int* __rest
     const int r0 = offs[t * kBinGroups], r1 = offs[(t + 1) * kBinGroups];
     const unsigned lmask = (1u << g.local_bits) - 1u;
     const unsigned kmax = static_cast<unsigned>(S) * 32u;
+    // LDS word of (pixel l, word j) is l*S + (j ^ (l & smask)): with S a
+    // power of two, consecutive entries of one cluster (same j) at
+    // different pixels fall in different banks
+    const unsigned smask = (S & (S - 1)) == 0 ? static_cast<unsigned>(S) - 1u : 0u;
     const int w = threadIdx.x / kWave;
     // Runs are taken kResolveThreads at a time: their lengths are scanned
     // into `pre`, then the chunk's entries are read as one flat range (each
@@ -191,7 +195,8 @@ tile_resolve_kernel(TileGrid g, const int* __restrict__ order, const int* __rest
             for (int u = 0; u < kResolveUnroll; ++u) {
                 const unsigned local = v[u] & lmask;
                 const unsigned k = v[u] >> g.local_bits;
-                if (k < kmax) atomicOr(&tile[local * S + (k >> 5)], 1u << (k & 31));
+                if (k < kmax)
+                    atomicOr(&tile[local * S + ((k >> 5) ^ (local & smask))], 1u << (k & 31));
             }
         }
         __syncthreads();  // pre / src / sh are rewritten by the next chunk
@@ -206,21 +211,29 @@ tile_resolve_kernel(TileGrid g, const int* __restrict__ order, const int* __rest
     if (VEC4) {
         const int q = S / 4;
         const int per_row = row_w * q;
-        const uint4* srcv = reinterpret_cast<const uint4*>(tile);
         for (int i = threadIdx.x; i < rows * per_row; i += kResolveThreads) {
             const int ly = i / per_row;
             const int rem = i - ly * per_row;
+            const int px = rem / q;  // pixel within the row
+            const unsigned l = static_cast<unsigned>(ly * side + px);
+            const int j0 = (rem - px * q) * 4;
+            const unsigned* lw = tile + l * S;
+            const uint4 v = make_uint4(lw[(j0 + 0) ^ (l & smask)], lw[(j0 + 1) ^ (l & smask)],
+                                       lw[(j0 + 2) ^ (l & smask)], lw[(j0 + 3) ^ (l & smask)]);
             const long long dst =
                 ((static_cast<long long>(b) * H + y0 + ly) * W + x0) * q + rem;
-            reinterpret_cast<uint4*>(bitset)[dst] = srcv[ly * side * q + rem];
+            reinterpret_cast<uint4*>(bitset)[dst] = v;
         }
     } else {
         const int per_row = row_w * S;
         for (int i = threadIdx.x; i < rows * per_row; i += kResolveThreads) {
             const int ly = i / per_row;
             const int rem = i - ly * per_row;
+            const int px = rem / S;
+            const unsigned l = static_cast<unsigned>(ly * side + px);
+            const int j = rem - px * S;
             const long long dst = ((static_cast<long long>(b) * H + y0 + ly) * W + x0) * S + rem;
-            bitset[dst] = static_cast<int>(tile[ly * side * S + rem]);
+            bitset[dst] = static_cast<int>(tile[l * S + (j ^ (l & smask))]);
         }
     }
 }
