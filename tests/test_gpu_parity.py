@@ -164,6 +164,37 @@ def test_dda_mask_long_scattered_edges(dev):
     np.testing.assert_array_equal(got, want)
 
 
+def _segments(rng, n, lo, hi, region, H, W):
+    """n segments of length in [lo, hi] px with random starts and directions
+    inside the top-left region x region pixels of an H x W image (one view,
+    w = 1)."""
+    p0 = rng.uniform(0, region, size=(n, 2))
+    ang = rng.uniform(0, 2 * np.pi, n)
+    p1 = np.clip(p0 + np.stack([np.cos(ang), np.sin(ang)], 1) * rng.uniform(lo, hi, n)[:, None],
+                 0, region - 0.01)
+    pts = np.stack([p0, p1], 1).reshape(-1, 2)
+    pos = np.zeros((1, 2 * n, 4), np.float32)
+    pos[0, :, 0] = 2 * pts[:, 0] / W - 1
+    pos[0, :, 1] = 1 - 2 * pts[:, 1] / H
+    pos[0, :, 3] = 1
+    return pos, np.arange(2 * n, dtype=np.int32).reshape(n, 2)
+
+
+@pytest.mark.parametrize("case", ["records_overflow", "stage_spill"])
+def test_dda_mask_workgroup_paths(dev, case):
+    """Workgroups (1,024 edges each) touching 513-1,024 distinct tiles (their
+    run records go to the overflow area) and workgroups whose threads have
+    more fragments than their LDS stage share (placed by a second walk)."""
+    report(f"dda->mask workgroup path: {case}")
+    rng = np.random.default_rng(5)
+    lo, hi, region = (1.5, 3.0, 448) if case == "records_overflow" else (10.0, 14.0, 320)
+    pos, edges = _segments(rng, 4096, lo, hi, region, 1024, 1024)
+    e2c = rng.integers(0, 64, size=edges.shape[0]).astype(np.int32)
+    want = oracle.dda_cluster_mask((1024, 1024), pos, edges, e2c, 64, threads=8)
+    got = fused_dda_mask(dev, (1024, 1024), pos, edges, e2c, 64)
+    np.testing.assert_array_equal(got, want)
+
+
 def test_dda_mask_depth_boundaries(dev):
     """Edges whose endpoint depths sit at / around the fast-path threshold
     (|z_ndc| = 1 - 1e-5) and at the clip limit, with w = 1 and with w
