@@ -84,9 +84,15 @@ def fit_section(dev, num_jobs=4096, plain_jobs=6, rates=(200.0, 1000.0, 5000.0))
           file=sys.stderr, flush=True)
     hyper = fitting.FitHyper()
     out = {}
+    from strns2cards_amd import autotune
+
+    tun = autotune.tune_fit(jobs[:1024], hyper, dev, iters=10)
+    out["autotune"] = {"chunk": tun.chunk, "order": tun.order,
+                       "ms_per_10_iters_1024_jobs": {k: round(v, 3) for k, v in tun.ms.items()}}
+    print(f"[bench] fit autotune: {out['autotune']}", file=sys.stderr, flush=True)
 
     # one persistent launch: every job runs all its iterations
-    srv = fitting.CardFitServer(num_jobs, hyper, dev)
+    srv = fitting.CardFitServer(num_jobs, hyper, dev, chunk=tun.chunk, order=tun.order)
     slots = srv.admit(jobs)
     srv.run(0)
     torch.cuda.synchronize()
@@ -135,7 +141,7 @@ def fit_section(dev, num_jobs=4096, plain_jobs=6, rates=(200.0, 1000.0, 5000.0))
     for lam in rates:
         n = min(num_jobs, max(64, int(lam * 1.0)))
         js = fitting.poisson_arrivals(jobs[:n], lam, seed=int(lam))
-        srv = fitting.CardFitServer(num_jobs, hyper, dev)
+        srv = fitting.CardFitServer(num_jobs, hyper, dev, chunk=tun.chunk, order=tun.order)
         rep = fitting.serve(js, srv, chunk=10)
         served[str(int(lam))] = {
             "jobs": n, "iters_per_s": round(rep.iters_per_s, 1),
@@ -250,8 +256,14 @@ def main():
     e2c = torch.from_numpy(w.edge2cluster).to(dev)
     bitset = torch.empty((B, w.H, w.W, S), dtype=torch.int32, device=dev)
 
+    from strns2cards_amd import autotune
+
+    dda_tune = autotune.tune_dda(pos, edges, e2c, bitset)
+    print(f"[bench] dda tile autotune: {dda_tune.ms} -> side {1 << dda_tune.tile_log2}",
+          file=sys.stderr, flush=True)
+
     def step():
-        _core.dda_cluster_mask(pos, edges, e2c, bitset)
+        _core.dda_cluster_mask(pos, edges, e2c, bitset, dda_tune.tile_log2)
 
     for _ in range(args.warmup):
         step()
@@ -336,7 +348,10 @@ def main():
         "config": {"workload": w.name, "views": B, "resolution": [w.H, w.W],
                    "strands": int(w.strands.shape[0]), "vertices_per_strand": int(w.strands.shape[1]),
                    "clusters": w.num_clusters, "bitset_words": S, "edges": E,
-                   "algorithmic_bytes_per_step": nbytes},
+                   "algorithmic_bytes_per_step": nbytes,
+                   "autotune": {"tile_side": 1 << dda_tune.tile_log2,
+                                "ms_per_tile_side": {str(k): round(v, 4)
+                                                     for k, v in dda_tune.ms.items()}}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": traffic_from_profiles(),

@@ -163,7 +163,8 @@ int s2c_skinning(int num_strands, int num_points, const float* strands,
 
 /* ---- fused forms of the reference's call sequences ------------------------
  * No single reference launcher corresponds; each equals the sequence cited,
- * with no fragment list in memory and no host synchronization. */
+ * with no fragment list in memory (the DDA form reads two counters back
+ * once per call to size its overflow buffers). */
 
 /* dda.py:5-76 then rasterize.py:170-214, as used at 3_fitting.py:254-261:
  * all B*E edges DDA-rasterized and OR-ed into bitset (B,H,W,num_slots),
@@ -172,6 +173,13 @@ int s2c_skinning(int num_strands, int num_points, const float* strands,
 int s2c_dda_cluster_mask(int B, int H, int W, int V, const float* pos, int E,
                          const int* edges, const int* edge2cluster,
                          int num_slots, int* bitset, void* stream);
+/* The same with the screen-tile side of the binning fixed to
+ * 1 << tile_log2 (2..6; 0 = chosen automatically, as above).  The result
+ * does not depend on it; a side whose tile bitset does not fit 32 KB of LDS
+ * is an S2C_ERR_ARG. */
+int s2c_dda_cluster_mask_tiled(int B, int H, int W, int V, const float* pos, int E,
+                               const int* edges, const int* edge2cluster, int num_slots,
+                               int tile_log2, int* bitset, void* stream);
 
 /* rasterize.py:6-53 then rasterize.py:170-214, as used at 3_fitting.py:154-163
  * (render_bitset_from_triangles); clears bitset first. */
@@ -207,12 +215,15 @@ int s2c_fit_targets(int num_jobs, const int* res, const int* bits_off,
  * renders the updated card and writes popcount(render XOR target) to
  * loss_out[job]; verts, Adam state and steps are updated in place.  Jobs
  * never share state, and each job's arithmetic order is fixed, so a job's
- * result does not depend on which other jobs run in the same call. */
+ * result does not depend on which other jobs run in the same call, on
+ * their order in `slots`, or on `chunk`: the silhouette samples a
+ * workgroup evaluates per pass (256, 512 or 1024; 0 = 512), a tuning knob
+ * trading LDS per workgroup against passes per iteration. */
 int s2c_fit_cards(int num_slots, const int* slots, int iters, const int* res,
                   const int* quads, const int* vert_off, const int* bits_off,
                   const float* mvp, float* verts, float* adam_m, float* adam_v,
                   int* steps, const unsigned* target_bits, const float* hyper,
-                  float* loss_out, void* stream);
+                  float* loss_out, int chunk, void* stream);
 
 /* compute_bishop_frame (curve_utils.py:104-165, a Python loop over the
  * vertices in the reference): rotation-minimizing frames of num_strands

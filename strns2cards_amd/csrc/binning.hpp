@@ -39,10 +39,20 @@ struct TileGrid {
 // Largest power-of-two side (<= 64) whose bitset fits kTileLdsBytes, shrunk
 // while the grid has fewer than `min_tiles` tiles.  Returns false when even
 // 4 x 4 does not fit or the packed entry cannot hold the cluster ids.
-inline bool choose_tiles(int B, int H, int W, int S, long long min_tiles, TileGrid& g) {
+// force_lg > 0 fixes the side to 1 << force_lg (false when it does not fit).
+inline bool choose_tiles(int B, int H, int W, int S, long long min_tiles, TileGrid& g,
+                         int force_lg = 0) {
+    auto fits = [&](int l) {
+        return l >= 2 && l <= 6 && (static_cast<long long>(1) << (2 * l)) * S * 4 <= kTileLdsBytes;
+    };
     int lg = 6;
-    while (lg >= 2 && (static_cast<long long>(1) << (2 * lg)) * S * 4 > kTileLdsBytes) --lg;
+    while (lg >= 2 && !fits(lg)) --lg;
     if (lg < 2) return false;
+    if (force_lg > 0) {
+        if (!fits(force_lg)) return false;
+        lg = force_lg;
+        min_tiles = 0;
+    }
     auto grid = [&](int l) {
         const int side = 1 << l;
         g.ts_log2 = l;

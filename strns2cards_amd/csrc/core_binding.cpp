@@ -504,7 +504,8 @@ PYBIND11_MODULE(_core, m) {
     });
 
     // Fused forms (no single reference binding; see the C ABI header).
-    m.def("dda_cluster_mask", [](Tensor pos, Tensor edges, Tensor edge2cluster, Tensor bitset) {
+    m.def("dda_cluster_mask", [](Tensor pos, Tensor edges, Tensor edge2cluster, Tensor bitset,
+                                 int tile_log2) {
         const char* fn = "dda_cluster_mask";
         need(pos, fn, "pos", at::kFloat, {-1, -1, 4});
         need(edges, fn, "edges", at::kInt, {-1, 2});
@@ -516,10 +517,11 @@ PYBIND11_MODULE(_core, m) {
         if (edge2cluster.size(0) != edges.size(0))
             throw std::runtime_error("dda_cluster_mask: edge2cluster must have E entries");
         DeviceGuard g(pos.device());
-        check(s2c_dda_cluster_mask(I(pos.size(0), fn), I(bitset.size(1), fn), I(bitset.size(2), fn),
-                                   I(pos.size(1), fn), P<float>(pos), I(edges.size(0), fn),
-                                   P<int>(edges), P<int>(edge2cluster), I(bitset.size(3), fn),
-                                   P<int>(bitset), stream_of(pos)),
+        check(s2c_dda_cluster_mask_tiled(I(pos.size(0), fn), I(bitset.size(1), fn),
+                                         I(bitset.size(2), fn), I(pos.size(1), fn), P<float>(pos),
+                                         I(edges.size(0), fn), P<int>(edges), P<int>(edge2cluster),
+                                         I(bitset.size(3), fn), tile_log2, P<int>(bitset),
+                                         stream_of(pos)),
               fn);
     });
 
@@ -545,7 +547,7 @@ PYBIND11_MODULE(_core, m) {
     m.def("fit_cards", [](Tensor slots, int iters, Tensor res, Tensor quads, Tensor vert_off,
                           Tensor bits_off, Tensor mvp, Tensor verts, Tensor adam_m, Tensor adam_v,
                           Tensor steps, Tensor target_bits, std::vector<float> hyper,
-                          Tensor loss_out) {
+                          Tensor loss_out, int chunk) {
         const char* fn = "fit_cards";
         need(slots, fn, "slots", at::kInt, {-1});
         need(res, fn, "res", at::kInt, {-1});
@@ -569,7 +571,7 @@ PYBIND11_MODULE(_core, m) {
         check(s2c_fit_cards(I(slots.size(0), fn), P<int>(slots), iters, P<int>(res), P<int>(quads),
                             P<int>(vert_off), P<int>(bits_off), P<float>(mvp), P<float>(verts),
                             P<float>(adam_m), P<float>(adam_v), P<int>(steps),
-                            P<unsigned>(target_bits), hyper.data(), P<float>(loss_out),
+                            P<unsigned>(target_bits), hyper.data(), P<float>(loss_out), chunk,
                             stream_of(res)),
               fn);
     });

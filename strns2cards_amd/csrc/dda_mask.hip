@@ -316,15 +316,28 @@ using namespace s2c;
 extern "C" int s2c_dda_cluster_mask(int B, int H, int W, int V, const float* pos, int E,
                                     const int* edges, const int* edge2cluster, int num_slots,
                                     int* bitset, void* stream) {
+    return s2c_dda_cluster_mask_tiled(B, H, W, V, pos, E, edges, edge2cluster, num_slots, 0,
+                                      bitset, stream);
+}
+
+extern "C" int s2c_dda_cluster_mask_tiled(int B, int H, int W, int V, const float* pos, int E,
+                                          const int* edges, const int* edge2cluster,
+                                          int num_slots, int tile_log2, int* bitset,
+                                          void* stream) {
     S2C_CHECK(B >= 0 && H >= 0 && W >= 0 && V >= 0 && E >= 0 && num_slots >= 0,
               "dda_cluster_mask: negative size");
+    S2C_CHECK(tile_log2 == 0 || (tile_log2 >= 2 && tile_log2 <= 6),
+              "dda_cluster_mask: tile_log2 must be 0 or in [2, 6]");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const long long words = static_cast<long long>(B) * H * W * num_slots;
     const long long P = static_cast<long long>(B) * E;
     if (words == 0) return S2C_OK;
     BinPlan plan;
     plan.blocks = (P + kPrimsPerBlock - 1) / kPrimsPerBlock;
-    if (P == 0 || !choose_tiles(B, H, W, num_slots, 4096, plan.g)) {
+    const bool tiled = P > 0 && choose_tiles(B, H, W, num_slots, 4096, plan.g, tile_log2);
+    S2C_CHECK(tiled || tile_log2 == 0 || P == 0,
+              "dda_cluster_mask: the requested tile bitset does not fit LDS");
+    if (!tiled) {
         // Empty input, or a tile bitset too large for LDS: clear + direct ORs.
         if (int r = fill_u32(bitset, 0u, words, s)) return r;
         if (P == 0) return S2C_OK;

@@ -30,7 +30,10 @@ constexpr int kFitMaxQuads = 32;
 constexpr int kFitMaxVerts = 2 * (kFitMaxQuads + 1);
 constexpr int kFitMaxTris = 2 * kFitMaxQuads;
 constexpr int kFitMaxEdges = 4 * kFitMaxQuads + 1;
-constexpr int kFitChunk = 512;  // silhouette samples evaluated per workgroup pass
+// Silhouette samples evaluated per workgroup pass: a template parameter
+// (LDS per workgroup against passes per iteration), chosen at run time from
+// kFitChunks; a job's sums do not depend on it.
+constexpr int kFitChunkDefault = 512;
 
 struct FitHyper {
     float lr0, b1, b2, gamma, w_pln, w_lap, rho, R;
@@ -127,6 +130,7 @@ __device__ __forceinline__ float block_max_nonneg(float v, float* red) {
     return r;
 }
 
+template <int kFitChunk>
 __global__ void __launch_bounds__(kBlock)
 fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict__ res_arr,
                  const int* __restrict__ quad_arr, const int* __restrict__ vert_off,
@@ -408,16 +412,24 @@ extern "C" int s2c_fit_cards(int num_slots, const int* slots, int iters, const i
                              const int* quads, const int* vert_off, const int* bits_off,
                              const float* mvp, float* verts, float* adam_m, float* adam_v,
                              int* steps, const unsigned* target_bits, const float* hyper,
-                             float* loss_out, void* stream) {
+                             float* loss_out, int chunk, void* stream) {
     S2C_CHECK(num_slots >= 0 && iters >= 0, "fit_cards: negative size");
     S2C_CHECK(hyper != nullptr, "fit_cards: null hyper-parameters");
+    if (chunk == 0) chunk = kFitChunkDefault;
+    S2C_CHECK(chunk == 256 || chunk == 512 || chunk == 1024,
+              "fit_cards: chunk must be 0, 256, 512 or 1024");
     if (num_slots == 0) return S2C_OK;
     FitHyper hp{hyper[0], hyper[1], hyper[2], hyper[3], hyper[4], hyper[5], hyper[6], hyper[7]};
     S2C_CHECK(hp.R > 0.f && hp.R < 64.f, "fit_cards: kernel radius must be in (0, 64)");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    fit_cards_kernel<<<num_slots, kBlock, 0, s>>>(slots, iters, res, quads, vert_off, bits_off,
-                                                  mvp, verts, adam_m, adam_v, steps, target_bits,
-                                                  hp, loss_out);
+#define S2C_FIT_LAUNCH(C)                                                                      \
+    fit_cards_kernel<C><<<num_slots, kBlock, 0, s>>>(slots, iters, res, quads, vert_off,     \
+                                                     bits_off, mvp, verts, adam_m, adam_v,   \
+                                                     steps, target_bits, hp, loss_out)
+    if (chunk == 256) S2C_FIT_LAUNCH(256);
+    else if (chunk == 512) S2C_FIT_LAUNCH(512);
+    else S2C_FIT_LAUNCH(1024);
+#undef S2C_FIT_LAUNCH
     S2C_LAUNCH_CHECK();
     return S2C_OK;
 }

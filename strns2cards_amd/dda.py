@@ -115,6 +115,7 @@ def dda_cluster_mask(
     edges: torch.Tensor,
     edge2cluster: torch.Tensor,
     cluster_count: Optional[int] = None,
+    tile_log2: Optional[int] = None,
 ) -> torch.Tensor:
     """Fused form of ``dda_compute_fragments`` followed by
     ``cluster_mask_from_fragments`` (the target-mask sequence of
@@ -123,6 +124,9 @@ def dda_cluster_mask(
     Returns a (B, H, W, ceil(K/32)) int32 bitset with B = pos.shape[0]; it
     equals the two-call result on every view (views that receive no fragment
     are all zero, where the two-call form drops trailing empty views).
+    ``tile_log2`` fixes the binning tile side (2..6; 0 = automatic); by
+    default the side ``autotune.tune_dda`` chose for this shape is used.
+    The result does not depend on it.
     """
     H, W = resolution
     if not (pos.is_cuda and edges.is_cuda and edge2cluster.is_cuda):
@@ -139,5 +143,9 @@ def dda_cluster_mask(
         cluster_count = int(edge2cluster.max().item()) + 1 if edge2cluster.numel() else 1
     S = (cluster_count + 31) // 32
     bitset = torch.empty((pos.shape[0], H, W, S), dtype=torch.int32, device=pos.device)
-    _core.dda_cluster_mask(pos.contiguous(), edges.contiguous(), edge2cluster.contiguous(), bitset)
+    if tile_log2 is None:
+        from .autotune import dda_tile_for
+        tile_log2 = dda_tile_for(pos.shape[0], H, W, S, edges.shape[0])
+    _core.dda_cluster_mask(pos.contiguous(), edges.contiguous(), edge2cluster.contiguous(), bitset,
+                           int(tile_log2))
     return bitset

@@ -234,8 +234,17 @@ class CardFitServer:
     the whole table (idle slots exit at once), so the launch can be captured
     once and replayed while jobs come and go."""
 
-    def __init__(self, capacity: int, hyper: FitHyper, dev, budget_bytes: float = 200e9):
+    def __init__(self, capacity: int, hyper: FitHyper, dev, budget_bytes: float = 200e9,
+                 chunk: int = 0, order: str = "admission"):
+        """``chunk`` (silhouette samples per workgroup pass: 0 = 512, 256,
+        512, 1024) and ``order`` ("admission": slot order; "cost": jobs
+        with the most work first) are launch knobs chosen by
+        ``autotune.tune_fit``; no job's result depends on them."""
+        if order not in ("admission", "cost"):
+            raise ValueError("order must be 'admission' or 'cost'")
         self.cap = capacity
+        self.chunk = int(chunk)
+        self.order = order
         self.hyper = hyper
         self.dev = dev
         self.budget = budget_bytes
@@ -254,6 +263,7 @@ class CardFitServer:
         self.adam_v = torch.zeros(capacity * MAX_VERTS, **f32)
         self.target = torch.zeros(capacity * MAX_WORDS, **i32)
         self.slots = torch.full((capacity,), -1, **i32)
+        self.launch = torch.full((capacity,), -1, **i32)  # workgroup -> slot ("cost" order)
         self.free = deque(range(capacity))
         self.state: dict[int, SlotState] = {}
         self.graph = None
@@ -295,7 +305,22 @@ class CardFitServer:
         for s, j in zip(slots, jobs):
             self.state[s] = SlotState(j, now)
             self.used_bytes += j.bytes
+        self._refresh_launch()
         return slots
+
+    @staticmethod
+    def job_cost(job: FitJob) -> float:
+        """Relative work of one iteration of a job: mask clears / loss over
+        res^2 bits and silhouette samples along ~res-long edges."""
+        return job.res * job.res / 32.0 + 4.0 * job.res * job.quads
+
+    def _refresh_launch(self) -> None:
+        if self.order != "cost":
+            return
+        act = sorted(self.state, key=lambda s: -self.job_cost(self.state[s].job))
+        arr = np.full(self.cap, -1, np.int32)
+        arr[: len(act)] = act
+        self.launch.copy_(torch.from_numpy(arr))
 
     def release(self, slots: list[int]) -> None:
         if not slots:
@@ -304,12 +329,14 @@ class CardFitServer:
         for s in slots:
             self.used_bytes -= self.state.pop(s).job.bytes
             self.free.append(s)
+        self._refresh_launch()
 
     # -- iterations ----------------------------------------------------
     def _launch(self, iters: int) -> None:
-        _core.fit_cards(self.slots, iters, self.res, self.quads, self.vert_off, self.bits_off, self.mvp,
-                        self.verts, self.adam_m, self.adam_v, self.steps, self.target,
-                        self.hyper.vector(), self.loss)
+        _core.fit_cards(self.launch if self.order == "cost" else self.slots, iters, self.res,
+                        self.quads, self.vert_off, self.bits_off, self.mvp, self.verts, self.adam_m,
+                        self.adam_v, self.steps, self.target, self.hyper.vector(), self.loss,
+                        self.chunk)
 
     def run(self, iters: int, graph: bool = False) -> None:
         """Advances every active slot by `iters` iterations (and refreshes

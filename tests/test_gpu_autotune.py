@@ -1,0 +1,68 @@
+"""The autotuner's knobs change launch shapes only: every fitting chunk and
+job order gives bit-identical jobs, every DDA tile side the same mask."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from conftest import report
+from strns2cards_amd import _core, autotune, fitting, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def test_fit_chunk_and_order_do_not_change_results(dev):
+    report("autotune: fit chunk x order give identical jobs")
+    jobs = fitting.make_jobs(24, seed=9)
+    hyper = fitting.FitHyper()
+    ref = None
+    for chunk in autotune.FIT_CHUNKS:
+        for order in autotune.FIT_ORDERS:
+            srv = fitting.CardFitServer(32, hyper, dev, chunk=chunk, order=order)
+            slots = srv.admit(jobs)
+            srv.run(7)
+            torch.cuda.synchronize()
+            got = (np.concatenate([srv.job_vertices(s) for s in slots]),
+                   srv.loss.cpu().numpy()[slots])
+            if ref is None:
+                ref = got
+            else:
+                np.testing.assert_array_equal(got[0], ref[0])
+                np.testing.assert_array_equal(got[1], ref[1])
+
+
+def test_dda_tile_sides_give_the_same_mask(dev):
+    report("autotune: every DDA tile side gives the oracle's mask")
+    w = synth.small_workload()
+    want = oracle.dda_cluster_mask((w.H, w.W), w.pos, w.edges, w.edge2cluster, 64)
+    pos = torch.from_numpy(w.pos).to(dev)
+    edges = torch.from_numpy(w.edges).to(dev)
+    e2c = torch.from_numpy(w.edge2cluster).to(dev)
+    bits = torch.empty((1, w.H, w.W, 2), dtype=torch.int32, device=dev)
+    for lg in autotune.dda_tile_candidates(2):
+        bits.fill_(-1)
+        _core.dda_cluster_mask(pos, edges, e2c, bits, lg)
+        np.testing.assert_array_equal(bits.cpu().numpy(), want)
+    with pytest.raises(RuntimeError):  # 128 x 128 x 2 words does not fit LDS
+        _core.dda_cluster_mask(pos, edges, e2c, bits, 7)
+
+
+def test_tuners_return_a_candidate(dev):
+    report("autotune: tuners return timed candidates")
+    jobs = fitting.make_jobs(16, seed=3)
+    t = autotune.tune_fit(jobs, fitting.FitHyper(), dev, iters=3, reps=1)
+    assert t.chunk in autotune.FIT_CHUNKS and t.order in autotune.FIT_ORDERS
+    assert len(t.ms) == len(autotune.FIT_CHUNKS) * len(autotune.FIT_ORDERS)
+    w = synth.small_workload()
+    pos = torch.from_numpy(w.pos).to(dev)
+    edges = torch.from_numpy(w.edges).to(dev)
+    e2c = torch.from_numpy(w.edge2cluster).to(dev)
+    bits = torch.empty((1, w.H, w.W, 2), dtype=torch.int32, device=dev)
+    d = autotune.tune_dda(pos, edges, e2c, bits, reps=1)
+    assert d.tile_log2 in autotune.dda_tile_candidates(2)
+    assert autotune.dda_tile_for(1, w.H, w.W, 2, w.edges.shape[0]) == d.tile_log2
+    # the Python API now uses the tuned side and still matches the oracle
+    import strns2cards_amd as s2c
+    got = s2c.dda_cluster_mask((w.H, w.W), pos, edges, e2c, 64).cpu().numpy()
+    np.testing.assert_array_equal(got, oracle.dda_cluster_mask((w.H, w.W), w.pos, w.edges,
+                                                               w.edge2cluster, 64))
