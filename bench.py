@@ -69,10 +69,11 @@ def cpu_baseline(w, seconds):
                       f"{dt * 1e3:.1f} ms/rep"}
 
 
-def fit_section(dev, num_jobs=4096, plain_jobs=6, rates=(200.0, 1000.0, 5000.0)):
+def fit_section(dev, num_jobs=4096, plain_jobs=6, rates=(200.0, 1000.0, 5000.0), tune=True):
     """Card fitting: batched persistent launch, plain per-job loop, served
     continuous batching under Poisson arrivals (all 100 iterations per
-    job)."""
+    job).  With tune, the launch shape is autotuned on the first 1024 jobs
+    first; otherwise the library defaults are used."""
     import numpy as np
     import torch
 
@@ -86,10 +87,13 @@ def fit_section(dev, num_jobs=4096, plain_jobs=6, rates=(200.0, 1000.0, 5000.0))
     out = {}
     from strns2cards_amd import autotune
 
-    tun = autotune.tune_fit(jobs[:1024], hyper, dev, iters=10)
-    out["autotune"] = {"chunk": tun.chunk, "order": tun.order,
-                       "ms_per_10_iters_1024_jobs": {k: round(v, 3) for k, v in tun.ms.items()}}
-    print(f"[bench] fit autotune: {out['autotune']}", file=sys.stderr, flush=True)
+    if tune:
+        tun = autotune.tune_fit(jobs[:1024], hyper, dev, iters=10)
+        out["autotune"] = {"chunk": tun.chunk, "order": tun.order,
+                           "ms_per_10_iters_1024_jobs": {k: round(v, 3) for k, v in tun.ms.items()}}
+        print(f"[bench] fit autotune: {out['autotune']}", file=sys.stderr, flush=True)
+    else:
+        tun = autotune.FitTuning(0, "admission")
 
     # one persistent launch: every job runs all its iterations
     srv = fitting.CardFitServer(num_jobs, hyper, dev, chunk=tun.chunk, order=tun.order)
@@ -229,6 +233,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--dda-only", action="store_true",
                     help="time the DDA->mask step only (for PMC runs)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the launch-shape autotuners (library defaults; for profiler runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 512 MiB device copy (PMC byte-counter scale)")
     args = ap.parse_args()
@@ -258,9 +264,12 @@ def main():
 
     from strns2cards_amd import autotune
 
-    dda_tune = autotune.tune_dda(pos, edges, e2c, bitset)
-    print(f"[bench] dda tile autotune: {dda_tune.ms} -> side {1 << dda_tune.tile_log2}",
-          file=sys.stderr, flush=True)
+    if args.no_autotune:
+        dda_tune = autotune.DdaTuning(0)
+    else:
+        dda_tune = autotune.tune_dda(pos, edges, e2c, bitset)
+        print(f"[bench] dda tile autotune: {dda_tune.ms} -> side {1 << dda_tune.tile_log2}",
+              file=sys.stderr, flush=True)
 
     def step():
         _core.dda_cluster_mask(pos, edges, e2c, bitset, dda_tune.tile_log2)
@@ -324,7 +333,8 @@ def main():
     print(f"[bench] card raster->mask {ms_tri:.3f} ms, antialias grad {ms_aa:.3f} ms",
           file=sys.stderr, flush=True)
 
-    fit = None if args.no_fit else fit_section(dev, num_jobs=args.fit_jobs)
+    fit = None if args.no_fit else fit_section(dev, num_jobs=args.fit_jobs,
+                                                   tune=not args.no_autotune)
     adj = None if args.no_adjoint else adjoint_section(dev)
 
     cpu = None
@@ -349,9 +359,9 @@ def main():
                    "strands": int(w.strands.shape[0]), "vertices_per_strand": int(w.strands.shape[1]),
                    "clusters": w.num_clusters, "bitset_words": S, "edges": E,
                    "algorithmic_bytes_per_step": nbytes,
-                   "autotune": {"tile_side": 1 << dda_tune.tile_log2,
-                                "ms_per_tile_side": {str(k): round(v, 4)
-                                                     for k, v in dda_tune.ms.items()}}},
+                   "autotune": None if args.no_autotune else
+                   {"tile_side": 1 << dda_tune.tile_log2,
+                    "ms_per_tile_side": {str(k): round(v, 4) for k, v in dda_tune.ms.items()}}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": traffic_from_profiles(),
