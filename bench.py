@@ -235,6 +235,8 @@ def main():
                     help="time the DDA->mask step only (for PMC runs)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the launch-shape autotuners (library defaults; for profiler runs)")
+    ap.add_argument("--dda-tile", type=int, default=0,
+                    help="DDA->mask tile side log2 (implies no DDA autotune; for profiler runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 512 MiB device copy (PMC byte-counter scale)")
     args = ap.parse_args()
@@ -264,8 +266,8 @@ def main():
 
     from strns2cards_amd import autotune
 
-    if args.no_autotune:
-        dda_tune = autotune.DdaTuning(0)
+    if args.no_autotune or args.dda_tile:
+        dda_tune = autotune.DdaTuning(args.dda_tile)
     else:
         dda_tune = autotune.tune_dda(pos, edges, e2c, bitset)
         print(f"[bench] dda tile autotune: {dda_tune.ms} -> side {1 << dda_tune.tile_log2}",
@@ -359,7 +361,7 @@ def main():
                    "strands": int(w.strands.shape[0]), "vertices_per_strand": int(w.strands.shape[1]),
                    "clusters": w.num_clusters, "bitset_words": S, "edges": E,
                    "algorithmic_bytes_per_step": nbytes,
-                   "autotune": None if args.no_autotune else
+                   "autotune": None if args.no_autotune or args.dda_tile else
                    {"tile_side": 1 << dda_tune.tile_log2,
                     "ms_per_tile_side": {str(k): round(v, 4) for k, v in dda_tune.ms.items()}}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,

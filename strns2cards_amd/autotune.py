@@ -12,7 +12,7 @@ the fastest:
   ("admission", or "cost": most work first, so the long jobs of a mixed
   batch do not start last);
 * DDA -> mask (``tune_dda``): the screen-tile side of the binning (every
-  power of two whose tile bitset fits 32 KB of LDS), which trades
+  power of two whose tile bitset fits 128 KB of LDS), which trades
   (workgroup, tile) runs and their atomics in the stage pass against tile
   workgroups and their balance in the resolve.
 
@@ -84,8 +84,10 @@ def tune_fit(jobs, hyper, dev, iters: int = 10, reps: int = 3) -> FitTuning:
 
 
 def dda_tile_candidates(S: int) -> list[int]:
-    """Tile sides (log2) whose (side^2 x S) word bitset fits 32 KB."""
-    return [lg for lg in range(2, 7) if (1 << (2 * lg)) * S * 4 <= 32768]
+    """Tile sides (log2) whose (side^2 x S) word bitset fits 128 KB of LDS
+    (sides above the default, whose bitset fits 32 KB, run one resolve
+    workgroup per CU)."""
+    return [lg for lg in range(2, 7) if (1 << (2 * lg)) * S * 4 <= 131072]
 
 
 def _dda_key(B, H, W, S, E):

@@ -125,13 +125,13 @@ bin_order_kernel(long long nt, long long blocks, const int* __restrict__ block_r
     }
 }
 
-template <bool VEC4>
+template <bool VEC4, int kLds>
 __global__ void __launch_bounds__(kResolveThreads)
 tile_resolve_kernel(TileGrid g, const int* __restrict__ order, const int* __restrict__ offs,
                     const BinRun* __restrict__ runs,
                     const unsigned* __restrict__ entries, int H, int W, int S,
                     int* __restrict__ bitset) {
-    __shared__ unsigned tile[kTileLdsBytes / 4];
+    __shared__ unsigned tile[kLds / 4];
     __shared__ int pre[kResolveThreads];
     __shared__ int src[kResolveThreads];
     __shared__ int sh[kResolveThreads / kWave + 1];
@@ -282,11 +282,19 @@ int bin_resolve(const BinPlan& p, int H, int W, int S, int* bitset, hipStream_t 
         p.order);
     S2C_LAUNCH_CHECK();
     const bool vec = (S % 4 == 0) && ((reinterpret_cast<uintptr_t>(bitset) & 15) == 0);
-    if (vec)
-        tile_resolve_kernel<true><<<static_cast<unsigned>(p.g.nt), kResolveThreads, 0, s>>>(
+    const bool small = tile_bitset_bytes(p.g.ts_log2, S) <= kTileLdsBytes;
+    const unsigned grid_t = static_cast<unsigned>(p.g.nt);
+    if (vec && small)
+        tile_resolve_kernel<true, kTileLdsBytes><<<grid_t, kResolveThreads, 0, s>>>(
+            p.g, p.order, p.runs_per_slot, p.runs, p.entries, H, W, S, bitset);
+    else if (vec)
+        tile_resolve_kernel<true, kTileLdsMax><<<grid_t, kResolveThreads, 0, s>>>(
+            p.g, p.order, p.runs_per_slot, p.runs, p.entries, H, W, S, bitset);
+    else if (small)
+        tile_resolve_kernel<false, kTileLdsBytes><<<grid_t, kResolveThreads, 0, s>>>(
             p.g, p.order, p.runs_per_slot, p.runs, p.entries, H, W, S, bitset);
     else
-        tile_resolve_kernel<false><<<static_cast<unsigned>(p.g.nt), kResolveThreads, 0, s>>>(
+        tile_resolve_kernel<false, kTileLdsMax><<<grid_t, kResolveThreads, 0, s>>>(
             p.g, p.order, p.runs_per_slot, p.runs, p.entries, H, W, S, bitset);
     S2C_LAUNCH_CHECK();
     return S2C_OK;

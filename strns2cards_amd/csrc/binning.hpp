@@ -23,7 +23,11 @@
 
 namespace s2c {
 
-constexpr int kTileLdsBytes = 32768;  // LDS bitset per tile workgroup
+// LDS bitset of a tile workgroup: the default tile side fits kTileLdsBytes
+// (several resolve workgroups per CU); an explicitly chosen side may use up
+// to kTileLdsMax (one per CU, fewer (workgroup, tile) runs).
+constexpr int kTileLdsBytes = 32768;
+constexpr int kTileLdsMax = 131072;
 constexpr int kHashLog2 = 10;
 constexpr int kHash = 1 << kHashLog2;  // LDS table slots per workgroup
 constexpr int kStage = 4096;           // fragments staged in LDS per workgroup
@@ -36,20 +40,26 @@ struct TileGrid {
     int local_bits;
 };
 
+// Bytes of a tile bitset of side 1 << l with S words per pixel.
+inline long long tile_bitset_bytes(int l, int S) {
+    return (static_cast<long long>(1) << (2 * l)) * S * 4;
+}
+
 // Largest power-of-two side (<= 64) whose bitset fits kTileLdsBytes, shrunk
 // while the grid has fewer than `min_tiles` tiles.  Returns false when even
 // 4 x 4 does not fit or the packed entry cannot hold the cluster ids.
-// force_lg > 0 fixes the side to 1 << force_lg (false when it does not fit).
+// force_lg > 0 fixes the side to 1 << force_lg, whose bitset may use up to
+// kTileLdsMax (false when it does not fit).
 inline bool choose_tiles(int B, int H, int W, int S, long long min_tiles, TileGrid& g,
                          int force_lg = 0) {
-    auto fits = [&](int l) {
-        return l >= 2 && l <= 6 && (static_cast<long long>(1) << (2 * l)) * S * 4 <= kTileLdsBytes;
+    auto fits = [&](int l, long long cap) {
+        return l >= 2 && l <= 6 && tile_bitset_bytes(l, S) <= cap;
     };
     int lg = 6;
-    while (lg >= 2 && !fits(lg)) --lg;
+    while (lg >= 2 && !fits(lg, kTileLdsBytes)) --lg;
     if (lg < 2) return false;
     if (force_lg > 0) {
-        if (!fits(force_lg)) return false;
+        if (!fits(force_lg, kTileLdsMax)) return false;
         lg = force_lg;
         min_tiles = 0;
     }
@@ -65,7 +75,7 @@ inline bool choose_tiles(int B, int H, int W, int S, long long min_tiles, TileGr
     while (g.nt < min_tiles && lg > 3) grid(--lg);
     // packed entries (cluster << local_bits | local pixel) stay below 2^22,
     // leaving 10 bits for a hash slot in the stage (always true while the
-    // tile bitset fits kTileLdsBytes)
+    // tile bitset fits kTileLdsMax)
     const long long max_cluster = static_cast<long long>(S) * 32;
     if (max_cluster > (1ll << (22 - g.local_bits))) return false;
     return g.nt * kBinGroups + 2 < INT_MAX;

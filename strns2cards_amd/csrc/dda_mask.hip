@@ -147,11 +147,12 @@ dda_mask_direct_kernel(long long P, int H, int W, int V, const float* __restrict
 // with one LDS atomic per run of same-tile fragments.  After the walk the
 // slots' counts are scanned into offsets within the workgroup's fixed
 // entry region, a run record per (workgroup, tile) pair goes to its fixed
-// record region, and each thread copies its runs out, ranked by one LDS
-// atomic per run.  A workgroup whose table overflows (> kHash distinct
-// tiles) makes one run per fragment in the overflow areas; one whose thread
-// has more fragments than its stage share takes its region from the
-// overflow area and walks again to place them.  Nothing is written past the
+// record region at an index from a block scan, and each thread copies its
+// column out of registers, one LDS atomic reserving each run's span.  A
+// workgroup whose table overflows (> kHash distinct tiles) makes one run
+// per fragment in the overflow areas; one whose thread has more fragments
+// than its stage share takes its region from the overflow area and walks
+// again to place them.  Nothing is written past the
 // overflow capacities; their use is counted regardless, so the host can
 // size them and launch again.  (choose_tiles keeps entries below 2^22.)
 constexpr int kPerThread = kStage / kBlock;
@@ -167,7 +168,7 @@ dda_bin_stage_kernel(long long P, int H, int W, int V, const float* __restrict__
     __shared__ int vals[kHash];   // fragments per slot, then slot offsets in the region
     __shared__ unsigned stage[kStage];
     __shared__ int deferred[kDeferred];
-    __shared__ int ndeferred, spill, full, nused, base_e, base_r;
+    __shared__ int ndeferred, spill, full, base_e, base_r;
     __shared__ int sh[kBlock / kWave + 1];
     const long long fixed_e = static_cast<long long>(gridDim.x) * kStage;
     const long long fixed_r = static_cast<long long>(gridDim.x) * kBinRecs;
@@ -176,7 +177,7 @@ dda_bin_stage_kernel(long long P, int H, int W, int V, const float* __restrict__
         keys[s] = -1;
         vals[s] = 0;
     }
-    if (threadIdx.x == 0) ndeferred = spill = full = nused = 0;
+    if (threadIdx.x == 0) ndeferred = spill = full = 0;
     __syncthreads();
     // thread t's i-th staged fragment lives at i * kBlock + t: consecutive
     // lanes touch consecutive LDS words
@@ -243,22 +244,20 @@ dda_bin_stage_kernel(long long P, int H, int W, int V, const float* __restrict__
         vals[threadIdx.x * per_thread + j] = run;  // now the slot's offset
         run += cnt[j];
     }
-    if (used) atomicAdd(&nused, used);
-    __syncthreads();
+    int nrec;
+    int rec = block_excl_scan(used, sh, &nrec);  // this thread's first record
     const bool staged = spill == 0;  // workgroup-uniform
     if (threadIdx.x == 0) {
         base_e = staged ? static_cast<int>(blockIdx.x) * kStage
                         : static_cast<int>(fixed_e) + atomicAdd(&counters[kEntriesOvf], total);
-        const bool in_fixed = nused <= kBinRecs;
+        const bool in_fixed = nrec <= kBinRecs;
         base_r = in_fixed ? static_cast<int>(blockIdx.x) * kBinRecs
-                          : static_cast<int>(fixed_r) + atomicAdd(&counters[kRunsOvf], nused);
-        block_runs[blockIdx.x] = in_fixed ? nused : 0;
-        nused = 0;  // reused as the run cursor below
+                          : static_cast<int>(fixed_r) + atomicAdd(&counters[kRunsOvf], nrec);
+        block_runs[blockIdx.x] = in_fixed ? nrec : 0;
     }
     __syncthreads();
     const int be = base_e, br = base_r;
     const bool e_fits = staged || static_cast<long long>(be) + total <= fixed_e + ovf_entries;
-    const bool r_fits = br < fixed_r || br - fixed_r + kHash <= ovf_runs;
     // run records; then the slots' keys become per-slot cursors
 #pragma unroll
     for (int j = 0; j < per_thread; ++j) {
@@ -267,8 +266,8 @@ dda_bin_stage_kernel(long long P, int H, int W, int V, const float* __restrict__
         if (key < 0) continue;
         const int ks = key * kBinGroups + group;
         const int k = atomicAdd(&runs_per_slot[ks + 1], 1);
-        const int r = br + atomicAdd(&nused, 1);
-        if (r_fits || r < fixed_r + ovf_runs) records[r] = BinRecord{ks, k, be + vals[s], cnt[j]};
+        const long long r = static_cast<long long>(br) + rec++;
+        if (r < fixed_r + ovf_runs) records[r] = BinRecord{ks, k, be + vals[s], cnt[j]};
     }
     if (!e_fits) return;
     if (staged) {
@@ -276,14 +275,34 @@ dda_bin_stage_kernel(long long P, int H, int W, int V, const float* __restrict__
 #pragma unroll
         for (int j = 0; j < per_thread; ++j) keys[threadIdx.x * per_thread + j] = 0;
         __syncthreads();
-        int i = 0;
-        while (i < mine) {
-            const unsigned s0 = stage[i * kBlock + threadIdx.x] >> 22;
-            int j = i + 1;
-            while (j < mine && (stage[j * kBlock + threadIdx.x] >> 22) == s0) ++j;
-            const int at = be + vals[s0] + atomicAdd(&keys[s0], j - i);
-            for (int k = i; k < j; ++k) entries[at + k - i] = stage[k * kBlock + threadIdx.x] & kEntryMask;
-            i = j;
+        // The thread's column in registers; each run of same-slot entries
+        // reserves its span at its last entry, and a backward pass writes
+        // every entry below its run's end.
+        unsigned v[kPerThread];
+#pragma unroll
+        for (int i = 0; i < kPerThread; ++i) v[i] = i < mine ? stage[i * kBlock + threadIdx.x] : 0u;
+        int end[kPerThread];
+        int len = 0;
+#pragma unroll
+        for (int i = 0; i < kPerThread; ++i) {
+            end[i] = 0;
+            if (i < mine) {
+                const unsigned sl = v[i] >> 22;
+                ++len;
+                const bool last = i + 1 == mine || (i + 1 < kPerThread && (v[i + 1] >> 22) != sl);
+                if (last) {  // end > 0: the run holds at least this entry
+                    end[i] = be + vals[sl] + atomicAdd(&keys[sl], len) + len;
+                    len = 0;
+                }
+            }
+        }
+        int at = 0;
+#pragma unroll
+        for (int i = kPerThread - 1; i >= 0; --i) {
+            if (i < mine) {
+                if (end[i]) at = end[i];
+                entries[--at] = v[i] & kEntryMask;
+            }
         }
         return;
     }
@@ -371,8 +390,7 @@ extern "C" int s2c_dda_cluster_mask_tiled(int B, int H, int W, int V, const floa
         S2C_LAUNCH_CHECK();
         long long ne = 0, nr = 0;
         if (int r = bin_read_overflow(plan, s, &ne, &nr)) return r;
-        // a workgroup writing records into the overflow area reserves up to kHash
-        if (ne <= want_e && nr + kHash <= want_r) break;
+        if (ne <= want_e && nr <= want_r) break;
         S2C_CHECK(attempt == 0, "dda_cluster_mask: buffer sizing did not converge");
         want_e = bin_entry_hint(ne + ne / 8 + 4 * kStage);
         want_r = bin_run_hint(nr + nr / 8 + 4 * kHash);

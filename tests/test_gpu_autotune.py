@@ -47,6 +47,28 @@ def test_dda_tile_sides_give_the_same_mask(dev):
         _core.dda_cluster_mask(pos, edges, e2c, bits, 7)
 
 
+@pytest.mark.parametrize("K", [1024, 96, 256])
+def test_dda_large_tile_bitsets_match_the_oracle(dev, K):
+    """Tile sides whose bitset exceeds 32 KB resolve in the 128 KB LDS
+    variant: S = 32 at side 32, S = 3 (scalar stores) and S = 8 at side 64."""
+    S = (K + 31) // 32
+    report(f"autotune: large tile bitsets, K={K} (S={S})")
+    w = synth.small_workload()
+    rng = np.random.default_rng(K)
+    e2c = rng.integers(0, K, w.edges.shape[0]).astype(np.int32)
+    want = oracle.dda_cluster_mask((w.H, w.W), w.pos, w.edges, e2c, K)
+    pos = torch.from_numpy(w.pos).to(dev)
+    edges = torch.from_numpy(w.edges).to(dev)
+    e2c_d = torch.from_numpy(e2c).to(dev)
+    bits = torch.empty((1, w.H, w.W, S), dtype=torch.int32, device=dev)
+    sides = [lg for lg in autotune.dda_tile_candidates(S) if lg >= 4]
+    assert any((1 << (2 * lg)) * S * 4 > 32768 for lg in sides)
+    for lg in sides:
+        bits.fill_(-1)
+        _core.dda_cluster_mask(pos, edges, e2c_d, bits, lg)
+        np.testing.assert_array_equal(bits.cpu().numpy(), want)
+
+
 def test_tuners_return_a_candidate(dev):
     report("autotune: tuners return timed candidates")
     jobs = fitting.make_jobs(16, seed=3)
