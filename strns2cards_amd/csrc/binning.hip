@@ -6,11 +6,12 @@
 
 namespace s2c {
 
-// A tile's entry count varies by more than an order of magnitude (strand
-// roots crowd a few tiles), so the resolve uses 1024-thread workgroups and
-// keeps kResolveUnroll loads in flight per thread: the hottest tiles set the
-// kernel's duration.
-constexpr int kResolveThreads = 1024;
+// A tile's resolve is a chain of dependent global round trips (order,
+// offsets, runs, entries, then the stores), so the kernel is bound by how
+// many tiles are in flight: 512-thread workgroups let four tiles share a CU
+// (LDS-bound at 32 KB of bitset each), twice as many as 1024-thread ones,
+// and each thread keeps kResolveUnroll entry loads in flight.
+constexpr int kResolveThreads = 512;
 constexpr int kResolveUnroll = 8;
 
 // Resolve order: tiles by decreasing run count in classes of powers of 4,
