@@ -16,7 +16,8 @@ constexpr int kResolveUnroll = 8;
 
 // Resolve order: tiles by decreasing run count in classes of powers of 4,
 // so the heaviest tiles start first and the light and empty ones fill the
-// tail.  One workgroup; each thread classifies kOrderPer consecutive tiles
+// tail.  Each entry is (tile, first run, end of its runs), so a resolve
+// workgroup starts with one load.  One workgroup; each thread classifies kOrderPer consecutive tiles
 // from one round of loads, and counts / places them with wave scans and one
 // LDS atomic per wave and class.  The order within a class is arbitrary.
 constexpr int kOrderThreads = 1024;
@@ -24,8 +25,7 @@ constexpr int kOrderClasses = 8;
 constexpr int kOrderPer = 16;
 
 __device__ __forceinline__ void order_classes(long long t0, long long nt, const int* offs,
-                                              int (&cls)[kOrderPer]) {
-    int v[kOrderPer + 1];
+                                              int (&cls)[kOrderPer], int (&v)[kOrderPer + 1]) {
 #pragma unroll
     for (int i = 0; i <= kOrderPer; ++i) v[i] = offs[dmin(t0 + i, nt) * kBinGroups];
 #pragma unroll
@@ -38,15 +38,15 @@ __device__ __forceinline__ void order_classes(long long t0, long long nt, const 
 }
 
 __device__ __forceinline__ void tile_order(long long nt, const int* __restrict__ offs,
-                                           int* __restrict__ order) {
+                                           int4* __restrict__ order) {
     __shared__ int hist[kOrderClasses];
     if (threadIdx.x < kOrderClasses) hist[threadIdx.x] = 0;
     __syncthreads();
     const int lane = lane_id();
     constexpr long long step = static_cast<long long>(kOrderThreads) * kOrderPer;
     for (long long c0 = 0; c0 < nt; c0 += step) {
-        int cls[kOrderPer];
-        order_classes(c0 + threadIdx.x * kOrderPer, nt, offs, cls);
+        int cls[kOrderPer], v[kOrderPer + 1];
+        order_classes(c0 + threadIdx.x * kOrderPer, nt, offs, cls, v);
 #pragma unroll
         for (int k = 0; k < kOrderClasses; ++k) {
             int n = 0;
@@ -68,8 +68,8 @@ __device__ __forceinline__ void tile_order(long long nt, const int* __restrict__
     __syncthreads();
     for (long long c0 = 0; c0 < nt; c0 += step) {
         const long long t0 = c0 + threadIdx.x * kOrderPer;
-        int cls[kOrderPer];
-        order_classes(t0, nt, offs, cls);
+        int cls[kOrderPer], v[kOrderPer + 1];
+        order_classes(t0, nt, offs, cls, v);
 #pragma unroll
         for (int k = 0; k < kOrderClasses; ++k) {
             int n = 0;
@@ -81,7 +81,7 @@ __device__ __forceinline__ void tile_order(long long nt, const int* __restrict__
             b = __shfl(b, kWave - 1, kWave) + incl - n;
 #pragma unroll
             for (int i = 0; i < kOrderPer; ++i)
-                if (cls[i] == k) order[b++] = static_cast<int>(t0 + i);
+                if (cls[i] == k) order[b++] = make_int4(static_cast<int>(t0 + i), v[i], v[i + 1], 0);
         }
     }
 }
@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(kOrderThreads)
 bin_order_kernel(long long nt, long long blocks, const int* __restrict__ block_runs, int ovf_cap,
                  const int* __restrict__ counters, const BinRecord* __restrict__ recs,
                  const int* __restrict__ offs, BinRun* __restrict__ runs,
-                 int* __restrict__ order) {
+                 int4* __restrict__ order) {
     if (blockIdx.x == 0) {
         tile_order(nt, offs, order);
         return;
@@ -128,7 +128,7 @@ bin_order_kernel(long long nt, long long blocks, const int* __restrict__ block_r
 
 template <bool VEC4, int kLds>
 __global__ void __launch_bounds__(kResolveThreads)
-tile_resolve_kernel(TileGrid g, const int* __restrict__ order, const int* __restrict__ offs,
+tile_resolve_kernel(TileGrid g, const int4* __restrict__ order,
                     const BinRun* __restrict__ runs,
                     const unsigned* __restrict__ entries, int H, int W, int S,
                     int* __restrict__ bitset) {
@@ -136,12 +136,15 @@ tile_resolve_kernel(TileGrid g, const int* __restrict__ order, const int* __rest
     __shared__ int pre[kResolveThreads];
     __shared__ int src[kResolveThreads];
     __shared__ int sh[kResolveThreads / kWave + 1];
-    const int t = order[blockIdx.x];
+    const int4 ord = order[blockIdx.x];
+    const int t = ord.x, r0 = ord.y, r1 = ord.z;
+    // the first chunk's run is loaded before the clear of the LDS bitset
+    BinRun next{0, 0};
+    if (r0 + static_cast<int>(threadIdx.x) < r1) next = runs[r0 + threadIdx.x];
     const int side = 1 << g.ts_log2;
     const int words = side * side * S;
     for (int i = threadIdx.x; i < words; i += kResolveThreads) tile[i] = 0u;
     __syncthreads();  // a tile without runs goes straight to the write-out
-    const int r0 = offs[t * kBinGroups], r1 = offs[(t + 1) * kBinGroups];
     const unsigned lmask = (1u << g.local_bits) - 1u;
     const unsigned kmax = static_cast<unsigned>(S) * 32u;
     // LDS word of (pixel l, word j) is l*S + (j ^ (l & smask)): with S a
@@ -155,8 +158,8 @@ tile_resolve_kernel(TileGrid g, const int* __restrict__ order, const int* __rest
     // consecutive entries of a run and kResolveUnroll loads are in flight.
     for (int rb = r0; rb < r1; rb += kResolveThreads) {
         const int r = rb + static_cast<int>(threadIdx.x);
-        BinRun run{0, 0};
-        if (r < r1) run = runs[r];
+        BinRun run = next;
+        if (rb != r0) run = r < r1 ? runs[r] : BinRun{0, 0};
         const int incl = wave_incl_scan(run.n);
         if (lane_id() == kWave - 1) sh[w] = incl;
         __syncthreads();
@@ -241,8 +244,8 @@ tile_resolve_kernel(TileGrid g, const int* __restrict__ order, const int* __rest
 
 size_t bin_plan_ints(const TileGrid& g, long long blocks) {
     const long long n = g.nt * kBinGroups;
-    return static_cast<size_t>(scan_scratch_ints(n + 1) + 4 + kBinCounters + (n + 2) + blocks +
-                               g.nt + 4);
+    return static_cast<size_t>(scan_scratch_ints(n + 1) + 4 + 4 * g.nt + kBinCounters + (n + 2) +
+                               blocks + 4);
 }
 
 void bin_carve(BinPlan& p, int* ints) {
@@ -250,10 +253,11 @@ void bin_carve(BinPlan& p, int* ints) {
     p.scratch = ints;  // int64 values first: 8-byte aligned
     long long off = static_cast<long long>(scan_scratch_ints(n + 1));
     off = (off + 3) & ~3ll;
+    p.order = reinterpret_cast<int4*>(ints + off);  // 16-byte aligned
+    off += 4 * p.g.nt;
     p.counters = ints + off;  // counters then run counts: one contiguous clear
     p.runs_per_slot = ints + off + kBinCounters;
     p.block_runs = ints + off + kBinCounters + n + 2;
-    p.order = p.block_runs + p.blocks;
 }
 
 int bin_clear(const BinPlan& p, hipStream_t s) {
@@ -287,16 +291,16 @@ int bin_resolve(const BinPlan& p, int H, int W, int S, int* bitset, hipStream_t 
     const unsigned grid_t = static_cast<unsigned>(p.g.nt);
     if (vec && small)
         tile_resolve_kernel<true, kTileLdsBytes><<<grid_t, kResolveThreads, 0, s>>>(
-            p.g, p.order, p.runs_per_slot, p.runs, p.entries, H, W, S, bitset);
+            p.g, p.order, p.runs, p.entries, H, W, S, bitset);
     else if (vec)
         tile_resolve_kernel<true, kTileLdsMax><<<grid_t, kResolveThreads, 0, s>>>(
-            p.g, p.order, p.runs_per_slot, p.runs, p.entries, H, W, S, bitset);
+            p.g, p.order, p.runs, p.entries, H, W, S, bitset);
     else if (small)
         tile_resolve_kernel<false, kTileLdsBytes><<<grid_t, kResolveThreads, 0, s>>>(
-            p.g, p.order, p.runs_per_slot, p.runs, p.entries, H, W, S, bitset);
+            p.g, p.order, p.runs, p.entries, H, W, S, bitset);
     else
         tile_resolve_kernel<false, kTileLdsMax><<<grid_t, kResolveThreads, 0, s>>>(
-            p.g, p.order, p.runs_per_slot, p.runs, p.entries, H, W, S, bitset);
+            p.g, p.order, p.runs, p.entries, H, W, S, bitset);
     S2C_LAUNCH_CHECK();
     return S2C_OK;
 }

@@ -158,7 +158,7 @@ struct BinPlan {
     int* runs_per_slot = nullptr;  // nt*kBinGroups + 1 ints, shifted by one; offsets after the scan
     int* counters = nullptr;       // kBinCounters ints
     int* block_runs = nullptr;     // runs in each workgroup's fixed record region
-    int* order = nullptr;          // nt ints: resolve order of the tiles
+    int4* order = nullptr;         // nt: resolve order (tile, first run, run end)
     int* scratch = nullptr;        // scan scratch
     unsigned* entries = nullptr;   // blocks*kStage fixed + ovf_entries
     BinRecord* records = nullptr;  // blocks*kBinRecs fixed + ovf_runs
