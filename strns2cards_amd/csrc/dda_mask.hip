@@ -147,14 +147,15 @@ dda_mask_direct_kernel(long long P, int H, int W, int V, const float* __restrict
 // with one LDS atomic per run of same-tile fragments.  After the walk the
 // slots' counts are scanned into offsets within the workgroup's fixed
 // entry region, a run record per (workgroup, tile) pair goes to its fixed
-// record region at an index from a block scan, and each thread copies its
-// column out of registers, one LDS atomic reserving each run's span.  A
-// workgroup whose table overflows (> kHash distinct tiles) makes one run
-// per fragment in the overflow areas; one whose thread has more fragments
-// than its stage share takes its region from the overflow area and walks
-// again to place them.  Nothing is written past the
-// overflow capacities; their use is counted regardless, so the host can
-// size them and launch again.  (choose_tiles keeps entries below 2^22.)
+// record region at an index from a block scan, each thread puts its column
+// back into the stage in slot order (one LDS atomic reserving each run's
+// span), and the workgroup copies the stage to its region with coalesced
+// stores.  A workgroup whose table overflows (> kHash distinct tiles) makes
+// one run per fragment in the overflow areas; one whose thread has more
+// fragments than its stage share takes its region from the overflow area
+// and walks again to place them.  Nothing is written past the overflow
+// capacities; their use is counted regardless, so the host can size them
+// and launch again.  (choose_tiles keeps entries below 2^22.)
 constexpr int kPerThread = kStage / kBlock;
 constexpr unsigned kEntryMask = (1u << 22) - 1u;
 
@@ -274,13 +275,15 @@ dda_bin_stage_kernel(long long P, int H, int W, int V, const float* __restrict__
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < per_thread; ++j) keys[threadIdx.x * per_thread + j] = 0;
-        __syncthreads();
-        // The thread's column in registers; each run of same-slot entries
-        // reserves its span at its last entry, and a backward pass writes
-        // every entry below its run's end.
+        // The thread's column in registers; after the barrier the stage is
+        // free.  Each run of same-slot entries reserves its span at its last
+        // entry, a backward pass puts every entry below its run's end in the
+        // stage, now in slot order, and the workgroup copies the stage to
+        // its region with coalesced 16-byte stores.
         unsigned v[kPerThread];
 #pragma unroll
         for (int i = 0; i < kPerThread; ++i) v[i] = i < mine ? stage[i * kBlock + threadIdx.x] : 0u;
+        __syncthreads();
         int end[kPerThread];
         int len = 0;
 #pragma unroll
@@ -291,7 +294,7 @@ dda_bin_stage_kernel(long long P, int H, int W, int V, const float* __restrict__
                 ++len;
                 const bool last = i + 1 == mine || (i + 1 < kPerThread && (v[i + 1] >> 22) != sl);
                 if (last) {  // end > 0: the run holds at least this entry
-                    end[i] = be + vals[sl] + atomicAdd(&keys[sl], len) + len;
+                    end[i] = vals[sl] + atomicAdd(&keys[sl], len) + len;
                     len = 0;
                 }
             }
@@ -301,9 +304,15 @@ dda_bin_stage_kernel(long long P, int H, int W, int V, const float* __restrict__
         for (int i = kPerThread - 1; i >= 0; --i) {
             if (i < mine) {
                 if (end[i]) at = end[i];
-                entries[--at] = v[i] & kEntryMask;
+                stage[--at] = v[i] & kEntryMask;
             }
         }
+        __syncthreads();
+        // total <= kStage here; the region is kStage entries from a 16-byte
+        // aligned start (be is a multiple of kStage)
+        uint4* dst = reinterpret_cast<uint4*>(entries + be);
+        const uint4* src = reinterpret_cast<const uint4*>(stage);
+        for (int q = threadIdx.x; q < (total + 3) / 4; q += kBlock) dst[q] = src[q];
         return;
     }
     // more fragments than the stage holds: walk again and place directly;
