@@ -7,7 +7,7 @@ re-exported at the top level.  All GPU work runs in hand-written HIP kernels
 in ``libstrns2cards_hip.so`` (C ABI: include/strns2cards_hip.h), reached
 through the ``_core`` extension; there is no CPU fallback, and importing the
 package fails if the native libraries are not built
-(``python -m strns2cards_amd.build``).
+(``python strns2cards_amd/build.py``).
 """
 import os as _os
 
@@ -20,7 +20,7 @@ try:
 except ImportError as _e:  # pragma: no cover - exercised only on broken installs
     raise ImportError(
         "strns2cards_amd: native extension not built or not loadable "
-        f"({_e}); run `python -m strns2cards_amd.build`") from _e
+        f"({_e}); run `python strns2cards_amd/build.py`") from _e
 
 from .adjoint import *  # noqa: E402,F401,F403
 from .antialias import *  # noqa: E402,F401,F403

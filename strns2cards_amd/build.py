@@ -4,7 +4,9 @@
   ``include/strns2cards_hip.h`` (hipcc, one object per ``csrc/*.hip``).
 * ``_core*.so`` — the PyTorch-ROCm pybind11 binding of that ABI (g++).
 
-Run ``python -m strns2cards_amd.build`` (or ``__graft_entry__.build()``).
+Run ``python strns2cards_amd/build.py`` (or ``__graft_entry__.build()``); it is
+loaded by path because the package itself does not import before these
+libraries exist.
 """
 from __future__ import annotations
 

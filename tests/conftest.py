@@ -27,8 +27,12 @@ def _ensure_built():
 
     core = glob.glob(os.path.join(ROOT, "strns2cards_amd", "_core*.so"))
     if not os.path.exists(lib) or not core:
-        from strns2cards_amd import build
+        import importlib.util
 
+        spec = importlib.util.spec_from_file_location(
+            "_s2c_build", os.path.join(ROOT, "strns2cards_amd", "build.py"))
+        build = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(build)
         build.build_all()
 
 
