@@ -67,7 +67,7 @@ __device__ __forceinline__ void walk_edges(long long P, int H, int W, int V,
 #pragma unroll
     for (int i = 0; i < kPrimsPerThread; ++i) {
         int e = e0 + i, b = b0;
-        if (e >= E) {  // at most one view boundary inside a thread's edges
+        while (e >= E) {  // views shorter than kPrimsPerThread edges wrap more than once
             e -= E;
             ++b;
         }

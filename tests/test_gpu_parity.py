@@ -151,6 +151,20 @@ def test_dda_mask_random(dev, K):
     np.testing.assert_array_equal(got, want)
 
 
+@pytest.mark.parametrize("B,E", [(37, 100), (9, 1), (5, 3), (3, 1023), (3, 1025)])
+def test_dda_mask_short_views(dev, B, E):
+    """Views with fewer edges than a workgroup walks (1,024): one workgroup
+    spans several views, including views of one edge, and the view changes
+    fall at and around workgroup boundaries."""
+    report(f"dda->mask short views B={B} E={E}")
+    rng = np.random.default_rng(B * 7919 + E)
+    pos, edges = random_lines(rng, B, 50, E, 40, 52, persp=True)
+    e2c = rng.integers(0, 70, size=E).astype(np.int32)
+    want = oracle.dda_cluster_mask((40, 52), pos, edges, e2c, 70)
+    got = fused_dda_mask(dev, (40, 52), pos, edges, e2c, 70)
+    np.testing.assert_array_equal(got, want)
+
+
 def test_dda_mask_long_scattered_edges(dev):
     """Long random edges over 1024^2: deferred (workgroup-walked) remainders,
     and workgroups touching more distinct tiles than their LDS table holds
