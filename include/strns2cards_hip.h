@@ -143,6 +143,20 @@ int s2c_backward_antialiased_cluster_bitset(
     const int* edge2cluster, float* d_pos, float kernel_radius, float rho,
     void* stream);
 
+/* s2c_backward_antialiased_cluster_bitset given the bitsets' view count B
+ * (num_views; 0 = unknown).  No reference launcher corresponds: the
+ * reference's takes no B.  With it, and num_slots >= 4, the residual bits
+ * are read from word planes (B, S, H, W) of the two bitsets built by a
+ * coalesced transpose into a temporary of B*H*W*S*8 bytes, instead of one
+ * word from each pixel's S-word record; fragments of views >= B add
+ * nothing.  Same result, up to the order of the d_pos atomics. */
+int s2c_backward_antialiased_cluster_bitset_views(
+    int num_views, int num_frags, const int* frag_pix, const float* frag_attrs_dda,
+    int H, int W, int num_slots, const int* bitset, const int* target_bitset, int V,
+    const float* pos, int E, const int* edges, const float* normals,
+    const int* edge2cluster, float* d_pos, float kernel_radius, float rho,
+    void* stream);
+
 /* compact_utils.cu:54-77 (compact_valid_ints), stable order. */
 int s2c_compact_valid_ints(int N, const int* input, int* output,
                            int* valid_count_out, void* stream);

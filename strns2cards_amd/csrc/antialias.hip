@@ -43,16 +43,62 @@ __device__ __forceinline__ unsigned bit_u(const int* bs, long long pix, int S, i
     return (static_cast<unsigned>(bs[pix * S + (k >> 5)]) >> (k & 31)) & 1u;
 }
 
-// Residual bits from the (B,H,W,S) bitsets, or from a per-wave LDS box.
+// Word planes of the two bitsets: planes[(b*S + j)*H*W + h*W + w] =
+// (bitset[b,h,w,j], target[b,h,w,j]).  With S words per pixel a window
+// read from the (B,H,W,S) layout takes one word of each S*4-byte pixel
+// record, so a window row touches a line per pixel; in word planes the
+// row is contiguous.  One workgroup transposes kPlanePix consecutive
+// pixels: coalesced reads of their records, coalesced plane-row writes.
+constexpr int kPlanePix = 64;
+constexpr int kPlaneMinSlots = 4;  // below this the records are already dense
+
+__global__ void __launch_bounds__(kBlock)
+residual_planes_kernel(long long NP, long long HW, int S, const int* __restrict__ bitset,
+                       const int* __restrict__ target, int2* __restrict__ planes) {
+    __shared__ int2 tile[kPlanePix][33];
+    const long long p0 = static_cast<long long>(blockIdx.x) * kPlanePix;
+    const int np = static_cast<int>(dmin(static_cast<long long>(kPlanePix), NP - p0));
+    for (int j0 = 0; j0 < S; j0 += 32) {
+        const int nj = dmin(32, S - j0);
+        for (int i = threadIdx.x; i < np * nj; i += kBlock) {
+            const int pl = i / nj, j = i - pl * nj;
+            const long long src = (p0 + pl) * S + j0 + j;
+            tile[pl][j] = make_int2(bitset[src], target[src]);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < np * nj; i += kBlock) {
+            const int j = i / np, pl = i - j * np;
+            const long long p = p0 + pl;
+            const long long b = p / HW, hw = p - b * HW;
+            planes[(b * S + j0 + j) * HW + hw] = tile[pl][j];
+        }
+        __syncthreads();
+    }
+}
+
+// Residual bits (bit 0 = rendered, bit 1 = target) of cluster k at a pixel
+// of the plane starting at `base` (plane of word k / 32).
+__device__ __forceinline__ unsigned plane_code(const int2* planes, long long base, int W, int h,
+                                               int w, int k) {
+    const int2 v = planes[base + w + static_cast<long long>(h) * W];
+    const int kb = k & 31;
+    return ((static_cast<unsigned>(v.x) >> kb) & 1u) |
+           (((static_cast<unsigned>(v.y) >> kb) & 1u) << 1);
+}
+
+// Residual bits from the (B,H,W,S) bitsets or their word planes, or from a
+// per-wave LDS box.
 struct ResidualSrc {
     const unsigned char* box;  // staged bytes, or nullptr for global reads
     int h0, w0, width;
     const int* bitset;
     const int* target;
-    long long plane;
+    const int2* planes;  // word planes, or nullptr to read the records
+    long long plane;     // first pixel of the view / first entry of the word plane
     int W, S, k;
     __device__ __forceinline__ unsigned operator()(int h, int w) const {
         if (box) return box[(h - h0) * width + (w - w0)];
+        if (planes) return plane_code(planes, plane, W, h, w, k);
         const long long pix = plane + w + static_cast<long long>(h) * W;
         return bit_u(bitset, pix, S, k) | (bit_u(target, pix, S, k) << 1);
     }
@@ -62,17 +108,20 @@ struct ResidualSrc {
 // its live samples share one view and cluster (consecutive silhouette
 // fragments of one card) and the union of their windows fits kAaBox bytes,
 // the wave stages that box's residual bits in LDS once; otherwise every
-// window pixel is read from the bitsets directly.
+// window pixel is read from the bitsets directly.  PLANES: the bits come
+// from the word planes of B views (residual_planes_kernel); fragments of
+// views >= B are dropped.
 constexpr int kAaWaves = kBlock / kWave;
 constexpr int kAaBox = 8192;
 
+template <bool PLANES>
 __global__ void __launch_bounds__(kBlock)
 antialias_bwd_kernel(int N, const int* __restrict__ frag_pix, const float* __restrict__ fa,
                      int H, int W, int S, const int* __restrict__ bitset,
                      const int* __restrict__ target, int V, const float* __restrict__ pos, int E,
                      const int* __restrict__ edges, const float* __restrict__ normals,
                      const int* __restrict__ edge2cluster, float* __restrict__ d_pos, float R,
-                     float rho) {
+                     float rho, int B, const int2* __restrict__ planes) {
     __shared__ unsigned char box[kAaWaves][kAaBox];
     const int wv = threadIdx.x / kWave;
     const long long i = (static_cast<long long>(blockIdx.x) * kAaWaves + wv) * kWave + lane_id();
@@ -96,7 +145,7 @@ antialias_bwd_kernel(int N, const int* __restrict__ frag_pix, const float* __res
             smp = aa_sample(H, W, h, w, a.x, a.y, p[v0], p[v1], n);
             // (h, w) outside the image only arises from a malformed fragment
             // list; such samples are dropped instead of read out of bounds.
-            live = smp.live && h >= 0 && h < H && w >= 0 && w < W && b >= 0;
+            live = smp.live && h >= 0 && h < H && w >= 0 && w < W && b >= 0 && (!PLANES || b < B);
             if (live) k = edge2cluster[e];
         }
     }
@@ -125,11 +174,18 @@ antialias_bwd_kernel(int N, const int* __restrict__ frag_pix, const float* __res
                 bwid = static_cast<int>(ww);
                 const int area = static_cast<int>(hh * ww);
                 const long long plane = static_cast<long long>(H) * W * b_ref;
+                const long long pbase = (static_cast<long long>(b_ref) * S + (k_ref >> 5)) *
+                                        (static_cast<long long>(H) * W);
                 for (int q = lane_id(); q < area; q += kWave) {
                     const int y = bh0 + q / bwid, x = bw0 + q % bwid;  // in [0,H) x [0,W)
-                    const long long pix = plane + x + static_cast<long long>(y) * W;
-                    box[wv][q] = static_cast<unsigned char>(bit_u(bitset, pix, S, k_ref) |
-                                                            (bit_u(target, pix, S, k_ref) << 1));
+                    if (PLANES) {
+                        box[wv][q] = static_cast<unsigned char>(
+                            plane_code(planes, pbase, W, y, x, k_ref));
+                    } else {
+                        const long long pix = plane + x + static_cast<long long>(y) * W;
+                        box[wv][q] = static_cast<unsigned char>(
+                            bit_u(bitset, pix, S, k_ref) | (bit_u(target, pix, S, k_ref) << 1));
+                    }
                 }
             }
         }
@@ -137,8 +193,11 @@ antialias_bwd_kernel(int N, const int* __restrict__ frag_pix, const float* __res
     __syncthreads();
     if (!live) return;
 
+    const long long hw = static_cast<long long>(H) * W;
     const ResidualSrc src{staged ? box[wv] : nullptr, bh0, bw0, bwid, bitset, target,
-                          static_cast<long long>(H) * W * b, W, S, k};
+                          PLANES ? planes : nullptr,
+                          PLANES ? (static_cast<long long>(b) * S + (k >> 5)) * hw : hw * b, W, S,
+                          k};
     float dpx, dpy, dpw;
     if (!aa_sample_grad(smp, H, W, R, rho, n, src, dpx, dpy, dpw)) return;
     float* dp = d_pos + static_cast<long long>(b) * V * 4;
@@ -175,13 +234,37 @@ extern "C" int s2c_backward_antialiased_cluster_bitset(
     const int* bitset, const int* target_bitset, int V, const float* pos, int E,
     const int* edges, const float* normals, const int* edge2cluster, float* d_pos,
     float kernel_radius, float rho, void* stream) {
-    S2C_CHECK(num_frags >= 0 && H >= 0 && W >= 0 && num_slots >= 0 && V >= 0 && E >= 0,
+    return s2c_backward_antialiased_cluster_bitset_views(
+        0, num_frags, frag_pix, frag_attrs_dda, H, W, num_slots, bitset, target_bitset, V, pos,
+        E, edges, normals, edge2cluster, d_pos, kernel_radius, rho, stream);
+}
+
+extern "C" int s2c_backward_antialiased_cluster_bitset_views(
+    int num_views, int num_frags, const int* frag_pix, const float* frag_attrs_dda, int H, int W,
+    int num_slots, const int* bitset, const int* target_bitset, int V, const float* pos, int E,
+    const int* edges, const float* normals, const int* edge2cluster, float* d_pos,
+    float kernel_radius, float rho, void* stream) {
+    S2C_CHECK(num_views >= 0 && num_frags >= 0 && H >= 0 && W >= 0 && num_slots >= 0 && V >= 0 &&
+                  E >= 0,
               "backward_antialiased_cluster_bitset: negative size");
     if (num_frags == 0) return S2C_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    antialias_bwd_kernel<<<grid_for(num_frags, kBlock), kBlock, 0, s>>>(
+    const long long np = static_cast<long long>(num_views) * H * W;
+    if (num_views == 0 || num_slots < kPlaneMinSlots || np == 0) {
+        antialias_bwd_kernel<false><<<grid_for(num_frags, kBlock), kBlock, 0, s>>>(
+            num_frags, frag_pix, frag_attrs_dda, H, W, num_slots, bitset, target_bitset, V, pos,
+            E, edges, normals, edge2cluster, d_pos, kernel_radius, rho, 0, nullptr);
+        S2C_LAUNCH_CHECK();
+        return S2C_OK;
+    }
+    Temp planes;
+    if (int r = planes.alloc(sizeof(int2) * static_cast<size_t>(np) * num_slots, s)) return r;
+    residual_planes_kernel<<<grid_for(np, kPlanePix), kBlock, 0, s>>>(
+        np, static_cast<long long>(H) * W, num_slots, bitset, target_bitset, planes.as<int2>());
+    S2C_LAUNCH_CHECK();
+    antialias_bwd_kernel<true><<<grid_for(num_frags, kBlock), kBlock, 0, s>>>(
         num_frags, frag_pix, frag_attrs_dda, H, W, num_slots, bitset, target_bitset, V, pos, E,
-        edges, normals, edge2cluster, d_pos, kernel_radius, rho);
+        edges, normals, edge2cluster, d_pos, kernel_radius, rho, num_views, planes.as<int2>());
     S2C_LAUNCH_CHECK();
     return S2C_OK;
 }

@@ -423,8 +423,8 @@ PYBIND11_MODULE(_core, m) {
                   throw std::runtime_error(
                       "backward_antialiased_bitset: d_pos must match pos shape.");
               DeviceGuard g(pos.device());
-              check(s2c_backward_antialiased_cluster_bitset(
-                        N, P<int>(fp), P<float>(fa), H, W, S, P<int>(bitset), P<int>(target), V,
+              check(s2c_backward_antialiased_cluster_bitset_views(
+                        B, N, P<int>(fp), P<float>(fa), H, W, S, P<int>(bitset), P<int>(target), V,
                         P<float>(pos), E, P<int>(edges), P<float>(normals), P<int>(edge2cluster),
                         P<float>(d_pos), kernel_radius, rho, stream_of(pos)),
                     fn);

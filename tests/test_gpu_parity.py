@@ -318,6 +318,45 @@ def test_triangles_card_workload(dev):
     np.testing.assert_array_equal(bits.cpu().numpy(), want)
 
 
+def fused_tri_mask(dev, res, pos, tri, t2c, K, fill=-1):
+    """s2c_triangle_cluster_mask into a bitset prefilled with `fill` (the
+    tiled path writes every word; the direct path clears first)."""
+    B, V, _ = pos.shape
+    S = (K + 31) // 32
+    bits = torch.full((B, *res, S), fill, dtype=torch.int32, device=dev)
+    call("s2c_triangle_cluster_mask", B, res[0], res[1], V, T(pos, dev), tri.shape[0],
+         T(tri, dev), T(t2c, dev), S, bits)
+    return bits.cpu().numpy()
+
+
+# K=1..1024: 16x16 tiles; 2048: 8x8 tiles; 16384: 4x4 tiles; 17000: direct ORs
+@pytest.mark.parametrize("K", [1, 33, 1024, 2048, 16384, 17000])
+def test_triangle_mask_tiles(dev, K):
+    """Tiled triangle -> bitset against the oracle: three perspective views
+    of random triangles over a ragged 70 x 45 image (partial edge tiles),
+    with screen-filling, off-screen, degenerate and z-clipped triangles."""
+    report(f"triangle mask tiles K={K}")
+    rng = np.random.default_rng(K + 3)
+    pos, tri = random_tris(rng, 3, 90, 400, persp=True)
+    pos[:, :3] = np.array([[-3, -3, 0, 1], [3, -3, 0, 1], [0, 3, 0, 1]], np.float32)
+    tri[0] = (0, 1, 2)  # covers the whole screen: every tile lists it
+    tri[1] = (5, 5, 6)  # degenerate
+    t2c = rng.integers(0, K, size=tri.shape[0]).astype(np.int32)
+    want = oracle.triangle_cluster_mask((45, 70), pos, tri, t2c, K)
+    got = fused_tri_mask(dev, (45, 70), pos, tri, t2c, K)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.slow
+def test_triangle_mask_full_cards(dev):
+    report("triangle mask full workload cards (1024 x 20 quads, 4 views 1024^2, K=1024)")
+    w = synth.full_workload()
+    c = w.cards
+    want = oracle.triangle_cluster_mask((w.H, w.W), w.card_pos, c.tri, c.tri2cluster, 1024)
+    got = fused_tri_mask(dev, (w.H, w.W), w.card_pos, c.tri, c.tri2cluster, 1024)
+    np.testing.assert_array_equal(got, want)
+
+
 # ---------------------------------------------------------------- bitset
 
 
@@ -396,6 +435,44 @@ def test_antialias_shuffled_edges(dev):
     prim, normals = oracle.mark_discontinuity_edges(res, w.card_pos, c.tri, edges, e2t)
     fp, fa = oracle.dda_compute_fragments(res, w.card_pos, edges, prim)
     check_antialias_backward(dev, w, cards, target, fp, fa, T(normals, dev), edges, e2t, e2c)
+
+
+@pytest.mark.parametrize("mult", [2, 16, 17])  # S = 4, 32, 34 words per pixel
+def test_antialias_word_planes(dev, mult):
+    """Antialias backward through the entry point that is given the view
+    count, which reads the residual bits from word planes (a transpose of
+    both bitsets) when S >= 4, and through the reference-shaped one, which
+    reads the pixel records.  Two views of the small workload (the second
+    shifted in clip space); cluster ids spread as mult*k + 1 so S = 4, 32
+    and 34 (a partial second chunk of 32 words)."""
+    report(f"antialias backward word planes mult={mult}")
+    w = synth.small_workload()
+    c = w.cards
+    res = (w.H, w.W)
+    shift = np.array([0.05, -0.03, 0.0, 0.0], np.float32)
+    pos2 = np.concatenate([w.pos, w.pos + shift * w.pos[..., 3:4]], 0)
+    cpos2 = np.concatenate([w.card_pos, w.card_pos + shift * w.card_pos[..., 3:4]], 0)
+    K = 63 * mult + 2
+    S = (K + 31) // 32
+    target = oracle.dda_cluster_mask(res, pos2, w.edges, (w.edge2cluster * mult + 1).astype(np.int32), K)
+    cards = oracle.triangle_cluster_mask(res, cpos2, c.tri, (c.tri2cluster * mult + 1).astype(np.int32), K)
+    e2c = (c.edge2cluster * mult + 1).astype(np.int32)
+    prim, normals = oracle.mark_discontinuity_edges(res, cpos2, c.tri, c.edges, c.edge2tri)
+    fp, fa = oracle.dda_compute_fragments(res, cpos2, c.edges, prim)
+    B, V, _ = cpos2.shape
+    E = c.edges.shape[0]
+    args = (fp.shape[0], T(fp, dev), T(fa, dev), res[0], res[1], S, T(cards, dev), T(target, dev), V,
+            T(cpos2, dev), E, T(c.edges, dev), T(normals, dev), T(e2c, dev))
+    for rho in (1.0, 0.5):
+        want = oracle.antialiased_cluster_bitset_grad(cards, target, cpos2, c.tri, c.edges,
+                                                      c.edge2tri, e2c, 7.0, rho)
+        assert np.abs(want[1]).max() > 0  # the second view contributes
+        d_planes = torch.zeros((B, V, 4), device=dev)
+        call("s2c_backward_antialiased_cluster_bitset_views", B, *args, d_planes, 7.0, rho)
+        normwise_close(d_planes.cpu().numpy(), want)
+        d_records = torch.zeros((B, V, 4), device=dev)
+        call("s2c_backward_antialiased_cluster_bitset", *args, d_records, 7.0, rho)
+        normwise_close(d_records.cpu().numpy(), want)
 
 
 # ---------------------------------------------------------------- curves
