@@ -125,15 +125,55 @@ __device__ __forceinline__ float aa_adjoint_sum(float x, float y, int H, int W, 
     return dc;
 }
 
-// antialias.cu:256-270: from the gated sum to the clip-space gradient of
-// the sample point.  Returns false when nothing is to be added.
-template <class Code>
-__device__ __forceinline__ bool aa_sample_grad(const AaSample& s, int H, int W, float R, float rho,
-                                               float2 n, const Code& code, float& dpx, float& dpy,
-                                               float& dpw) {
-    const float c01 = 1.f - static_cast<float>(code(s.ph, s.pw_) & 1u);
-    if (c01 == 0.f) return false;  // dc = 0 * sum (antialias.cu:261-262)
-    const float dc = c01 * aa_adjoint_sum(s.fx, s.fy, H, W, R, rho, code);
+// Widest window row aa_adjoint_sum_rows takes (2 * floor(R) + 1 pixels).
+constexpr int kAaRowMax = 32;
+
+// aa_adjoint_sum for residuals read a window row at a time:
+// rows(h, w0, n, diff, tgt) gives, for pixels w0 .. w0 + n - 1 of row h
+// (1 <= n <= kAaRowMax), the bits where the rendered and target bits differ
+// and the target bits, bit i for pixel w0 + i.  Rows without a difference
+// are skipped whole and only the differing pixels are visited, by
+// increasing w within increasing h: the terms aa_adjoint_sum adds (all
+// others are exact zeros there), in the same order, with the same
+// arithmetic, so the same sum.  The caller ensures 2 * floor(R) + 1 <=
+// kAaRowMax.
+template <class Rows>
+__device__ __forceinline__ float aa_adjoint_sum_rows(float x, float y, int H, int W, float R,
+                                                     float rho, const Rows& rows) {
+    constexpr float pi = 3.14159265358979323846f;
+    const float sigma = R / 4.f;
+    const float c = 1.f / (sigma * sigma * 2.f * pi);
+    const float alpha = 0.5f / (sigma * sigma);
+    const float R2 = R * R;
+    const int Ri = f2i(floorf(R));
+    const int cw = f2i(floorf(x)), ch = f2i(floorf(y));
+    const int h0 = dmax(ch - Ri, 0), h1 = dmin(ch + Ri, H - 1) + 1;
+    const int w0 = dmax(cw - Ri, 0), w1 = dmin(cw + Ri, W - 1) + 1;
+    const int n = w1 - w0;
+    float dc = 0.f;
+    if (n <= 0) return dc;
+    for (int h = h0; h < h1; ++h) {
+        unsigned diff, tgt;
+        rows(h, w0, n, diff, tgt);
+        if (diff == 0u) continue;
+        const float dy = static_cast<float>(h) + 0.5f - y;
+        while (diff != 0u) {
+            const int i = __builtin_ctz(diff);
+            diff &= diff - 1u;
+            const float dx = static_cast<float>(w0 + i) + 0.5f - x;
+            const float n2 = dx * dx + dy * dy;
+            if (!(n2 < R2)) continue;
+            const unsigned c1 = (tgt >> i) & 1u, c0 = c1 ^ 1u;
+            const float wt = c * expf(-alpha * n2) * 1.f;
+            dc += wt * (static_cast<float>(c0) - static_cast<float>(c1)) * (c1 == 0u ? rho : 1.f);
+        }
+    }
+    return dc;
+}
+
+// antialias.cu:264-269 for a gated sum dc.
+__device__ __forceinline__ bool aa_clip_grad(const AaSample& s, int H, int W, float2 n, float dc,
+                                             float& dpx, float& dpy, float& dpw) {
     if (dc == 0.f) return false;
     const float fH = static_cast<float>(H), fW = static_cast<float>(W);
     const float dx = dc * 0.5f * fW * n.x;
@@ -144,4 +184,30 @@ __device__ __forceinline__ bool aa_sample_grad(const AaSample& s, int H, int W, 
     return isfinite(dpx) && isfinite(dpy) && isfinite(dpw);
 }
 
+// antialias.cu:256-270: from the gated sum to the clip-space gradient of
+// the sample point.  Returns false when nothing is to be added.
+template <class Code>
+__device__ __forceinline__ bool aa_sample_grad(const AaSample& s, int H, int W, float R, float rho,
+                                               float2 n, const Code& code, float& dpx, float& dpy,
+                                               float& dpw) {
+    const float c01 = 1.f - static_cast<float>(code(s.ph, s.pw_) & 1u);
+    if (c01 == 0.f) return false;  // dc = 0 * sum (antialias.cu:261-262)
+    const float dc = c01 * aa_adjoint_sum(s.fx, s.fy, H, W, R, rho, code);
+    return aa_clip_grad(s, H, W, n, dc, dpx, dpy, dpw);
+}
+
+// aa_sample_grad with the window read a row at a time (aa_adjoint_sum_rows)
+// when it is at most kAaRowMax pixels wide; the same result.
+template <class Code, class Rows>
+__device__ __forceinline__ bool aa_sample_grad_rows(const AaSample& s, int H, int W, float R,
+                                                    float rho, float2 n, const Code& code,
+                                                    const Rows& rows, float& dpx, float& dpy,
+                                                    float& dpw) {
+    const float c01 = 1.f - static_cast<float>(code(s.ph, s.pw_) & 1u);
+    if (c01 == 0.f) return false;  // dc = 0 * sum (antialias.cu:261-262)
+    const float sum = 2 * f2i(floorf(R)) + 1 <= kAaRowMax
+                          ? aa_adjoint_sum_rows(s.fx, s.fy, H, W, R, rho, rows)
+                          : aa_adjoint_sum(s.fx, s.fy, H, W, R, rho, code);
+    return aa_clip_grad(s, H, W, n, c01 * sum, dpx, dpy, dpw);
+}
 }  // namespace s2c

@@ -84,6 +84,28 @@ struct FitCode {
     }
 };
 
+// Bits p0 .. p0 + n - 1 (1 <= n <= 32) of a packed mask, bit i for p0 + i.
+__device__ __forceinline__ unsigned get_bits(const unsigned* bits, int p0, int n) {
+    const int wi = p0 >> 5, sh = p0 & 31;
+    unsigned long long v = bits[wi];
+    if (sh + n > 32) v |= static_cast<unsigned long long>(bits[wi + 1]) << 32;
+    v >>= sh;
+    return n == 32 ? static_cast<unsigned>(v) : static_cast<unsigned>(v) & ((1u << n) - 1u);
+}
+
+// Window rows of the job's masks for aa_adjoint_sum_rows.
+struct FitRows {
+    const unsigned* render;
+    const unsigned* target;
+    int res;
+    __device__ __forceinline__ void operator()(int h, int w0, int n, unsigned& diff,
+                                               unsigned& tgt) const {
+        const int p0 = h * res + w0;
+        tgt = get_bits(target, p0, n);
+        diff = get_bits(render, p0, n) ^ tgt;
+    }
+};
+
 // Rasterize the strip into `render` (cleared by the caller): every covered
 // pixel of every triangle (rasterize.cu:90-172) sets its bit.
 __device__ __forceinline__ void fit_raster(int res, int Q, const float4* clip, unsigned* render,
@@ -179,6 +201,7 @@ fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict
     }
     int step = steps[job];
     const FitCode code{render, target, res};
+    const FitRows rows{render, target, res};
     __syncthreads();
 
     for (int it = 0; it <= iters; ++it) {
@@ -243,7 +266,8 @@ fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict
                 if (dda_frag(res, res, sp, j - eexcl[lo], P0, P1, fr)) {
                     const AaSample s = aa_sample(res, res, fr.h, fr.w, sp.axis, fr.t1, P0, P1, n);
                     float dpx, dpy, dpw;
-                    if (s.live && aa_sample_grad(s, res, res, hp.R, hp.rho, n, code, dpx, dpy, dpw)) {
+                    if (s.live &&
+                        aa_sample_grad_rows(s, res, res, hp.R, hp.rho, n, code, rows, dpx, dpy, dpw)) {
                         c[0] = dpx * s.t0; c[1] = dpy * s.t0; c[2] = dpw * s.t0;
                         c[3] = dpx * s.t1; c[4] = dpy * s.t1; c[5] = dpw * s.t1;
                     }
@@ -282,16 +306,31 @@ fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict
         // ---- per-vertex gradient and UniformAdam (optimizer.py:33-68)
         float gx = 0.f, gy = 0.f, gz = 0.f, rms = 0.f;
         if (tid < V) {
-            float cx = 0.f, cy = 0.f, cw = 0.f;  // d loss / d clip (x, y, w)
-            for (int e = 0; e < E; ++e) {
-                const StripEdge se = strip_edge(e, Q);
-                if (se.v0 == tid) { cx += acc[e][0]; cy += acc[e][1]; cw += acc[e][2]; }
-                if (se.v1 == tid) { cx += acc[e][3]; cy += acc[e][4]; cw += acc[e][5]; }
+            // d loss / d clip (x, y, w): the vertex's incident edges (at most
+            // four; strip_edge's numbering) in increasing edge index, each
+            // adding its end's sums -- the order of a scan over all edges
+            const int r = tid / N, c = tid - r * N;
+            float cx = 0.f, cy = 0.f, cw = 0.f;
+            // edge e's sums for its end k (0: v0, 3: v1)
+            auto add = [&](int e, int k) {
+                cx += acc[e][k + 0];
+                cy += acc[e][k + 1];
+                cw += acc[e][k + 2];
+            };
+            if (r == 0) {
+                add(c, 0);                              // width edge, v0
+                if (c > 0) add(N + c - 1, 3);           // row 0 left, v1
+                if (c < Q) add(N + c, 0);               // row 0 right, v0
+                if (c < Q) add(N + 2 * Q + c, 0);       // diagonal, v0
+            } else {
+                add(c, 3);                              // width edge, v1
+                if (c > 0) add(N + Q + c - 1, 3);       // row 1 left, v1
+                if (c < Q) add(N + Q + c, 0);           // row 1 right, v0
+                if (c > 0) add(N + 2 * Q + c - 1, 3);   // diagonal, v1
             }
             gx = cx * M[0] + cy * M[4] + cw * M[12];
             gy = cx * M[1] + cy * M[5] + cw * M[13];
             gz = cx * M[2] + cy * M[6] + cw * M[14];
-            const int r = tid / N, c = tid - r * N;
             // d/dp of w_lap * sum |L p|^2 = 2 w_lap * L (L p)
             {
                 float deg = 1.f;
