@@ -136,8 +136,10 @@ constexpr int kAaRowMax = 32;
 // increasing w within increasing h: the terms aa_adjoint_sum adds (all
 // others are exact zeros there), in the same order, with the same
 // arithmetic, so the same sum.  The caller ensures 2 * floor(R) + 1 <=
-// kAaRowMax.
-template <class Rows>
+// kAaRowMax.  FAST_EXP evaluates the Gaussian with __expf, as the
+// reference kernel does (antialias.cu:131), instead of expf (the CPU
+// oracle's function); the two differ by about an ulp.
+template <bool FAST_EXP, class Rows>
 __device__ __forceinline__ float aa_adjoint_sum_rows(float x, float y, int H, int W, float R,
                                                      float rho, const Rows& rows) {
     constexpr float pi = 3.14159265358979323846f;
@@ -164,7 +166,7 @@ __device__ __forceinline__ float aa_adjoint_sum_rows(float x, float y, int H, in
             const float n2 = dx * dx + dy * dy;
             if (!(n2 < R2)) continue;
             const unsigned c1 = (tgt >> i) & 1u, c0 = c1 ^ 1u;
-            const float wt = c * expf(-alpha * n2) * 1.f;
+            const float wt = c * (FAST_EXP ? __expf(-alpha * n2) : expf(-alpha * n2)) * 1.f;
             dc += wt * (static_cast<float>(c0) - static_cast<float>(c1)) * (c1 == 0u ? rho : 1.f);
         }
     }
@@ -197,8 +199,9 @@ __device__ __forceinline__ bool aa_sample_grad(const AaSample& s, int H, int W, 
 }
 
 // aa_sample_grad with the window read a row at a time (aa_adjoint_sum_rows)
-// when it is at most kAaRowMax pixels wide; the same result.
-template <class Code, class Rows>
+// when it is at most kAaRowMax pixels wide; the same result (with FAST_EXP,
+// up to the exponential's last bits).
+template <bool FAST_EXP, class Code, class Rows>
 __device__ __forceinline__ bool aa_sample_grad_rows(const AaSample& s, int H, int W, float R,
                                                     float rho, float2 n, const Code& code,
                                                     const Rows& rows, float& dpx, float& dpy,
@@ -206,7 +209,7 @@ __device__ __forceinline__ bool aa_sample_grad_rows(const AaSample& s, int H, in
     const float c01 = 1.f - static_cast<float>(code(s.ph, s.pw_) & 1u);
     if (c01 == 0.f) return false;  // dc = 0 * sum (antialias.cu:261-262)
     const float sum = 2 * f2i(floorf(R)) + 1 <= kAaRowMax
-                          ? aa_adjoint_sum_rows(s.fx, s.fy, H, W, R, rho, rows)
+                          ? aa_adjoint_sum_rows<FAST_EXP>(s.fx, s.fy, H, W, R, rho, rows)
                           : aa_adjoint_sum(s.fx, s.fy, H, W, R, rho, code);
     return aa_clip_grad(s, H, W, n, c01 * sum, dpx, dpy, dpw);
 }

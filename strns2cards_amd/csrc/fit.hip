@@ -267,7 +267,8 @@ fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict
                     const AaSample s = aa_sample(res, res, fr.h, fr.w, sp.axis, fr.t1, P0, P1, n);
                     float dpx, dpy, dpw;
                     if (s.live &&
-                        aa_sample_grad_rows(s, res, res, hp.R, hp.rho, n, code, rows, dpx, dpy, dpw)) {
+                        aa_sample_grad_rows<true>(s, res, res, hp.R, hp.rho, n, code, rows, dpx,
+                                                  dpy, dpw)) {
                         c[0] = dpx * s.t0; c[1] = dpy * s.t0; c[2] = dpw * s.t0;
                         c[3] = dpx * s.t1; c[4] = dpy * s.t1; c[5] = dpw * s.t1;
                     }
