@@ -104,6 +104,42 @@ struct ResidualSrc {
     }
 };
 
+// Adds (x, y, w) to components 0, 1, 3 of d_pos[key] for every live lane
+// of the wave, with one atomic per run of lanes that share a key: the
+// wave's consecutive samples lie on a few edges, so most lanes would
+// otherwise add into the same two vertices.  A run's values are summed by
+// a segmented wave scan; the run's last lane issues the atomics.  Every
+// lane of the wave must call it.
+__device__ __forceinline__ void wave_add_runs(float* d_pos, bool live, long long key, float x,
+                                              float y, float w) {
+    const int lane = lane_id();
+    if (!live) key = -1 - lane;  // a run of its own, never added
+    const long long prev = __shfl_up(key, 1, kWave);
+    int start = (lane == 0 || prev != key) ? lane : 0;  // first lane of the run
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int t = __shfl_up(start, d, kWave);
+        if (lane >= d) start = dmax(start, t);
+    }
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const float tx = __shfl_up(x, d, kWave), ty = __shfl_up(y, d, kWave),
+                    tw = __shfl_up(w, d, kWave);
+        if (lane - d >= start) {
+            x += tx;
+            y += ty;
+            w += tw;
+        }
+    }
+    const long long next = __shfl_down(key, 1, kWave);
+    if (live && (lane == kWave - 1 || next != key)) {
+        float* dp = d_pos + key * 4;
+        atomicAdd(&dp[0], x);
+        atomicAdd(&dp[1], y);
+        atomicAdd(&dp[3], w);
+    }
+}
+
 // antialias.cu:179-279.  A wave takes 64 consecutive fragments.  When all of
 // its live samples share one view and cluster (consecutive silhouette
 // fragments of one card) and the union of their windows fits kAaBox bytes,
@@ -191,22 +227,21 @@ antialias_bwd_kernel(int N, const int* __restrict__ frag_pix, const float* __res
         }
     }
     __syncthreads();
-    if (!live) return;
 
-    const long long hw = static_cast<long long>(H) * W;
-    const ResidualSrc src{staged ? box[wv] : nullptr, bh0, bw0, bwid, bitset, target,
-                          PLANES ? planes : nullptr,
-                          PLANES ? (static_cast<long long>(b) * S + (k >> 5)) * hw : hw * b, W, S,
-                          k};
-    float dpx, dpy, dpw;
-    if (!aa_sample_grad(smp, H, W, R, rho, n, src, dpx, dpy, dpw)) return;
-    float* dp = d_pos + static_cast<long long>(b) * V * 4;
-    atomicAdd(&dp[v0 * 4 + 0], dpx * smp.t0);
-    atomicAdd(&dp[v0 * 4 + 1], dpy * smp.t0);
-    atomicAdd(&dp[v0 * 4 + 3], dpw * smp.t0);
-    atomicAdd(&dp[v1 * 4 + 0], dpx * smp.t1);
-    atomicAdd(&dp[v1 * 4 + 1], dpy * smp.t1);
-    atomicAdd(&dp[v1 * 4 + 3], dpw * smp.t1);
+    // the sample's contributions to its two vertices (antialias.cu:272-277)
+    bool adds = false;
+    float dpx = 0.f, dpy = 0.f, dpw = 0.f;
+    if (live) {
+        const long long hw = static_cast<long long>(H) * W;
+        const ResidualSrc src{staged ? box[wv] : nullptr, bh0, bw0, bwid, bitset, target,
+                              PLANES ? planes : nullptr,
+                              PLANES ? (static_cast<long long>(b) * S + (k >> 5)) * hw : hw * b, W,
+                              S, k};
+        adds = aa_sample_grad(smp, H, W, R, rho, n, src, dpx, dpy, dpw);
+    }
+    const long long vb = static_cast<long long>(b) * V;
+    wave_add_runs(d_pos, adds, vb + v0, dpx * smp.t0, dpy * smp.t0, dpw * smp.t0);
+    wave_add_runs(d_pos, adds, vb + v1, dpx * smp.t1, dpy * smp.t1, dpw * smp.t1);
 }
 
 }  // namespace s2c

@@ -128,6 +128,16 @@ __device__ __forceinline__ float aa_adjoint_sum(float x, float y, int H, int W, 
 // Widest window row aa_adjoint_sum_rows takes (2 * floor(R) + 1 pixels).
 constexpr int kAaRowMax = 32;
 
+// Bits c .. c + n - 1 (1 <= n <= 32) of a packed bit row, bit i for c + i.
+// Reads the next word only when the range reaches into it.
+__device__ __forceinline__ unsigned bits_at(const unsigned* row, int c, int n) {
+    const int wi = c >> 5, sh = c & 31;
+    unsigned long long v = row[wi];
+    if (sh + n > 32) v |= static_cast<unsigned long long>(row[wi + 1]) << 32;
+    v >>= sh;
+    return n == 32 ? static_cast<unsigned>(v) : static_cast<unsigned>(v) & ((1u << n) - 1u);
+}
+
 // aa_adjoint_sum for residuals read a window row at a time:
 // rows(h, w0, n, diff, tgt) gives, for pixels w0 .. w0 + n - 1 of row h
 // (1 <= n <= kAaRowMax), the bits where the rendered and target bits differ

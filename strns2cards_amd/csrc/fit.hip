@@ -84,15 +84,6 @@ struct FitCode {
     }
 };
 
-// Bits p0 .. p0 + n - 1 (1 <= n <= 32) of a packed mask, bit i for p0 + i.
-__device__ __forceinline__ unsigned get_bits(const unsigned* bits, int p0, int n) {
-    const int wi = p0 >> 5, sh = p0 & 31;
-    unsigned long long v = bits[wi];
-    if (sh + n > 32) v |= static_cast<unsigned long long>(bits[wi + 1]) << 32;
-    v >>= sh;
-    return n == 32 ? static_cast<unsigned>(v) : static_cast<unsigned>(v) & ((1u << n) - 1u);
-}
-
 // Window rows of the job's masks for aa_adjoint_sum_rows.
 struct FitRows {
     const unsigned* render;
@@ -101,8 +92,8 @@ struct FitRows {
     __device__ __forceinline__ void operator()(int h, int w0, int n, unsigned& diff,
                                                unsigned& tgt) const {
         const int p0 = h * res + w0;
-        tgt = get_bits(target, p0, n);
-        diff = get_bits(render, p0, n) ^ tgt;
+        tgt = bits_at(target, p0, n);
+        diff = bits_at(render, p0, n) ^ tgt;
     }
 };
 
