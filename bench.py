@@ -284,16 +284,22 @@ def main():
            for _ in range(args.steps)]
     t_all0 = torch.cuda.Event(enable_timing=True)
     t_all1 = torch.cuda.Event(enable_timing=True)
+    # whole-job rate: K steps back to back between two events
     torch.cuda.synchronize()
     t_all0.record(stream)
-    for a, b in evs:
-        a.record(stream)
+    for _ in range(args.steps):
         step()
-        b.record(stream)
     t_all1.record(stream)
     torch.cuda.synchronize()
     ms_total = t_all0.elapsed_time(t_all1)
     ms_step = ms_total / args.steps
+    # average launch duration: the same K steps, each between its own events
+    # (the events' own packets stay out of the rate above)
+    for a, b in evs:
+        a.record(stream)
+        step()
+        b.record(stream)
+    torch.cuda.synchronize()
     per_launch = sorted(a.elapsed_time(b) for a, b in evs)
     ms_launch = sum(per_launch) / len(per_launch)
     nbytes = dda_mask_bytes(B, V, E, w.H, w.W, S)

@@ -3,6 +3,8 @@
 #include "binning.hpp"
 
 #include <atomic>
+#include <map>
+#include <utility>
 
 namespace s2c {
 
@@ -95,9 +97,19 @@ constexpr int kStagePerOrder = kOrderThreads / kBlock;
 
 __global__ void __launch_bounds__(kOrderThreads)
 bin_order_kernel(long long nt, long long blocks, const int* __restrict__ block_runs, int ovf_cap,
-                 const int* __restrict__ counters, const BinRecord* __restrict__ recs,
-                 const int* __restrict__ offs, BinRun* __restrict__ runs,
-                 int4* __restrict__ order) {
+                 int ovf_entry_cap, const int* __restrict__ counters,
+                 const BinRecord* __restrict__ recs, const int* __restrict__ offs,
+                 BinRun* __restrict__ runs, int4* __restrict__ order) {
+    // An attempt whose overflow areas were too small (the host relaunches
+    // it) gets an order of empty tiles: its resolve writes zeros and reads
+    // no run or entry.
+    const int ce = counters[kEntriesOvf], cr = counters[kRunsOvf];
+    if (ce < 0 || ce > ovf_entry_cap || cr < 0 || cr > ovf_cap) {
+        if (blockIdx.x == 0)
+            for (long long t = threadIdx.x; t < nt; t += kOrderThreads)
+                order[t] = make_int4(static_cast<int>(t), 0, 0, 0);
+        return;
+    }
     if (blockIdx.x == 0) {
         tile_order(nt, offs, order);
         return;
@@ -264,17 +276,27 @@ int bin_clear(const BinPlan& p, hipStream_t s) {
     return fill_u32(p.counters, 0u, kBinCounters + p.slots() + 1, s);
 }
 
-int bin_read_overflow(const BinPlan& p, hipStream_t s, long long* entries, long long* runs) {
-    int h[kBinCounters];
-    S2C_HIP(hipMemcpyAsync(h, p.counters, sizeof(h), hipMemcpyDeviceToHost, s));
-    S2C_HIP(hipStreamSynchronize(s));
+int* bin_pinned_counters() {
+    thread_local int* p = nullptr;
+    if (!p && hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(int) * kBinCounters,
+                            hipHostMallocPortable) != hipSuccess)
+        p = nullptr;
+    return p;
+}
+
+int bin_counters_async(const BinPlan& p, int* host, hipStream_t s) {
+    S2C_HIP(hipMemcpyAsync(host, p.counters, sizeof(int) * kBinCounters, hipMemcpyDeviceToHost, s));
+    return S2C_OK;
+}
+
+int bin_check_overflow(const int* host, long long* entries, long long* runs) {
     // int32 atomics: a wrapped (negative) value means more than 2^31
-    if (h[kEntriesOvf] < 0 || h[kRunsOvf] < 0) {
+    if (host[kEntriesOvf] < 0 || host[kRunsOvf] < 0) {
         set_error("dda_cluster_mask: overflow area beyond 2^31 entries");
         return S2C_ERR_OVERFLOW;
     }
-    *entries = h[kEntriesOvf];
-    *runs = h[kRunsOvf];
+    *entries = host[kEntriesOvf];
+    *runs = host[kRunsOvf];
     return S2C_OK;
 }
 
@@ -283,8 +305,8 @@ int bin_resolve(const BinPlan& p, int H, int W, int S, int* bitset, hipStream_t 
     const long long grid = 1 + (p.blocks + kStagePerOrder - 1) / kStagePerOrder +
                            (p.ovf_runs + kOrderThreads - 1) / kOrderThreads;
     bin_order_kernel<<<static_cast<unsigned>(grid), kOrderThreads, 0, s>>>(
-        p.g.nt, p.blocks, p.block_runs, p.ovf_runs, p.counters, p.records, p.runs_per_slot, p.runs,
-        p.order);
+        p.g.nt, p.blocks, p.block_runs, p.ovf_runs, p.ovf_entries, p.counters, p.records,
+        p.runs_per_slot, p.runs, p.order);
     S2C_LAUNCH_CHECK();
     const bool vec = (S % 4 == 0) && ((reinterpret_cast<uintptr_t>(bitset) & 15) == 0);
     const bool small = tile_bitset_bytes(p.g.ts_log2, S) <= kTileLdsBytes;
@@ -302,6 +324,38 @@ int bin_resolve(const BinPlan& p, int H, int W, int S, int* bitset, hipStream_t 
         tile_resolve_kernel<false, kTileLdsMax><<<grid_t, kResolveThreads, 0, s>>>(
             p.g, p.order, p.runs, p.entries, H, W, S, bitset);
     S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
+
+namespace {
+struct WorkspaceBuffers {
+    void* p[BinWorkspace::kSlots] = {};
+    size_t n[BinWorkspace::kSlots] = {};
+};
+std::map<std::pair<int, hipStream_t>, WorkspaceBuffers>& workspaces() {
+    static std::map<std::pair<int, hipStream_t>, WorkspaceBuffers> m;
+    return m;
+}
+}  // namespace
+
+std::mutex& BinWorkspace::lock() {
+    static std::mutex mu;
+    return mu;
+}
+
+int BinWorkspace::get(hipStream_t s, Slot slot, size_t bytes, void** out) {
+    int dev = 0;
+    S2C_HIP(hipGetDevice(&dev));
+    WorkspaceBuffers& w = workspaces()[{dev, s}];
+    if (w.n[slot] < bytes) {
+        if (w.p[slot]) S2C_HIP(hipFreeAsync(w.p[slot], s));  // ordered after its last use
+        w.p[slot] = nullptr;
+        w.n[slot] = 0;
+        const size_t grown = bytes + bytes / 8;  // headroom for the next, larger call
+        S2C_HIP(hipMalloc(&w.p[slot], grown));
+        w.n[slot] = grown;
+    }
+    *out = w.p[slot];
     return S2C_OK;
 }
 

@@ -19,6 +19,8 @@
 // when it did not fit.
 #pragma once
 
+#include <mutex>
+
 #include "common.hpp"
 
 namespace s2c {
@@ -176,11 +178,34 @@ size_t bin_plan_ints(const TileGrid& g, long long blocks);
 void bin_carve(BinPlan& p, int* ints);
 // Zeroes the run counters and the global counters.
 int bin_clear(const BinPlan& p, hipStream_t s);
-// Reads the overflow-area use of the stage pass (a host sync).
-int bin_read_overflow(const BinPlan& p, hipStream_t s, long long* entries, long long* runs);
+// Pinned host words for the stage pass's counters (one block per thread,
+// kept for the process); nullptr if it cannot be allocated.
+int* bin_pinned_counters();
+// Queues the copy of the counters into `host`.
+int bin_counters_async(const BinPlan& p, int* host, hipStream_t s);
+// The overflow-area use of the stage pass from the copied counters.
+int bin_check_overflow(const int* host, long long* entries, long long* runs);
 // Run offsets per tile, runs scattered into tile order, then one workgroup
-// per tile ORs its entries into LDS and writes the tile bitset.
+// per tile ORs its entries into LDS and writes the tile bitset.  If the
+// stage pass overflowed its areas, every tile is written as zeros instead.
 int bin_resolve(const BinPlan& p, int H, int W, int S, int* bitset, hipStream_t s);
+
+// Device buffers of the binned pass kept per (device, stream) across calls,
+// so that a call allocates nothing once its sizes have been seen (a
+// stream-ordered allocation behind the previous call's pending frees can
+// hold the host until they complete).  Calls on one stream are ordered, so
+// they can share the buffers; `lock` must be held from the first get() to
+// the last launch of a call, so that two threads enqueueing on one stream
+// cannot interleave.  Buffers only grow and are kept until the process
+// ends (for the 50k-strand, 4-view 1024^2 workload about 0.6 GB per
+// stream).
+struct BinWorkspace {
+    enum Slot { kPlanInts = 0, kEntries = 1, kRecords = 2, kRuns = 3, kSlots = 4 };
+    static std::mutex& lock();
+    // Device memory of at least `bytes` for `slot`, for calls on stream s of
+    // the current device.
+    static int get(hipStream_t s, Slot slot, size_t bytes, void** out);
+};
 
 // Running estimates of the overflow-area sizes, per process.
 long long bin_entry_hint(long long want);

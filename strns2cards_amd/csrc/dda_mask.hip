@@ -376,36 +376,65 @@ extern "C" int s2c_dda_cluster_mask_tiled(int B, int H, int W, int V, const floa
     }
     S2C_CHECK(plan.blocks * kStage < INT_MAX / 2 && plan.blocks * kBinRecs < INT_MAX / 2,
               "dda_cluster_mask: too many primitives for 32-bit entry offsets");
-    Temp work;
-    if (int r = work.alloc(sizeof(int) * bin_plan_ints(plan.g, plan.blocks), s)) return r;
-    bin_carve(plan, work.as<int>());
+    std::lock_guard<std::mutex> held(BinWorkspace::lock());
+    void* work = nullptr;
+    if (int r = BinWorkspace::get(s, BinWorkspace::kPlanInts,
+                                  sizeof(int) * bin_plan_ints(plan.g, plan.blocks), &work))
+        return r;
+    bin_carve(plan, static_cast<int*>(work));
     long long want_e = bin_entry_hint(4 * kStage), want_r = bin_run_hint(4 * kHash);
-    Temp ebuf, rbuf, obuf;
+    // The overflow counters are checked without stalling the device: the
+    // run ordering and the resolve are queued right behind the stage pass
+    // (an attempt that overflowed resolves to zeros), and the host waits for
+    // the stage pass and the counter copy alone, relaunching in the rare
+    // case the areas were too small.
+    int* host = bin_pinned_counters();
+    S2C_CHECK(host != nullptr, "dda_cluster_mask: no pinned host memory for the counters");
+    struct Event {
+        hipEvent_t e = nullptr;
+        ~Event() {
+            if (e) (void)hipEventDestroy(e);
+        }
+    } ev;
+    S2C_HIP(hipEventCreateWithFlags(&ev.e, hipEventDisableTiming));
     for (int attempt = 0;; ++attempt) {
         want_e = dmin(want_e, static_cast<long long>(INT_MAX / 2));
         want_r = dmin(want_r, static_cast<long long>(INT_MAX / 2));
         plan.ovf_entries = static_cast<int>(want_e);
         plan.ovf_runs = static_cast<int>(want_r);
-        if (int r = ebuf.alloc(sizeof(unsigned) * static_cast<size_t>(plan.fixed_entries() + want_e), s))
+        void *ebuf = nullptr, *rbuf = nullptr;
+        if (int r = BinWorkspace::get(
+                s, BinWorkspace::kEntries,
+                sizeof(unsigned) * static_cast<size_t>(plan.fixed_entries() + want_e), &ebuf))
             return r;
-        if (int r = rbuf.alloc(sizeof(BinRecord) * static_cast<size_t>(plan.fixed_runs() + want_r), s))
+        if (int r = BinWorkspace::get(s, BinWorkspace::kRecords,
+                                      sizeof(BinRecord) *
+                                          static_cast<size_t>(plan.fixed_runs() + want_r),
+                                      &rbuf))
             return r;
-        plan.entries = ebuf.as<unsigned>();
-        plan.records = rbuf.as<BinRecord>();
+        plan.entries = static_cast<unsigned*>(ebuf);
+        plan.records = static_cast<BinRecord*>(rbuf);
         if (int r = bin_clear(plan, s)) return r;
         dda_bin_stage_kernel<<<static_cast<unsigned>(plan.blocks), kBlock, 0, s>>>(
             P, H, W, V, pos, E, edges, edge2cluster, plan.g, plan.runs_per_slot, plan.counters,
             plan.block_runs, plan.entries, plan.ovf_entries, plan.records, plan.ovf_runs);
         S2C_LAUNCH_CHECK();
+        if (int r = bin_counters_async(plan, host, s)) return r;
+        S2C_HIP(hipEventRecord(ev.e, s));
+        void* obuf = nullptr;
+        if (int r = BinWorkspace::get(
+                s, BinWorkspace::kRuns,
+                sizeof(BinRun) * static_cast<size_t>(plan.fixed_runs() + plan.ovf_runs), &obuf))
+            return r;
+        plan.runs = static_cast<BinRun*>(obuf);
+        if (int r = bin_resolve(plan, H, W, num_slots, bitset, s)) return r;
+        S2C_HIP(hipEventSynchronize(ev.e));
         long long ne = 0, nr = 0;
-        if (int r = bin_read_overflow(plan, s, &ne, &nr)) return r;
+        if (int r = bin_check_overflow(host, &ne, &nr)) return r;
         if (ne <= want_e && nr <= want_r) break;
         S2C_CHECK(attempt == 0, "dda_cluster_mask: buffer sizing did not converge");
         want_e = bin_entry_hint(ne + ne / 8 + 4 * kStage);
         want_r = bin_run_hint(nr + nr / 8 + 4 * kHash);
     }
-    if (int r = obuf.alloc(sizeof(BinRun) * static_cast<size_t>(plan.fixed_runs() + plan.ovf_runs), s))
-        return r;
-    plan.runs = obuf.as<BinRun>();
-    return bin_resolve(plan, H, W, num_slots, bitset, s);
+    return S2C_OK;
 }
