@@ -119,7 +119,9 @@ __device__ __forceinline__ float aa_adjoint_sum(float x, float y, int H, int W, 
             const unsigned c0 = cc & 1u, c1 = cc >> 1;
             if (c0 == c1) continue;
             const float wt = c * expf(-alpha * n2) * 1.f;
-            dc += wt * (static_cast<float>(c0) - static_cast<float>(c1)) * (c1 == 0u ? rho : 1.f);
+            // wt * (c0 - c1) * (c1 == 0 ? rho : 1) with c0 != c1, exactly:
+            // multiplying by +-1 is exact
+            dc += c1 ? -wt : wt * rho;
         }
     }
     return dc;
@@ -175,9 +177,9 @@ __device__ __forceinline__ float aa_adjoint_sum_rows(float x, float y, int H, in
             const float dx = static_cast<float>(w0 + i) + 0.5f - x;
             const float n2 = dx * dx + dy * dy;
             if (!(n2 < R2)) continue;
-            const unsigned c1 = (tgt >> i) & 1u, c0 = c1 ^ 1u;
+            const unsigned c1 = (tgt >> i) & 1u;
             const float wt = c * (FAST_EXP ? __expf(-alpha * n2) : expf(-alpha * n2)) * 1.f;
-            dc += wt * (static_cast<float>(c0) - static_cast<float>(c1)) * (c1 == 0u ? rho : 1.f);
+            dc += c1 ? -wt : wt * rho;  // as in aa_adjoint_sum
         }
     }
     return dc;
