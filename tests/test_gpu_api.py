@@ -89,6 +89,40 @@ def test_rasterize_and_interpolate(dev, small):
     np.testing.assert_array_equal(fused.cpu().numpy(), want_bits)
 
 
+def test_dda_cluster_mask_two_streams(dev):
+    """The fused DDA mask keeps its binning buffers per (device, stream):
+    two streams working at the same time, on different workloads and
+    resolutions (so the second call on each stream grows its buffers), give
+    the oracle's masks."""
+    report("dda_cluster_mask on two streams at once")
+    rng = np.random.default_rng(77)
+    cases = []
+    for res, n, seg in (((300, 260), 3000, 20.0), ((512, 512), 12000, 40.0)):
+        H, W = res
+        p0 = rng.uniform(0, [W, H], size=(n, 2))
+        p1 = p0 + rng.normal(scale=seg, size=(n, 2))
+        pts = np.stack([p0, p1], 1).reshape(-1, 2)
+        pos = np.zeros((2, 2 * n, 4), np.float32)
+        pos[:, :, 0] = 2 * pts[:, 0] / W - 1
+        pos[:, :, 1] = 1 - 2 * pts[:, 1] / H
+        pos[1, :, 0] += 0.01
+        pos[..., 3] = 1
+        edges = np.arange(2 * n, dtype=np.int32).reshape(n, 2)
+        e2c = rng.integers(0, 200, size=n).astype(np.int32)
+        cases.append((res, pos, edges, e2c, oracle.dda_cluster_mask(res, pos, edges, e2c, 200)))
+    streams = [torch.cuda.Stream(device=dev), torch.cuda.Stream(device=dev)]
+    for order in ((0, 1), (1, 0)):  # each stream sees the small and the large case
+        outs = []
+        for si, ci in enumerate(order):
+            res, pos, edges, e2c, _ = cases[ci]
+            with torch.cuda.stream(streams[si]):
+                outs.append(s2c.dda_cluster_mask(res, T(pos, dev), T(edges, dev), T(e2c, dev),
+                                                 cluster_count=200))
+        torch.cuda.synchronize()
+        for out, ci in zip(outs, order):
+            np.testing.assert_array_equal(out.cpu().numpy(), cases[ci][4])
+
+
 def test_bitset_api(dev, small):
     report("api: accumulate_bitset_rgb / popcount_bitset")
     w = small
