@@ -206,15 +206,21 @@ int s2c_triangle_cluster_mask(int B, int H, int W, int V, const float* pos,
  * per-cluster fitting jobs at once, the iteration of 3_fitting.py:381-407
  * (triangle mask, antialias edge gradient, planarity and uniform-Laplacian
  * terms, UniformAdam of optimizer.py:33-68 with ExponentialLR decay), one
- * workgroup per job, without the largesteps reparameterization.
+ * workgroup per job, with the largesteps reparameterization of
+ * 3_fitting.py:347-384 (param = M pos, pos = M^{-1} param, M = I + lam
+ * (L_h + w L_w) of the strip) unless lam = 0.
  *
  * Job j: a res[j] x res[j] view (res <= 256), a Q = quads[j] <= 32 strip of
  * 2(Q+1) vertices (row r, column c -> r*(Q+1) + c) stored from
  * verts[vert_off[j]] (x,y,z fp32), a row-major 4x4 clip matrix mvp[16*j],
  * and a packed target mask (bit p = pixel h*res + w of word p/32) starting
- * at target_bits[bits_off[j]].  adam_m (3 per vertex), adam_v (1 per vertex)
- * and steps[j] hold the optimizer state.  hyper is a HOST array of 8 floats:
- * lr0, beta1, beta2, gamma (lr decay per step), w_pln, w_lap, rho, radius. */
+ * at target_bits[bits_off[j]].  params (3 per vertex, the largesteps
+ * parameter, set from verts at a job's first step), adam_m (3 per vertex),
+ * adam_v (1 per vertex) and steps[j] hold the optimizer state; verts holds
+ * the positions after each call.  hyper is a HOST array of 10 floats: lr0,
+ * beta1, beta2, gamma (lr decay per step), w_pln, w_lap, rho, radius, lam
+ * (laplacian_lambda; 0 = optimize positions directly), w_weight
+ * (laplacian_w_weight). */
 
 /* Target masks: for job j, num_strands[j] strands of verts_per_strand
  * clip-space float4 vertices from strand_clip[strand_off[j]]; every edge is
@@ -227,7 +233,7 @@ int s2c_fit_targets(int num_jobs, const int* res, const int* bits_off,
 /* Runs `iters` fitting iterations for the jobs slots[0..num_slots) (a
  * negative entry is an idle slot and is skipped), then
  * renders the updated card and writes popcount(render XOR target) to
- * loss_out[job]; verts, Adam state and steps are updated in place.  Jobs
+ * loss_out[job]; verts, params, Adam state and steps are updated in place.  Jobs
  * never share state, and each job's arithmetic order is fixed, so a job's
  * result does not depend on which other jobs run in the same call, on
  * their order in `slots`, or on `chunk`: the silhouette samples a
@@ -235,8 +241,8 @@ int s2c_fit_targets(int num_jobs, const int* res, const int* bits_off,
  * trading LDS per workgroup against passes per iteration. */
 int s2c_fit_cards(int num_slots, const int* slots, int iters, const int* res,
                   const int* quads, const int* vert_off, const int* bits_off,
-                  const float* mvp, float* verts, float* adam_m, float* adam_v,
-                  int* steps, const unsigned* target_bits, const float* hyper,
+                  const float* mvp, float* verts, float* params, float* adam_m,
+                  float* adam_v, int* steps, const unsigned* target_bits, const float* hyper,
                   float* loss_out, int chunk, void* stream);
 
 /* compute_bishop_frame (curve_utils.py:104-165, a Python loop over the
@@ -277,6 +283,41 @@ int s2c_card_coverage_forward(int B, int H, int W, int V, const float* pos, int 
 int s2c_card_coverage_backward(int B, int H, int W, int V, const float* pos, int T,
                                const int* tri, const float* d_coverage, float* d_pos,
                                void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Large-steps parameterization of grid meshes                            */
+/* ---------------------------------------------------------------------- */
+/* M = I + lam * (L_h + w_weight * L_w) of 3_fitting.py:347-356, for the
+ * concatenated grid meshes of texture_utils.py:525-655: grid g has rows[g]
+ * rows of cols[g] vertices, vertex (i, j) at voff[g] + i * cols[g] + j;
+ * L_h / L_w are the uniform Laplacians (optimizer.py:70-99) of its h_edges
+ * (i, j)-(i, j+1) and w_edges (i, j)-(i+1, j).  x and u are (V, 3) float32
+ * with V = voff[last] + rows[last] * cols[last]; rows, cols, voff and
+ * fac_off are device int32 arrays of num_grids entries.  lam, w_weight >= 0.
+ *
+ * s2c_grid_system_apply: u = M x.  Replaces largesteps.parameterize's
+ * to_differential(M, x) (called at 3_fitting.py:365).
+ *
+ * s2c_grid_system_factor: the factor of M for the solve, computed once per
+ * (lam, w_weight) and layout, as the reference builds its Cholesky solver
+ * once per M.  Writes rows[g] * cols[g] floats at factor + fac_off[g] (the
+ * pivot reciprocals of the grid's decoupled tridiagonal systems); they
+ * depend only on (rows[g], cols[g], lam, w_weight), so grids of one shape
+ * may share one offset (and the call may list each shape once).
+ *
+ * s2c_grid_system_solve: x = M^{-1} u with the factor of the same lam.
+ * Replaces largesteps.parameterize's from_differential(M, u)
+ * (3_fitting.py:384, 457).  max_verts / max_rows bound rows[g] * cols[g]
+ * and rows[g] over all grids and size the workgroup's LDS
+ * (24 * max_verts + 4 * max_rows^2 bytes, at most 65536: e.g. 2 rows of up
+ * to 1365 columns); a grid beyond them is written as NaN. */
+int s2c_grid_system_apply(int num_grids, const int* rows, const int* cols, const int* voff,
+                          float lam, float w_weight, const float* x, float* u, void* stream);
+int s2c_grid_system_factor(int num_grids, const int* rows, const int* cols, const int* fac_off,
+                           float lam, float w_weight, float* factor, void* stream);
+int s2c_grid_system_solve(int num_grids, const int* rows, const int* cols, const int* voff,
+                          const int* fac_off, const float* factor, int max_verts, int max_rows,
+                          float lam, const float* u, float* x, void* stream);
 
 #ifdef __cplusplus
 }

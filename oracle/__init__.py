@@ -447,3 +447,64 @@ def resample_strands_by_arclength(num_samples, points, target):
     out = np.zeros((ns.size, int(target), 3), np.float32)
     lib().orc_resample_strands_by_arclength(ns.size, ns, _f(points), int(target), out)
     return out
+
+
+# ------------------------------------------------ large steps (grid meshes)
+# The reference's third-party solve: largesteps 0.2.2 (parameterize.py's
+# to_differential(L, v) = L @ v and from_differential(L, u), a cholespy
+# 2.1.0 sparse Cholesky solve of L x = u; requirements-detail.txt:1,14),
+# neither installed here.  Restated in float64 with numpy/scipy: the
+# published algorithm is a Cholesky factorization and two triangular solves.
+
+def wisp_grid_edges(S: int, N: int):
+    """h_edges and w_edges of one (S, N) wisp, loop for loop as
+    texture_utils.py:566-584 (Python loops: small cases only)."""
+    h = [(i * N + j, i * N + j + 1) for i in range(S) for j in range(N - 1)]
+    w = [(i * N + j, (i + 1) * N + j) for i in range(S - 1) for j in range(N)]
+    return np.array(h, np.int64).reshape(-1, 2), np.array(w, np.int64).reshape(-1, 2)
+
+
+def combined_grid_edges(shapes):
+    """Edges of several (S, N) wisps concatenated with running vertex
+    offsets, as combine_wisps_with_cluster_labels (texture_utils.py:634-655).
+    Returns (num_verts, h_edges, w_edges)."""
+    hs, ws, off = [], [], 0
+    for S, N in shapes:
+        h, w = wisp_grid_edges(S, N)
+        hs.append(h + off)
+        ws.append(w + off)
+        off += S * N
+    cat = lambda a: np.concatenate(a) if a else np.zeros((0, 2), np.int64)  # noqa: E731
+    return off, cat(hs), cat(ws)
+
+
+def laplacian_edges(V: int, edges) -> np.ndarray:
+    """Dense float64 uniform Laplacian, as optimizer.py:70-99: the distinct
+    directed pairs of the symmetrized edge list get -1, each pair adds +1 at
+    its first vertex's diagonal (duplicates summed, as coalesce does)."""
+    e = np.asarray(edges, np.int64).reshape(-1, 2)
+    adj = np.unique(np.concatenate([e, e[:, ::-1]]), axis=0)
+    L = np.zeros((V, V))
+    np.add.at(L, (adj[:, 0], adj[:, 1]), -1.0)
+    np.add.at(L, (adj[:, 0], adj[:, 0]), 1.0)
+    return L
+
+
+def largesteps_matrix(V: int, h_edges, w_edges, lam: float, w_weight: float) -> np.ndarray:
+    """M = I + lam * (L_h + w_weight * L_w) (3_fitting.py:347-356)."""
+    return np.eye(V) + lam * (laplacian_edges(V, h_edges) + w_weight * laplacian_edges(V, w_edges))
+
+
+def to_differential(M: np.ndarray, x) -> np.ndarray:
+    """u = M x (largesteps to_differential)."""
+    return M @ np.asarray(x, np.float64)
+
+
+def from_differential(M: np.ndarray, u) -> np.ndarray:
+    """x = M^{-1} u through the Cholesky factor M = C C^T (largesteps
+    from_differential with its default Cholesky method)."""
+    from scipy.linalg import solve_triangular
+
+    C = np.linalg.cholesky(M)
+    y = solve_triangular(C, np.asarray(u, np.float64), lower=True)
+    return solve_triangular(C.T, y, lower=False)

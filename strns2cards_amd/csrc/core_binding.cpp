@@ -545,8 +545,8 @@ PYBIND11_MODULE(_core, m) {
     });
 
     m.def("fit_cards", [](Tensor slots, int iters, Tensor res, Tensor quads, Tensor vert_off,
-                          Tensor bits_off, Tensor mvp, Tensor verts, Tensor adam_m, Tensor adam_v,
-                          Tensor steps, Tensor target_bits, std::vector<float> hyper,
+                          Tensor bits_off, Tensor mvp, Tensor verts, Tensor params, Tensor adam_m,
+                          Tensor adam_v, Tensor steps, Tensor target_bits, std::vector<float> hyper,
                           Tensor loss_out, int chunk) {
         const char* fn = "fit_cards";
         need(slots, fn, "slots", at::kInt, {-1});
@@ -557,20 +557,21 @@ PYBIND11_MODULE(_core, m) {
         need(bits_off, fn, "bits_off", at::kInt, {J});
         need(mvp, fn, "mvp", at::kFloat, {J, 4, 4});
         need(verts, fn, "verts", at::kFloat, {-1, 3});
+        need(params, fn, "params", at::kFloat, {verts.size(0), 3});
         need(adam_m, fn, "adam_m", at::kFloat, {verts.size(0), 3});
         need(adam_v, fn, "adam_v", at::kFloat, {verts.size(0)});
         need(steps, fn, "steps", at::kInt, {J});
         need(target_bits, fn, "target_bits", at::kInt, {-1});
         need(loss_out, fn, "loss_out", at::kFloat, {J});
-        same_device(fn, {&slots, &res, &quads, &vert_off, &bits_off, &mvp, &verts, &adam_m, &adam_v,
-                         &steps, &target_bits, &loss_out});
-        if (hyper.size() != 8)
-            throw std::runtime_error("fit_cards: hyper must hold 8 values");
+        same_device(fn, {&slots, &res, &quads, &vert_off, &bits_off, &mvp, &verts, &params,
+                         &adam_m, &adam_v, &steps, &target_bits, &loss_out});
+        if (hyper.size() != 10)
+            throw std::runtime_error("fit_cards: hyper must hold 10 values");
         if (iters < 0) throw std::runtime_error("fit_cards: iters must be >= 0");
         DeviceGuard g(res.device());
         check(s2c_fit_cards(I(slots.size(0), fn), P<int>(slots), iters, P<int>(res), P<int>(quads),
                             P<int>(vert_off), P<int>(bits_off), P<float>(mvp), P<float>(verts),
-                            P<float>(adam_m), P<float>(adam_v), P<int>(steps),
+                            P<float>(params), P<float>(adam_m), P<float>(adam_v), P<int>(steps),
                             P<unsigned>(target_bits), hyper.data(), P<float>(loss_out), chunk,
                             stream_of(res)),
               fn);
@@ -656,6 +657,56 @@ PYBIND11_MODULE(_core, m) {
                                         I(bitset.size(2), fn), I(pos.size(1), fn), P<float>(pos),
                                         I(tri.size(0), fn), P<int>(tri), P<int>(tri2cluster),
                                         I(bitset.size(3), fn), P<int>(bitset), stream_of(pos)),
+              fn);
+    });
+
+    // Grid-mesh large-steps system (no reference binding: replaces
+    // largesteps.parameterize.to_differential / from_differential).
+    auto grid_layout = [](const char* fn, const Tensor& rows, const Tensor& cols,
+                          const Tensor& voff, const Tensor& a, const Tensor& b) {
+        need(rows, fn, "rows", at::kInt, {-1});
+        need(cols, fn, "cols", at::kInt, {rows.size(0)});
+        need(voff, fn, "voff", at::kInt, {rows.size(0)});
+        need(a, fn, "input", at::kFloat, {-1, 3});
+        need(b, fn, "output", at::kFloat, {a.size(0), 3});
+        same_device(fn, {&rows, &cols, &voff, &a, &b});
+        return I(rows.size(0), fn);
+    };
+    m.def("grid_system_apply", [grid_layout](Tensor rows, Tensor cols, Tensor voff, float lam,
+                                             float w_weight, Tensor x, Tensor u) {
+        const char* fn = "grid_system_apply";
+        const int G = grid_layout(fn, rows, cols, voff, x, u);
+        DeviceGuard g(x.device());
+        check(s2c_grid_system_apply(G, P<int>(rows), P<int>(cols), P<int>(voff), lam, w_weight,
+                                    P<float>(x), P<float>(u), stream_of(x)),
+              fn);
+    });
+    m.def("grid_system_factor", [](Tensor rows, Tensor cols, Tensor fac_off, float lam,
+                                   float w_weight, Tensor factor) {
+        const char* fn = "grid_system_factor";
+        need(rows, fn, "rows", at::kInt, {-1});
+        need(cols, fn, "cols", at::kInt, {rows.size(0)});
+        need(fac_off, fn, "fac_off", at::kInt, {rows.size(0)});
+        need(factor, fn, "factor", at::kFloat, {-1});
+        same_device(fn, {&rows, &cols, &fac_off, &factor});
+        DeviceGuard g(factor.device());
+        check(s2c_grid_system_factor(I(rows.size(0), fn), P<int>(rows), P<int>(cols),
+                                     P<int>(fac_off), lam, w_weight, P<float>(factor),
+                                     stream_of(factor)),
+              fn);
+    });
+    m.def("grid_system_solve", [grid_layout](Tensor rows, Tensor cols, Tensor voff,
+                                             Tensor fac_off, Tensor factor, int max_verts,
+                                             int max_rows, float lam, Tensor u, Tensor x) {
+        const char* fn = "grid_system_solve";
+        const int G = grid_layout(fn, rows, cols, voff, u, x);
+        need(fac_off, fn, "fac_off", at::kInt, {rows.size(0)});
+        need(factor, fn, "factor", at::kFloat, {-1});
+        same_device(fn, {&rows, &fac_off, &factor});
+        DeviceGuard g(u.device());
+        check(s2c_grid_system_solve(G, P<int>(rows), P<int>(cols), P<int>(voff), P<int>(fac_off),
+                                    P<float>(factor), max_verts, max_rows, lam, P<float>(u),
+                                    P<float>(x), stream_of(u)),
               fn);
     });
 }

@@ -1,12 +1,18 @@
 // Batched / served card fitting: one workgroup per fitting job.
 //
 // The per-iteration sequence is 3_fitting.py:381-407 for one card strip of
-// a cluster, without the largesteps reparameterization (not available):
+// a cluster:
+//   pos       = from_differential(M, param)           (3_fitting.py:384)
 //   pos_ndc_h = [pos, 1] @ MVP^T                      (3_fitting.py:385-386)
 //   bitset    = render_bitset_from_triangles(...)     (3_fitting.py:154-163)
 //   loss      = antialiased_cluster_bitset_loss(...)  (antialias.py:66-130)
 //             + w_pln * planarity + w_lap * ||L pos||^2 (3_fitting.py:400-401)
-//   UniformAdam step, then ExponentialLR decay        (optimizer.py:33-68)
+//   UniformAdam step on param, then ExponentialLR decay (optimizer.py:33-68)
+// with M = I + lam (L_h + w L_w) of the strip (3_fitting.py:347-356) and
+// param = M pos at the job's first step (3_fitting.py:365); the gradient
+// reaches param through M^{-1} (M is symmetric).  Both solves use the
+// strip's pivots, factored once per launch (grid_math.hpp).  lam = 0 turns
+// the reparameterization off (pos itself is optimized).
 // A job's render and target masks (<= 256 x 256, one bit per pixel) live
 // in LDS as packed bits, its vertices and Adam moments in registers, so a
 // launch runs many iterations with only workgroup barriers and no HBM
@@ -20,6 +26,7 @@
 // [c, N+c, N+c+1] and [c, N+c+1, c+1].
 #include "antialias_math.hpp"
 #include "dda_math.hpp"
+#include "grid_math.hpp"
 #include "raster_math.hpp"
 
 namespace s2c {
@@ -36,7 +43,7 @@ constexpr int kFitMaxEdges = 4 * kFitMaxQuads + 1;
 constexpr int kFitChunkDefault = 512;
 
 struct FitHyper {
-    float lr0, b1, b2, gamma, w_pln, w_lap, rho, R;
+    float lr0, b1, b2, gamma, w_pln, w_lap, rho, R, lam, ww;
 };
 
 __device__ __forceinline__ int3 strip_tri(int t, int N) {
@@ -148,14 +155,20 @@ __global__ void __launch_bounds__(kBlock)
 fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict__ res_arr,
                  const int* __restrict__ quad_arr, const int* __restrict__ vert_off,
                  const int* __restrict__ bits_off, const float* __restrict__ mvp_arr,
-                 float* __restrict__ verts, float* __restrict__ adam_m,
-                 float* __restrict__ adam_v, int* __restrict__ steps,
+                 float* __restrict__ verts, float* __restrict__ params,
+                 float* __restrict__ adam_m, float* __restrict__ adam_v,
+                 int* __restrict__ steps,
                  const unsigned* __restrict__ target_bits, FitHyper hp,
                  float* __restrict__ loss_out) {
     __shared__ unsigned render[kFitMaxWords];
     __shared__ unsigned target[kFitMaxWords];
     __shared__ float4 clip[kFitMaxVerts];
     __shared__ float P[kFitMaxVerts][3];
+    __shared__ float U[kFitMaxVerts][3];     // param (largesteps)
+    __shared__ float G[kFitMaxVerts][3];     // gradient handed to the solve
+    __shared__ float Ysol[3 * kFitMaxVerts]; // modes of a solve
+    __shared__ float IM[kFitMaxVerts];       // pivots of the strip's systems
+    __shared__ float Qb[4];                  // 2 x 2 row basis
     __shared__ float lp[kFitMaxVerts][3];
     __shared__ float acc[kFitMaxEdges][6];
     __shared__ DdaSpan espan[kFitMaxEdges];
@@ -193,9 +206,36 @@ fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict
     int step = steps[job];
     const FitCode code{render, target, res};
     const FitRows rows{render, target, res};
+    const bool ls = hp.lam > 0.f;  // workgroup-uniform
+    if (ls) {
+        if (tid < 4) Qb[tid] = grid_dct_q(tid >> 1, tid & 1, 2);
+        if (tid < 2) grid_factor(2, N, tid, hp.lam, hp.ww, IM + tid * N);
+        if (step > 0 && tid < V)
+            for (int k = 0; k < 3; ++k) U[tid][k] = params[(vo + tid) * 3 + k];
+    }
     __syncthreads();
+    if (ls && step == 0) {  // param = to_differential(M, pos)
+        for (int q = tid; q < 3 * V; q += kBlock)
+            (&U[0][0])[q] = grid_apply_at(2, N, hp.lam, hp.ww, &P[0][0], q);
+        __syncthreads();
+    }
+    // in-place M^{-1} over a (V, 3) LDS array (three workgroup barriers)
+    auto solve = [&](float* a) {
+        grid_project<kBlock>(2, N, Qb, a, Ysol, tid);
+        __syncthreads();
+        if (tid < kWave) grid_solve_wave(2, N, hp.lam, IM, Ysol, tid);
+        __syncthreads();
+        grid_back_project<kBlock>(2, N, Qb, Ysol, [&](int q, float v) { a[q] = v; }, tid);
+        __syncthreads();
+    };
 
     for (int it = 0; it <= iters; ++it) {
+        // ---- pos = from_differential(M, param) (3_fitting.py:384)
+        if (ls) {
+            for (int q = tid; q < 3 * V; q += kBlock) (&P[0][0])[q] = (&U[0][0])[q];
+            __syncthreads();
+            solve(&P[0][0]);
+        }
         // ---- project (3_fitting.py:385-386) and clear
         if (tid < V) {
             float c[4];
@@ -345,6 +385,22 @@ fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict
                 gy += sgn * ((P[i0][1] + P[i2][1]) - (P[i1][1] + P[i3][1]));
                 gz += sgn * ((P[i0][2] + P[i2][2]) - (P[i1][2] + P[i3][2]));
             }
+        }
+        if (ls) {  // d loss / d param = M^{-1} d loss / d pos
+            if (tid < V) {
+                G[tid][0] = gx;
+                G[tid][1] = gy;
+                G[tid][2] = gz;
+            }
+            __syncthreads();
+            solve(&G[0][0]);
+            if (tid < V) {
+                gx = G[tid][0];
+                gy = G[tid][1];
+                gz = G[tid][2];
+            }
+        }
+        if (tid < V) {
             const float t = static_cast<float>(step + 1);
             g1x = hp.b1 * g1x + (1.f - hp.b1) * gx;
             g1y = hp.b1 * g1y + (1.f - hp.b1) * gy;
@@ -358,9 +414,10 @@ fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict
             const float t = static_cast<float>(step + 1);
             const float c1 = 1.f - powf(hp.b1, t);
             const float lr = hp.lr0 * powf(hp.gamma, static_cast<float>(step));
-            P[tid][0] -= lr * ((g1x / c1) / denom);
-            P[tid][1] -= lr * ((g1y / c1) / denom);
-            P[tid][2] -= lr * ((g1z / c1) / denom);
+            float* p = ls ? U[tid] : P[tid];
+            p[0] -= lr * ((g1x / c1) / denom);
+            p[1] -= lr * ((g1y / c1) / denom);
+            p[2] -= lr * ((g1z / c1) / denom);
         }
         ++step;
         __syncthreads();
@@ -377,6 +434,8 @@ fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict
     }
     if (tid < V) {
         for (int k = 0; k < 3; ++k) verts[(vo + tid) * 3 + k] = P[tid][k];
+        if (ls)
+            for (int k = 0; k < 3; ++k) params[(vo + tid) * 3 + k] = U[tid][k];
         adam_m[(vo + tid) * 3 + 0] = g1x;
         adam_m[(vo + tid) * 3 + 1] = g1y;
         adam_m[(vo + tid) * 3 + 2] = g1z;
@@ -441,7 +500,8 @@ extern "C" int s2c_fit_targets(int num_jobs, const int* res, const int* bits_off
 
 extern "C" int s2c_fit_cards(int num_slots, const int* slots, int iters, const int* res,
                              const int* quads, const int* vert_off, const int* bits_off,
-                             const float* mvp, float* verts, float* adam_m, float* adam_v,
+                             const float* mvp, float* verts, float* params, float* adam_m,
+                             float* adam_v,
                              int* steps, const unsigned* target_bits, const float* hyper,
                              float* loss_out, int chunk, void* stream) {
     S2C_CHECK(num_slots >= 0 && iters >= 0, "fit_cards: negative size");
@@ -450,12 +510,16 @@ extern "C" int s2c_fit_cards(int num_slots, const int* slots, int iters, const i
     S2C_CHECK(chunk == 256 || chunk == 512 || chunk == 1024,
               "fit_cards: chunk must be 0, 256, 512 or 1024");
     if (num_slots == 0) return S2C_OK;
-    FitHyper hp{hyper[0], hyper[1], hyper[2], hyper[3], hyper[4], hyper[5], hyper[6], hyper[7]};
+    FitHyper hp{hyper[0], hyper[1], hyper[2], hyper[3], hyper[4],
+                hyper[5], hyper[6], hyper[7], hyper[8], hyper[9]};
     S2C_CHECK(hp.R > 0.f && hp.R < 64.f, "fit_cards: kernel radius must be in (0, 64)");
+    S2C_CHECK(hp.lam >= 0.f && hp.ww >= 0.f,
+              "fit_cards: laplacian lambda and w weight must be >= 0");
     hipStream_t s = static_cast<hipStream_t>(stream);
 #define S2C_FIT_LAUNCH(C)                                                                      \
     fit_cards_kernel<C><<<num_slots, kBlock, 0, s>>>(slots, iters, res, quads, vert_off,     \
-                                                     bits_off, mvp, verts, adam_m, adam_v,   \
+                                                     bits_off, mvp, verts, params, adam_m,   \
+                                                     adam_v,                                 \
                                                      steps, target_bits, hp, loss_out)
     if (chunk == 256) S2C_FIT_LAUNCH(256);
     else if (chunk == 512) S2C_FIT_LAUNCH(512);

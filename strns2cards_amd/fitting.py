@@ -5,9 +5,10 @@ A fitting job is one cluster (wisp) of strands and one card strip fitted to
 it in one orthographic view, with the iteration of 3_fitting.py:381-407:
 triangle cluster mask, antialiased silhouette gradient (rho, radius 7),
 planarity and uniform-Laplacian regularizers, UniformAdam and an
-exponential learning-rate decay.  The largesteps reparameterization of the
-reference (3_fitting.py:362-384) is not available offline, so vertex
-positions are optimized directly, in both forms.
+exponential learning-rate decay, over the largesteps parameterization of
+3_fitting.py:347-384 (param = M pos, pos = M^{-1} param, M = I + lam (L_h +
+w L_w)) in both forms: ``optimizer.GridSystem`` in the plain loop, the same
+factor and solve inside the batched kernel.
 
 * ``plain_fit`` runs one job at a time through the drop-in API
   (``compute_fragments`` -> ``cluster_mask_from_fragments`` ->
@@ -32,7 +33,7 @@ from . import _core
 from . import synth
 from .antialias import antialiased_cluster_bitset_loss
 from .bitset_utils import popcount_bitset
-from .optimizer import UniformAdam, laplacian_edges
+from .optimizer import GridSystem, UniformAdam, from_differential, laplacian_edges, to_differential
 from .rasterize import cluster_mask_from_fragments, compute_fragments
 
 MAX_RES = 256
@@ -44,8 +45,10 @@ MAX_WORDS = MAX_RES * MAX_RES // 32
 @dataclass
 class FitHyper:
     """3_fitting.py:506-525 defaults (lr 0.1, alpha 0.5, w_pln 10,
-    w_lap 1e-2, kernel radius 7) with the decay to 1e-2 of the initial lr
-    over the job's iterations (3_fitting.py:376)."""
+    w_lap 1e-2, kernel radius 7, laplacian_lambda 100, laplacian_w_weight
+    1/100) with the decay to 1e-2 of the initial lr over the job's
+    iterations (3_fitting.py:376).  lam = 0 optimizes the positions
+    directly (no largesteps)."""
     lr: float = 0.1
     beta1: float = 0.9
     beta2: float = 0.999
@@ -54,6 +57,8 @@ class FitHyper:
     w_lap: float = 1e-2
     rho: float = 0.5
     radius: float = 7.0
+    lam: float = 100.0
+    w_weight: float = 1.0 / 100
 
     @property
     def gamma(self) -> float:
@@ -61,7 +66,7 @@ class FitHyper:
 
     def vector(self) -> list[float]:
         return [self.lr, self.beta1, self.beta2, self.gamma, self.w_pln, self.w_lap, self.rho,
-                self.radius]
+                self.radius, self.lam, self.w_weight]
 
 
 @dataclass
@@ -178,9 +183,10 @@ def plain_target(job: FitJob, dev) -> torch.Tensor:
 
 
 def plain_fit(job: FitJob, hyper: FitHyper, dev, iters: int | None = None):
-    """One job through the drop-in API, as 3_fitting.py:381-407 without
-    largesteps.  Returns (final vertices, final XOR pixel count, target
-    pixel count)."""
+    """One job through the drop-in API, as 3_fitting.py:362-407: param =
+    to_differential(M, pos), and each iteration pos = from_differential(M,
+    param) (``GridSystem`` of the strip; positions directly when lam = 0).
+    Returns (final vertices, final XOR pixel count, target pixel count)."""
     iters = hyper.iters if iters is None else iters
     tri_np, quads_np, edges_np, e2t_np, lap_np = strip_topology(job.quads)
     tri = torch.from_numpy(tri_np).to(dev)
@@ -190,9 +196,16 @@ def plain_fit(job: FitJob, hyper: FitHyper, dev, iters: int | None = None):
     e2c = torch.zeros(edges.shape[0], dtype=torch.int32, device=dev)
     mvp = torch.from_numpy(job.mvp).to(dev)
     target = plain_target(job, dev)
-    pos = torch.from_numpy(job.verts.copy()).to(dev).requires_grad_(True)
-    L = laplacian_edges(pos.detach(), torch.from_numpy(lap_np).to(dev))
-    opt = UniformAdam([pos], lr=hyper.lr, betas=(hyper.beta1, hyper.beta2))
+    pos0 = torch.from_numpy(job.verts.copy()).to(dev)
+    L = laplacian_edges(pos0, torch.from_numpy(lap_np).to(dev))
+    if hyper.lam > 0:
+        M = GridSystem([2], [job.quads + 1], hyper.lam, hyper.w_weight, device=dev)
+        param = to_differential(M, pos0).detach().requires_grad_(True)
+        positions = lambda: from_differential(M, param)  # noqa: E731
+    else:
+        param = pos0.clone().requires_grad_(True)
+        positions = lambda: param  # noqa: E731
+    opt = UniformAdam([param], lr=hyper.lr, betas=(hyper.beta1, hyper.beta2))
     sched = torch.optim.lr_scheduler.ExponentialLR(opt, gamma=hyper.gamma)
 
     def project(p):
@@ -201,6 +214,7 @@ def plain_fit(job: FitJob, hyper: FitHyper, dev, iters: int | None = None):
 
     for _ in range(iters):
         opt.zero_grad()
+        pos = positions()
         pos_ndc = project(pos)
         bits = render_mask(job.res, pos_ndc.detach(), tri)
         loss = antialiased_cluster_bitset_loss(bits, target, pos_ndc, tri, edges, e2t, e2c,
@@ -211,6 +225,7 @@ def plain_fit(job: FitJob, hyper: FitHyper, dev, iters: int | None = None):
         opt.step()
         sched.step()
     with torch.no_grad():
+        pos = positions()
         bits = render_mask(job.res, project(pos), tri)
         xor = int(popcount_bitset(torch.bitwise_xor(bits, target)).sum().item())
         tpix = int(popcount_bitset(target).sum().item())
@@ -259,6 +274,7 @@ class CardFitServer:
         self.steps = torch.zeros(capacity, **i32)
         self.loss = torch.zeros(capacity, **f32)
         self.verts = torch.zeros((capacity * MAX_VERTS, 3), **f32)
+        self.params = torch.zeros((capacity * MAX_VERTS, 3), **f32)
         self.adam_m = torch.zeros((capacity * MAX_VERTS, 3), **f32)
         self.adam_v = torch.zeros(capacity * MAX_VERTS, **f32)
         self.target = torch.zeros(capacity * MAX_WORDS, **i32)
@@ -334,9 +350,9 @@ class CardFitServer:
     # -- iterations ----------------------------------------------------
     def _launch(self, iters: int) -> None:
         _core.fit_cards(self.launch if self.order == "cost" else self.slots, iters, self.res,
-                        self.quads, self.vert_off, self.bits_off, self.mvp, self.verts, self.adam_m,
-                        self.adam_v, self.steps, self.target, self.hyper.vector(), self.loss,
-                        self.chunk)
+                        self.quads, self.vert_off, self.bits_off, self.mvp, self.verts,
+                        self.params, self.adam_m, self.adam_v, self.steps, self.target,
+                        self.hyper.vector(), self.loss, self.chunk)
 
     def run(self, iters: int, graph: bool = False) -> None:
         """Advances every active slot by `iters` iterations (and refreshes
