@@ -208,6 +208,93 @@ def adjoint_section(dev, steps=200):
     return out
 
 
+def strip_layout_default(seed=0, guides=300, num_quads=15000):
+    """Strip grids of 2_texture.py's defaults: `guides` guide strands with
+    lognormal lengths share num_quads quads in proportion to length
+    (allocate_vertex_counts), each swept into two crossed 2-row strips
+    (res_w 2, tangent rotations -45 and 45 degrees)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    length = rng.lognormal(0.0, 0.5, size=guides)
+    quads = np.maximum(1, np.rint(num_quads / 2 * length / length.sum())).astype(np.int64)
+    cols = np.repeat(quads + 1, 2)
+    return np.full_like(cols, 2), cols
+
+
+def largesteps_section(dev, reps=50):
+    """The largesteps system of 3_fitting.py:347-384 on grid meshes: factor
+    once, then apply (to_differential) and solve (from_differential) timed
+    with HIP events on the current stream, for the strips of 2_texture.py's
+    defaults and for a batch of 16,384 strips of 2 x 128.  Algorithmic
+    bytes: 24 per vertex (x or u read, the result written).  CPU
+    counterpart: scipy's sparse LU of the same M (factored once), one solve
+    of the default strips on one core."""
+    import numpy as np
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    import torch
+
+    from strns2cards_amd.optimizer import GridSystem, grid_edges
+
+    stream = torch.cuda.current_stream()
+    out = {}
+    for name, (rows, cols) in [("reference_default_strips", strip_layout_default()),
+                               ("batch_16384x2x128", (np.full(16384, 2), np.full(16384, 128)))]:
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        ev[0].record(stream)
+        M = GridSystem(rows, cols, 100.0, 0.01, device=dev)
+        ev[1].record(stream)
+        g = torch.Generator(device=dev).manual_seed(1)
+        x = torch.randn(M.num_verts, 3, device=dev, generator=g)
+        u = M.apply(x)
+        M.solve(u)
+        torch.cuda.synchronize()
+        ev[2].record(stream)
+        for _ in range(reps):
+            M.apply(x)
+        ev[3].record(stream)
+        for _ in range(reps):
+            M.solve(u)
+        ev[4].record(stream)
+        torch.cuda.synchronize()
+        apply_ms = ev[2].elapsed_time(ev[3]) / reps
+        solve_ms = ev[3].elapsed_time(ev[4]) / reps
+        nbytes = 24 * M.num_verts
+        err = ((M.solve(u) - x).norm() / x.norm()).item()
+        out[name] = {"grids": int(M.num_grids), "vertices": int(M.num_verts),
+                     "max_columns": int(np.max(cols)),
+                     "factor_ms_incl_setup": round(ev[0].elapsed_time(ev[1]), 4),
+                     "apply_us": round(apply_ms * 1e3, 2), "solve_us": round(solve_ms * 1e3, 2),
+                     "solve_GBps": round(nbytes / solve_ms / 1e6, 1),
+                     "solve_frac_of_hbm": round(nbytes / solve_ms / 1e6 / HBM_PEAK_GBS, 4),
+                     "round_trip_rel_err": float(f"{err:.3e}")}
+        if name == "reference_default_strips":
+            V = M.num_verts
+            eh, ew = grid_edges(M.rows_host, M.cols_host)
+
+            def lap(e):
+                e = np.concatenate([e, e[:, ::-1]])
+                A = sp.coo_matrix((np.ones(len(e)), (e[:, 0], e[:, 1])), shape=(V, V)).tocsr()
+                return sp.diags(np.asarray(A.sum(1)).ravel()) - A
+
+            Ms = (sp.eye(V) + 100.0 * (lap(eh) + 0.01 * lap(ew))).tocsc()
+            t0 = time.perf_counter()
+            lu = spla.splu(Ms)
+            t1 = time.perf_counter()
+            ub = u.cpu().numpy().astype(np.float64)
+            n = 0
+            while time.perf_counter() - t1 < 1.0 or n < 3:
+                lu.solve(ub)
+                n += 1
+            t2 = time.perf_counter()
+            out[name]["cpu_scipy_splu"] = {"factor_ms": round((t1 - t0) * 1e3, 3),
+                                           "solve_us": round((t2 - t1) / n * 1e6, 1),
+                                           "cores": 1, "dtype": "f64"}
+    print(f"[bench] largesteps: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def copy_calibration(dev, nbytes=1 << 29, reps=3):
     """A plain device-to-device copy of known size, so PMC byte counters can
     be scaled against it."""
@@ -231,6 +318,9 @@ def main():
     ap.add_argument("--no-fit", action="store_true")
     ap.add_argument("--fit-jobs", type=int, default=4096)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-largesteps", action="store_true")
+    ap.add_argument("--largesteps-only", action="store_true",
+                    help="only the large-steps factor / apply / solve section")
     ap.add_argument("--dda-only", action="store_true",
                     help="time the DDA->mask step only (for PMC runs)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -251,6 +341,9 @@ def main():
     from strns2cards_amd import _core, synth
 
     dev = torch.device("cuda", 0)
+    if args.largesteps_only:
+        print(json.dumps({"largesteps": largesteps_section(dev)}), flush=True)
+        return
     t_gen = time.perf_counter()
     w = synth.full_workload()
     gen_s = time.perf_counter() - t_gen
@@ -344,6 +437,7 @@ def main():
     fit = None if args.no_fit else fit_section(dev, num_jobs=args.fit_jobs,
                                                    tune=not args.no_autotune)
     adj = None if args.no_adjoint else adjoint_section(dev)
+    lsteps = None if args.no_largesteps else largesteps_section(dev)
 
     cpu = None
     if not args.no_cpu:
@@ -382,6 +476,7 @@ def main():
                   "cards": int(c.tri.shape[0] // 40), "card_triangles": int(c.tri.shape[0])},
         "card_fit": fit,
         "adjoint_fit": adj,
+        "largesteps": lsteps,
     }
     print(json.dumps(out), flush=True)
 

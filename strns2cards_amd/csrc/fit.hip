@@ -219,7 +219,8 @@ fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict
             (&U[0][0])[q] = grid_apply_at(2, N, hp.lam, hp.ww, &P[0][0], q);
         __syncthreads();
     }
-    // in-place M^{-1} over a (V, 3) LDS array (three workgroup barriers)
+    // in-place M^{-1} over a (V, 3) LDS array (three workgroup barriers; the
+    // sweeps over LDS: their register form would cost this kernel occupancy)
     auto solve = [&](float* a) {
         grid_project<kBlock>(2, N, Qb, a, Ysol, tid);
         __syncthreads();

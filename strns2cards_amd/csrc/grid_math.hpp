@@ -23,6 +23,23 @@
 
 namespace s2c {
 
+// v / n and v % n for 0 <= v < 2^22 without an integer divide: the float
+// quotient is within 1 of the exact one (two roundings of relative size
+// 2^-24 on a value below 2^22), and one correction step fixes it.
+struct DivN {
+    int n;
+    float inv;
+    __device__ __forceinline__ explicit DivN(int n_) : n(n_), inv(1.f / static_cast<float>(n_)) {}
+    __device__ __forceinline__ int div(int v, int& rem) const {
+        int d = static_cast<int>(static_cast<float>(v) * inv);
+        int r = v - d * n;
+        if (r < 0) { --d; r += n; }
+        else if (r >= n) { ++d; r -= n; }
+        rem = r;
+        return d;
+    }
+};
+
 __device__ __forceinline__ float grid_dct_q(int i, int k, int S) {
     const float pi = 3.14159265358979323846f;
     const float fs = static_cast<float>(S);
@@ -45,6 +62,12 @@ __device__ __forceinline__ void grid_factor(int S, int N, int k, float lam, floa
     }
 }
 
+// Chunk length of a lane in the wave solve of a grid with 3S systems of N:
+// odd, so that the lanes' strided LDS accesses fall on distinct banks.
+__device__ __forceinline__ int grid_chunk(int S, int N) {
+    return ((3 * S * N + kWave - 1) / kWave) | 1;
+}
+
 // One wave: y <- T^{-1} y for the 3S systems of one grid stored back to
 // back in Y[(k*3 + c)*N + j], system k*3 + c using the pivots im[k*N ..].
 // Both sweeps are first-order affine recurrences over the whole array
@@ -57,9 +80,11 @@ __device__ __forceinline__ void grid_factor(int S, int N, int k, float lam, floa
 __device__ __forceinline__ void grid_solve_wave(int S, int N, float lam, const float* im,
                                                 float* Y, int lane) {
     const int n = 3 * S * N;
-    const int L = (n + kWave - 1) / kWave;
+    const int L = grid_chunk(S, N);
     const int lo = min(lane * L, n), hi = min(lo + L, n);
-    const int kc0 = lo / N, j0 = lo - kc0 * N;
+    const DivN dn(N);
+    int j0;
+    const int kc0 = dn.div(lo, j0);
     // forward sweep: the chunk maps d_in to A d_in + B
     float A = 1.f, B = 0.f;
     {
@@ -92,7 +117,8 @@ __device__ __forceinline__ void grid_solve_wave(int S, int N, float lam, const f
         }
     }
     // backward sweep over the same chunk, last element first
-    const int kc1 = (hi - 1) / N, j1 = (hi - 1) - kc1 * N;
+    int j1;
+    const int kc1 = dn.div(max(hi - 1, 0), j1);
     A = 1.f;
     B = 0.f;
     {
@@ -124,13 +150,91 @@ __device__ __forceinline__ void grid_solve_wave(int S, int N, float lam, const f
     }
 }
 
+// One wave, for grids whose chunk fits kL registers: the projection
+// Y = Q^T in (per column) and both elimination sweeps of grid_solve_wave in
+// registers, the result written to Y[(k*3 + c)*N + j].  `in` is the grid
+// vertex-major in LDS, `im` the pivots (im[k*N + j]).  Same chunks and the
+// same operations per element as grid_project + grid_solve_wave.
+template <int kL>
+__device__ __forceinline__ void grid_solve_wave_regs(int S, int N, float lam, const float* Qb,
+                                                     const float* in, const float* im, float* Y,
+                                                     int lane) {
+    const int n = 3 * S * N;
+    const int L = grid_chunk(S, N);
+    const int lo = min(lane * L, n), hi = min(lo + L, n);
+    const int cnt = hi - lo;
+    const DivN dn(N);
+    int j;
+    int kc = dn.div(lo, j);
+    // per element: value r, pivot m and the sweep coefficients (forward a,
+    // backward c); elements past the chunk are identity maps (a = c = 1,
+    // r = m = 0)
+    float r[kL], m[kL], a[kL], c[kL];
+#pragma unroll
+    for (int e = 0; e < kL; ++e) {
+        const bool ok = e < cnt;
+        const int k = ok ? kc / 3 : 0, cc = ok ? kc - 3 * (kc / 3) : 0, jj = ok ? j : 0;
+        float acc = 0.f;
+        for (int i = 0; i < S; ++i) acc += Qb[i * S + k] * in[(i * N + jj) * 3 + cc];
+        const float mm = im[k * N + jj];
+        r[e] = ok ? acc : 0.f;
+        m[e] = ok ? mm : 0.f;
+        a[e] = ok ? (jj == 0 ? 0.f : lam * mm) : 1.f;
+        c[e] = ok ? (jj == N - 1 ? 0.f : lam * mm) : 1.f;
+        if (++j == N) { j = 0; ++kc; }
+    }
+    float A = 1.f, B = 0.f;
+#pragma unroll
+    for (int e = 0; e < kL; ++e) {
+        A = a[e] * A;
+        B = a[e] * B + m[e] * r[e];
+    }
+    for (int s = 1; s < kWave; s <<= 1) {
+        const float pA = __shfl_up(A, s, kWave), pB = __shfl_up(B, s, kWave);
+        if (lane >= s) {
+            B = A * pB + B;
+            A = A * pA;
+        }
+    }
+    float d = __shfl_up(B, 1, kWave);
+    if (lane == 0) d = 0.f;
+#pragma unroll
+    for (int e = 0; e < kL; ++e) {
+        d = a[e] * d + m[e] * r[e];
+        r[e] = e < cnt ? d : 0.f;  // padding stays an identity map
+    }
+    A = 1.f;
+    B = 0.f;
+#pragma unroll
+    for (int e = kL - 1; e >= 0; --e) {
+        A = c[e] * A;
+        B = c[e] * B + r[e];
+    }
+    for (int s = 1; s < kWave; s <<= 1) {
+        const float nA = __shfl_down(A, s, kWave), nB = __shfl_down(B, s, kWave);
+        if (lane + s < kWave) {
+            B = A * nB + B;
+            A = A * nA;
+        }
+    }
+    float y = __shfl_down(B, 1, kWave);
+    if (lane == kWave - 1) y = 0.f;
+#pragma unroll
+    for (int e = kL - 1; e >= 0; --e) {
+        y = c[e] * y + r[e];
+        if (e < cnt) Y[lo + e] = y;
+    }
+}
+
 // Y[(k*3 + c)*N + j] = sum_i Q[i][k] * in[(i*N + j)*3 + c]   (Q^T per column)
 template <int kThreads>
 __device__ __forceinline__ void grid_project(int S, int N, const float* Qb, const float* in,
                                              float* Y, int tid) {
     const int n = 3 * S * N;
+    const DivN dn(N);
     for (int q = tid; q < n; q += kThreads) {
-        const int kc = q / N, j = q - kc * N;
+        int j;
+        const int kc = dn.div(q, j);
         const int k = kc / 3, c = kc - 3 * k;
         float r = 0.f;
         for (int i = 0; i < S; ++i) r += Qb[i * S + k] * in[(i * N + j) * 3 + c];
@@ -143,9 +247,11 @@ template <int kThreads, class Out>
 __device__ __forceinline__ void grid_back_project(int S, int N, const float* Qb, const float* Y,
                                                   Out out, int tid) {
     const int n = 3 * S * N;
+    const DivN dn(N);
     for (int q = tid; q < n; q += kThreads) {
         const int v = q / 3, c = q - 3 * v;
-        const int i = v / N, j = v - i * N;
+        int j;
+        const int i = dn.div(v, j);
         float acc = 0.f;
         for (int k = 0; k < S; ++k) acc += Qb[i * S + k] * Y[(k * 3 + c) * N + j];
         out(q, acc);

@@ -6,7 +6,8 @@
 // Thomas pivots of each grid shape's decoupled tridiagonal systems
 // (grid_math.hpp) and a solve is one workgroup (one wave) per grid:
 // the grid's 3 S N floats staged in LDS with coalesced loads, projected onto
-// the row basis, both elimination sweeps as wave scans, and the
+// the row basis, both elimination sweeps as wave scans (over registers when
+// a lane's chunk has at most 16 elements, else over LDS), and the
 // back-projection stored coalesced.
 #include "grid_math.hpp"
 
@@ -70,12 +71,19 @@ grid_solve_kernel(const int* __restrict__ rows, const int* __restrict__ cols,
     for (int q = t; q < n; q += kWave) U[q] = u[base + q];
     for (int q = t; q < S * S; q += kWave) Qb[q] = grid_dct_q(q / S, q % S, S);
     __syncthreads();
-    grid_project<kWave>(S, N, Qb, U, Y, t);
-    __syncthreads();
     const float* im = factor + fac_off[g];
-    for (int q = t; q < S * N; q += kWave) U[q] = im[q];
-    __syncthreads();
-    grid_solve_wave(S, N, lam, U, Y, t);
+    const int L = grid_chunk(S, N);
+    if (L <= 8) {  // chunks in registers (wave-uniform branch)
+        grid_solve_wave_regs<8>(S, N, lam, Qb, U, im, Y, t);
+    } else if (L <= 16) {
+        grid_solve_wave_regs<16>(S, N, lam, Qb, U, im, Y, t);
+    } else {  // long grids: the sweeps over LDS
+        grid_project<kWave>(S, N, Qb, U, Y, t);
+        __syncthreads();
+        for (int q = t; q < S * N; q += kWave) U[q] = im[q];
+        __syncthreads();
+        grid_solve_wave(S, N, lam, U, Y, t);
+    }
     __syncthreads();
     float* xg = x + base;
     grid_back_project<kWave>(S, N, Qb, Y, [&](int q, float v) { xg[q] = v; }, t);
