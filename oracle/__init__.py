@@ -15,6 +15,9 @@ force / invariants: DDA span+fragments, triangle raster, depth test, cluster
 masks, bitset RGB/popcount, silhouette marking, compaction, smoothing,
 skinning.  The antialias backward is pinned by hand-computable cases (one
 straight edge, translation equivariance, rho scaling) — see its tests.
+The large-steps section (float64 numpy: Laplacian, system matrix, Cholesky
+solve) is pinned by hand-computed matrices and the closed-form spectrum of
+a grid's system matrix (tests/test_oracle_largesteps.py).
 """
 from __future__ import annotations
 
