@@ -295,6 +295,58 @@ def largesteps_section(dev, reps=50):
     return out
 
 
+def card_mesh_section(dev, reps=20):
+    """Card mesh inputs on the GPU for 2_texture.py's defaults (300 guide
+    strands of lognormal lengths sharing 7,500 quads, two crossed 2-row
+    strips each): strips (strand_to_strip_mesh), then the grid meshes'
+    faces / quads / uvs / h_edges / w_edges / face clusters
+    (combine_wisps_with_cluster_labels), timed with HIP events.  CPU
+    baseline: the oracle's numpy restatement of the same two functions
+    (the reference computes them with numpy on the host), one core."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from strns2cards_amd import cardmesh
+
+    rng = np.random.default_rng(3)
+    rows, cols = strip_layout_default()
+    counts = cols[::2]
+    strands = [np.cumsum(rng.normal(size=(n, 3)) * 0.01, 0).astype(np.float32) for n in counts]
+    rects = rng.uniform(0, 1, (len(cols), 4))
+    g2c = rng.integers(0, 300, len(cols))
+    pts, cnt, offs, _ = cardmesh._ragged(strands, dev)
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    cardmesh.strands_to_strips_device(pts, cnt, offs, 2, 0.1, [-45.0, 45.0])
+    cardmesh.grid_mesh_topology_device(rows, cols, rects, g2c, device=dev)
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    for _ in range(reps):
+        cardmesh.strands_to_strips_device(pts, cnt, offs, 2, 0.1, [-45.0, 45.0])
+    ev[1].record(stream)
+    for _ in range(reps):
+        cardmesh.grid_mesh_topology_device(rows, cols, rects, g2c, device=dev)
+    ev[2].record(stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in strands:
+        oracle.strand_to_strip_mesh(s, 2, 0.1, [-45.0, 45.0])
+    t1 = time.perf_counter()
+    oracle.combine_grid_meshes(list(zip(rows.tolist(), cols.tolist())), rects.tolist(),
+                               g2c.tolist())
+    t2 = time.perf_counter()
+    out = {"guides": len(strands), "strips": int(len(cols)), "vertices": int(2 * cols.sum()),
+           "quads": int(((rows - 1) * (cols - 1)).sum()),
+           "strips_ms": round(ev[0].elapsed_time(ev[1]) / reps, 4),
+           "topology_ms_incl_host_layout": round(ev[1].elapsed_time(ev[2]) / reps, 4),
+           "cpu_baseline": {"strips_ms": round((t1 - t0) * 1e3, 2),
+                            "topology_ms": round((t2 - t1) * 1e3, 2), "cores": 1,
+                            "kind": "port (numpy, the reference's host algorithm)"}}
+    print(f"[bench] card mesh: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def copy_calibration(dev, nbytes=1 << 29, reps=3):
     """A plain device-to-device copy of known size, so PMC byte counters can
     be scaled against it."""
@@ -320,7 +372,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-largesteps", action="store_true")
     ap.add_argument("--largesteps-only", action="store_true",
-                    help="only the large-steps factor / apply / solve section")
+                    help="only the large-steps solve and card-mesh sections")
     ap.add_argument("--dda-only", action="store_true",
                     help="time the DDA->mask step only (for PMC runs)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -342,7 +394,8 @@ def main():
 
     dev = torch.device("cuda", 0)
     if args.largesteps_only:
-        print(json.dumps({"largesteps": largesteps_section(dev)}), flush=True)
+        print(json.dumps({"largesteps": largesteps_section(dev),
+                          "card_mesh": card_mesh_section(dev)}), flush=True)
         return
     t_gen = time.perf_counter()
     w = synth.full_workload()
@@ -438,6 +491,7 @@ def main():
                                                    tune=not args.no_autotune)
     adj = None if args.no_adjoint else adjoint_section(dev)
     lsteps = None if args.no_largesteps else largesteps_section(dev)
+    cmesh = None if args.no_largesteps else card_mesh_section(dev)
 
     cpu = None
     if not args.no_cpu:
@@ -477,6 +531,7 @@ def main():
         "card_fit": fit,
         "adjoint_fit": adj,
         "largesteps": lsteps,
+        "card_mesh": cmesh,
     }
     print(json.dumps(out), flush=True)
 

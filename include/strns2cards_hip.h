@@ -319,6 +319,43 @@ int s2c_grid_system_solve(int num_grids, const int* rows, const int* cols, const
                           const int* fac_off, const float* factor, int max_verts, int max_rows,
                           float lam, const float* u, float* x, void* stream);
 
+/* ---------------------------------------------------------------------- */
+/* Card mesh inputs                                                       */
+/* ---------------------------------------------------------------------- */
+/* Replaces strand_to_strip_mesh (2_texture.py:17-96, host numpy in the
+ * reference) for many strands: strand s has counts[s] >= 2 float32 xyz
+ * points at points + 3 * offsets[s] (int64 point offsets, the prefix of
+ * counts).  For every strand and each of the num_rot (1..16) tangent
+ * rotations rot_deg (a HOST array, degrees) one (res_w, counts[s], 3)
+ * strip is written at out + 3 * res_w * (num_rot * offsets[s] +
+ * r * counts[s]): the strips in the order 2_texture.py:304-312 collects
+ * them, i.e. the vertex array of their combined grid meshes.  res_w >= 2;
+ * width as the reference's float argument. */
+int s2c_strands_to_strips(int num_strands, const int* counts, const long long* offsets,
+                          const float* points, int res_w, double width, int num_rot,
+                          const float* rot_deg, float* out, void* stream);
+
+/* Replaces wisp_to_grid_mesh + combine_wisps_with_cluster_labels
+ * (texture_utils.py:525-655, host numpy): grid g has rows[g] x cols[g]
+ * vertices from voff[g]; its (rows-1)(cols-1) quads start at qoff[g]
+ * (faces at 2 * qoff[g]), its rows*(cols-1) h_edges at hoff[g] and its
+ * (rows-1)*cols w_edges at woff[g] (int32 arrays, prefixes of those
+ * counts).  faces (F,3), quads (Q,4), h_edges (H,2), w_edges (W,2) int32.
+ * Optional pairs (both NULL or both given): grid2cluster (int64 per grid)
+ * -> face2cluster (int64 per face); rects (float64 [u0, v0, u1, v1] per
+ * grid) -> uvs (float64 (V,2)), with v1 reduced by margin_top. */
+int s2c_grid_mesh_topology(int num_grids, const int* rows, const int* cols, const int* voff,
+                           const int* qoff, const int* hoff, const int* woff,
+                           const long long* grid2cluster, const double* rects, double margin_top,
+                           int* faces, int* quads, int* h_edges, int* w_edges,
+                           long long* face2cluster, double* uvs, void* stream);
+
+/* Replaces build_clustered_strand_edges (3_fitting.py:17-60): edges (E,2)
+ * between consecutive points of each strand (counts[s] >= 1 points from
+ * point offset offsets[s]) and edge2cluster[e] = cluster_ids[strand]. */
+int s2c_strand_edges(int num_strands, const int* counts, const long long* offsets,
+                     const int* cluster_ids, int* edges, int* edge2cluster, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

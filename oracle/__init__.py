@@ -511,3 +511,109 @@ def from_differential(M: np.ndarray, u) -> np.ndarray:
     C = np.linalg.cholesky(M)
     y = solve_triangular(C, np.asarray(u, np.float64), lower=True)
     return solve_triangular(C.T, y, lower=False)
+
+
+# --------------------------------------------------------------- card meshes
+
+def strand_to_strip_mesh(strand, res_w=2, width=0.5, tangent_rotations_deg=(0.0,)):
+    """Strips of one strand, as 2_texture.py:17-96: central-difference
+    tangents, a root frame from the change of tangent (or a fixed axis when
+    it is below 1e-4), rotated about the root tangent by each angle (Rodrigues),
+    propagated by the double-reflection rotation-minimizing frame, and the
+    strip swept along the binormal at res_w offsets across `width`.  float32
+    like the reference's arrays (its angle terms are float64 and rounded
+    into the float32 frame).  Returns a list of (res_w, S, 3) float32."""
+    p = np.asarray(strand, np.float32)
+    S = p.shape[0]
+    d = p[1:] - p[:-1]
+    t = np.zeros_like(p)
+    t[:-1] += d
+    t[1:] += d
+    t /= np.maximum(np.linalg.norm(t, axis=1, keepdims=True), np.float32(1e-8))
+    T0 = t[0]
+    dT = t[1] - t[0]
+    if np.linalg.norm(dT) < 1e-4:
+        n = np.array([0.0, 1.0, 0.0], np.float32)
+        if abs(np.dot(T0, n)) > 0.9:
+            n = np.array([0.0, 0.0, 1.0], np.float32)
+    else:
+        n = -dT / np.linalg.norm(dT)
+    b = np.cross(T0, n)
+    b /= max(np.linalg.norm(b), np.float32(1e-8))
+    n = np.cross(b, T0)
+    off = np.linspace(-0.5 * width, 0.5 * width, res_w, dtype=np.float32)
+    strips = []
+    for deg in tangent_rotations_deg:
+        th = np.deg2rad(deg)
+        N0 = np.cos(th) * n + np.sin(th) * np.cross(T0, n) + (1.0 - np.cos(th)) * np.dot(T0, n) * T0
+        Ni = N0.astype(np.float32)
+        Bs = np.zeros_like(p)
+        Bs[0] = np.cross(T0, N0)
+        for i in range(1, S):
+            v = p[i] - p[i - 1]
+            vn = np.linalg.norm(v)
+            if vn < 1e-8:
+                Bs[i] = Bs[i - 1]
+                continue
+            v = v / vn
+            nl = Ni - 2 * np.dot(Ni, v) * v
+            tl = t[i - 1] - 2 * np.dot(t[i - 1], v) * v
+            v2 = t[i] - tl
+            v2 /= max(np.linalg.norm(v2), np.float32(1e-8))
+            n1 = nl - 2 * np.dot(nl, v2) * v2
+            n1 /= max(np.linalg.norm(n1), np.float32(1e-8))
+            Ni = n1
+            Bs[i] = np.cross(t[i], n1)
+        strips.append((p[None] + off[:, None, None] * Bs[None]).astype(np.float32))
+    return strips
+
+
+def grid_mesh(S: int, N: int, uv_rect=None, margin_top=1e-4):
+    """Faces, quads, h_edges, w_edges (and uvs) of one (S, N) wisp, as
+    texture_utils.py:525-599 (Python loops: small cases)."""
+    faces, quads = [], []
+    for i in range(S - 1):
+        for j in range(N - 1):
+            a, b, c, d = i * N + j, (i + 1) * N + j, (i + 1) * N + j + 1, i * N + j + 1
+            faces += [[a, b, c], [a, c, d]]
+            quads.append([a, b, c, d])
+    h, w = wisp_grid_edges(S, N)
+    out = dict(faces=np.array(faces, np.int32).reshape(-1, 3),
+               quads=np.array(quads, np.int32).reshape(-1, 4),
+               h_edges=h.astype(np.int32), w_edges=w.astype(np.int32))
+    if uv_rect is not None:
+        u0, v0, u1, v1 = uv_rect
+        uu, vv = np.meshgrid(np.linspace(u0, u1, S), np.linspace(v0, v1 - margin_top, N),
+                             indexing="ij")
+        out["uvs"] = np.stack([uu, vv], -1).reshape(-1, 2)
+    return out
+
+
+def combine_grid_meshes(shapes, rects, grid2cluster):
+    """texture_utils.py:597-655: the wisps' meshes concatenated with running
+    vertex offsets and a cluster label per face.  Returns (faces, uvs,
+    face2cluster, quads, h_edges, w_edges)."""
+    parts = {k: [] for k in ("faces", "uvs", "f2c", "quads", "h_edges", "w_edges")}
+    off = 0
+    for (S, N), rect, c in zip(shapes, rects, grid2cluster):
+        m = grid_mesh(S, N, rect)
+        for k in ("faces", "quads", "h_edges", "w_edges"):
+            parts[k].append(m[k] + off)
+        parts["uvs"].append(m["uvs"])
+        parts["f2c"].append(np.full(m["faces"].shape[0], c))
+        off += S * N
+    cat = np.concatenate
+    return (cat(parts["faces"]), cat(parts["uvs"]), cat(parts["f2c"]), cat(parts["quads"]),
+            cat(parts["h_edges"]), cat(parts["w_edges"]))
+
+
+def clustered_strand_edges(counts, cluster_ids):
+    """Edges between consecutive points of each strand and the strand's
+    cluster per edge, as 3_fitting.py:17-60 (points concatenated)."""
+    edges, e2c, off = [], [], 0
+    for n, c in zip(counts, cluster_ids):
+        edges += [[off + i, off + i + 1] for i in range(n - 1)]
+        e2c += [c] * (n - 1)
+        off += n
+    return (np.array(edges, np.int32).reshape(-1, 2), np.array(e2c, np.int32),
+            max([c + 1 for c in cluster_ids], default=0))

@@ -26,7 +26,8 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["scan.hip", "binning.hip", "dda.hip", "dda_mask.hip", "raster.hip", "bitset.hip",
-               "antialias.hip", "curve.hip", "fit.hip", "coverage.hip", "gridsolve.hip"]
+               "antialias.hip", "curve.hip", "fit.hip", "coverage.hip", "gridsolve.hip",
+               "cardmesh.hip"]
 
 # Contraction off and correctly rounded divide/sqrt are the numerics contract
 # with the CPU oracle; packed fp32 is disabled for the device code.

@@ -12,6 +12,7 @@
 #include <ATen/hip/impl/HIPStreamMasqueradingAsCUDA.h>
 
 #include <string>
+#include <optional>
 #include <vector>
 
 #include "../../include/strns2cards_hip.h"
@@ -25,7 +26,8 @@ using DeviceGuard = c10::hip::HIPGuardMasqueradingAsCUDA;
 namespace {
 
 const char* dtype_name(at::ScalarType t) {
-    return t == at::kFloat ? "float32" : t == at::kInt ? "int32" : "other";
+    return t == at::kFloat ? "float32" : t == at::kInt ? "int32" : t == at::kLong ? "int64"
+           : t == at::kDouble ? "float64" : "other";
 }
 
 // Shape spec: -1 = any extent.
@@ -707,6 +709,79 @@ PYBIND11_MODULE(_core, m) {
         check(s2c_grid_system_solve(G, P<int>(rows), P<int>(cols), P<int>(voff), P<int>(fac_off),
                                     P<float>(factor), max_verts, max_rows, lam, P<float>(u),
                                     P<float>(x), stream_of(u)),
+              fn);
+    });
+
+    // Card mesh inputs (no reference binding: the scripts build these on
+    // the host, 2_texture.py:17-96, texture_utils.py:525-655,
+    // 3_fitting.py:17-60).
+    m.def("strands_to_strips", [](Tensor counts, Tensor offsets, Tensor points, int res_w,
+                                  double width, std::vector<float> rot_deg, Tensor out) {
+        const char* fn = "strands_to_strips";
+        need(counts, fn, "counts", at::kInt, {-1});
+        need(offsets, fn, "offsets", at::kLong, {counts.size(0)});
+        need(points, fn, "points", at::kFloat, {-1, 3});
+        need(out, fn, "out", at::kFloat, {-1, 3});
+        same_device(fn, {&counts, &offsets, &points, &out});
+        if (out.size(0) != points.size(0) * res_w * static_cast<int64_t>(rot_deg.size()))
+            throw std::runtime_error("strands_to_strips: out must hold res_w * rotations * points rows");
+        DeviceGuard g(points.device());
+        check(s2c_strands_to_strips(I(counts.size(0), fn), P<int>(counts), P<long long>(offsets),
+                                    P<float>(points), res_w, width, I(rot_deg.size(), fn),
+                                    rot_deg.data(), P<float>(out), stream_of(points)),
+              fn);
+    });
+    m.def("grid_mesh_topology", [](Tensor rows, Tensor cols, Tensor voff, Tensor qoff, Tensor hoff,
+                                   Tensor woff, std::optional<Tensor> grid2cluster,
+                                   std::optional<Tensor> rects, double margin_top, Tensor faces,
+                                   Tensor quads, Tensor h_edges, Tensor w_edges,
+                                   std::optional<Tensor> face2cluster, std::optional<Tensor> uvs) {
+        const char* fn = "grid_mesh_topology";
+        need(rows, fn, "rows", at::kInt, {-1});
+        const int64_t G = rows.size(0);
+        for (auto* t : {&cols, &voff, &qoff, &hoff, &woff}) need(*t, fn, "layout", at::kInt, {G});
+        need(faces, fn, "faces", at::kInt, {-1, 3});
+        need(quads, fn, "quads", at::kInt, {-1, 4});
+        need(h_edges, fn, "h_edges", at::kInt, {-1, 2});
+        need(w_edges, fn, "w_edges", at::kInt, {-1, 2});
+        if (faces.size(0) != 2 * quads.size(0))
+            throw std::runtime_error("grid_mesh_topology: faces must have 2 rows per quad");
+        same_device(fn, {&rows, &cols, &voff, &qoff, &hoff, &woff, &faces, &quads, &h_edges, &w_edges});
+        if (grid2cluster.has_value() != face2cluster.has_value() || rects.has_value() != uvs.has_value())
+            throw std::runtime_error("grid_mesh_topology: grid2cluster/face2cluster and rects/uvs go in pairs");
+        if (grid2cluster) {
+            need(*grid2cluster, fn, "grid2cluster", at::kLong, {G});
+            need(*face2cluster, fn, "face2cluster", at::kLong, {faces.size(0)});
+            same_device(fn, {&rows, &*grid2cluster, &*face2cluster});
+        }
+        if (rects) {
+            need(*rects, fn, "rects", at::kDouble, {G, 4});
+            need(*uvs, fn, "uvs", at::kDouble, {-1, 2});
+            same_device(fn, {&rows, &*rects, &*uvs});
+        }
+        DeviceGuard g(rows.device());
+        check(s2c_grid_mesh_topology(
+                  I(G, fn), P<int>(rows), P<int>(cols), P<int>(voff), P<int>(qoff), P<int>(hoff),
+                  P<int>(woff), grid2cluster ? P<long long>(*grid2cluster) : nullptr,
+                  rects ? P<double>(*rects) : nullptr, margin_top, P<int>(faces), P<int>(quads),
+                  P<int>(h_edges), P<int>(w_edges),
+                  face2cluster ? P<long long>(*face2cluster) : nullptr,
+                  uvs ? P<double>(*uvs) : nullptr, stream_of(rows)),
+              fn);
+    });
+    m.def("strand_edges", [](Tensor counts, Tensor offsets, Tensor cluster_ids, Tensor edges,
+                             Tensor edge2cluster) {
+        const char* fn = "strand_edges";
+        need(counts, fn, "counts", at::kInt, {-1});
+        need(offsets, fn, "offsets", at::kLong, {counts.size(0)});
+        need(cluster_ids, fn, "cluster_ids", at::kInt, {counts.size(0)});
+        need(edges, fn, "edges", at::kInt, {-1, 2});
+        need(edge2cluster, fn, "edge2cluster", at::kInt, {edges.size(0)});
+        same_device(fn, {&counts, &offsets, &cluster_ids, &edges, &edge2cluster});
+        DeviceGuard g(counts.device());
+        check(s2c_strand_edges(I(counts.size(0), fn), P<int>(counts), P<long long>(offsets),
+                               P<int>(cluster_ids), P<int>(edges), P<int>(edge2cluster),
+                               stream_of(counts)),
               fn);
     });
 }

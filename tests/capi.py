@@ -34,6 +34,8 @@ def _ctype(param: str):
         return ctypes.c_void_p
     if param.startswith("float"):
         return ctypes.c_float
+    if param.startswith("double"):
+        return ctypes.c_double
     if param.startswith("int"):
         return ctypes.c_int
     raise ValueError(f"unsupported parameter {param!r}")

@@ -1,0 +1,119 @@
+"""GPU parity of the card-mesh inputs against the oracle.
+
+* Topology (faces, quads, edges, face clusters) and strand edges: bit-exact.
+  uvs: bit-exact (float64, numpy's linspace restated).
+* Strips: the oracle follows the reference's numpy float32 arithmetic,
+  whose 3-vector norms and dots go through BLAS; the kernel evaluates them
+  in a fixed order.  Tolerance: |strip - strip_ref| <= 2e-5 * width + 4 ulp
+  of the coordinates (frame directions agree to 4e-5 rad along 200-point
+  strands).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from strns2cards_amd import cardmesh
+from strns2cards_amd.optimizer import GridSystem
+
+pytestmark = pytest.mark.gpu
+
+
+def _strands(count, seed, lo=2, hi=80):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.integers(lo, hi + 1))
+        steps = rng.normal(size=(n, 3)) * 0.05 + [0.0, -0.1, 0.02]
+        s = np.cumsum(steps, 0) + rng.normal(size=3)
+        if i % 7 == 3 and n > 4:
+            s[n // 2] = s[n // 2 - 1]  # a zero-length segment
+        if i % 11 == 5:
+            s = np.stack([np.linspace(0, 1, n), np.zeros(n), np.zeros(n)], 1) + rng.normal(size=3)
+        out.append(s.astype(np.float32))
+    return out
+
+
+def _check_strips(got, want, width):
+    scale = np.abs(want).max()
+    tol = 2e-5 * width + 4 * np.spacing(np.float32(scale))
+    err = np.abs(got - want).max()
+    assert err <= tol, (err, tol)
+
+
+@pytest.mark.parametrize("res_w,width,rots", [(2, 0.1, [-45.0, 45.0]), (3, 0.5, [0.0]),
+                                              (4, 1.0, [30.0, 60.0, 90.0, 180.0])])
+def test_strips_match_oracle(dev, res_w, width, rots):
+    print(f"[strips] res_w={res_w} rots={rots}", flush=True)
+    strands = _strands(64, res_w)
+    pts, counts, offsets, counts_np = cardmesh._ragged(strands, dev)
+    out = cardmesh.strands_to_strips_device(pts, counts, offsets, res_w, width, rots)
+    got = out.cpu().numpy()
+    off = 0
+    for s in strands:
+        for strip in oracle.strand_to_strip_mesh(s, res_w, width, rots):
+            n = strip.size // 3
+            _check_strips(got[off:off + n].reshape(strip.shape), strip, width)
+            off += n
+    assert off == got.shape[0]
+
+
+def test_strip_mesh_drop_in(dev):
+    s = _strands(1, 9, 50, 50)[0]
+    got = cardmesh.strand_to_strip_mesh(s, 2, 0.1, [-45.0, 45.0])
+    want = oracle.strand_to_strip_mesh(s, 2, 0.1, [-45.0, 45.0])
+    assert len(got) == 2
+    for g, w in zip(got, want):
+        assert g.shape == w.shape and g.dtype == np.float32
+        _check_strips(g, w, 0.1)
+
+
+def test_topology_bit_exact(dev):
+    rng = np.random.default_rng(4)
+    shapes = [(int(rng.integers(1, 5)), int(rng.integers(1, 40))) for _ in range(50)]
+    rects = [tuple(rng.uniform(0, 1, 4)) for _ in shapes]
+    g2c = rng.integers(0, 300, len(shapes)).tolist()
+    want = oracle.combine_grid_meshes(shapes, rects, g2c)
+    wisps = [np.zeros((S, N, 3), np.float32) for S, N in shapes]
+    layout = [(i,) + r for i, r in enumerate(rects)]
+    got = cardmesh.combine_wisps_with_cluster_labels(wisps, layout, g2c)
+    for g, w, name in zip(got[1:], want, ["faces", "uvs", "face2cluster", "quads", "h", "w"]):
+        assert g.dtype == w.dtype or name in ("face2cluster",), (name, g.dtype, w.dtype)
+        np.testing.assert_array_equal(g, w, err_msg=name)
+    # one wisp, with and without uvs
+    v, f, uv, q, h, w = cardmesh.wisp_to_grid_mesh(np.zeros((3, 7, 3), np.float32), (0.1, 0.2, 0.5, 0.9))
+    m = oracle.grid_mesh(3, 7, (0.1, 0.2, 0.5, 0.9))
+    for a, b in ((f, m["faces"]), (uv, m["uvs"]), (q, m["quads"]), (h, m["h_edges"]),
+                 (w, m["w_edges"])):
+        np.testing.assert_array_equal(a, b)
+    assert len(cardmesh.wisp_to_grid_mesh(np.zeros((2, 2, 3), np.float32))) == 5
+
+
+def test_strand_edges_bit_exact(dev):
+    strands = _strands(40, 5, 1, 30)
+    ids = np.random.default_rng(6).integers(0, 1000, len(strands)).tolist()
+    pts, e, c, k = cardmesh.build_clustered_strand_edges(strands, ids)
+    we, wc, wk = oracle.clustered_strand_edges([len(s) for s in strands], ids)
+    np.testing.assert_array_equal(e, we)
+    np.testing.assert_array_equal(c, wc)
+    assert k == wk
+    np.testing.assert_array_equal(pts, np.concatenate(strands))
+
+
+def test_default_strips_feed_the_grid_solve(dev):
+    """2_texture.py's defaults (300 guides, two crossed 2-row strips each,
+    15,000 quads): strips -> topology -> GridSystem.from_edges recovers the
+    layout, and the edges' degrees match the solve's grids."""
+    rng = np.random.default_rng(7)
+    length = rng.lognormal(0.0, 0.5, 300)
+    counts = np.maximum(2, np.rint(7500 * length / length.sum()) + 1).astype(int)
+    strands = [np.cumsum(rng.normal(size=(n, 3)) * 0.01, 0).astype(np.float32) for n in counts]
+    pts, cnt, offs, _ = cardmesh._ragged(strands, dev)
+    verts = cardmesh.strands_to_strips_device(pts, cnt, offs, 2, 0.1, [-45.0, 45.0])
+    rows = np.full(2 * len(strands), 2)
+    cols = np.repeat(counts, 2)
+    topo = cardmesh.grid_mesh_topology_device(rows, cols, device=dev)
+    M = GridSystem.from_edges(verts.shape[0], topo["h_edges"], topo["w_edges"], device=dev)
+    assert M.rows_host.tolist() == rows.tolist() and M.cols_host.tolist() == cols.tolist()
+    x = M.solve(M.apply(verts))
+    assert ((x - verts).norm() / verts.norm()).item() < 1e-5

@@ -1,0 +1,103 @@
+"""Pins for the card-mesh oracle (CPU): strips of 2_texture.py:17-96, the
+grid meshes of texture_utils.py:525-655 and the strand edges of
+3_fitting.py:17-60.
+
+The strip sweep is pinned by closed forms on straight strands (whose
+rotation-minimizing frame is constant), by the Rodrigues rotation of the
+root binormal, by rigid-motion equivariance, and by frame invariants
+(the binormal is a unit vector normal to the tangent everywhere).  The
+topology is pinned by hand-enumerated meshes.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+
+def _binormals(strip, width):
+    """Per-point binormal of a strip: (last row - first row) / width."""
+    return (strip[-1] - strip[0]).astype(np.float64) / width
+
+
+def test_straight_strand_closed_form():
+    s = np.stack([np.linspace(0, 1, 5), np.zeros(5), np.zeros(5)], 1).astype(np.float32)
+    a, b = oracle.strand_to_strip_mesh(s, 3, 0.5, [0.0, 90.0])
+    # tangent +x, no curvature: the fallback normal +y, binormal x * y = +z;
+    # rotated by 90 degrees about +x the normal is +z and the binormal -y
+    for strip, axis in ((a, np.array([0, 0, 1.0])), (b, np.array([0, -1.0, 0]))):
+        assert strip.shape == (3, 5, 3)
+        for w, off in enumerate((-0.25, 0.0, 0.25)):
+            np.testing.assert_allclose(strip[w], s + off * axis, atol=1e-7)
+    # a strand along +y takes the other fallback axis (+z), binormal y * z = +x
+    s2 = s[:, [1, 0, 2]]
+    (c,) = oracle.strand_to_strip_mesh(s2, 2, 1.0)
+    np.testing.assert_allclose(c[1] - c[0], np.tile([1.0, 0, 0], (5, 1)), atol=1e-7)
+
+
+def _curve(n=40, seed=0):
+    rng = np.random.default_rng(seed)
+    t = np.linspace(0, 1, n)
+    c = np.stack([t, 0.3 * np.sin(3 * t + rng.uniform()), 0.2 * np.cos(2 * t) + 0.1 * t ** 2], 1)
+    return c.astype(np.float32)
+
+
+def test_frames_are_unit_normal_and_rotate_at_the_root():
+    s = _curve()
+    width = 0.5
+    strips = oracle.strand_to_strip_mesh(s, 2, width, [0.0, 30.0, -45.0])
+    d = np.gradient(s.astype(np.float64), axis=0)
+    t = d / np.linalg.norm(d, axis=1, keepdims=True)
+    for st in strips:
+        b = _binormals(st, width)
+        np.testing.assert_allclose(np.linalg.norm(b, axis=1), 1.0, atol=1e-5)
+        np.testing.assert_allclose(np.abs((b * t).sum(1)).max(), 0.0, atol=1e-5)
+    # the root binormal of angle a is the one of angle 0 turned by a about T0
+    T0 = t[0]
+    b0 = _binormals(strips[0], width)[0]
+    for st, deg in zip(strips[1:], (30.0, -45.0)):
+        th = np.deg2rad(deg)
+        want = np.cos(th) * b0 + np.sin(th) * np.cross(T0, b0) + (1 - np.cos(th)) * np.dot(T0, b0) * T0
+        np.testing.assert_allclose(_binormals(st, width)[0], want, atol=1e-5)
+
+
+def test_rigid_motion_equivariance():
+    s = _curve(60, 1)
+    q, _ = np.linalg.qr(np.random.default_rng(2).normal(size=(3, 3)))
+    if np.linalg.det(q) < 0:
+        q[:, 0] *= -1
+    moved = (s.astype(np.float64) @ q.T + [0.3, -1.0, 2.0]).astype(np.float32)
+    a = oracle.strand_to_strip_mesh(s, 3, 0.2, [45.0])[0].astype(np.float64)
+    b = oracle.strand_to_strip_mesh(moved, 3, 0.2, [45.0])[0].astype(np.float64)
+    np.testing.assert_allclose(b, a @ q.T + [0.3, -1.0, 2.0], atol=2e-5)
+
+
+def test_repeated_point_keeps_the_frame():
+    s = _curve(10, 3)
+    s = np.insert(s, 5, s[5], axis=0)  # a zero-length segment
+    (st,) = oracle.strand_to_strip_mesh(s, 2, 1.0)
+    b = _binormals(st, 1.0)
+    np.testing.assert_array_equal(b[6], b[5])
+
+
+def test_grid_mesh_by_hand():
+    m = oracle.grid_mesh(2, 3, (0.0, 0.0, 1.0, 1.0), margin_top=0.1)
+    assert m["faces"].tolist() == [[0, 3, 4], [0, 4, 1], [1, 4, 5], [1, 5, 2]]
+    assert m["quads"].tolist() == [[0, 3, 4, 1], [1, 4, 5, 2]]
+    assert m["h_edges"].tolist() == [[0, 1], [1, 2], [3, 4], [4, 5]]
+    assert m["w_edges"].tolist() == [[0, 3], [1, 4], [2, 5]]
+    np.testing.assert_allclose(m["uvs"], [[0, 0], [0, 0.45], [0, 0.9], [1, 0], [1, 0.45], [1, 0.9]])
+    faces, uvs, f2c, quads, h, w = oracle.combine_grid_meshes(
+        [(2, 2), (1, 3), (2, 2)], [(0, 0, 1, 1)] * 3, [7, 8, 9])
+    assert faces.tolist() == [[0, 2, 3], [0, 3, 1], [7, 9, 10], [7, 10, 8]]
+    assert f2c.tolist() == [7, 7, 9, 9]
+    assert quads.tolist() == [[0, 2, 3, 1], [7, 9, 10, 8]]
+    assert h.tolist() == [[0, 1], [2, 3], [4, 5], [5, 6], [7, 8], [9, 10]]
+    assert w.tolist() == [[0, 2], [1, 3], [7, 9], [8, 10]]
+    assert uvs.shape == (11, 2)
+
+
+def test_clustered_strand_edges_by_hand():
+    e, c, k = oracle.clustered_strand_edges([3, 1, 2], [4, 0, 2])
+    assert e.tolist() == [[0, 1], [1, 2], [4, 5]]
+    assert c.tolist() == [4, 4, 2]
+    assert k == 5
