@@ -356,6 +356,35 @@ int s2c_grid_mesh_topology(int num_grids, const int* rows, const int* cols, cons
 int s2c_strand_edges(int num_strands, const int* counts, const long long* offsets,
                      const int* cluster_ids, int* edges, int* edge2cluster, void* stream);
 
+/* ---------------------------------------------------------------------- */
+/* Software-MSAA texture patches (render_texture, texture_utils.py:106-288) */
+/* ---------------------------------------------------------------------- */
+/* The reference rasterizes each wisp's tube mesh once per sample of an
+ * 8 x 8 jitter grid (64 rasterize + interpolate calls) and accumulates with
+ * torch.  Here: s2c_msaa_jitter makes the jittered copies (num_samples, V, 4)
+ * = (x + dx[s], y + dy[s], z, 1) of the (V, 3) NDC vertices (dx, dy:
+ * device float32 arrays) for one batched rasterize; s2c_msaa_resolve
+ * reduces the (num_samples, H, W, 4) raster over its samples per pixel, in
+ * sample order, into rgb (H, W, 3) = 0.5 (mean interpolated attr over the
+ * covered samples + 1) ((0, 1, 0) + 1 halved where none) and alpha (H, W)
+ * = sum of weights[s] over covered samples (attr (V, 3), tri (T, 3),
+ * weights device float32); s2c_blur3x3_reflect blurs with the separable
+ * 3-tap kernel3 (a HOST array), reflect borders, x then y (tmp, out:
+ * (H, W)); s2c_atlas_paste writes the (atlas_y1 - atlas_y0) x (atlas_x1 -
+ * atlas_x0) rectangle of the R x R byte atlases from the vertically
+ * flipped patch starting at (crop_y0, crop_x0), values clamped to [0, 1]
+ * and scaled by 255 with truncation. */
+int s2c_msaa_jitter(int V, const float* pos, int num_samples, const float* dx, const float* dy,
+                    float* out, void* stream);
+int s2c_msaa_resolve(int num_samples, int H, int W, const float* rast, const float* attr,
+                     const int* tri, const float* weights, float* rgb, float* alpha,
+                     void* stream);
+int s2c_blur3x3_reflect(int H, int W, const float* kernel3, const float* in, float* tmp,
+                        float* out, void* stream);
+int s2c_atlas_paste(int H, int W, const float* rgb, const float* alpha, int crop_x0, int crop_y0,
+                    int atlas_x0, int atlas_y0, int atlas_x1, int atlas_y1, int R,
+                    unsigned char* atlas_rgb, unsigned char* atlas_alpha, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

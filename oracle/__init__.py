@@ -617,3 +617,54 @@ def clustered_strand_edges(counts, cluster_ids):
         off += n
     return (np.array(edges, np.int32).reshape(-1, 2), np.array(e2c, np.int32),
             max([c + 1 for c in cluster_ids], default=0))
+
+
+# ------------------------------------------------------- MSAA texture patch
+
+def blur3x3_reflect(a, k):
+    """Separable 3-tap blur, reflect borders, along x then y, each output
+    (k0 a[-1] + k1 a[0]) + k2 a[+1] in float32 (kornia's gaussian_blur2d
+    with a 3 x 3 kernel, texture_utils.py:250-254, without its conv's
+    unspecified summation order)."""
+    a = np.asarray(a, np.float32)
+    k0, k1, k2 = (np.float32(v) for v in k)
+    H, W = a.shape
+    xl = np.abs(np.arange(W) - 1)
+    xr = np.arange(W) + 1
+    xr[xr >= W] = 2 * W - 2 - xr[xr >= W]
+    t = (k0 * a[:, xl] + k1 * a) + k2 * a[:, xr]
+    yl = np.abs(np.arange(H) - 1)
+    yr = np.arange(H) + 1
+    yr[yr >= H] = 2 * H - 2 - yr[yr >= H]
+    return ((k0 * t[yl, :] + k1 * t) + k2 * t[yr, :]).astype(np.float32)
+
+
+def msaa_patch(pos, tri, attr, H, W, dx, dy, weights, kernel3):
+    """One wisp patch of texture_utils.py:203-254: per jitter s a separate
+    rasterize + interpolate of the (V, 3) NDC mesh moved by (dx[s], dy[s])
+    (float32), rgb averaged over covered samples ((0, 1, 0) where none),
+    mapped by 0.5 (rgb + 1), alpha the weighted coverage, blurred.
+    Returns (rgb (H, W, 3), alpha (H, W)) float32, before the flip."""
+    pos = np.asarray(pos, np.float32)
+    tri = np.asarray(tri, np.int32)
+    attr = np.asarray(attr, np.float32)
+    V = pos.shape[0]
+    acc = np.zeros((H, W, 3), np.float32)
+    cnt = np.zeros((H, W), np.float32)
+    al = np.zeros((H, W), np.float32)
+    for s in range(len(dx)):
+        p = np.zeros((1, V, 4), np.float32)
+        p[0, :, 0] = pos[:, 0] + np.float32(dx[s])
+        p[0, :, 1] = pos[:, 1] + np.float32(dy[s])
+        p[0, :, 2] = pos[:, 2]
+        p[0, :, 3] = 1.0
+        rast = rasterize((H, W), p, tri)
+        img = interpolate_triangle_attributes(rast, attr, tri)[0]
+        v = (rast[0, ..., 3] > 0).astype(np.float32)
+        acc += img * v[..., None]
+        cnt += v
+        al += np.float32(weights[s]) * v
+    rgb = acc / np.maximum(cnt, np.float32(1))[..., None]
+    rgb[cnt == 0] = np.array([0, 1, 0], np.float32)
+    rgb = np.float32(0.5) * (rgb + np.float32(1))
+    return rgb.astype(np.float32), blur3x3_reflect(al, kernel3)

@@ -27,7 +27,7 @@ namespace {
 
 const char* dtype_name(at::ScalarType t) {
     return t == at::kFloat ? "float32" : t == at::kInt ? "int32" : t == at::kLong ? "int64"
-           : t == at::kDouble ? "float64" : "other";
+           : t == at::kDouble ? "float64" : t == at::kByte ? "uint8" : "other";
 }
 
 // Shape spec: -1 = any extent.
@@ -782,6 +782,67 @@ PYBIND11_MODULE(_core, m) {
         check(s2c_strand_edges(I(counts.size(0), fn), P<int>(counts), P<long long>(offsets),
                                P<int>(cluster_ids), P<int>(edges), P<int>(edge2cluster),
                                stream_of(counts)),
+              fn);
+    });
+
+    // Software-MSAA texture patches (no reference binding: render_texture,
+    // texture_utils.py:106-288, loops over rasterize on the host).
+    m.def("msaa_jitter", [](Tensor pos, Tensor dx, Tensor dy, Tensor out) {
+        const char* fn = "msaa_jitter";
+        need(pos, fn, "pos", at::kFloat, {-1, 3});
+        need(dx, fn, "dx", at::kFloat, {-1});
+        need(dy, fn, "dy", at::kFloat, {dx.size(0)});
+        need(out, fn, "out", at::kFloat, {dx.size(0), pos.size(0), 4});
+        same_device(fn, {&pos, &dx, &dy, &out});
+        DeviceGuard g(pos.device());
+        check(s2c_msaa_jitter(I(pos.size(0), fn), P<float>(pos), I(dx.size(0), fn), P<float>(dx),
+                              P<float>(dy), P<float>(out), stream_of(pos)),
+              fn);
+    });
+    m.def("msaa_resolve", [](Tensor rast, Tensor attr, Tensor tri, Tensor weights, Tensor rgb,
+                             Tensor alpha) {
+        const char* fn = "msaa_resolve";
+        need(rast, fn, "rast", at::kFloat, {-1, -1, -1, 4});
+        need(attr, fn, "attr", at::kFloat, {-1, 3});
+        need(tri, fn, "tri", at::kInt, {-1, 3});
+        need(weights, fn, "weights", at::kFloat, {rast.size(0)});
+        need(rgb, fn, "rgb", at::kFloat, {rast.size(1), rast.size(2), 3});
+        need(alpha, fn, "alpha", at::kFloat, {rast.size(1), rast.size(2)});
+        same_device(fn, {&rast, &attr, &tri, &weights, &rgb, &alpha});
+        DeviceGuard g(rast.device());
+        check(s2c_msaa_resolve(I(rast.size(0), fn), I(rast.size(1), fn), I(rast.size(2), fn),
+                               P<float>(rast), P<float>(attr), P<int>(tri), P<float>(weights),
+                               P<float>(rgb), P<float>(alpha), stream_of(rast)),
+              fn);
+    });
+    m.def("blur3x3_reflect", [](std::vector<float> kernel3, Tensor in, Tensor tmp, Tensor out) {
+        const char* fn = "blur3x3_reflect";
+        need(in, fn, "in", at::kFloat, {-1, -1});
+        need(tmp, fn, "tmp", at::kFloat, {in.size(0), in.size(1)});
+        need(out, fn, "out", at::kFloat, {in.size(0), in.size(1)});
+        same_device(fn, {&in, &tmp, &out});
+        if (kernel3.size() != 3) throw std::runtime_error("blur3x3_reflect: kernel3 needs 3 taps");
+        DeviceGuard g(in.device());
+        check(s2c_blur3x3_reflect(I(in.size(0), fn), I(in.size(1), fn), kernel3.data(),
+                                  P<float>(in), P<float>(tmp), P<float>(out), stream_of(in)),
+              fn);
+    });
+    m.def("atlas_paste", [](Tensor rgb, Tensor alpha, int crop_x0, int crop_y0, int atlas_x0,
+                            int atlas_y0, int atlas_x1, int atlas_y1, Tensor atlas_rgb,
+                            Tensor atlas_alpha) {
+        const char* fn = "atlas_paste";
+        need(rgb, fn, "rgb", at::kFloat, {-1, -1, 3});
+        need(alpha, fn, "alpha", at::kFloat, {rgb.size(0), rgb.size(1)});
+        need(atlas_rgb, fn, "atlas_rgb", at::kByte, {-1, -1, 3});
+        need(atlas_alpha, fn, "atlas_alpha", at::kByte, {atlas_rgb.size(0), atlas_rgb.size(1)});
+        same_device(fn, {&rgb, &alpha, &atlas_rgb, &atlas_alpha});
+        if (atlas_rgb.size(0) != atlas_rgb.size(1))
+            throw std::runtime_error("atlas_paste: the atlas must be square");
+        DeviceGuard g(rgb.device());
+        check(s2c_atlas_paste(I(rgb.size(0), fn), I(rgb.size(1), fn), P<float>(rgb),
+                              P<float>(alpha), crop_x0, crop_y0, atlas_x0, atlas_y0, atlas_x1,
+                              atlas_y1, I(atlas_rgb.size(0), fn), P<unsigned char>(atlas_rgb),
+                              P<unsigned char>(atlas_alpha), stream_of(rgb)),
               fn);
     });
 }

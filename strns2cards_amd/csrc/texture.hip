@@ -1,0 +1,181 @@
+// Software-MSAA texture patches (render_texture, texture_utils.py:106-288).
+//
+// The reference renders each wisp's tube mesh 64 times, once per jitter of
+// an 8 x 8 sample grid, each a separate rasterize + interpolate call, and
+// accumulates on the host side with torch ops.  Here the 64 jittered copies
+// of the vertices are made in one kernel and rasterized as 64 views of one
+// batched rasterize (the drop-in raster kernels; per-view results are those
+// of separate calls), then one kernel resolves every pixel over its samples
+// in the reference's order:
+//   acc_rgb += rgb * valid, count += valid, alpha += w_s * valid   (s = 0..63)
+//   rgb = acc_rgb / max(count, 1), (0, 1, 0) where count = 0, then 0.5 (rgb + 1)
+// The alpha is blurred with a normalized 3-tap Gaussian (reflect borders),
+// horizontally then vertically, and the patch is flipped vertically,
+// clamped, quantized to bytes and pasted into the atlas.
+#include "common.hpp"
+
+namespace s2c {
+
+// out[s, v] = (x + dx[s], y + dy[s], z, 1) for the (V, 3) vertices.
+__global__ void __launch_bounds__(kBlock)
+msaa_jitter_kernel(int V, const float* __restrict__ pos, int S, const float* __restrict__ dx,
+                   const float* __restrict__ dy, float* __restrict__ out) {
+    const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= static_cast<long long>(S) * V) return;
+    const int s = static_cast<int>(i / V), v = static_cast<int>(i - static_cast<long long>(s) * V);
+    const float* p = pos + 3ll * v;
+    reinterpret_cast<float4*>(out)[i] = make_float4(p[0] + dx[s], p[1] + dy[s], p[2], 1.f);
+}
+
+// One thread per pixel: the samples' interpolated attribute (as
+// rasterize.cu:367-400 interpolates) averaged over the covered samples and
+// the weighted coverage.
+__global__ void __launch_bounds__(kBlock)
+msaa_resolve_kernel(int S, long long npix, const float* __restrict__ rast,
+                    const float* __restrict__ attr, const int* __restrict__ tri,
+                    const float* __restrict__ weights, float* __restrict__ rgb,
+                    float* __restrict__ alpha) {
+    const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= npix) return;
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, cnt = 0.f, al = 0.f;
+    for (int s = 0; s < S; ++s) {
+        const float4 r = reinterpret_cast<const float4*>(rast)[static_cast<long long>(s) * npix + i];
+        const bool valid = r.w > 0.f;
+        float c0 = 0.f, c1 = 0.f, c2 = 0.f;
+        const int t = f2i(r.w) - 1;
+        if (t >= 0) {
+            const float b0 = r.x, b1 = r.y, b2 = 1.f - b0 - b1;
+            const float* p0 = attr + 3ll * tri[t * 3 + 0];
+            const float* p1 = attr + 3ll * tri[t * 3 + 1];
+            const float* p2 = attr + 3ll * tri[t * 3 + 2];
+            c0 = b0 * p0[0] + b1 * p1[0] + b2 * p2[0];
+            c1 = b0 * p0[1] + b1 * p1[1] + b2 * p2[1];
+            c2 = b0 * p0[2] + b1 * p1[2] + b2 * p2[2];
+        }
+        const float v = valid ? 1.f : 0.f;
+        a0 += c0 * v;
+        a1 += c1 * v;
+        a2 += c2 * v;
+        cnt += v;
+        al += weights[s] * v;
+    }
+    float r0 = 0.f, r1 = 1.f, r2 = 0.f;
+    if (cnt > 0.f) {
+        r0 = a0 / cnt;
+        r1 = a1 / cnt;
+        r2 = a2 / cnt;
+    }
+    rgb[3 * i + 0] = 0.5f * (r0 + 1.f);
+    rgb[3 * i + 1] = 0.5f * (r1 + 1.f);
+    rgb[3 * i + 2] = 0.5f * (r2 + 1.f);
+    alpha[i] = al;
+}
+
+__device__ __forceinline__ int reflect_index(int i, int n) {
+    return i < 0 ? -i : (i >= n ? 2 * n - 2 - i : i);
+}
+
+// One 3-tap pass with reflect borders: along x (dir 0) or y (dir 1),
+// out = (k0 in[-1] + k1 in[0]) + k2 in[+1].
+__global__ void __launch_bounds__(kBlock)
+blur3_kernel(int H, int W, int dir, float k0, float k1, float k2, const float* __restrict__ in,
+             float* __restrict__ out) {
+    const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= static_cast<long long>(H) * W) return;
+    const int y = static_cast<int>(i / W), x = static_cast<int>(i - static_cast<long long>(y) * W);
+    float a, b = in[i], c;
+    if (dir == 0) {
+        a = in[static_cast<long long>(y) * W + reflect_index(x - 1, W)];
+        c = in[static_cast<long long>(y) * W + reflect_index(x + 1, W)];
+    } else {
+        a = in[static_cast<long long>(reflect_index(y - 1, H)) * W + x];
+        c = in[static_cast<long long>(reflect_index(y + 1, H)) * W + x];
+    }
+    out[i] = (k0 * a + k1 * b) + k2 * c;
+}
+
+__device__ __forceinline__ unsigned char to_byte(float v) {
+    const float c = fminf(fmaxf(v, 0.f), 1.f);  // NaN -> 0 (fmaxf returns the number)
+    return static_cast<unsigned char>(static_cast<int>(c * 255.f));
+}
+
+// atlas[ay0 + r, ax0 + c] = flip(patch)[cy0 + r, cx0 + c] for the
+// (ay1 - ay0) x (ax1 - ax0) atlas rectangle, rgb and alpha as bytes.
+__global__ void __launch_bounds__(kBlock)
+atlas_paste_kernel(int H, int W, const float* __restrict__ rgb, const float* __restrict__ alpha,
+                   int cx0, int cy0, int ax0, int ay0, int aw, int ah, int R,
+                   unsigned char* __restrict__ atlas_rgb, unsigned char* __restrict__ atlas_alpha) {
+    const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= static_cast<long long>(aw) * ah) return;
+    const int r = static_cast<int>(i / aw), c = static_cast<int>(i - static_cast<long long>(r) * aw);
+    const int py = H - 1 - (cy0 + r), px = cx0 + c;  // flipped patch row
+    const long long p = static_cast<long long>(py) * W + px;
+    const long long a = static_cast<long long>(ay0 + r) * R + (ax0 + c);
+    atlas_rgb[3 * a + 0] = to_byte(rgb[3 * p + 0]);
+    atlas_rgb[3 * a + 1] = to_byte(rgb[3 * p + 1]);
+    atlas_rgb[3 * a + 2] = to_byte(rgb[3 * p + 2]);
+    atlas_alpha[a] = to_byte(alpha[p]);
+}
+
+}  // namespace s2c
+
+using namespace s2c;
+
+extern "C" int s2c_msaa_jitter(int V, const float* pos, int num_samples, const float* dx,
+                               const float* dy, float* out, void* stream) {
+    S2C_CHECK(V >= 0 && num_samples >= 0, "msaa_jitter: negative size");
+    if (V == 0 || num_samples == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long n = static_cast<long long>(V) * num_samples;
+    msaa_jitter_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(V, pos, num_samples, dx, dy, out);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
+
+extern "C" int s2c_msaa_resolve(int num_samples, int H, int W, const float* rast,
+                                const float* attr, const int* tri, const float* weights,
+                                float* rgb, float* alpha, void* stream) {
+    S2C_CHECK(num_samples >= 0 && H >= 0 && W >= 0, "msaa_resolve: negative size");
+    const long long npix = static_cast<long long>(H) * W;
+    if (npix == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    msaa_resolve_kernel<<<grid_for(npix, kBlock), kBlock, 0, s>>>(num_samples, npix, rast, attr,
+                                                                  tri, weights, rgb, alpha);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
+
+extern "C" int s2c_blur3x3_reflect(int H, int W, const float* kernel3, const float* in,
+                                   float* tmp, float* out, void* stream) {
+    S2C_CHECK(H >= 2 && W >= 2, "blur3x3_reflect: reflect borders need H, W >= 2");
+    S2C_CHECK(kernel3 != nullptr, "blur3x3_reflect: null kernel");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long n = static_cast<long long>(H) * W;
+    blur3_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(H, W, 0, kernel3[0], kernel3[1],
+                                                        kernel3[2], in, tmp);
+    S2C_LAUNCH_CHECK();
+    blur3_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(H, W, 1, kernel3[0], kernel3[1],
+                                                        kernel3[2], tmp, out);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
+
+extern "C" int s2c_atlas_paste(int H, int W, const float* rgb, const float* alpha, int crop_x0,
+                               int crop_y0, int atlas_x0, int atlas_y0, int atlas_x1,
+                               int atlas_y1, int R, unsigned char* atlas_rgb,
+                               unsigned char* atlas_alpha, void* stream) {
+    const int aw = atlas_x1 - atlas_x0, ah = atlas_y1 - atlas_y0;
+    S2C_CHECK(aw >= 0 && ah >= 0, "atlas_paste: empty or inverted atlas rectangle");
+    S2C_CHECK(atlas_x0 >= 0 && atlas_y0 >= 0 && atlas_x1 <= R && atlas_y1 <= R,
+              "atlas_paste: rectangle outside the %d x %d atlas", R, R);
+    S2C_CHECK(crop_x0 >= 0 && crop_y0 >= 0 && crop_x0 + aw <= W && crop_y0 + ah <= H,
+              "atlas_paste: crop outside the %d x %d patch", H, W);
+    if (aw == 0 || ah == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long n = static_cast<long long>(aw) * ah;
+    atlas_paste_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(
+        H, W, rgb, alpha, crop_x0, crop_y0, atlas_x0, atlas_y0, aw, ah, R, atlas_rgb,
+        atlas_alpha);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}

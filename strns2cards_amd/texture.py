@@ -1,0 +1,163 @@
+"""Software-MSAA texture patches of wisps (render_texture,
+texture_utils.py:106-288).
+
+The reference renders each wisp's tube mesh once per sample of an 8 x 8
+jitter grid spanning two pixels, 64 separate ``rasterize`` +
+``interpolate_triangle_attributes`` calls with host synchronizations, and
+accumulates with torch ops.  ``render_patch`` makes the 64 jittered vertex
+sets in one kernel, rasterizes them as 64 views of one batched
+``rasterize`` (the drop-in raster kernels; each view is what a separate
+call gives), and resolves every pixel over its samples in one kernel:
+
+* rgb: the mean of the interpolated attribute over the covered samples,
+  (0, 1, 0) where none is covered, mapped by 0.5 (rgb + 1);
+* alpha: the Mitchell-Netravali (B = C = 1/3) weighted coverage, blurred
+  by a normalized 3-tap Gaussian (sigma 1.5, reflect borders) as kornia's
+  ``gaussian_blur2d`` with kernel (3, 3) does.
+
+``render_texture`` runs the reference's per-wisp loop on top of it and
+pastes each patch (flipped vertically, as bytes) into the atlas on the GPU.
+"""
+from __future__ import annotations
+
+import math
+import os
+
+import numpy as np
+import torch
+
+from . import _core
+from .curve_utils import strands_to_tube
+from .rasterize import rasterize
+
+
+def mitchell_netravali(x, B: float = 1.0 / 3, C: float = 1.0 / 3):
+    """The cubic reconstruction filter of texture_utils.py:189-201."""
+    ax = np.abs(np.asarray(x, np.float64))
+    near = ((12 - 9 * B - 6 * C) * ax ** 3 + (-18 + 12 * B + 6 * C) * ax ** 2 + (6 - 2 * B))
+    far = ((-B - 6 * C) * ax ** 3 + (6 * B + 30 * C) * ax ** 2 + (-12 * B - 48 * C) * ax
+           + (8 * B + 24 * C))
+    return (near * (ax < 1) + far * ((ax >= 1) & (ax < 2))) / 6.0
+
+
+def msaa_samples(grid_res: int = 8, span: float = 2.0):
+    """Sample offsets in pixels (y-major, x fastest) and their normalized
+    filter weights, as texture_utils.py:203-215."""
+    g = span * ((np.arange(grid_res) + 0.5) / grid_res - 0.5)
+    offsets = [(float(dx), float(dy)) for dy in g for dx in g]
+    w = np.array([mitchell_netravali(dx) * mitchell_netravali(dy) for dx, dy in offsets])
+    return offsets, (w / w.sum()).tolist()
+
+
+def gaussian_kernel3(sigma: float = 1.5) -> list[float]:
+    """Normalized float32 taps at -1, 0, 1 of exp(-x^2 / (2 sigma^2)), the
+    1-D kernel kornia's gaussian_blur2d builds for a kernel size of 3."""
+    x = np.arange(3, dtype=np.float32) - np.float32(1)
+    s = np.float32(sigma)
+    g = np.exp(-(x * x) / (np.float32(2) * s * s)).astype(np.float32)
+    return (g / g.sum(dtype=np.float32)).astype(np.float32).tolist()
+
+
+def render_patch(pos: torch.Tensor, tri: torch.Tensor, attr: torch.Tensor,
+                 patch_res: tuple[int, int], grid_res: int = 8, span: float = 2.0):
+    """(rgb (H, W, 3), alpha (H, W)) float32 device tensors of one wisp's
+    NDC mesh (pos (V, 3), tri (T, 3) int32, attr (V, 3)): the patch of
+    texture_utils.py:203-260 before its vertical flip."""
+    H, W = patch_res
+    dev = pos.device
+    offsets, weights = msaa_samples(grid_res, span)
+    # the reference adds Python floats to a float32 tensor: float32 offsets
+    dx = torch.tensor([o[0] * 2.0 / W for o in offsets], dtype=torch.float32, device=dev)
+    dy = torch.tensor([o[1] * 2.0 / H for o in offsets], dtype=torch.float32, device=dev)
+    S = len(offsets)
+    pos = pos.contiguous().float()
+    jit = torch.empty((S, pos.shape[0], 4), dtype=torch.float32, device=dev)
+    _core.msaa_jitter(pos, dx, dy, jit)
+    rast = rasterize((H, W), jit, tri.contiguous())
+    rgb = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+    alpha = torch.empty((H, W), dtype=torch.float32, device=dev)
+    _core.msaa_resolve(rast, attr.contiguous().float(), tri.contiguous(),
+                       torch.tensor(weights, dtype=torch.float32, device=dev), rgb, alpha)
+    tmp = torch.empty_like(alpha)
+    blurred = torch.empty_like(alpha)
+    _core.blur3x3_reflect(gaussian_kernel3(1.5), alpha, tmp, blurred)
+    return rgb, blurred
+
+
+def atlas_rect(uv_rect, resolution: int, patch_res: tuple[int, int]):
+    """Atlas rectangle of a wisp and the crop of its patch, as
+    texture_utils.py:264-276: (crop_x0, crop_y0, x0, y0, x1, y1)."""
+    u0, v0, u1, v1 = uv_rect
+    x0, y0 = int(np.floor(u0 * resolution)), int(np.floor(v0 * resolution))
+    x1, y1 = int(np.floor(u1 * resolution)), int(np.floor(v1 * resolution))
+    crop_x0 = int((patch_res[1] - (x1 - x0)) / 2)
+    return crop_x0, 0, x0, y0, x1, y1
+
+
+def wisp_ndc_mesh(wisp: np.ndarray, uv_rect, packed_bbox_size, to_ndc_scale, tube_radius,
+                  n_circle):
+    """The tube mesh of one (N, S, 3) wisp with its tangents as attribute,
+    placed in the patch's NDC frame (texture_utils.py:155-178; host numpy
+    in the reference, kept so)."""
+    tangents = np.zeros_like(wisp)
+    deltas = np.diff(wisp, axis=1)
+    tangents[:, :-1, :] += deltas
+    tangents[:, 1:, :] += deltas
+    tangents /= np.linalg.norm(tangents, axis=2, keepdims=True)
+    verts, faces, attr = strands_to_tube(wisp, radius=tube_radius, n_circle=n_circle,
+                                         attr=tangents)
+    u0, v0, u1, v1 = uv_rect
+    u0w, u1w = u0 * packed_bbox_size[0], u1 * packed_bbox_size[0]
+    v0w = v0 * packed_bbox_size[1]
+    verts[:, 0] -= 0.5 * (u0w + u1w)
+    verts[:, 1] -= v0w
+    verts *= to_ndc_scale
+    verts[:, 1] -= 1.0
+    return verts, faces, attr
+
+
+def render_texture(packed_wisps, uv_coords, tube_radius, n_circle, resolution=1024,
+                   output_dirname="./output", subsampling_ratio=None, device="cuda"):
+    """texture_utils.py:106-288: the tangent (rgb) and alpha atlases of the
+    packed wisps, (resolution, resolution, 3) and (resolution, resolution)
+    uint8 numpy arrays.  Written as tangent/alpha .png/.tga when
+    `output_dirname` is given and imageio is importable."""
+    max_u = max(u1 - u0 for _, u0, _, u1, _ in uv_coords)
+    max_v = max(v1 - v0 for _, _, v0, _, v1 in uv_coords)
+    patch_res = (int(math.ceil(max_v * resolution)), int(math.ceil(max_u * resolution)))
+    packed_points = np.concatenate([w.reshape(-1, 3) for w in packed_wisps], axis=0)
+    packed_bbox_size = packed_points[:, :2].max(axis=0) - packed_points[:, :2].min(axis=0)
+    bbox_size = np.zeros(3)
+    for wisp in packed_wisps:
+        p = wisp.reshape(-1, 3)
+        bbox_size = np.maximum(bbox_size, p.max(axis=0) - p.min(axis=0))
+    to_ndc_scale = 2.0 * np.ones(3) / bbox_size
+    to_ndc_scale[2] *= 0.2
+    dev = torch.device(device)
+    atlas_rgb = torch.zeros((resolution, resolution, 3), dtype=torch.uint8, device=dev)
+    atlas_alpha = torch.zeros((resolution, resolution), dtype=torch.uint8, device=dev)
+    for wisp, (_, u0, v0, u1, v1) in zip(packed_wisps, uv_coords):
+        if subsampling_ratio is not None and subsampling_ratio < 1.0:
+            S = wisp.shape[0]
+            keep = np.random.choice(S, size=max(1, int(S * subsampling_ratio)), replace=False)
+            wisp = wisp[keep]
+        verts, faces, attr = wisp_ndc_mesh(wisp, (u0, v0, u1, v1), packed_bbox_size,
+                                           to_ndc_scale, tube_radius, n_circle)
+        rgb, alpha = render_patch(torch.from_numpy(verts).to(dev, torch.float32),
+                                  torch.from_numpy(faces).to(dev, torch.int32),
+                                  torch.from_numpy(np.ascontiguousarray(attr)).to(dev, torch.float32),
+                                  patch_res)
+        cx0, cy0, x0, y0, x1, y1 = atlas_rect((u0, v0, u1, v1), resolution, patch_res)
+        _core.atlas_paste(rgb, alpha, cx0, cy0, x0, y0, x1, y1, atlas_rgb, atlas_alpha)
+    rgb_np, alpha_np = atlas_rgb.cpu().numpy(), atlas_alpha.cpu().numpy()
+    if output_dirname is not None:
+        try:
+            import imageio.v3 as iio
+        except ImportError:
+            iio = None
+        if iio is not None:
+            os.makedirs(output_dirname, exist_ok=True)
+            for ext in ("png", "tga"):
+                iio.imwrite(os.path.join(output_dirname, f"tangent.{ext}"), rgb_np)
+                iio.imwrite(os.path.join(output_dirname, f"alpha.{ext}"), alpha_np)
+    return rgb_np, alpha_np
