@@ -347,6 +347,89 @@ def card_mesh_section(dev, reps=20):
     return out
 
 
+def texture_section(dev, reps=5):
+    """One wisp's software-MSAA texture patch (render_texture,
+    texture_utils.py:203-254): a 200-strand x 64-point wisp as 8-gon tubes
+    (102,400 vertices, 201,600 triangles) into a 256 x 128 patch, 64
+    jitters.  Fused: one 64-view depth-key pass + resolve + blur
+    (texture.render_patch); batched: the same through one 64-view drop-in
+    rasterize (fragment lists).  Reference-shaped: the 64 single-view
+    rasterize + interpolate calls and torch accumulation of the reference's
+    loop, on the same drop-in kernels.  CPU baseline: the oracle's C
+    rasterizer and numpy accumulation on 4 of the 64 jitters, scaled."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from strns2cards_amd import texture
+    from strns2cards_amd.rasterize import interpolate_triangle_attributes, rasterize
+
+    rng = np.random.default_rng(5)
+    base = np.cumsum(rng.normal(size=(64, 3)) * [0.01, 0.05, 0.01] + [0, 0.05, 0], 0)
+    wisp = np.stack([base + rng.normal(size=3) * [0.05, 0.01, 0.05] for _ in range(200)])
+    wisp = wisp.astype(np.float32)
+    p = wisp.reshape(-1, 3)
+    bbox = p[:, :2].max(0) - p[:, :2].min(0)
+    scale = 2.0 * np.ones(3) / (p.max(0) - p.min(0))
+    scale[2] *= 0.2
+    verts, faces, attr = texture.wisp_ndc_mesh(wisp, (0.0, 0.0, 0.5, 1.0), bbox, scale, 0.002, 8)
+    verts[:, 0] *= 0.9  # keep the strands inside the patch
+    H, W = 256, 128
+    pos = torch.from_numpy(verts.astype(np.float32)).to(dev)
+    tri = torch.from_numpy(faces).to(dev)
+    at = torch.from_numpy(np.ascontiguousarray(attr, np.float32)).to(dev)
+    offs, wts = texture.msaa_samples()
+
+    def batched():
+        return texture.render_patch(pos, tri, at, (H, W))
+
+    def batched_raster():
+        return texture.render_patch(pos, tri, at, (H, W), fused=False)
+
+    def looped():
+        acc = torch.zeros((H, W, 3), device=dev)
+        cnt = torch.zeros((H, W), device=dev)
+        acc_alpha = 0
+        for i, (ox, oy) in enumerate(offs):
+            pj = pos.clone()
+            pj[:, 0] += ox * 2.0 / W
+            pj[:, 1] += oy * 2.0 / H
+            ph = torch.cat([pj, torch.ones_like(pos[:, :1])], -1)
+            rast = rasterize((H, W), ph[None], tri)
+            rgb = interpolate_triangle_attributes(rast, at, tri)[0]
+            a = (rast[0, ..., -1] > 0).float()
+            acc += rgb * a[..., None]
+            cnt += a
+            acc_alpha = acc_alpha + wts[i] * a
+        return acc, cnt, acc_alpha
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    ms_b, ms_r, ms_l = timed(batched), timed(batched_raster), timed(looped)
+    dx = np.array([o[0] * 2.0 / W for o in offs], np.float32)
+    dy = np.array([o[1] * 2.0 / H for o in offs], np.float32)
+    t0 = time.perf_counter()
+    oracle.msaa_patch(verts, faces, attr, H, W, dx[:4], dy[:4], np.array(wts[:4], np.float32),
+                      texture.gaussian_kernel3(1.5))
+    cpu_ms = (time.perf_counter() - t0) * 1e3 * 16
+    out = {"strands": 200, "points": 64, "tube_sides": 8, "vertices": int(verts.shape[0]),
+           "triangles": int(faces.shape[0]), "patch": [H, W], "samples": 64,
+           "fused_ms": round(ms_b, 3), "batched_rasterize_ms": round(ms_r, 3),
+           "per_jitter_loop_ms": round(ms_l, 3), "fused_vs_loop": round(ms_l / ms_b, 2),
+           "cpu_baseline": {"ms": round(cpu_ms, 1), "cores": 1,
+                            "kind": "port (C raster oracle + numpy)",
+                            "sample": "4 of 64 jitters, x16"}}
+    print(f"[bench] texture patch: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def copy_calibration(dev, nbytes=1 << 29, reps=3):
     """A plain device-to-device copy of known size, so PMC byte counters can
     be scaled against it."""
@@ -370,9 +453,10 @@ def main():
     ap.add_argument("--no-fit", action="store_true")
     ap.add_argument("--fit-jobs", type=int, default=4096)
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--no-largesteps", action="store_true")
+    ap.add_argument("--no-largesteps", action="store_true",
+                    help="skip the large-steps, card-mesh and texture sections")
     ap.add_argument("--largesteps-only", action="store_true",
-                    help="only the large-steps solve and card-mesh sections")
+                    help="only the large-steps, card-mesh and texture sections")
     ap.add_argument("--dda-only", action="store_true",
                     help="time the DDA->mask step only (for PMC runs)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -395,7 +479,8 @@ def main():
     dev = torch.device("cuda", 0)
     if args.largesteps_only:
         print(json.dumps({"largesteps": largesteps_section(dev),
-                          "card_mesh": card_mesh_section(dev)}), flush=True)
+                          "card_mesh": card_mesh_section(dev),
+                          "texture_patch": texture_section(dev)}), flush=True)
         return
     t_gen = time.perf_counter()
     w = synth.full_workload()
@@ -492,6 +577,7 @@ def main():
     adj = None if args.no_adjoint else adjoint_section(dev)
     lsteps = None if args.no_largesteps else largesteps_section(dev)
     cmesh = None if args.no_largesteps else card_mesh_section(dev)
+    tex = None if args.no_largesteps else texture_section(dev)
 
     cpu = None
     if not args.no_cpu:
@@ -532,6 +618,7 @@ def main():
         "adjoint_fit": adj,
         "largesteps": lsteps,
         "card_mesh": cmesh,
+        "texture_patch": tex,
     }
     print(json.dumps(out), flush=True)
 

@@ -381,6 +381,17 @@ int s2c_msaa_resolve(int num_samples, int H, int W, const float* rast, const flo
                      void* stream);
 int s2c_blur3x3_reflect(int H, int W, const float* kernel3, const float* in, float* tmp,
                         float* out, void* stream);
+/* The fused form of rasterize + s2c_msaa_resolve for the jittered views
+ * (num_samples, V, 4) of s2c_msaa_jitter: no fragment lists; per (view,
+ * pixel) the minimum (bits(z + 2) << 32 | triangle) over the covering
+ * triangles, which picks the fragment the drop-in depth test keeps
+ * (rasterize.cu:198-228; at one pixel its fragments of one view are ordered
+ * by triangle), in keys (num_samples, H, W) int64 scratch; then the resolve
+ * with the winners' barycentrics recomputed.  Same rgb / alpha as
+ * rasterize + s2c_msaa_resolve. */
+int s2c_msaa_patch(int V, int T, const float* jittered, const int* tri, const float* attr,
+                   int num_samples, int H, int W, const float* weights, long long* keys,
+                   float* rgb, float* alpha, void* stream);
 int s2c_atlas_paste(int H, int W, const float* rgb, const float* alpha, int crop_x0, int crop_y0,
                     int atlas_x0, int atlas_y0, int atlas_x1, int atlas_y1, int R,
                     unsigned char* atlas_rgb, unsigned char* atlas_alpha, void* stream);

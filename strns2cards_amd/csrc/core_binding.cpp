@@ -845,4 +845,22 @@ PYBIND11_MODULE(_core, m) {
                               P<unsigned char>(atlas_alpha), stream_of(rgb)),
               fn);
     });
+    m.def("msaa_patch", [](Tensor jittered, Tensor tri, Tensor attr, Tensor weights, int H, int W,
+                           Tensor keys, Tensor rgb, Tensor alpha) {
+        const char* fn = "msaa_patch";
+        need(jittered, fn, "jittered", at::kFloat, {-1, -1, 4});
+        need(tri, fn, "tri", at::kInt, {-1, 3});
+        need(attr, fn, "attr", at::kFloat, {jittered.size(1), 3});
+        need(weights, fn, "weights", at::kFloat, {jittered.size(0)});
+        need(keys, fn, "keys", at::kLong, {jittered.size(0), H, W});
+        need(rgb, fn, "rgb", at::kFloat, {H, W, 3});
+        need(alpha, fn, "alpha", at::kFloat, {H, W});
+        same_device(fn, {&jittered, &tri, &attr, &weights, &keys, &rgb, &alpha});
+        DeviceGuard g(jittered.device());
+        check(s2c_msaa_patch(I(jittered.size(1), fn), I(tri.size(0), fn), P<float>(jittered),
+                             P<int>(tri), P<float>(attr), I(jittered.size(0), fn), H, W,
+                             P<float>(weights), P<long long>(keys), P<float>(rgb),
+                             P<float>(alpha), stream_of(jittered)),
+              fn);
+    });
 }

@@ -12,7 +12,7 @@
 // The alpha is blurred with a normalized 3-tap Gaussian (reflect borders),
 // horizontally then vertically, and the patch is flipped vertically,
 // clamped, quantized to bytes and pasted into the atlas.
-#include "common.hpp"
+#include "raster_math.hpp"
 
 namespace s2c {
 
@@ -53,6 +53,117 @@ msaa_resolve_kernel(int S, long long npix, const float* __restrict__ rast,
             c2 = b0 * p0[2] + b1 * p1[2] + b2 * p2[2];
         }
         const float v = valid ? 1.f : 0.f;
+        a0 += c0 * v;
+        a1 += c1 * v;
+        a2 += c2 * v;
+        cnt += v;
+        al += weights[s] * v;
+    }
+    float r0 = 0.f, r1 = 1.f, r2 = 0.f;
+    if (cnt > 0.f) {
+        r0 = a0 / cnt;
+        r1 = a1 / cnt;
+        r2 = a2 / cnt;
+    }
+    rgb[3 * i + 0] = 0.5f * (r0 + 1.f);
+    rgb[3 * i + 1] = 0.5f * (r1 + 1.f);
+    rgb[3 * i + 2] = 0.5f * (r2 + 1.f);
+    alpha[i] = al;
+}
+
+// ---------------------------------------------------------------- fused form
+// The same patch without fragment lists.  At one pixel a triangle yields at
+// most one fragment, and the drop-in depth test orders fragments of one
+// view by their stable index, i.e. by triangle index at a given pixel: so
+// the minimum of (bits(z + 2) << 32 | t) over the covering triangles
+// (signed, as rasterize.cu:198-228) picks the same fragment, and its
+// barycentrics are recomputed by the same coverage test.
+
+struct MsaaPrim {
+    float4 P0, P1, P2;
+    int4 r;
+    int b, t;
+};
+
+// Depth keys (S, H, W) of the S jittered views (pos (S, V, 4)), filled with
+// INT64_MAX by the caller: one lane per (view, triangle) counts its
+// candidates, a block scan spreads them over the workgroup, and each
+// covered candidate takes an atomic minimum.
+__global__ void __launch_bounds__(kBlock)
+msaa_depth_kernel(long long BT, int H, int W, int V, const float* __restrict__ pos, int T,
+                  const int* __restrict__ tri, long long* __restrict__ keys) {
+    __shared__ MsaaPrim prim[kBlock];
+    __shared__ int excl[kBlock];
+    __shared__ int sh[kBlock / kWave + 1];
+    const long long q = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    int cnt = 0;
+    if (q < BT) {
+        const int b = static_cast<int>(q / T);
+        const int t = static_cast<int>(q - static_cast<long long>(b) * T);
+        const long long vb = static_cast<long long>(b) * V;
+        MsaaPrim m;
+        m.P0 = ld4(pos, vb + tri[t * 3 + 0]);
+        m.P1 = ld4(pos, vb + tri[t * 3 + 1]);
+        m.P2 = ld4(pos, vb + tri[t * 3 + 2]);
+        cnt = tri_rect(H, W, m.P0, m.P1, m.P2, m.r);
+        m.b = b;
+        m.t = t;
+        prim[threadIdx.x] = m;
+    }
+    int total;
+    excl[threadIdx.x] = block_excl_scan(cnt, sh, &total);
+    __syncthreads();
+#pragma unroll 1
+    for (int j = threadIdx.x; j < total; j += kBlock) {
+        int lo = 0, hi = kBlock;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (excl[mid] <= j) lo = mid;
+            else hi = mid;
+        }
+        const MsaaPrim& m = prim[lo];
+        const int l = j - excl[lo];
+        const int y = m.r.x + l / m.r.w;
+        const int x = m.r.z + l % m.r.w;
+        float b0, b1, z;
+        if (!tri_cover(H, W, x, y, m.P0, m.P1, m.P2, b0, b1, z)) continue;
+        const unsigned long long k =
+            (static_cast<unsigned long long>(__float_as_uint(z + 2.f)) << 32) |
+            static_cast<unsigned>(m.t);
+        atomicMin(&keys[(static_cast<long long>(m.b) * H + y) * W + x], static_cast<long long>(k));
+    }
+}
+
+// The resolve of msaa_resolve_kernel from depth keys: the winning
+// triangle's barycentrics recomputed at the pixel in its view.
+__global__ void __launch_bounds__(kBlock)
+msaa_resolve_keys_kernel(int S, int H, int W, int V, const float* __restrict__ pos,
+                         const long long* __restrict__ keys, const float* __restrict__ attr,
+                         const int* __restrict__ tri, const float* __restrict__ weights,
+                         float* __restrict__ rgb, float* __restrict__ alpha) {
+    const long long npix = static_cast<long long>(H) * W;
+    const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= npix) return;
+    const int y = static_cast<int>(i / W), x = static_cast<int>(i - static_cast<long long>(y) * W);
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, cnt = 0.f, al = 0.f;
+    for (int s = 0; s < S; ++s) {
+        const int t = static_cast<int>(static_cast<unsigned>(keys[s * npix + i] & 0xFFFFFFFFll));
+        float c0 = 0.f, c1 = 0.f, c2 = 0.f, v = 0.f;
+        if (t >= 0) {
+            const long long vb = static_cast<long long>(s) * V;
+            const int i0 = tri[t * 3 + 0], i1 = tri[t * 3 + 1], i2 = tri[t * 3 + 2];
+            float b0, b1, z;
+            tri_cover(H, W, x, y, ld4(pos, vb + i0), ld4(pos, vb + i1), ld4(pos, vb + i2), b0, b1,
+                      z);
+            const float b2 = 1.f - b0 - b1;
+            const float* p0 = attr + 3ll * i0;
+            const float* p1 = attr + 3ll * i1;
+            const float* p2 = attr + 3ll * i2;
+            c0 = b0 * p0[0] + b1 * p1[0] + b2 * p2[0];
+            c1 = b0 * p0[1] + b1 * p1[1] + b2 * p2[1];
+            c2 = b0 * p0[2] + b1 * p1[2] + b2 * p2[2];
+            v = 1.f;
+        }
         a0 += c0 * v;
         a1 += c1 * v;
         a2 += c2 * v;
@@ -141,6 +252,32 @@ extern "C" int s2c_msaa_resolve(int num_samples, int H, int W, const float* rast
     hipStream_t s = static_cast<hipStream_t>(stream);
     msaa_resolve_kernel<<<grid_for(npix, kBlock), kBlock, 0, s>>>(num_samples, npix, rast, attr,
                                                                   tri, weights, rgb, alpha);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
+
+extern "C" int s2c_msaa_patch(int V, int T, const float* jittered, const int* tri,
+                              const float* attr, int num_samples, int H, int W,
+                              const float* weights, long long* keys, float* rgb, float* alpha,
+                              void* stream) {
+    S2C_CHECK(V >= 0 && T >= 0 && num_samples >= 0 && H >= 0 && W >= 0,
+              "msaa_patch: negative size");
+    const long long npix = static_cast<long long>(H) * W;
+    if (npix == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long nkeys = npix * num_samples;
+    if (nkeys > 0) {
+        fill_i64_kernel<<<grid_for(nkeys, kBlock), kBlock, 0, s>>>(keys, LLONG_MAX, nkeys);
+        S2C_LAUNCH_CHECK();
+    }
+    const long long BT = static_cast<long long>(num_samples) * T;
+    if (BT > 0) {
+        msaa_depth_kernel<<<grid_for(BT, kBlock), kBlock, 0, s>>>(BT, H, W, V, jittered, T, tri,
+                                                                  keys);
+        S2C_LAUNCH_CHECK();
+    }
+    msaa_resolve_keys_kernel<<<grid_for(npix, kBlock), kBlock, 0, s>>>(
+        num_samples, H, W, V, jittered, keys, attr, tri, weights, rgb, alpha);
     S2C_LAUNCH_CHECK();
     return S2C_OK;
 }

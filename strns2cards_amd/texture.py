@@ -5,9 +5,10 @@ The reference renders each wisp's tube mesh once per sample of an 8 x 8
 jitter grid spanning two pixels, 64 separate ``rasterize`` +
 ``interpolate_triangle_attributes`` calls with host synchronizations, and
 accumulates with torch ops.  ``render_patch`` makes the 64 jittered vertex
-sets in one kernel, rasterizes them as 64 views of one batched
-``rasterize`` (the drop-in raster kernels; each view is what a separate
-call gives), and resolves every pixel over its samples in one kernel:
+sets in one kernel, depth-tests them as 64 views in one kernel that keeps
+only the winning (depth, triangle) key per sample and pixel (or, with
+``fused=False``, rasterizes them as one batched ``rasterize``), and
+resolves every pixel over its samples in one kernel:
 
 * rgb: the mean of the interpolated attribute over the covered samples,
   (0, 1, 0) where none is covered, mapped by 0.5 (rgb + 1);
@@ -59,25 +60,34 @@ def gaussian_kernel3(sigma: float = 1.5) -> list[float]:
 
 
 def render_patch(pos: torch.Tensor, tri: torch.Tensor, attr: torch.Tensor,
-                 patch_res: tuple[int, int], grid_res: int = 8, span: float = 2.0):
+                 patch_res: tuple[int, int], grid_res: int = 8, span: float = 2.0,
+                 fused: bool = True):
     """(rgb (H, W, 3), alpha (H, W)) float32 device tensors of one wisp's
     NDC mesh (pos (V, 3), tri (T, 3) int32, attr (V, 3)): the patch of
-    texture_utils.py:203-260 before its vertical flip."""
+    texture_utils.py:203-260 before its vertical flip.  `fused` (default)
+    keeps only a depth key per (sample, pixel) instead of the batched
+    rasterize's fragment lists; both give the same patch."""
     H, W = patch_res
     dev = pos.device
     offsets, weights = msaa_samples(grid_res, span)
     # the reference adds Python floats to a float32 tensor: float32 offsets
     dx = torch.tensor([o[0] * 2.0 / W for o in offsets], dtype=torch.float32, device=dev)
     dy = torch.tensor([o[1] * 2.0 / H for o in offsets], dtype=torch.float32, device=dev)
+    w = torch.tensor(weights, dtype=torch.float32, device=dev)
     S = len(offsets)
     pos = pos.contiguous().float()
+    attr = attr.contiguous().float()
+    tri = tri.contiguous()
     jit = torch.empty((S, pos.shape[0], 4), dtype=torch.float32, device=dev)
     _core.msaa_jitter(pos, dx, dy, jit)
-    rast = rasterize((H, W), jit, tri.contiguous())
     rgb = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
     alpha = torch.empty((H, W), dtype=torch.float32, device=dev)
-    _core.msaa_resolve(rast, attr.contiguous().float(), tri.contiguous(),
-                       torch.tensor(weights, dtype=torch.float32, device=dev), rgb, alpha)
+    if fused:
+        keys = torch.empty((S, H, W), dtype=torch.int64, device=dev)
+        _core.msaa_patch(jit, tri, attr, w, H, W, keys, rgb, alpha)
+    else:
+        rast = rasterize((H, W), jit, tri)
+        _core.msaa_resolve(rast, attr, tri, w, rgb, alpha)
     tmp = torch.empty_like(alpha)
     blurred = torch.empty_like(alpha)
     _core.blur3x3_reflect(gaussian_kernel3(1.5), alpha, tmp, blurred)

@@ -2,8 +2,8 @@
 
 The oracle rasterizes each jitter separately (as the reference loops over
 64 rasterize calls) with the C raster oracle and accumulates in numpy in
-sample order; the GPU path rasterizes the 64 jitters as one batched call
-and resolves in one kernel.  rgb, the weighted coverage and its blur (in
+sample order; the GPU path depth-tests the 64 jitters in one fused kernel
+(or rasterizes them as one batched call) and resolves in one kernel.  rgb, the weighted coverage and its blur (in
 the oracle's summation order) are bit-exact, and so are the atlases.
 """
 import numpy as np
@@ -31,13 +31,15 @@ def _mesh(seed, uv=(0.1, 0.2, 0.4, 0.9)):
     return texture.wisp_ndc_mesh(wisp, uv, bbox, scale, 0.01, 6)
 
 
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "batched_rasterize"])
 @pytest.mark.parametrize("seed,H,W", [(0, 24, 16), (1, 40, 33), (2, 7, 5)])
-def test_patch_matches_oracle(dev, seed, H, W):
-    print(f"[msaa patch] seed={seed} {H}x{W}", flush=True)
+def test_patch_matches_oracle(dev, seed, H, W, fused):
+    print(f"[msaa patch] seed={seed} {H}x{W} fused={fused}", flush=True)
     verts, faces, attr = _mesh(seed)
     rgb, alpha = texture.render_patch(torch.from_numpy(verts.astype(np.float32)).to(dev),
                                       torch.from_numpy(faces).to(dev),
-                                      torch.from_numpy(attr.astype(np.float32)).to(dev), (H, W))
+                                      torch.from_numpy(attr.astype(np.float32)).to(dev), (H, W),
+                                      fused=fused)
     offs, w = texture.msaa_samples()
     dx = np.array([o[0] * 2.0 / W for o in offs], np.float32)
     dy = np.array([o[1] * 2.0 / H for o in offs], np.float32)
