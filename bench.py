@@ -426,6 +426,79 @@ def texture_section(dev, reps=5):
            "cpu_baseline": {"ms": round(cpu_ms, 1), "cores": 1,
                             "kind": "port (C raster oracle + numpy)",
                             "sample": "4 of 64 jitters, x16"}}
+    # wisp masks of render_patches_reduced: 300 wisps of 50 strands x 64
+    # points into 256 x 128 patches, one launch against the reference's
+    # per-wisp dda_compute_fragments + depth_test + dda_interpolate loop
+    from strns2cards_amd.dda import dda_compute_fragments, dda_interpolate_attributes
+    from strns2cards_amd.rasterize import depth_test
+
+    wisps = []
+    for _ in range(300):
+        b = np.cumsum(rng.normal(size=(64, 3)) * [0.01, 0.05, 0.01] + [0, 0.05, 0], 0)
+        wisps.append(np.stack([b + rng.normal(size=3) * [0.05, 0.01, 0.05]
+                               for _ in range(50)]).astype(np.float32))
+    uv = [(i, 0.0, 0.0, 0.5, 1.0) for i in range(len(wisps))]
+    Rm = 256
+    t0 = time.perf_counter()
+    masks = texture.patch_masks(wisps, uv, Rm, device=dev)
+    torch.cuda.synchronize()
+    ms_masks = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    masks = texture.patch_masks(wisps, uv, Rm, device=dev)
+    torch.cuda.synchronize()
+    ms_masks = min(ms_masks, (time.perf_counter() - t0) * 1e3)
+    Hm, Wm = masks.shape[1:]
+    # the one launch alone, on device-resident points
+    _, _, pbm, scm = texture.packing_frame(wisps)
+    posm = torch.ones((len(wisps) * 50 * 64, 4), device=dev)
+    posm[:, :3] = torch.from_numpy(np.concatenate(
+        [texture.wisp_ndc_points(w_, r[1:5], pbm, scm) for w_, r in zip(wisps, uv)])).to(dev)
+    evm = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    texture.patch_masks_device(posm, [50] * len(wisps), [64] * len(wisps), (Hm, Wm))
+    torch.cuda.synchronize()
+    evm[0].record(torch.cuda.current_stream())
+    for _ in range(reps):
+        texture.patch_masks_device(posm, [50] * len(wisps), [64] * len(wisps), (Hm, Wm))
+    evm[1].record(torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    ms_dev = evm[0].elapsed_time(evm[1]) / reps
+    allp = np.concatenate([w.reshape(-1, 3) for w in wisps])
+    bb = allp[:, :2].max(0) - allp[:, :2].min(0)
+    sz = np.zeros(3)
+    for w_ in wisps:
+        q = w_.reshape(-1, 3)
+        sz = np.maximum(sz, q.max(0) - q.min(0))
+    sc = 2.0 * np.ones(3) / sz
+    sc[2] *= 0.2
+    idx = np.arange(50 * 64).reshape(50, 64)
+    edges_np = np.stack([idx[:, :-1].reshape(-1), idx[:, 1:].reshape(-1)], 1).astype(np.int32)
+    edges_t = torch.from_numpy(edges_np).to(dev)
+    loop_pos = []
+    for w_, r in zip(wisps, uv):
+        ph = torch.ones((50 * 64, 4), device=dev)
+        ph[:, :3] = torch.from_numpy(texture.wisp_ndc_points(w_, r[1:5], bb, sc)).to(dev)
+        loop_pos.append(ph[None])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for ph in loop_pos:
+        fp, fa = dda_compute_fragments((Hm, Wm), ph, edges_t)
+        rast = depth_test(1, (Hm, Wm), fp, fa)
+        mk = dda_interpolate_attributes(rast, torch.ones_like(ph[0, :, :1]), edges_t)[0]
+        mk = (torch.flip(mk[..., 0], dims=(0,)).clamp(0, 1) * 255).to(torch.uint8)
+    torch.cuda.synchronize()
+    ms_loop = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    for w_, r in zip(wisps[:5], uv[:5]):
+        oracle.patch_mask(texture.wisp_ndc_points(w_, r[1:5], bb, sc), 50, 64, Hm, Wm)
+    cpu_masks = (time.perf_counter() - t0) * 1e3 * len(wisps) / 5
+    out["patch_masks"] = {"wisps": len(wisps), "strands_per_wisp": 50, "points": 64,
+                          "patch": [int(Hm), int(Wm)],
+                          "one_launch_ms": round(ms_dev, 3),
+                          "with_host_prep_and_copy_ms": round(ms_masks, 2),
+                          "per_wisp_loop_ms": round(ms_loop, 2),
+                          "cpu_baseline": {"ms": round(cpu_masks, 1), "cores": 1,
+                                           "kind": "port (C DDA oracle + numpy)",
+                                           "sample": "5 of 300 wisps, scaled"}}
     print(f"[bench] texture patch: {out}", file=sys.stderr, flush=True)
     return out
 

@@ -381,6 +381,18 @@ int s2c_msaa_resolve(int num_samples, int H, int W, const float* rast, const flo
                      void* stream);
 int s2c_blur3x3_reflect(int H, int W, const float* kernel3, const float* in, float* tmp,
                         float* out, void* stream);
+/* The wisp masks of render_patches_reduced (texture_utils.py:290-411) for
+ * num_wisps (<= 65535) wisps in one call: wisp g has rows[g] strands of
+ * cols[g] points, clip-space float4 vertices from pos + 4 * voff[g] (int64
+ * vertex offsets), and the edges of build_edge_indices_from_wisp
+ * (__init__.py:165-175).  out (num_wisps, H, W) bytes = the DDA fragments'
+ * depth test and the interpolation of a constant 1 (dda.cu), flipped
+ * vertically, clamped and scaled by 255 with truncation, 0 where nothing
+ * is drawn.  keys: (num_wisps, H, W) int64 scratch; max_edges bounds
+ * rows[g] * (cols[g] - 1). */
+int s2c_patch_masks(int num_wisps, int H, int W, const int* rows, const int* cols,
+                    const long long* voff, int max_edges, const float* pos, long long* keys,
+                    unsigned char* out, void* stream);
 /* The fused form of rasterize + s2c_msaa_resolve for the jittered views
  * (num_samples, V, 4) of s2c_msaa_jitter: no fragment lists; per (view,
  * pixel) the minimum (bits(z + 2) << 32 | triangle) over the covering

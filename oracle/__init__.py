@@ -668,3 +668,21 @@ def msaa_patch(pos, tri, attr, H, W, dx, dy, weights, kernel3):
     rgb[cnt == 0] = np.array([0, 1, 0], np.float32)
     rgb = np.float32(0.5) * (rgb + np.float32(1))
     return rgb.astype(np.float32), blur3x3_reflect(al, kernel3)
+
+
+def patch_mask(pts, rows, cols, H, W):
+    """One wisp's mask of texture_utils.py:346-358: dda_compute_fragments
+    + depth_test + dda_interpolate_attributes of a constant 1 over the
+    edges of build_edge_indices_from_wisp, flipped, clamped, bytes.
+    pts (rows*cols, 3) float32 NDC points."""
+    pts = np.asarray(pts, np.float32)
+    pos = np.ones((1, pts.shape[0], 4), np.float32)
+    pos[0, :, :3] = pts
+    idx = np.arange(rows * cols).reshape(rows, cols)
+    edges = np.stack([idx[:, :-1].reshape(-1), idx[:, 1:].reshape(-1)], 1).astype(np.int32)
+    fp, fa = dda_compute_fragments((H, W), pos, edges)
+    rast = depth_test(1, (H, W), fp, fa)
+    m = dda_interpolate_attributes(rast, np.ones((pts.shape[0], 1), np.float32), edges)[0, ..., 0]
+    m = np.clip(m[::-1], 0, 1)
+    return (m * 255).astype(np.uint8)
+

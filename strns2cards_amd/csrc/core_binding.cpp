@@ -863,4 +863,21 @@ PYBIND11_MODULE(_core, m) {
                              P<float>(alpha), stream_of(jittered)),
               fn);
     });
+    m.def("patch_masks", [](Tensor rows, Tensor cols, Tensor voff, int max_edges, Tensor pos,
+                            Tensor keys, Tensor out) {
+        const char* fn = "patch_masks";
+        need(rows, fn, "rows", at::kInt, {-1});
+        need(cols, fn, "cols", at::kInt, {rows.size(0)});
+        need(voff, fn, "voff", at::kLong, {rows.size(0)});
+        need(pos, fn, "pos", at::kFloat, {-1, 4});
+        need(out, fn, "out", at::kByte, {rows.size(0), -1, -1});
+        need(keys, fn, "keys", at::kLong, {rows.size(0), out.size(1), out.size(2)});
+        same_device(fn, {&rows, &cols, &voff, &pos, &keys, &out});
+        DeviceGuard g(pos.device());
+        check(s2c_patch_masks(I(rows.size(0), fn), I(out.size(1), fn), I(out.size(2), fn),
+                              P<int>(rows), P<int>(cols), P<long long>(voff), max_edges,
+                              P<float>(pos), P<long long>(keys), P<unsigned char>(out),
+                              stream_of(pos)),
+              fn);
+    });
 }

@@ -12,6 +12,7 @@
 // The alpha is blurred with a normalized 3-tap Gaussian (reflect borders),
 // horizontally then vertically, and the patch is flipped vertically,
 // clamped, quantized to bytes and pasted into the atlas.
+#include "dda_math.hpp"
 #include "raster_math.hpp"
 
 namespace s2c {
@@ -182,6 +183,95 @@ msaa_resolve_keys_kernel(int S, int H, int W, int V, const float* __restrict__ p
     alpha[i] = al;
 }
 
+// --------------------------------------------------------------- patch masks
+// render_patches_reduced (texture_utils.py:290-411): per wisp, the DDA
+// fragments of its strands' polylines, the depth test and the
+// interpolation of a constant 1 (1 - t1 + t1, which may round below 1),
+// flipped and quantized.  All wisps in one launch, without fragment lists:
+// a DDA walk visits distinct major-axis columns, so an edge has at most one
+// fragment per pixel, the drop-in depth test's tie order at a pixel is the
+// edge order, and the minimum of (bits(zw + 2) << 32 | edge) keeps its
+// fragment; the resolve recomputes that edge's fragment at the pixel.
+// Wisp g: rows[g] strands of cols[g] points from vertex voff[g] (clip
+// float4, w = 1 as the reference builds them); edge e of the wisp joins
+// point (e / (cols - 1), e % (cols - 1)) and the next one
+// (build_edge_indices_from_wisp, __init__.py:165-175).
+
+__device__ __forceinline__ void wisp_edge(int e, int N, int& v0) {
+    const int n = e / (N - 1);
+    v0 = n * N + (e - n * (N - 1));
+}
+
+__global__ void __launch_bounds__(kBlock)
+patch_mask_depth_kernel(int H, int W, const int* __restrict__ rows, const int* __restrict__ cols,
+                        const long long* __restrict__ voff, const float* __restrict__ pos,
+                        long long* __restrict__ keys) {
+    const int g = blockIdx.y;
+    const int N = cols[g];
+    if (N < 2) return;
+    const int E = rows[g] * (N - 1);
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= E) return;
+    int v0;
+    wisp_edge(e, N, v0);
+    const float4 P0 = load_pos(pos, voff[g] + v0), P1 = load_pos(pos, voff[g] + v0 + 1);
+    DdaSpan sp;
+    bool wrote;
+    const int cnt = dda_span(H, W, P0, P1, sp, wrote);
+    long long* kg = keys + static_cast<long long>(g) * H * W;
+    for (int k = 0; k < cnt; ++k) {
+        DdaFrag f;
+        if (!dda_frag(H, W, sp, k, P0, P1, f)) continue;
+        const unsigned long long key =
+            (static_cast<unsigned long long>(__float_as_uint(f.zw + 2.f)) << 32) |
+            static_cast<unsigned>(e);
+        atomicMin(&kg[static_cast<long long>(f.h) * W + f.w], static_cast<long long>(key));
+    }
+}
+
+// out[g, H - 1 - h, w] = byte(clamp(t0 * 1 + t1 * 1)) of the winner, 0
+// where no fragment.
+__global__ void __launch_bounds__(kBlock)
+patch_mask_resolve_kernel(int H, int W, const int* __restrict__ cols,
+                          const long long* __restrict__ voff, const float* __restrict__ pos,
+                          const long long* __restrict__ keys, unsigned char* __restrict__ out) {
+    const int g = blockIdx.y;
+    const long long npix = static_cast<long long>(H) * W;
+    const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= npix) return;
+    const int h = static_cast<int>(i / W), w = static_cast<int>(i - static_cast<long long>(h) * W);
+    const int e = static_cast<int>(static_cast<unsigned>(keys[g * npix + i] & 0xFFFFFFFFll));
+    float m = 0.f;
+    if (e >= 0) {
+        int v0;
+        wisp_edge(e, cols[g], v0);
+        const float4 P0 = load_pos(pos, voff[g] + v0), P1 = load_pos(pos, voff[g] + v0 + 1);
+        DdaSpan sp;
+        bool wrote;
+        dda_span(H, W, P0, P1, sp, wrote);
+        // the candidate on this pixel's major-axis column (k - 1 and k + 1
+        // as well, should a clamped end of the walk round across a column)
+        // (were two of them on the pixel, the depth test keeps the smaller
+        // key, then the earlier candidate: so does this search)
+        const int k = (sp.axis == 0.f ? w : h) - f2i(sp.axis == 0.f ? sp.sx : sp.sy);
+        unsigned best = 0xFFFFFFFFu;
+        for (int kk = k - 1; kk <= k + 1; ++kk) {
+            DdaFrag f;
+            if (kk >= 0 && dda_frag(H, W, sp, kk, P0, P1, f) && f.h == h && f.w == w) {
+                const unsigned zb = __float_as_uint(f.zw + 2.f);
+                if (best == 0xFFFFFFFFu || static_cast<int>(zb) < static_cast<int>(best)) {
+                    best = zb;
+                    const float t1 = f.t1, t0 = 1.f - t1;
+                    m = t0 * 1.f + t1 * 1.f;
+                }
+            }
+        }
+    }
+    const float c = fminf(fmaxf(m, 0.f), 1.f);
+    out[g * npix + static_cast<long long>(H - 1 - h) * W + w] =
+        static_cast<unsigned char>(static_cast<int>(c * 255.f));
+}
+
 __device__ __forceinline__ int reflect_index(int i, int n) {
     return i < 0 ? -i : (i >= n ? 2 * n - 2 - i : i);
 }
@@ -278,6 +368,28 @@ extern "C" int s2c_msaa_patch(int V, int T, const float* jittered, const int* tr
     }
     msaa_resolve_keys_kernel<<<grid_for(npix, kBlock), kBlock, 0, s>>>(
         num_samples, H, W, V, jittered, keys, attr, tri, weights, rgb, alpha);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
+
+extern "C" int s2c_patch_masks(int num_wisps, int H, int W, const int* rows, const int* cols,
+                               const long long* voff, int max_edges, const float* pos,
+                               long long* keys, unsigned char* out, void* stream) {
+    S2C_CHECK(num_wisps >= 0 && H >= 0 && W >= 0 && max_edges >= 0, "patch_masks: negative size");
+    S2C_CHECK(num_wisps <= 65535, "patch_masks: at most 65535 wisps per call");
+    const long long npix = static_cast<long long>(H) * W;
+    if (num_wisps == 0 || npix == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long nkeys = npix * num_wisps;
+    fill_i64_kernel<<<grid_for(nkeys, kBlock), kBlock, 0, s>>>(keys, LLONG_MAX, nkeys);
+    S2C_LAUNCH_CHECK();
+    if (max_edges > 0) {
+        patch_mask_depth_kernel<<<dim3(grid_for(max_edges, kBlock), num_wisps), kBlock, 0, s>>>(
+            H, W, rows, cols, voff, pos, keys);
+        S2C_LAUNCH_CHECK();
+    }
+    patch_mask_resolve_kernel<<<dim3(grid_for(npix, kBlock), num_wisps), kBlock, 0, s>>>(
+        H, W, cols, voff, pos, keys, out);
     S2C_LAUNCH_CHECK();
     return S2C_OK;
 }

@@ -18,6 +18,10 @@ resolves every pixel over its samples in one kernel:
 
 ``render_texture`` runs the reference's per-wisp loop on top of it and
 pastes each patch (flipped vertically, as bytes) into the atlas on the GPU.
+
+``patch_masks`` renders the wisp masks of ``render_patches_reduced``
+(texture_utils.py:290-411), all wisps in one launch, with the same
+depth-key idea for DDA fragments.
 """
 from __future__ import annotations
 
@@ -104,6 +108,25 @@ def atlas_rect(uv_rect, resolution: int, patch_res: tuple[int, int]):
     return crop_x0, 0, x0, y0, x1, y1
 
 
+def packing_frame(packed_wisps):
+    """(points, point offsets, packed_bbox_size, to_ndc_scale) of the packed
+    wisps (texture_utils.py:120-141): the xy extent of all points, and 2
+    over the largest per-wisp extent (z scaled by 0.2 more).  Per-wisp
+    extents come from one segmented min / max over the concatenated points
+    (the same exact values as the reference's per-wisp reductions)."""
+    pts = np.concatenate([np.asarray(w).reshape(-1, 3) for w in packed_wisps], axis=0)
+    counts = np.array([np.asarray(w).reshape(-1, 3).shape[0] for w in packed_wisps])
+    if counts.min() < 1:
+        raise ValueError("every packed wisp needs at least one point")
+    offs = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    packed_bbox_size = pts[:, :2].max(axis=0) - pts[:, :2].min(axis=0)
+    ext = np.maximum.reduceat(pts, offs, axis=0) - np.minimum.reduceat(pts, offs, axis=0)
+    bbox_size = np.maximum(np.zeros(3), ext.max(axis=0))
+    to_ndc_scale = 2.0 * np.ones(3) / bbox_size
+    to_ndc_scale[2] *= 0.2
+    return pts, counts, packed_bbox_size, to_ndc_scale
+
+
 def wisp_ndc_mesh(wisp: np.ndarray, uv_rect, packed_bbox_size, to_ndc_scale, tube_radius,
                   n_circle):
     """The tube mesh of one (N, S, 3) wisp with its tangents as attribute,
@@ -135,14 +158,7 @@ def render_texture(packed_wisps, uv_coords, tube_radius, n_circle, resolution=10
     max_u = max(u1 - u0 for _, u0, _, u1, _ in uv_coords)
     max_v = max(v1 - v0 for _, _, v0, _, v1 in uv_coords)
     patch_res = (int(math.ceil(max_v * resolution)), int(math.ceil(max_u * resolution)))
-    packed_points = np.concatenate([w.reshape(-1, 3) for w in packed_wisps], axis=0)
-    packed_bbox_size = packed_points[:, :2].max(axis=0) - packed_points[:, :2].min(axis=0)
-    bbox_size = np.zeros(3)
-    for wisp in packed_wisps:
-        p = wisp.reshape(-1, 3)
-        bbox_size = np.maximum(bbox_size, p.max(axis=0) - p.min(axis=0))
-    to_ndc_scale = 2.0 * np.ones(3) / bbox_size
-    to_ndc_scale[2] *= 0.2
+    _, _, packed_bbox_size, to_ndc_scale = packing_frame(packed_wisps)
     dev = torch.device(device)
     atlas_rgb = torch.zeros((resolution, resolution, 3), dtype=torch.uint8, device=dev)
     atlas_alpha = torch.zeros((resolution, resolution), dtype=torch.uint8, device=dev)
@@ -171,3 +187,99 @@ def render_texture(packed_wisps, uv_coords, tube_radius, n_circle, resolution=10
                 iio.imwrite(os.path.join(output_dirname, f"tangent.{ext}"), rgb_np)
                 iio.imwrite(os.path.join(output_dirname, f"alpha.{ext}"), alpha_np)
     return rgb_np, alpha_np
+
+
+def wisp_ndc_points(wisp: np.ndarray, uv_rect, packed_bbox_size, to_ndc_scale) -> np.ndarray:
+    """(N*S, 3) float32 strand points of one wisp in its patch's NDC frame
+    (texture_utils.py:334-343; host numpy in the reference, kept so)."""
+    u0, v0, u1, v1 = uv_rect
+    verts = np.array(wisp, copy=True).reshape(-1, 3)
+    u0w, u1w = u0 * packed_bbox_size[0], u1 * packed_bbox_size[0]
+    verts[:, 0] -= 0.5 * (u0w + u1w)
+    verts[:, 1] -= v0 * packed_bbox_size[1]
+    verts *= to_ndc_scale
+    verts[:, 1] -= 1.0
+    return verts.astype(np.float32)
+
+
+def patch_masks_device(pos: torch.Tensor, rows, cols, patch_res: tuple[int, int]) -> torch.Tensor:
+    """(num_wisps, H, W) uint8 device masks of wisps whose clip-space points
+    (float4, w = 1) lie back to back in `pos` (wisp g: rows[g] strands of
+    cols[g] points): one launch for all wisps."""
+    H, W = patch_res
+    rows = np.asarray(rows, np.int64)
+    cols = np.asarray(cols, np.int64)
+    counts = rows * cols
+    voff = np.zeros_like(counts)
+    if counts.size:
+        voff[1:] = np.cumsum(counts)[:-1]
+    dev = pos.device
+    G = int(rows.size)
+    keys = torch.empty((G, H, W), dtype=torch.int64, device=dev)
+    out = torch.empty((G, H, W), dtype=torch.uint8, device=dev)
+    max_edges = int((rows * np.maximum(cols - 1, 0)).max()) if G else 0
+    _core.patch_masks(torch.from_numpy(rows.astype(np.int32)).to(dev),
+                      torch.from_numpy(cols.astype(np.int32)).to(dev),
+                      torch.from_numpy(voff).to(dev), max_edges, pos.contiguous(), keys, out)
+    return out
+
+
+def patch_masks(packed_wisps, uv_coords, resolution, device="cuda") -> np.ndarray:
+    """The per-wisp masks of render_patches_reduced (texture_utils.py:
+    304-358) as one (num_wisps, H, W) uint8 array: every wisp's strands
+    DDA-rasterized, depth-tested and interpolated (a constant 1) into the
+    shared patch resolution, flipped and quantized, all wisps in one
+    launch."""
+    max_u = max(u1 - u0 for _, u0, _, u1, _ in uv_coords)
+    max_v = max(v1 - v0 for _, _, v0, _, v1 in uv_coords)
+    H, W = int(math.ceil(max_v * resolution)), int(math.ceil(max_u * resolution))
+    _, _, packed_bbox_size, to_ndc_scale = packing_frame(packed_wisps)
+    # per wisp, so that the shifts keep the reference's scalar types (numpy
+    # float64 uv values make the float32 subtraction round once from double)
+    allp = np.concatenate([wisp_ndc_points(w, r[1:5], packed_bbox_size, to_ndc_scale)
+                           for w, r in zip(packed_wisps, uv_coords)])
+    dev = torch.device(device)
+    pos = torch.ones((allp.shape[0], 4), dtype=torch.float32, device=dev)
+    pos[:, :3] = torch.from_numpy(allp).to(dev)
+    rows = [w.shape[0] for w in packed_wisps]
+    cols = [w.shape[1] for w in packed_wisps]
+    return patch_masks_device(pos, rows, cols, (H, W)).cpu().numpy()
+
+
+def render_patches_reduced(packed_wisps, uv_coords, resolution, n_clusters=20,
+                           output_dirname="./output", dump_patches=False):
+    """texture_utils.py:290-411: the wisp masks (on the GPU, `patch_masks`),
+    then k-means over them (scikit-learn, as the reference) and the member
+    of each cluster closest to its binarized centre.  Returns
+    (centroid_indices, labels); the mask images are written when imageio is
+    importable."""
+    from sklearn.cluster import KMeans
+
+    masks = patch_masks(packed_wisps, uv_coords, resolution)
+    mask_all = masks.reshape(len(masks), -1)
+    kmeans = KMeans(n_clusters=n_clusters, random_state=42)
+    labels = kmeans.fit_predict(mask_all)
+    centroid_indices = []
+    for k in range(n_clusters):
+        member = labels == k
+        pts = mask_all[member]
+        if len(pts) == 0:
+            continue
+        center = kmeans.cluster_centers_[k].copy()
+        center[center > 0] = 1.0
+        dist = np.sum(np.minimum(pts, center), axis=1)
+        centroid_indices.append(np.where(member)[0][np.argmin(dist)])
+    try:
+        import imageio.v3 as iio
+    except ImportError:
+        iio = None
+    if iio is not None and output_dirname is not None:
+        if dump_patches:
+            os.makedirs(os.path.join(output_dirname, "tmp"), exist_ok=True)
+            for idx, m in enumerate(masks):
+                iio.imwrite(os.path.join(output_dirname, "tmp", f"mask_{idx}.png"), m)
+        os.makedirs(os.path.join(output_dirname, "centroids"), exist_ok=True)
+        for i, idx in enumerate(centroid_indices):
+            iio.imwrite(os.path.join(output_dirname, "centroids", f"centroid_{i}.png"), masks[idx])
+    return centroid_indices, labels
+

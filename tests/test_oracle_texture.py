@@ -90,3 +90,39 @@ def test_patch_half_plane():
     np.testing.assert_array_equal(alpha[:, 20:], 0.0)
     np.testing.assert_allclose(rgb[:, :12], np.broadcast_to([1.0, 0.5, 0.0], (H, 12, 3)), atol=1e-6)
     assert np.all(np.diff(alpha[3]) <= 1e-6)  # monotone across the edge
+
+
+def test_patch_mask_segment():
+    # a horizontal segment at NDC y = 0.5 across x in [-0.5, 0.5] of an
+    # 8 x 16 patch: pixel row 0.5 (1 - 0.5) 8 = 2, flipped to row 5,
+    # columns 4 .. 11 drawn at (1 - t1) + t1 (255, or 254 when it rounds
+    # below 1), nothing else
+    pts = np.array([[-0.5, 0.5, 0.0], [0.5, 0.5, 0.0]], np.float32)
+    m = oracle.patch_mask(pts, 1, 2, 8, 16)
+    assert m.dtype == np.uint8 and m.shape == (8, 16)
+    assert np.all(m[5, 4:12] >= 254)
+    m[5, 4:12] = 0
+    assert m[5, 12] == 0 or m[5, 12] >= 254  # the end pixel x = 12.0 itself
+    m[5, 12] = 0
+    assert not m.any()
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_packing_frame_equals_per_wisp_reductions(dt):
+    """The segmented min / max gives the reference's per-wisp extents
+    (texture_utils.py:128-141) exactly, in the same dtypes."""
+    rng = np.random.default_rng(0)
+    wisps = [np.cumsum(rng.normal(size=(int(rng.integers(1, 6)), int(rng.integers(1, 40)), 3)),
+                       1).astype(dt) for _ in range(40)]
+    _, _, pb, sc = texture.packing_frame(wisps)
+    pts = np.concatenate([w.reshape(-1, 3) for w in wisps])
+    pb_ref = pts[:, :2].max(axis=0) - pts[:, :2].min(axis=0)
+    size = np.zeros(3)
+    for w in wisps:
+        p = w.reshape(-1, 3)
+        size = np.maximum(size, p.max(axis=0) - p.min(axis=0))
+    sc_ref = 2.0 * np.ones(3) / size
+    sc_ref[2] *= 0.2
+    assert pb.dtype == pb_ref.dtype and sc.dtype == sc_ref.dtype
+    np.testing.assert_array_equal(pb, pb_ref)
+    np.testing.assert_array_equal(sc, sc_ref)
