@@ -40,6 +40,8 @@ def strands_to_strips_device(points: torch.Tensor, counts: torch.Tensor, offsets
     strip res_w rows by counts[s] columns).  Its grid layout is rows =
     res_w and cols = counts repeated per rotation."""
     rot = [float(a) for a in tangent_rotations_deg]
+    if counts.numel() and int(counts.min()) < 2:
+        raise ValueError("strands_to_strips: every strand needs at least 2 points")
     out = torch.empty((points.shape[0] * res_w * len(rot), 3), dtype=torch.float32,
                       device=points.device)
     _core.strands_to_strips(counts, offsets, points.contiguous(), int(res_w), float(width), rot,

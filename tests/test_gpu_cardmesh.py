@@ -117,3 +117,11 @@ def test_default_strips_feed_the_grid_solve(dev):
     assert M.rows_host.tolist() == rows.tolist() and M.cols_host.tolist() == cols.tolist()
     x = M.solve(M.apply(verts))
     assert ((x - verts).norm() / verts.norm()).item() < 1e-5
+
+
+def test_short_strands_are_rejected(dev):
+    pts, counts, offsets, _ = cardmesh._ragged([np.zeros((3, 3), np.float32),
+                                                np.zeros((1, 3), np.float32)], dev)
+    with pytest.raises(ValueError, match="2 points"):
+        cardmesh.strands_to_strips_device(pts, counts, offsets)
+
