@@ -503,6 +503,29 @@ def texture_section(dev, reps=5):
     return out
 
 
+def joint_fit_section(dev, iters=50):
+    """3_fitting.py's loop at its defaults (13 views at 200 x 200, 300
+    clusters, about 15,000 card quads in 600 crossed 2-row strips, large
+    steps lam 100 / w 1/100, lr 0.1) on synthetic strands (10,000 x 64
+    points): ms per iteration through the drop-in API, and the XOR count
+    over all views before and after `iters` iterations."""
+    from strns2cards_amd import fitting
+
+    t0 = time.perf_counter()
+    inp = fitting.joint_fit_inputs(device=dev)
+    prep = time.perf_counter() - t0
+    fitting.joint_fit(**inp, num_iters=3)  # warm-up
+    rep = fitting.joint_fit(**inp, num_iters=iters)
+    out = {"strands": 10000, "points": 64, "clusters": 300, "views": 13, "resolution": 200,
+           "card_vertices": int(inp["card_pos"].shape[0]), "quads": int(inp["quads"].shape[0]),
+           "iters": iters, "ms_per_iter": round(rep.ms_per_iter, 3),
+           "iters_per_s": round(1e3 / rep.ms_per_iter, 1),
+           "xor_first": rep.xor_first, "xor_last": rep.xor_last,
+           "input_prep_s": round(prep, 2)}
+    print(f"[bench] joint fit: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def copy_calibration(dev, nbytes=1 << 29, reps=3):
     """A plain device-to-device copy of known size, so PMC byte counters can
     be scaled against it."""
@@ -529,7 +552,7 @@ def main():
     ap.add_argument("--no-largesteps", action="store_true",
                     help="skip the large-steps, card-mesh and texture sections")
     ap.add_argument("--largesteps-only", action="store_true",
-                    help="only the large-steps, card-mesh and texture sections")
+                    help="only the large-steps, card-mesh, texture and joint-fit sections")
     ap.add_argument("--dda-only", action="store_true",
                     help="time the DDA->mask step only (for PMC runs)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -553,7 +576,8 @@ def main():
     if args.largesteps_only:
         print(json.dumps({"largesteps": largesteps_section(dev),
                           "card_mesh": card_mesh_section(dev),
-                          "texture_patch": texture_section(dev)}), flush=True)
+                          "texture_patch": texture_section(dev),
+                          "joint_fit": joint_fit_section(dev)}), flush=True)
         return
     t_gen = time.perf_counter()
     w = synth.full_workload()
@@ -651,6 +675,7 @@ def main():
     lsteps = None if args.no_largesteps else largesteps_section(dev)
     cmesh = None if args.no_largesteps else card_mesh_section(dev)
     tex = None if args.no_largesteps else texture_section(dev)
+    jfit = None if args.no_fit else joint_fit_section(dev)
 
     cpu = None
     if not args.no_cpu:
@@ -692,6 +717,7 @@ def main():
         "largesteps": lsteps,
         "card_mesh": cmesh,
         "texture_patch": tex,
+        "joint_fit": jfit,
     }
     print(json.dumps(out), flush=True)
 

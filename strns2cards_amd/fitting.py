@@ -451,3 +451,128 @@ def poisson_arrivals(jobs: list[FitJob], rate: float, seed: int = 0) -> list[Fit
     for j, a in zip(jobs, t):
         j.arrival = float(a)
     return jobs
+
+
+# ------------------------------------------------------------ joint fit
+
+
+@dataclass
+class JointFitReport:
+    pos: torch.Tensor          # (V, 3) fitted card vertices
+    xor_first: int             # popcount(render XOR target) over all views, before
+    xor_last: int              # ... and after the iterations
+    ms_per_iter: float
+
+
+def joint_fit(points: torch.Tensor, edges: torch.Tensor, edge2cluster: torch.Tensor,
+              cluster_count: int, card_pos: torch.Tensor, tri: torch.Tensor,
+              tri2cluster: torch.Tensor, quads: torch.Tensor, h_edges: torch.Tensor,
+              w_edges: torch.Tensor, resolution: int = 200, num_iters: int = 1000,
+              lr: float = 0.1, alpha: float = 0.5, lam: float = 100.0, w_weight: float = 1 / 100,
+              w_pln: float = 10.0, w_lap: float = 1e-2, margin: float = 0.3,
+              mvp: torch.Tensor | None = None) -> JointFitReport:
+    """The fitting loop of 3_fitting.py:232-407 (all cards, all views) on
+    the drop-in API, with its defaults: 13 orthographic views
+    (build_ortho_cameras), target cluster masks from the DDA of every strand
+    edge, and per iteration pos = from_differential(M, param), the card
+    cluster masks (render_bitset_from_triangles), the antialiased loss with
+    kernel radius 7 and rho = alpha, planarity and uniform-Laplacian terms,
+    UniformAdam and an exponential lr decay to 1e-2 over num_iters.  M is
+    the card grids' GridSystem (lam, w_weight).  Device tensors: points
+    (N, 3), edges (E, 2), edge2cluster (E,), card_pos (V, 3), tri (T, 3),
+    tri2cluster (T,), quads (Q, 4), h_edges, w_edges."""
+    from . import strand_io
+    from .dda import dda_compute_fragments
+
+    dev = points.device
+    H = W = int(resolution)
+    if mvp is None:
+        mvp = torch.from_numpy(synth.ortho_cameras(points.detach().cpu().numpy(), 13, margin))
+    mvp = mvp.to(dev, torch.float32)
+    ph = torch.cat([points, torch.ones_like(points[:, :1])], -1)[None]
+    pts_ndc = torch.matmul(ph, mvp.transpose(1, 2)).contiguous()
+    fp, fa = dda_compute_fragments((H, W), pts_ndc, edges)
+    target = cluster_mask_from_fragments((H, W), fp, fa, edge2cluster, cluster_count=cluster_count)
+
+    cedges_np, e2t_np = strand_io.build_edges_from_triangles(tri.cpu().numpy().astype(np.int32))
+    cedges = torch.from_numpy(cedges_np).to(dev)
+    e2t = torch.from_numpy(e2t_np).to(dev)
+    ce2c = tri2cluster[e2t[:, 0].long()].contiguous()
+    V = card_pos.shape[0]
+    M = GridSystem.from_edges(V, h_edges, w_edges, lam, w_weight, device=dev)
+    quad_edges = torch.cat([h_edges, w_edges], 0)
+    L_uniform = laplacian_edges(card_pos, quad_edges)
+    param = to_differential(M, card_pos.float()).detach().requires_grad_(True)
+    opt = UniformAdam([param], lr=lr, betas=(0.9, 0.999))
+    sched = torch.optim.lr_scheduler.ExponentialLR(opt, gamma=1e-2 ** (1 / num_iters))
+    mt = mvp.transpose(1, 2)
+
+    def render(pos):
+        pos_h = torch.cat([pos, torch.ones_like(pos[..., :1])], -1)
+        pos_ndc_h = torch.matmul(pos_h, mt).contiguous()
+        f_pix, f_attr = compute_fragments((H, W), pos_ndc_h.detach(), tri)
+        bits = cluster_mask_from_fragments((H, W), f_pix, f_attr, tri2cluster,
+                                           cluster_count=cluster_count)
+        return pos_ndc_h, bits
+
+    def xor_count(bits):
+        return int(popcount_bitset(torch.bitwise_xor(bits, target)).sum().item())
+
+    with torch.no_grad():
+        xor_first = xor_count(render(from_differential(M, param))[1])
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(num_iters):
+        opt.zero_grad()
+        pos = from_differential(M, param)
+        pos_ndc_h, bits = render(pos)
+        loss = antialiased_cluster_bitset_loss(bits, target, pos_ndc_h, tri, cedges, e2t, ce2c,
+                                               kernel_radius=7.0, rho=alpha)
+        loss = loss + w_pln * planarity_loss(pos, quads)
+        loss = loss + w_lap * torch.sum(torch.sparse.mm(L_uniform, pos) ** 2)
+        loss.backward()
+        opt.step()
+        sched.step()
+    torch.cuda.synchronize(dev)
+    ms = (time.perf_counter() - t0) * 1e3 / max(num_iters, 1)
+    with torch.no_grad():
+        pos = from_differential(M, param)
+        xor_last = xor_count(render(pos)[1])
+    return JointFitReport(pos.detach(), xor_first, xor_last, ms)
+
+
+def joint_fit_inputs(num_strands: int = 10000, num_vertices: int = 64, clusters: int = 300,
+                     num_quads: int = 15000, width: float = 0.1, seed: int = 0, device="cuda"):
+    """Synthetic inputs of the joint fit at 3_fitting.py's scale: seeded
+    strands (cm), k-means clusters of the roots, and per cluster its mean
+    strand as guide, swept into two crossed 2-row strips (2_texture.py's
+    defaults) with quads shared in proportion to guide length, on the GPU
+    (cardmesh).  Returns a dict of device tensors for ``joint_fit``."""
+    from . import cardmesh
+
+    dev = torch.device(device)
+    strands = synth.generate_strands(num_strands, num_vertices, 0.004, seed) * 100.0
+    labels = synth.kmeans_clusters(strands[:, 0], clusters, seed)
+    ids = np.unique(labels)
+    guides, lengths = [], []
+    for c in ids:
+        g = strands[labels == c].mean(0)
+        guides.append(g)
+        lengths.append(np.linalg.norm(np.diff(g, axis=0), axis=1).sum())
+    lengths = np.array(lengths)
+    quads = np.maximum(1, np.rint(num_quads / 2 * lengths / lengths.sum())).astype(int)
+    guides = [synth.resample_by_arclength(g[None], int(q) + 1)[0].astype(np.float32)
+              for g, q in zip(guides, quads)]
+    pts, cnt, offs, _ = cardmesh._ragged(guides, dev)
+    card_pos = cardmesh.strands_to_strips_device(pts, cnt, offs, 2, width, [-45.0, 45.0])
+    rows = np.full(2 * len(guides), 2)
+    cols = np.repeat(quads + 1, 2)
+    topo = cardmesh.grid_mesh_topology_device(rows, cols, grid2cluster=np.repeat(ids, 2),
+                                              device=dev)
+    points = torch.from_numpy(strands.reshape(-1, 3).astype(np.float32)).to(dev)
+    edges = torch.from_numpy(synth.strand_edges(num_strands, num_vertices)).to(dev)
+    e2c = torch.from_numpy(np.repeat(labels, num_vertices - 1).astype(np.int32)).to(dev)
+    return dict(points=points, edges=edges, edge2cluster=e2c, cluster_count=int(clusters),
+                card_pos=card_pos, tri=topo["faces"],
+                tri2cluster=topo["face2cluster"].to(torch.int32), quads=topo["quads"],
+                h_edges=topo["h_edges"], w_edges=topo["w_edges"])

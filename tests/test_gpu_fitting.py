@@ -102,3 +102,14 @@ def test_serve_continuous_batching(dev):
     alone.run(20)
     torch.cuda.synchronize()
     assert alone.loss[s].item() == rep.losses[js[5].jid]
+
+
+def test_joint_fit_reduces_the_mask_error(dev):
+    """3_fitting.py's loop (13 views, large steps, antialiased loss,
+    planarity and Laplacian terms) on a small model lowers the XOR count
+    between the card masks and the strand masks."""
+    print("[joint fit] 1500 strands, 24 clusters, 64^2 x 13 views", flush=True)
+    inp = fitting.joint_fit_inputs(1500, 32, 24, 1200, 0.1, seed=3, device=dev)
+    rep = fitting.joint_fit(**inp, resolution=64, num_iters=30)
+    assert torch.isfinite(rep.pos).all()
+    assert rep.xor_last < rep.xor_first, (rep.xor_first, rep.xor_last)
