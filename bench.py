@@ -507,8 +507,10 @@ def joint_fit_section(dev, iters=50):
     """3_fitting.py's loop at its defaults (13 views at 200 x 200, 300
     clusters, about 15,000 card quads in 600 crossed 2-row strips, large
     steps lam 100 / w 1/100, lr 0.1) on synthetic strands (10,000 x 64
-    points): ms per iteration through the drop-in API, and the XOR count
-    over all views before and after `iters` iterations."""
+    points): ms per iteration through the drop-in API as the script calls
+    it, and with `fused=True` (triangle_cluster_mask and the grid-Laplacian
+    kernel), and the XOR count over all views before and after `iters`
+    iterations."""
     from strns2cards_amd import fitting
 
     t0 = time.perf_counter()
@@ -516,11 +518,16 @@ def joint_fit_section(dev, iters=50):
     prep = time.perf_counter() - t0
     fitting.joint_fit(**inp, num_iters=3)  # warm-up
     rep = fitting.joint_fit(**inp, num_iters=iters)
+    fitting.joint_fit(**inp, num_iters=3, fused=True)
+    rep_f = fitting.joint_fit(**inp, num_iters=iters, fused=True)
     out = {"strands": 10000, "points": 64, "clusters": 300, "views": 13, "resolution": 200,
            "card_vertices": int(inp["card_pos"].shape[0]), "quads": int(inp["quads"].shape[0]),
            "iters": iters, "ms_per_iter": round(rep.ms_per_iter, 3),
            "iters_per_s": round(1e3 / rep.ms_per_iter, 1),
            "xor_first": rep.xor_first, "xor_last": rep.xor_last,
+           "fused_ms_per_iter": round(rep_f.ms_per_iter, 3),
+           "fused_iters_per_s": round(1e3 / rep_f.ms_per_iter, 1),
+           "fused_xor_last": rep_f.xor_last,
            "input_prep_s": round(prep, 2)}
     print(f"[bench] joint fit: {out}", file=sys.stderr, flush=True)
     return out

@@ -313,6 +313,12 @@ int s2c_card_coverage_backward(int B, int H, int W, int V, const float* pos, int
  * to 1365 columns); a grid beyond them is written as NaN. */
 int s2c_grid_system_apply(int num_grids, const int* rows, const int* cols, const int* voff,
                           float lam, float w_weight, const float* x, float* u, void* stream);
+/* y = (L_h + w_weight L_w) x, the grids' uniform Laplacians (no identity,
+ * no lam): with w_weight = 1 the laplacian_edges of h_edges + w_edges that
+ * 3_fitting.py:359 builds for its w_lap term, applied without a sparse
+ * matrix.  Symmetric, so it is also its own adjoint. */
+int s2c_grid_laplacian(int num_grids, const int* rows, const int* cols, const int* voff,
+                       float w_weight, const float* x, float* y, void* stream);
 int s2c_grid_system_factor(int num_grids, const int* rows, const int* cols, const int* fac_off,
                            float lam, float w_weight, float* factor, void* stream);
 int s2c_grid_system_solve(int num_grids, const int* rows, const int* cols, const int* voff,

@@ -683,6 +683,15 @@ PYBIND11_MODULE(_core, m) {
                                     P<float>(x), P<float>(u), stream_of(x)),
               fn);
     });
+    m.def("grid_laplacian", [grid_layout](Tensor rows, Tensor cols, Tensor voff, float w_weight,
+                                          Tensor x, Tensor y) {
+        const char* fn = "grid_laplacian";
+        const int G = grid_layout(fn, rows, cols, voff, x, y);
+        DeviceGuard g(x.device());
+        check(s2c_grid_laplacian(G, P<int>(rows), P<int>(cols), P<int>(voff), w_weight,
+                                 P<float>(x), P<float>(y), stream_of(x)),
+              fn);
+    });
     m.def("grid_system_factor", [](Tensor rows, Tensor cols, Tensor fac_off, float lam,
                                    float w_weight, Tensor factor) {
         const char* fn = "grid_system_factor";

@@ -258,6 +258,21 @@ __device__ __forceinline__ void grid_back_project(int S, int N, const float* Qb,
     }
 }
 
+// (L_h + w L_w) x at vertex-major index q = v*3 + c of one grid: the
+// degree-weighted differences with the row (h) and column (w) neighbours.
+__device__ __forceinline__ float grid_laplacian_at(int S, int N, float ww, const float* x,
+                                                   int q) {
+    const int v = q / 3;
+    const int i = v / N, j = v - i * N;
+    const float xv = x[q];
+    float h = 0.f, w = 0.f;
+    if (j > 0) h += xv - x[q - 3];
+    if (j < N - 1) h += xv - x[q + 3];
+    if (i > 0) w += xv - x[q - 3 * N];
+    if (i < S - 1) w += xv - x[q + 3 * N];
+    return h + ww * w;
+}
+
 // u = M x at vertex-major index q = v*3 + c of one grid (x vertex-major).
 __device__ __forceinline__ float grid_apply_at(int S, int N, float lam, float ww, const float* x,
                                                int q) {

@@ -46,6 +46,21 @@ grid_apply_kernel(const int* __restrict__ rows, const int* __restrict__ cols,
         u[base + q] = grid_apply_at(S, N, lam, ww, x + base, q);
 }
 
+// y = (L_h + w L_w) x, one workgroup per grid: the uniform Laplacian of
+// the grid's quad edges with the w_edges weighted (w = 1: the
+// laplacian_edges of h_edges + w_edges, 3_fitting.py:359).
+__global__ void __launch_bounds__(kBlock)
+grid_laplacian_kernel(const int* __restrict__ rows, const int* __restrict__ cols,
+                      const int* __restrict__ voff, float ww, const float* __restrict__ x,
+                      float* __restrict__ y) {
+    const int g = blockIdx.x;
+    const int S = rows[g], N = cols[g];
+    const long long base = 3ll * voff[g];
+    const int n = 3 * S * N;
+    for (int q = threadIdx.x; q < n; q += kBlock)
+        y[base + q] = grid_laplacian_at(S, N, ww, x + base, q);
+}
+
 // x = M^{-1} u, one workgroup (one wave) per grid.
 __global__ void __launch_bounds__(kWave)
 grid_solve_kernel(const int* __restrict__ rows, const int* __restrict__ cols,
@@ -112,6 +127,17 @@ extern "C" int s2c_grid_system_apply(int num_grids, const int* rows, const int* 
     if (num_grids == 0) return S2C_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
     grid_apply_kernel<<<num_grids, kBlock, 0, s>>>(rows, cols, voff, lam, w_weight, x, u);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
+
+extern "C" int s2c_grid_laplacian(int num_grids, const int* rows, const int* cols,
+                                  const int* voff, float w_weight, const float* x, float* y,
+                                  void* stream) {
+    S2C_CHECK(num_grids >= 0, "grid_laplacian: num_grids must be >= 0");
+    if (num_grids == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    grid_laplacian_kernel<<<num_grids, kBlock, 0, s>>>(rows, cols, voff, w_weight, x, y);
     S2C_LAUNCH_CHECK();
     return S2C_OK;
 }

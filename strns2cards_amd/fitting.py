@@ -33,8 +33,9 @@ from . import _core
 from . import synth
 from .antialias import antialiased_cluster_bitset_loss
 from .bitset_utils import popcount_bitset
-from .optimizer import GridSystem, UniformAdam, from_differential, laplacian_edges, to_differential
-from .rasterize import cluster_mask_from_fragments, compute_fragments
+from .optimizer import (GridSystem, UniformAdam, from_differential, grid_laplacian,
+                        laplacian_edges, to_differential)
+from .rasterize import cluster_mask_from_fragments, compute_fragments, triangle_cluster_mask
 
 MAX_RES = 256
 MAX_QUADS = 32
@@ -470,7 +471,7 @@ def joint_fit(points: torch.Tensor, edges: torch.Tensor, edge2cluster: torch.Ten
               w_edges: torch.Tensor, resolution: int = 200, num_iters: int = 1000,
               lr: float = 0.1, alpha: float = 0.5, lam: float = 100.0, w_weight: float = 1 / 100,
               w_pln: float = 10.0, w_lap: float = 1e-2, margin: float = 0.3,
-              mvp: torch.Tensor | None = None) -> JointFitReport:
+              mvp: torch.Tensor | None = None, fused: bool = False) -> JointFitReport:
     """The fitting loop of 3_fitting.py:232-407 (all cards, all views) on
     the drop-in API, with its defaults: 13 orthographic views
     (build_ortho_cameras), target cluster masks from the DDA of every strand
@@ -480,7 +481,12 @@ def joint_fit(points: torch.Tensor, edges: torch.Tensor, edge2cluster: torch.Ten
     UniformAdam and an exponential lr decay to 1e-2 over num_iters.  M is
     the card grids' GridSystem (lam, w_weight).  Device tensors: points
     (N, 3), edges (E, 2), edge2cluster (E,), card_pos (V, 3), tri (T, 3),
-    tri2cluster (T,), quads (Q, 4), h_edges, w_edges."""
+    tri2cluster (T,), quads (Q, 4), h_edges, w_edges.  `fused` renders the
+    card masks with triangle_cluster_mask (the same masks in one pass, no
+    fragment list or host sync) instead of the script's compute_fragments +
+    cluster_mask_from_fragments, and applies the uniform Laplacian of the
+    w_lap term with the grid kernel (grid_laplacian) instead of a sparse
+    matrix product, whose backward re-sorts the matrix every iteration."""
     from . import strand_io
     from .dda import dda_compute_fragments
 
@@ -510,6 +516,9 @@ def joint_fit(points: torch.Tensor, edges: torch.Tensor, edge2cluster: torch.Ten
     def render(pos):
         pos_h = torch.cat([pos, torch.ones_like(pos[..., :1])], -1)
         pos_ndc_h = torch.matmul(pos_h, mt).contiguous()
+        if fused:
+            return pos_ndc_h, triangle_cluster_mask((H, W), pos_ndc_h.detach(), tri, tri2cluster,
+                                                    cluster_count)
         f_pix, f_attr = compute_fragments((H, W), pos_ndc_h.detach(), tri)
         bits = cluster_mask_from_fragments((H, W), f_pix, f_attr, tri2cluster,
                                            cluster_count=cluster_count)
@@ -529,7 +538,8 @@ def joint_fit(points: torch.Tensor, edges: torch.Tensor, edge2cluster: torch.Ten
         loss = antialiased_cluster_bitset_loss(bits, target, pos_ndc_h, tri, cedges, e2t, ce2c,
                                                kernel_radius=7.0, rho=alpha)
         loss = loss + w_pln * planarity_loss(pos, quads)
-        loss = loss + w_lap * torch.sum(torch.sparse.mm(L_uniform, pos) ** 2)
+        lap = grid_laplacian(M, pos) if fused else torch.sparse.mm(L_uniform, pos)
+        loss = loss + w_lap * torch.sum(lap ** 2)
         loss.backward()
         opt.step()
         sched.step()

@@ -200,6 +200,33 @@ class GridSystem:
         return x
 
 
+def grid_laplacian(M: GridSystem, x: torch.Tensor, w_weight: float = 1.0) -> torch.Tensor:
+    """(L_h + w_weight L_w) x on M's grids, differentiable (L is symmetric,
+    so its backward is itself).  With w_weight = 1 it is
+    laplacian_edges(x, cat(h_edges, w_edges)) @ x, the uniform Laplacian of
+    3_fitting.py:359, without a sparse matrix."""
+    if not isinstance(M, GridSystem):
+        raise TypeError("grid_laplacian: M must be a GridSystem")
+    return _GridLaplacian.apply(x, M, float(w_weight))
+
+
+class _GridLaplacian(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, M, ww):
+        ctx.M, ctx.ww = M, ww
+        x = M._check(x, "grid_laplacian")
+        y = torch.empty_like(x)
+        _core.grid_laplacian(M.rows, M.cols, M.voff, ww, x, y)
+        return y
+
+    @staticmethod
+    def backward(ctx, grad):
+        g = ctx.M._check(grad, "grid_laplacian")
+        y = torch.empty_like(g)
+        _core.grid_laplacian(ctx.M.rows, ctx.M.cols, ctx.M.voff, ctx.ww, g, y)
+        return y, None, None
+
+
 def grid_edges(rows, cols):
     """h_edges and w_edges of concatenated grid meshes, in the order of
     wisp_to_grid_mesh (texture_utils.py:566-584) with the vertex offsets of

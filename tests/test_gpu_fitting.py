@@ -113,3 +113,15 @@ def test_joint_fit_reduces_the_mask_error(dev):
     rep = fitting.joint_fit(**inp, resolution=64, num_iters=30)
     assert torch.isfinite(rep.pos).all()
     assert rep.xor_last < rep.xor_first, (rep.xor_first, rep.xor_last)
+
+
+def test_joint_fit_fused_follows_the_same_path(dev):
+    """The fused card masks are the script's masks and the grid Laplacian is
+    its sparse Laplacian, so the fit takes the same steps (to the rounding
+    of the antialias gradient's atomics and of the Laplacian sums)."""
+    inp = fitting.joint_fit_inputs(800, 24, 12, 600, 0.1, seed=5, device=dev)
+    a = fitting.joint_fit(**inp, resolution=48, num_iters=3)
+    b = fitting.joint_fit(**inp, resolution=48, num_iters=3, fused=True)
+    assert a.xor_first == b.xor_first
+    assert torch.allclose(a.pos, b.pos, rtol=0, atol=1e-3)
+

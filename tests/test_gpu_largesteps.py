@@ -111,3 +111,22 @@ def test_layout_limits(dev):
     empty = GridSystem([], [], device=dev)
     assert from_differential(empty, torch.zeros(0, 3, device=dev)).shape == (0, 3)
     assert optimizer.to_differential(empty, torch.zeros(0, 3, device=dev)).shape == (0, 3)
+
+
+@pytest.mark.parametrize("name,shapes,lam,ww", CASES[:3], ids=[c[0] for c in CASES[:3]])
+def test_grid_laplacian_matches_oracle(dev, name, shapes, lam, ww):
+    """(L_h + w L_w) x against the oracle's laplacian_edges, elementwise
+    within 1e-6 * (|L| |x|); the backward is L applied to the gradient."""
+    V, h, w = oracle.combined_grid_edges(shapes)
+    L = oracle.laplacian_edges(V, h) + ww * oracle.laplacian_edges(V, w)
+    sys_ = GridSystem.from_edges(V, h, w, lam, ww, device=dev)
+    x = np.random.default_rng(8).standard_normal((V, 3)).astype(np.float32)
+    xt = torch.from_numpy(x).to(dev).requires_grad_(True)
+    y = optimizer.grid_laplacian(sys_, xt, ww)
+    bound = 1e-6 * (np.abs(L) @ np.abs(x.astype(np.float64)))
+    assert np.all(np.abs(y.detach().cpu().numpy() - L @ x) <= bound + 1e-30)
+    g = np.random.default_rng(9).standard_normal((V, 3)).astype(np.float32)
+    (y * torch.from_numpy(g).to(dev)).sum().backward()
+    bound = 1e-6 * (np.abs(L) @ np.abs(g.astype(np.float64)))
+    assert np.all(np.abs(xt.grad.cpu().numpy() - L @ g) <= bound + 1e-30)
+
