@@ -506,8 +506,7 @@ def joint_fit(points: torch.Tensor, edges: torch.Tensor, edge2cluster: torch.Ten
     ce2c = tri2cluster[e2t[:, 0].long()].contiguous()
     V = card_pos.shape[0]
     M = GridSystem.from_edges(V, h_edges, w_edges, lam, w_weight, device=dev)
-    quad_edges = torch.cat([h_edges, w_edges], 0)
-    L_uniform = laplacian_edges(card_pos, quad_edges)
+    L_uniform = None if fused else laplacian_edges(card_pos, torch.cat([h_edges, w_edges], 0))
     param = to_differential(M, card_pos.float()).detach().requires_grad_(True)
     opt = UniformAdam([param], lr=lr, betas=(0.9, 0.999))
     sched = torch.optim.lr_scheduler.ExponentialLR(opt, gamma=1e-2 ** (1 / num_iters))
