@@ -686,3 +686,20 @@ def patch_mask(pts, rows, cols, H, W):
     m = np.clip(m[::-1], 0, 1)
     return (m * 255).astype(np.uint8)
 
+
+
+def from_differential_sparse(V: int, h_edges, w_edges, lam: float, w_weight: float, u) -> np.ndarray:
+    """x = M^{-1} u in float64 through scipy's sparse LU of the same M
+    (laplacian_edges restated on sparse matrices): the check at sizes the
+    dense Cholesky cannot hold."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    def lap(e):
+        e = np.asarray(e, np.int64).reshape(-1, 2)
+        adj = np.unique(np.concatenate([e, e[:, ::-1]]), axis=0)
+        A = sp.coo_matrix((np.ones(len(adj)), (adj[:, 0], adj[:, 1])), shape=(V, V)).tocsr()
+        return sp.diags(np.asarray(A.sum(1)).ravel()) - A
+
+    M = (sp.eye(V) + lam * (lap(h_edges) + w_weight * lap(w_edges))).tocsc()
+    return spla.splu(M).solve(np.asarray(u, np.float64))

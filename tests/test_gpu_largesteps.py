@@ -130,3 +130,19 @@ def test_grid_laplacian_matches_oracle(dev, name, shapes, lam, ww):
     bound = 1e-6 * (np.abs(L) @ np.abs(g.astype(np.float64)))
     assert np.all(np.abs(xt.grad.cpu().numpy() - L @ g) <= bound + 1e-30)
 
+
+
+def test_solve_reference_default_size(dev):
+    """2_texture.py's default strips (600 crossed 2-row strips sharing
+    15,000 quads, 31,200 vertices) against a float64 sparse LU of the same
+    M: forward error within 1e-6 + 3e-8 kappa."""
+    rng = np.random.default_rng(11)
+    length = rng.lognormal(0.0, 0.5, 300)
+    quads = np.maximum(1, np.rint(7500 * length / length.sum())).astype(int)
+    shapes = [(2, int(q) + 1) for q in np.repeat(quads, 2)]
+    V, h, w = oracle.combined_grid_edges(shapes)
+    sys_ = GridSystem([s[0] for s in shapes], [s[1] for s in shapes], 100.0, 0.01, device=dev)
+    u = rng.standard_normal((V, 3)).astype(np.float32)
+    x = from_differential(sys_, torch.from_numpy(u).to(dev)).cpu().numpy().astype(np.float64)
+    want = oracle.from_differential_sparse(V, h, w, 100.0, 0.01, u)
+    assert np.linalg.norm(x - want) / np.linalg.norm(want) <= 1e-6 + 3e-8 * 405

@@ -95,3 +95,11 @@ def test_grid_layout_rejects_other_meshes():
         GridSystem([2, 0], [3, 3], device="cpu")
     with pytest.raises(ValueError):
         GridSystem([2], [3], lam=-1.0, device="cpu")
+
+
+def test_sparse_solve_agrees_with_cholesky():
+    V, h, w = oracle.combined_grid_edges([(2, 40), (3, 9), (1, 7)])
+    M = oracle.largesteps_matrix(V, h, w, 100.0, 0.01)
+    u = np.random.default_rng(2).standard_normal((V, 3))
+    np.testing.assert_allclose(oracle.from_differential_sparse(V, h, w, 100.0, 0.01, u),
+                               oracle.from_differential(M, u), rtol=1e-9, atol=1e-11)
