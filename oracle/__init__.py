@@ -4,7 +4,8 @@ numpy front end over ``liboracle.so`` (built from ``s2c_oracle.c`` by
 ``oracle/Makefile``), a plain-C, element-for-element restatement of the
 reference CUDA kernels (kenji-tojo/strns2cards, pystrns2cards/src/*.cu).
 Only tests, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg may
-use it, and only as the checker / the timed CPU path; the product never does.
+use it, and only as the reference the GPU results are compared against / the
+timed CPU path; the product never does.
 
 Functions take and return numpy arrays and follow the reference's *Python*
 API semantics (allocation, -1 fill, stable compaction), e.g.

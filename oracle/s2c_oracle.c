@@ -3,8 +3,8 @@
  *
  * This file restates, element for element, the arithmetic of the reference
  * CUDA kernels in kenji-tojo/strns2cards (paths relative to that repository).
- * It is the checker the HIP library is compared against and the CPU
- * baseline that bench.py times; nothing in the shipped library links it.
+ * The tests compare the HIP library against it, and bench.py times it as
+ * the CPU baseline; nothing in the shipped library links it.
  *
  * Evaluation policy (shared with the HIP kernels, which is what makes the
  * integer outputs bit-exact): every float expression is evaluated in the
