@@ -319,6 +319,17 @@ int s2c_grid_system_apply(int num_grids, const int* rows, const int* cols, const
  * matrix.  Symmetric, so it is also its own adjoint. */
 int s2c_grid_laplacian(int num_grids, const int* rows, const int* cols, const int* voff,
                        float w_weight, const float* x, float* y, void* stream);
+/* D (Q, 3) = (p0 + p2) - (p1 + p3) for every quad [p0, p1, p2, p3] =
+ * [(i,j), (i+1,j), (i+1,j+1), (i,j+1)] of the grids (the planarity term of
+ * 3_fitting.py:181-199 over the quads of texture_utils.py:557-565); grid
+ * g's quads are rows qoff[g] .. qoff[g] + (rows-1)(cols-1) in row-major
+ * (i, j) order.  The backward writes dpos (V, 3) = the adjoint of D for
+ * dD, gathered per vertex (no atomics). */
+int s2c_grid_quad_diagonals(int num_grids, const int* rows, const int* cols, const int* voff,
+                            const int* qoff, const float* pos, float* D, void* stream);
+int s2c_grid_quad_diagonals_backward(int num_grids, const int* rows, const int* cols,
+                                     const int* voff, const int* qoff, const float* dD,
+                                     float* dpos, void* stream);
 int s2c_grid_system_factor(int num_grids, const int* rows, const int* cols, const int* fac_off,
                            float lam, float w_weight, float* factor, void* stream);
 int s2c_grid_system_solve(int num_grids, const int* rows, const int* cols, const int* voff,

@@ -692,6 +692,29 @@ PYBIND11_MODULE(_core, m) {
                                  P<float>(x), P<float>(y), stream_of(x)),
               fn);
     });
+    m.def("grid_quad_diagonals", [](Tensor rows, Tensor cols, Tensor voff, Tensor qoff,
+                                    Tensor pos, Tensor D, bool backward) {
+        const char* fn = "grid_quad_diagonals";
+        need(rows, fn, "rows", at::kInt, {-1});
+        need(cols, fn, "cols", at::kInt, {rows.size(0)});
+        need(voff, fn, "voff", at::kInt, {rows.size(0)});
+        need(qoff, fn, "qoff", at::kInt, {rows.size(0)});
+        need(pos, fn, "pos", at::kFloat, {-1, 3});
+        need(D, fn, "D", at::kFloat, {-1, 3});
+        same_device(fn, {&rows, &cols, &voff, &qoff, &pos, &D});
+        DeviceGuard g(pos.device());
+        const int G = I(rows.size(0), fn);
+        if (backward)  // D holds dD, pos receives dpos
+            check(s2c_grid_quad_diagonals_backward(G, P<int>(rows), P<int>(cols), P<int>(voff),
+                                                   P<int>(qoff), P<float>(D), P<float>(pos),
+                                                   stream_of(pos)),
+                  fn);
+        else
+            check(s2c_grid_quad_diagonals(G, P<int>(rows), P<int>(cols), P<int>(voff),
+                                          P<int>(qoff), P<float>(pos), P<float>(D),
+                                          stream_of(pos)),
+                  fn);
+    });
     m.def("grid_system_factor", [](Tensor rows, Tensor cols, Tensor fac_off, float lam,
                                    float w_weight, Tensor factor) {
         const char* fn = "grid_system_factor";

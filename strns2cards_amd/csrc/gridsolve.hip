@@ -61,6 +61,51 @@ grid_laplacian_kernel(const int* __restrict__ rows, const int* __restrict__ cols
         y[base + q] = grid_laplacian_at(S, N, ww, x + base, q);
 }
 
+// Quad diagonals of the grids (the planarity term, 3_fitting.py:181-199):
+// quad (i, j) of grid g, at row qoff[g] + i (N - 1) + j, has the corners
+// [(i,j), (i+1,j), (i+1,j+1), (i,j+1)] of texture_utils.py:557-565 and
+// D = (p0 + p2) - (p1 + p3).
+__global__ void __launch_bounds__(kBlock)
+grid_quad_diag_kernel(const int* __restrict__ rows, const int* __restrict__ cols,
+                      const int* __restrict__ voff, const int* __restrict__ qoff,
+                      const float* __restrict__ pos, float* __restrict__ D) {
+    const int g = blockIdx.x;
+    const int S = rows[g], N = cols[g];
+    const long long v0 = voff[g], q0 = qoff[g];
+    const int nq = (S - 1) * (N - 1);
+    for (int q = threadIdx.x; q < 3 * nq; q += kBlock) {
+        const int k = q / 3, c = q - 3 * k;
+        const int i = k / (N - 1), j = k - i * (N - 1);
+        const long long a = v0 + static_cast<long long>(i) * N + j;
+        const float p0 = pos[3 * a + c], p1 = pos[3 * (a + N) + c];
+        const float p2 = pos[3 * (a + N + 1) + c], p3 = pos[3 * (a + 1) + c];
+        D[3 * (q0 + k) + c] = (p0 + p2) - (p1 + p3);
+    }
+}
+
+// The adjoint: dpos[v] = sum over the quads at v of +dD (corners 0, 2) or
+// -dD (corners 1, 3), in the fixed order: v as corner 0 of quad (i, j),
+// corner 1 of (i-1, j), corner 2 of (i-1, j-1), corner 3 of (i, j-1).
+__global__ void __launch_bounds__(kBlock)
+grid_quad_diag_bwd_kernel(const int* __restrict__ rows, const int* __restrict__ cols,
+                          const int* __restrict__ voff, const int* __restrict__ qoff,
+                          const float* __restrict__ dD, float* __restrict__ dpos) {
+    const int g = blockIdx.x;
+    const int S = rows[g], N = cols[g];
+    const long long v0 = voff[g], q0 = qoff[g];
+    for (int q = threadIdx.x; q < 3 * S * N; q += kBlock) {
+        const int v = q / 3, c = q - 3 * v;
+        const int i = v / N, j = v - i * N;
+        auto dq = [&](int qi, int qj) { return dD[3 * (q0 + qi * (N - 1) + qj) + c]; };
+        float acc = 0.f;
+        if (i < S - 1 && j < N - 1) acc += dq(i, j);
+        if (i > 0 && j < N - 1) acc -= dq(i - 1, j);
+        if (i > 0 && j > 0) acc += dq(i - 1, j - 1);
+        if (i < S - 1 && j > 0) acc -= dq(i, j - 1);
+        dpos[3 * (v0 + v) + c] = acc;
+    }
+}
+
 // x = M^{-1} u, one workgroup (one wave) per grid.
 __global__ void __launch_bounds__(kWave)
 grid_solve_kernel(const int* __restrict__ rows, const int* __restrict__ cols,
@@ -138,6 +183,28 @@ extern "C" int s2c_grid_laplacian(int num_grids, const int* rows, const int* col
     if (num_grids == 0) return S2C_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
     grid_laplacian_kernel<<<num_grids, kBlock, 0, s>>>(rows, cols, voff, w_weight, x, y);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
+
+extern "C" int s2c_grid_quad_diagonals(int num_grids, const int* rows, const int* cols,
+                                       const int* voff, const int* qoff, const float* pos,
+                                       float* D, void* stream) {
+    S2C_CHECK(num_grids >= 0, "grid_quad_diagonals: num_grids must be >= 0");
+    if (num_grids == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    grid_quad_diag_kernel<<<num_grids, kBlock, 0, s>>>(rows, cols, voff, qoff, pos, D);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
+
+extern "C" int s2c_grid_quad_diagonals_backward(int num_grids, const int* rows, const int* cols,
+                                                const int* voff, const int* qoff,
+                                                const float* dD, float* dpos, void* stream) {
+    S2C_CHECK(num_grids >= 0, "grid_quad_diagonals_backward: num_grids must be >= 0");
+    if (num_grids == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    grid_quad_diag_bwd_kernel<<<num_grids, kBlock, 0, s>>>(rows, cols, voff, qoff, dD, dpos);
     S2C_LAUNCH_CHECK();
     return S2C_OK;
 }

@@ -34,7 +34,7 @@ from . import synth
 from .antialias import antialiased_cluster_bitset_loss
 from .bitset_utils import popcount_bitset
 from .optimizer import (GridSystem, UniformAdam, from_differential, grid_laplacian,
-                        laplacian_edges, to_differential)
+                        grid_quad_diagonals, laplacian_edges, to_differential)
 from .rasterize import cluster_mask_from_fragments, compute_fragments, triangle_cluster_mask
 
 MAX_RES = 256
@@ -484,9 +484,13 @@ def joint_fit(points: torch.Tensor, edges: torch.Tensor, edge2cluster: torch.Ten
     tri2cluster (T,), quads (Q, 4), h_edges, w_edges.  `fused` renders the
     card masks with triangle_cluster_mask (the same masks in one pass, no
     fragment list or host sync) instead of the script's compute_fragments +
-    cluster_mask_from_fragments, and applies the uniform Laplacian of the
-    w_lap term with the grid kernel (grid_laplacian) instead of a sparse
-    matrix product, whose backward re-sorts the matrix every iteration."""
+    cluster_mask_from_fragments, applies the uniform Laplacian of the w_lap
+    term with the grid kernel (grid_laplacian) instead of a sparse matrix
+    product, whose backward re-sorts the matrix every iteration, and forms
+    the planarity term with grid_quad_diagonals instead of four gathers,
+    whose backward scatters sort their indices.  `quads` must then be the
+    grids' quads in wisp_to_grid_mesh order (as combine_wisps_with_cluster_
+    labels writes them)."""
     from . import strand_io
     from .dda import dda_compute_fragments
 
@@ -536,7 +540,10 @@ def joint_fit(points: torch.Tensor, edges: torch.Tensor, edge2cluster: torch.Ten
         pos_ndc_h, bits = render(pos)
         loss = antialiased_cluster_bitset_loss(bits, target, pos_ndc_h, tri, cedges, e2t, ce2c,
                                                kernel_radius=7.0, rho=alpha)
-        loss = loss + w_pln * planarity_loss(pos, quads)
+        if fused:
+            loss = loss + w_pln * torch.sum(grid_quad_diagonals(M, pos) ** 2)
+        else:
+            loss = loss + w_pln * planarity_loss(pos, quads)
         lap = grid_laplacian(M, pos) if fused else torch.sparse.mm(L_uniform, pos)
         loss = loss + w_lap * torch.sum(lap ** 2)
         loss.backward()

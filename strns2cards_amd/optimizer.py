@@ -176,6 +176,18 @@ class GridSystem:
                              "grid meshes (texture_utils.wisp_to_grid_mesh)")
         return sys_
 
+    def quad_layout(self):
+        """(per-grid quad offsets (device int32), total quads) of the grids'
+        (rows - 1)(cols - 1) quads, concatenated."""
+        if not hasattr(self, "_qoff"):
+            nq = (self.rows_host.astype(np.int64) - 1) * (self.cols_host.astype(np.int64) - 1)
+            off = np.zeros_like(nq)
+            if nq.size:
+                off[1:] = np.cumsum(nq)[:-1]
+            self._qoff = torch.from_numpy(off.astype(np.int32)).to(self.rows.device)
+            self._nq = int(nq.sum())
+        return self._qoff, self._nq
+
     def _check(self, t: torch.Tensor, who: str) -> torch.Tensor:
         if t.dim() != 2 or t.shape[0] != self.num_verts or t.shape[1] != 3:
             raise ValueError(f"{who}: expected a ({self.num_verts}, 3) tensor, got "
@@ -225,6 +237,37 @@ class _GridLaplacian(torch.autograd.Function):
         y = torch.empty_like(g)
         _core.grid_laplacian(ctx.M.rows, ctx.M.cols, ctx.M.voff, ctx.ww, g, y)
         return y, None, None
+
+
+def grid_quad_diagonals(M: GridSystem, pos: torch.Tensor) -> torch.Tensor:
+    """(Q, 3) (p0 + p2) - (p1 + p3) of every quad of M's grids, in the quad
+    order of wisp_to_grid_mesh (texture_utils.py:557-565), differentiable:
+    sum(D ** 2) is the planarity loss of 3_fitting.py:181-199.  The backward
+    gathers each vertex's at most four quads in a fixed order (no
+    accumulating scatter, no index sort)."""
+    if not isinstance(M, GridSystem):
+        raise TypeError("grid_quad_diagonals: M must be a GridSystem")
+    return _GridQuadDiagonals.apply(pos, M)
+
+
+class _GridQuadDiagonals(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, pos, M):
+        ctx.M = M
+        pos = M._check(pos, "grid_quad_diagonals")
+        qoff, nq = M.quad_layout()
+        D = torch.empty((nq, 3), dtype=torch.float32, device=pos.device)
+        _core.grid_quad_diagonals(M.rows, M.cols, M.voff, qoff, pos, D, False)
+        return D
+
+    @staticmethod
+    def backward(ctx, grad):
+        M = ctx.M
+        qoff, _ = M.quad_layout()
+        dpos = torch.empty((M.num_verts, 3), dtype=torch.float32, device=grad.device)
+        _core.grid_quad_diagonals(M.rows, M.cols, M.voff, qoff, dpos, grad.contiguous().float(),
+                                  True)
+        return dpos, None
 
 
 def grid_edges(rows, cols):

@@ -146,3 +146,31 @@ def test_solve_reference_default_size(dev):
     x = from_differential(sys_, torch.from_numpy(u).to(dev)).cpu().numpy().astype(np.float64)
     want = oracle.from_differential_sparse(V, h, w, 100.0, 0.01, u)
     assert np.linalg.norm(x - want) / np.linalg.norm(want) <= 1e-6 + 3e-8 * 405
+
+
+def test_grid_quad_diagonals(dev):
+    """(p0 + p2) - (p1 + p3) per quad, bit-exact against numpy on the
+    oracle's grid-mesh quads (same float32 operations); the backward equals
+    the adjoint scatter of the quads (float64, within 1e-6 * sum |dD|)."""
+    shapes = RAGGED + [(3, 2), (2, 3)]
+    V = sum(S * N for S, N in shapes)
+    quads, off = [], 0
+    for S, N in shapes:
+        quads.append(oracle.grid_mesh(S, N)["quads"] + off)
+        off += S * N
+    quads = np.concatenate(quads)
+    sys_ = GridSystem([s[0] for s in shapes], [s[1] for s in shapes], device=dev)
+    pos = np.random.default_rng(12).standard_normal((V, 3)).astype(np.float32)
+    pt = torch.from_numpy(pos).to(dev).requires_grad_(True)
+    D = optimizer.grid_quad_diagonals(sys_, pt)
+    want = (pos[quads[:, 0]] + pos[quads[:, 2]]) - (pos[quads[:, 1]] + pos[quads[:, 3]])
+    np.testing.assert_array_equal(D.detach().cpu().numpy(), want)
+    g = np.random.default_rng(13).standard_normal(want.shape).astype(np.float32)
+    (D * torch.from_numpy(g).to(dev)).sum().backward()
+    adj = np.zeros((V, 3))
+    for k, sgn in enumerate((1.0, -1.0, 1.0, -1.0)):
+        np.add.at(adj, quads[:, k], sgn * g.astype(np.float64))
+    bound = np.zeros((V, 3))
+    for k in range(4):
+        np.add.at(bound, quads[:, k], np.abs(g.astype(np.float64)))
+    assert np.all(np.abs(pt.grad.cpu().numpy() - adj) <= 1e-6 * bound + 1e-30)
