@@ -145,7 +145,7 @@ int s2c_backward_antialiased_cluster_bitset(
 
 /* s2c_backward_antialiased_cluster_bitset given the bitsets' view count B
  * (num_views; 0 = unknown).  No reference launcher corresponds: the
- * reference's takes no B.  With it, and num_slots >= 4, the residual bits
+ * reference's takes no B.  With it, and num_slots >= 16, the residual bits
  * are read from word planes (B, S, H, W) of the two bitsets built by a
  * coalesced transpose into a temporary of B*H*W*S*8 bytes, instead of one
  * word from each pixel's S-word record; fragments of views >= B add

@@ -437,14 +437,15 @@ def test_antialias_shuffled_edges(dev):
     check_antialias_backward(dev, w, cards, target, fp, fa, T(normals, dev), edges, e2t, e2c)
 
 
-@pytest.mark.parametrize("mult", [2, 16, 17])  # S = 4, 32, 34 words per pixel
+@pytest.mark.parametrize("mult", [2, 8, 16, 17])  # S = 4, 16, 32, 34 words per pixel
 def test_antialias_word_planes(dev, mult):
     """Antialias backward through the entry point that is given the view
     count, which reads the residual bits from word planes (a transpose of
-    both bitsets) when S >= 4, and through the reference-shaped one, which
-    reads the pixel records.  Two views of the small workload (the second
-    shifted in clip space); cluster ids spread as mult*k + 1 so S = 4, 32
-    and 34 (a partial second chunk of 32 words)."""
+    both bitsets) when S >= 16 and the pixel records below, and through the
+    reference-shaped one, which always reads the records.  Two views of the
+    small workload (the second shifted in clip space); cluster ids spread as
+    mult*k + 1 so S = 4 (records), 16 (the threshold), 32 and 34 (a partial
+    second chunk of 32 words)."""
     report(f"antialias backward word planes mult={mult}")
     w = synth.small_workload()
     c = w.cards
