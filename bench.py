@@ -508,8 +508,8 @@ def joint_fit_section(dev, iters=50):
     clusters, about 15,000 card quads in 600 crossed 2-row strips, large
     steps lam 100 / w 1/100, lr 0.1) on synthetic strands (10,000 x 64
     points): ms per iteration through the drop-in API as the script calls
-    it, and with `fused=True` (triangle_cluster_mask and the grid-Laplacian
-    kernel), and the XOR count over all views before and after `iters`
+    it, and with `fused=True` (triangle_cluster_mask, the grid kernels and the
+    antialias gradient from the marked edges, no host sync), and the XOR count over all views before and after `iters`
     iterations."""
     from strns2cards_amd import fitting
 

@@ -157,6 +157,21 @@ int s2c_backward_antialiased_cluster_bitset_views(
     const int* edge2cluster, float* d_pos, float kernel_radius, float rho,
     void* stream);
 
+/* The composition antialias.py:106-122 runs for the gradient
+ * (compact_valid_ints of prim_ids, dda_compute_fragments,
+ * backward_antialiased_cluster_bitset) as one launch sequence with no host
+ * synchronization: prim_ids (B*E) and normals (B*E,2) are the outputs of
+ * s2c_mark_discontinuity_edges before compaction, and each workgroup DDA-walks
+ * the marked edges it holds (dda.cu:9-87, 185-216) straight into the
+ * per-sample gradient.  num_views = B, the view count of pos and of the two
+ * bitsets.  Same result as the three calls, up to the order of the d_pos
+ * atomics; the word-plane path applies as above. */
+int s2c_backward_antialiased_cluster_bitset_edges(
+    int num_views, int H, int W, int num_slots, const int* bitset,
+    const int* target_bitset, int V, const float* pos, int E, const int* edges,
+    const int* prim_ids, const float* normals, const int* edge2cluster,
+    float* d_pos, float kernel_radius, float rho, void* stream);
+
 /* compact_utils.cu:54-77 (compact_valid_ints), stable order. */
 int s2c_compact_valid_ints(int N, const int* input, int* output,
                            int* valid_count_out, void* stream);

@@ -64,6 +64,8 @@ def antialiased_cluster_bitset_loss(
     kernel_radius: float = 7.0,
     rho: float = 1.0,
     count_xor_error: bool = False,
+    *,
+    fused: bool = False,
 ) -> torch.Tensor:
     """Antialiased silhouette loss between a rendered and a target cluster
     bitset (reference: antialias.py:66-130).
@@ -74,6 +76,11 @@ def antialiased_cluster_bitset_loss(
     edge's cluster over a (2*floor(R)+1)^2 window, false positives weighted
     by ``rho``, is pushed along the edge normal.  The value is 1, or the XOR
     popcount of the two bitsets when ``count_xor_error`` is set.
+
+    ``fused=True`` (no reference counterpart) computes the same gradient
+    without compacting the marked edges or materializing the fragment list:
+    the backward kernel DDA-walks the marked edges itself, so the call makes
+    no host synchronization (d_pos agrees up to the order of float atomics).
     """
     device = bitset.device
     assert target_bitset.device == device
@@ -84,14 +91,25 @@ def antialiased_cluster_bitset_loss(
     assert edge2cluster.device == device
 
     _, H, W, _ = target_bitset.shape
-    prim_ids, normals = mark_discontinuity_edges((H, W), pos, tri, edges, edge2tri)
-    frag_pix, frag_attrs = dda_compute_fragments((H, W), pos, edges, prim_ids)
-
     d_pos = torch.zeros_like(pos)
-    _core.backward_antialiased_cluster_bitset(
-        frag_pix.contiguous(), frag_attrs.contiguous(), bitset.contiguous(),
-        target_bitset.contiguous(), pos.contiguous(), edges.contiguous(), normals.contiguous(),
-        edge2cluster.contiguous(), d_pos, float(kernel_radius), float(rho))
+    if fused:
+        B, E = pos.shape[0], edges.shape[0]
+        marked = torch.full((B * E,), -1, dtype=torch.int32, device=device)
+        normals = torch.zeros((B * E, 2), dtype=torch.float32, device=device)
+        _core.mark_discontinuity_edges(H, W, tri.contiguous(), edges.contiguous(),
+                                       edge2tri.contiguous(), pos.contiguous(), marked, normals)
+        _core.backward_antialiased_cluster_bitset_edges(
+            marked, normals, bitset.contiguous(), target_bitset.contiguous(), pos.contiguous(),
+            edges.contiguous(), edge2cluster.contiguous(), d_pos, float(kernel_radius),
+            float(rho))
+    else:
+        prim_ids, normals = mark_discontinuity_edges((H, W), pos, tri, edges, edge2tri)
+        frag_pix, frag_attrs = dda_compute_fragments((H, W), pos, edges, prim_ids)
+        _core.backward_antialiased_cluster_bitset(
+            frag_pix.contiguous(), frag_attrs.contiguous(), bitset.contiguous(),
+            target_bitset.contiguous(), pos.contiguous(), edges.contiguous(),
+            normals.contiguous(), edge2cluster.contiguous(), d_pos, float(kernel_radius),
+            float(rho))
 
     if count_xor_error:
         loss_value = popcount_bitset(torch.bitwise_xor(bitset, target_bitset)).sum()

@@ -1,6 +1,7 @@
 // Silhouette-edge marking and the antialiased cluster-bitset edge gradient
 // (reference: pystrns2cards/src/antialias.cu).
 #include "antialias_math.hpp"
+#include "dda_math.hpp"
 
 namespace s2c {
 
@@ -153,40 +154,33 @@ __device__ __forceinline__ void wave_add_runs(float* d_pos, bool live, long long
 constexpr int kAaWaves = kBlock / kWave;
 constexpr int kAaBox = 8192;
 
+// One sample per lane, every lane of the wave calling (the body contains
+// __syncthreads(): every thread of the workgroup must call it as often).
+// `has`: the lane holds a fragment (b, h, w) of edge e with the DDA axis and
+// t1 of frag_attrs_dda (dda.cu:218-224).
 template <bool PLANES>
-__global__ void __launch_bounds__(kBlock)
-antialias_bwd_kernel(int N, const int* __restrict__ frag_pix, const float* __restrict__ fa,
-                     int H, int W, int S, const int* __restrict__ bitset,
-                     const int* __restrict__ target, int V, const float* __restrict__ pos, int E,
-                     const int* __restrict__ edges, const float* __restrict__ normals,
-                     const int* __restrict__ edge2cluster, float* __restrict__ d_pos, float R,
-                     float rho, int B, const int2* __restrict__ planes) {
-    __shared__ unsigned char box[kAaWaves][kAaBox];
-    const int wv = threadIdx.x / kWave;
-    const long long i = (static_cast<long long>(blockIdx.x) * kAaWaves + wv) * kWave + lane_id();
-
+__device__ __forceinline__ void aa_wave_fragment(
+    bool has, int b, int h, int w, float axis, float t1, int e, int H, int W, int S,
+    const int* __restrict__ bitset, const int* __restrict__ target, int V,
+    const float* __restrict__ pos, int E, const int* __restrict__ edges,
+    const float* __restrict__ normals, const int* __restrict__ edge2cluster,
+    float* __restrict__ d_pos, float R, float rho, int B, const int2* __restrict__ planes,
+    unsigned char* box) {
     // ---- per-sample setup, as the reference (antialias.cu:198-256)
     bool live = false;
-    int b = 0, h = 0, w = 0, k = 0, v0 = 0, v1 = 0;
+    int k = 0, v0 = 0, v1 = 0;
     float2 n = make_float2(0.f, 0.f);
     AaSample smp{};
-    if (i < N) {
-        b = frag_pix[i * 3 + 0];
-        h = frag_pix[i * 3 + 1];
-        w = frag_pix[i * 3 + 2];
-        const float4 a = reinterpret_cast<const float4*>(fa)[i];
-        const int e = f2i(a.w) - 1;
-        if (e >= 0) {
-            const float4* p = reinterpret_cast<const float4*>(pos + static_cast<long long>(b) * V * 4);
-            v0 = edges[e * 2 + 0];
-            v1 = edges[e * 2 + 1];
-            n = reinterpret_cast<const float2*>(normals + static_cast<long long>(b) * E * 2)[e];
-            smp = aa_sample(H, W, h, w, a.x, a.y, p[v0], p[v1], n);
-            // (h, w) outside the image only arises from a malformed fragment
-            // list; such samples are dropped instead of read out of bounds.
-            live = smp.live && h >= 0 && h < H && w >= 0 && w < W && b >= 0 && (!PLANES || b < B);
-            if (live) k = edge2cluster[e];
-        }
+    if (has && e >= 0) {
+        const float4* p = reinterpret_cast<const float4*>(pos + static_cast<long long>(b) * V * 4);
+        v0 = edges[e * 2 + 0];
+        v1 = edges[e * 2 + 1];
+        n = reinterpret_cast<const float2*>(normals + static_cast<long long>(b) * E * 2)[e];
+        smp = aa_sample(H, W, h, w, axis, t1, p[v0], p[v1], n);
+        // (h, w) outside the image only arises from a malformed fragment
+        // list; such samples are dropped instead of read out of bounds.
+        live = smp.live && h >= 0 && h < H && w >= 0 && w < W && b >= 0 && (!PLANES || b < B);
+        if (live) k = edge2cluster[e];
     }
     const int ch = live ? f2i(floorf(smp.fy)) : 0, cw = live ? f2i(floorf(smp.fx)) : 0;
     const int Ri = f2i(floorf(R));
@@ -218,11 +212,11 @@ antialias_bwd_kernel(int N, const int* __restrict__ frag_pix, const float* __res
                 for (int q = lane_id(); q < area; q += kWave) {
                     const int y = bh0 + q / bwid, x = bw0 + q % bwid;  // in [0,H) x [0,W)
                     if (PLANES) {
-                        box[wv][q] = static_cast<unsigned char>(
+                        box[q] = static_cast<unsigned char>(
                             plane_code(planes, pbase, W, y, x, k_ref));
                     } else {
                         const long long pix = plane + x + static_cast<long long>(y) * W;
-                        box[wv][q] = static_cast<unsigned char>(
+                        box[q] = static_cast<unsigned char>(
                             bit_u(bitset, pix, S, k_ref) | (bit_u(target, pix, S, k_ref) << 1));
                     }
                 }
@@ -236,7 +230,7 @@ antialias_bwd_kernel(int N, const int* __restrict__ frag_pix, const float* __res
     float dpx = 0.f, dpy = 0.f, dpw = 0.f;
     if (live) {
         const long long hw = static_cast<long long>(H) * W;
-        const ResidualSrc src{staged ? box[wv] : nullptr, bh0, bw0, bwid, bitset, target,
+        const ResidualSrc src{staged ? box : nullptr, bh0, bw0, bwid, bitset, target,
                               PLANES ? planes : nullptr,
                               PLANES ? (static_cast<long long>(b) * S + (k >> 5)) * hw : hw * b, W,
                               S, k};
@@ -245,6 +239,106 @@ antialias_bwd_kernel(int N, const int* __restrict__ frag_pix, const float* __res
     const long long vb = static_cast<long long>(b) * V;
     wave_add_runs(d_pos, adds, vb + v0, dpx * smp.t0, dpy * smp.t0, dpw * smp.t0);
     wave_add_runs(d_pos, adds, vb + v1, dpx * smp.t1, dpy * smp.t1, dpw * smp.t1);
+    __syncthreads();  // the box is reused by the wave's next sample
+}
+
+template <bool PLANES>
+__global__ void __launch_bounds__(kBlock)
+antialias_bwd_kernel(int N, const int* __restrict__ frag_pix, const float* __restrict__ fa,
+                     int H, int W, int S, const int* __restrict__ bitset,
+                     const int* __restrict__ target, int V, const float* __restrict__ pos, int E,
+                     const int* __restrict__ edges, const float* __restrict__ normals,
+                     const int* __restrict__ edge2cluster, float* __restrict__ d_pos, float R,
+                     float rho, int B, const int2* __restrict__ planes) {
+    __shared__ unsigned char box[kAaWaves][kAaBox];
+    const int wv = threadIdx.x / kWave;
+    const long long i = (static_cast<long long>(blockIdx.x) * kAaWaves + wv) * kWave + lane_id();
+    int b = 0, h = 0, w = 0, e = -1;
+    float axis = 0.f, t1 = 0.f;
+    if (i < N) {
+        b = frag_pix[i * 3 + 0];
+        h = frag_pix[i * 3 + 1];
+        w = frag_pix[i * 3 + 2];
+        const float4 a = reinterpret_cast<const float4*>(fa)[i];
+        e = f2i(a.w) - 1;
+        axis = a.x;
+        t1 = a.y;
+    }
+    aa_wave_fragment<PLANES>(i < N, b, h, w, axis, t1, e, H, W, S, bitset, target, V, pos, E,
+                             edges, normals, edge2cluster, d_pos, R, rho, B, planes, box[wv]);
+}
+
+// The same samples without a fragment list: the silhouette edges marked in
+// prim_ids (mark_edges_kernel: b*E + e, or -1) are DDA-walked in the
+// workgroup that holds them (a block scan spreads their candidates over the
+// workgroup, in (edge, candidate) order, as the stable fragment list orders
+// them), and every candidate that dda_compute_fragments would keep
+// (dda.cu:185-216) becomes the lane's sample.  No count leaves the device.
+struct AaEdgeItem {
+    DdaSpan sp;
+    int b, e;
+};
+
+template <bool PLANES>
+__global__ void __launch_bounds__(kBlock)
+antialias_edges_bwd_kernel(long long BE, int H, int W, int S, const int* __restrict__ bitset,
+                           const int* __restrict__ target, int V, const float* __restrict__ pos,
+                           int E, const int* __restrict__ edges,
+                           const int* __restrict__ prim_ids, const float* __restrict__ normals,
+                           const int* __restrict__ edge2cluster, float* __restrict__ d_pos,
+                           float R, float rho, int B, const int2* __restrict__ planes) {
+    __shared__ unsigned char box[kAaWaves][kAaBox];
+    __shared__ AaEdgeItem item[kBlock];
+    __shared__ int excl[kBlock];
+    __shared__ int sh[kBlock / kWave + 1];
+    const int wv = threadIdx.x / kWave;
+    const long long q = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    int cnt = 0;
+    if (q < BE && prim_ids[q] >= 0) {
+        const int b = static_cast<int>(q / E);
+        const int e = static_cast<int>(q - static_cast<long long>(b) * E);
+        const float4* p = reinterpret_cast<const float4*>(pos + static_cast<long long>(b) * V * 4);
+        AaEdgeItem it;
+        bool wrote;
+        cnt = dda_span(H, W, p[edges[e * 2 + 0]], p[edges[e * 2 + 1]], it.sp, wrote);
+        it.b = b;
+        it.e = e;
+        item[threadIdx.x] = it;
+    }
+    int total;
+    excl[threadIdx.x] = block_excl_scan(cnt, sh, &total);
+    __syncthreads();
+#pragma unroll 1
+    for (int base = 0; base < total; base += kBlock) {  // uniform trip count
+        const int j = base + static_cast<int>(threadIdx.x);
+        bool has = false;
+        int b = 0, h = 0, w = 0, e = -1;
+        float axis = 0.f, t1 = 0.f;
+        if (j < total) {
+            int lo = 0, hi = kBlock;  // last item with excl <= j
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (excl[mid] <= j) lo = mid;
+                else hi = mid;
+            }
+            const AaEdgeItem it = item[lo];
+            const float4* p =
+                reinterpret_cast<const float4*>(pos + static_cast<long long>(it.b) * V * 4);
+            DdaFrag f;
+            if (dda_frag(H, W, it.sp, j - excl[lo], p[edges[it.e * 2 + 0]],
+                         p[edges[it.e * 2 + 1]], f)) {
+                has = true;
+                b = it.b;
+                h = f.h;
+                w = f.w;
+                e = it.e;
+                axis = it.sp.axis;
+                t1 = f.t1;
+            }
+        }
+        aa_wave_fragment<PLANES>(has, b, h, w, axis, t1, e, H, W, S, bitset, target, V, pos, E,
+                                 edges, normals, edge2cluster, d_pos, R, rho, B, planes, box[wv]);
+    }
 }
 
 }  // namespace s2c
@@ -275,6 +369,36 @@ extern "C" int s2c_backward_antialiased_cluster_bitset(
     return s2c_backward_antialiased_cluster_bitset_views(
         0, num_frags, frag_pix, frag_attrs_dda, H, W, num_slots, bitset, target_bitset, V, pos,
         E, edges, normals, edge2cluster, d_pos, kernel_radius, rho, stream);
+}
+
+extern "C" int s2c_backward_antialiased_cluster_bitset_edges(
+    int num_views, int H, int W, int num_slots, const int* bitset, const int* target_bitset,
+    int V, const float* pos, int E, const int* edges, const int* prim_ids, const float* normals,
+    const int* edge2cluster, float* d_pos, float kernel_radius, float rho, void* stream) {
+    S2C_CHECK(num_views >= 0 && H >= 0 && W >= 0 && num_slots >= 0 && V >= 0 && E >= 0,
+              "backward_antialiased_cluster_bitset_edges: negative size");
+    const long long BE = static_cast<long long>(num_views) * E;
+    S2C_CHECK(BE <= INT_MAX, "backward_antialiased_cluster_bitset_edges: B*E exceeds INT32_MAX");
+    const long long np = static_cast<long long>(num_views) * H * W;
+    if (BE == 0 || np == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (num_slots < kPlaneMinSlots) {
+        antialias_edges_bwd_kernel<false><<<grid_for(BE, kBlock), kBlock, 0, s>>>(
+            BE, H, W, num_slots, bitset, target_bitset, V, pos, E, edges, prim_ids, normals,
+            edge2cluster, d_pos, kernel_radius, rho, num_views, nullptr);
+        S2C_LAUNCH_CHECK();
+        return S2C_OK;
+    }
+    Temp planes;
+    if (int r = planes.alloc(sizeof(int2) * static_cast<size_t>(np) * num_slots, s)) return r;
+    residual_planes_kernel<<<grid_for(np, kPlanePix), kBlock, 0, s>>>(
+        np, static_cast<long long>(H) * W, num_slots, bitset, target_bitset, planes.as<int2>());
+    S2C_LAUNCH_CHECK();
+    antialias_edges_bwd_kernel<true><<<grid_for(BE, kBlock), kBlock, 0, s>>>(
+        BE, H, W, num_slots, bitset, target_bitset, V, pos, E, edges, prim_ids, normals,
+        edge2cluster, d_pos, kernel_radius, rho, num_views, planes.as<int2>());
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
 }
 
 extern "C" int s2c_backward_antialiased_cluster_bitset_views(

@@ -432,6 +432,49 @@ PYBIND11_MODULE(_core, m) {
                     fn);
           });
 
+    // The same gradient from the marked edges themselves (prim_ids of
+    // mark_discontinuity_edges before compaction: b * E + e or -1), with the
+    // DDA walk inside the kernel: no fragment list and no host sync.
+    m.def("backward_antialiased_cluster_bitset_edges",
+          [](Tensor prim_ids, Tensor normals, Tensor bitset, Tensor target, Tensor pos,
+             Tensor edges, Tensor edge2cluster, Tensor d_pos, float kernel_radius, float rho) {
+              const char* fn = "backward_antialiased_cluster_bitset_edges";
+              need(prim_ids, fn, "prim_ids", at::kInt, {-1});
+              need(normals, fn, "normals", at::kFloat, {-1, 2});
+              need(bitset, fn, "bitset", at::kInt, {-1, -1, -1, -1});
+              need(target, fn, "target_bitset", at::kInt, {-1, -1, -1, -1});
+              need(pos, fn, "pos", at::kFloat, {-1, -1, 4});
+              need(edges, fn, "edges", at::kInt, {-1, 2});
+              need(edge2cluster, fn, "edge2cluster", at::kInt, {-1});
+              need(d_pos, fn, "d_pos", at::kFloat, {-1, -1, 4});
+              same_device(fn, {&prim_ids, &normals, &bitset, &target, &pos, &edges,
+                               &edge2cluster, &d_pos});
+              const int B = I(bitset.size(0), fn), H = I(bitset.size(1), fn);
+              const int W = I(bitset.size(2), fn), S = I(bitset.size(3), fn);
+              const int V = I(pos.size(1), fn), E = I(edges.size(0), fn);
+              if (pos.size(0) != B)
+                  throw std::runtime_error(std::string(fn) +
+                                           ": pos must hold one view per bitset view.");
+              if (prim_ids.size(0) != pos.size(0) * edges.size(0) ||
+                  normals.size(0) != prim_ids.size(0))
+                  throw std::runtime_error(std::string(fn) +
+                                           ": prim_ids and normals must have B * E rows.");
+              if (edge2cluster.size(0) != E)
+                  throw std::runtime_error(std::string(fn) +
+                                           ": edge2cluster must have one entry per edge.");
+              if (target.sizes() != bitset.sizes())
+                  throw std::runtime_error(std::string(fn) +
+                                           ": target_bitset must match bitset shape.");
+              if (d_pos.sizes() != pos.sizes())
+                  throw std::runtime_error(std::string(fn) + ": d_pos must match pos shape.");
+              DeviceGuard g(pos.device());
+              check(s2c_backward_antialiased_cluster_bitset_edges(
+                        B, H, W, S, P<int>(bitset), P<int>(target), V, P<float>(pos), E,
+                        P<int>(edges), P<int>(prim_ids), P<float>(normals), P<int>(edge2cluster),
+                        P<float>(d_pos), kernel_radius, rho, stream_of(pos)),
+                    fn);
+          });
+
     // pystrns2cards.cpp:774-786
     m.def("compact_valid_ints", [](Tensor in, Tensor out) -> int {
         const char* fn = "compact_valid_ints";

@@ -488,7 +488,10 @@ def joint_fit(points: torch.Tensor, edges: torch.Tensor, edge2cluster: torch.Ten
     term with the grid kernel (grid_laplacian) instead of a sparse matrix
     product, whose backward re-sorts the matrix every iteration, and forms
     the planarity term with grid_quad_diagonals instead of four gathers,
-    whose backward scatters sort their indices.  `quads` must then be the
+    whose backward scatters sort their indices, and takes the silhouette
+    gradient from the marked edges in one launch sequence
+    (antialiased_cluster_bitset_loss(fused=True)), so the iteration makes
+    no host synchronization.  `quads` must then be the
     grids' quads in wisp_to_grid_mesh order (as combine_wisps_with_cluster_
     labels writes them)."""
     from . import strand_io
@@ -539,7 +542,7 @@ def joint_fit(points: torch.Tensor, edges: torch.Tensor, edge2cluster: torch.Ten
         pos = from_differential(M, param)
         pos_ndc_h, bits = render(pos)
         loss = antialiased_cluster_bitset_loss(bits, target, pos_ndc_h, tri, cedges, e2t, ce2c,
-                                               kernel_radius=7.0, rho=alpha)
+                                               kernel_radius=7.0, rho=alpha, fused=fused)
         if fused:
             loss = loss + w_pln * torch.sum(grid_quad_diagonals(M, pos) ** 2)
         else:

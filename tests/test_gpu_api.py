@@ -156,6 +156,13 @@ def test_antialias_loss_api(dev, small):
                                                         c.edge2tri, c.edge2cluster, 7.0, 0.5)
     assert rel_err(pos.grad.cpu().numpy(), want) <= 1e-5
     assert loss.item() == oracle.popcount_bitset(np.bitwise_xor(bits, target)).sum()
+    pos.grad = None
+    loss = s2c.antialiased_cluster_bitset_loss(T(bits, dev), T(target, dev), pos, T(c.tri, dev),
+                                               T(c.edges, dev), T(c.edge2tri, dev),
+                                               T(c.edge2cluster, dev), 7.0, 0.5, True, fused=True)
+    (3.0 * loss).backward()
+    assert rel_err(pos.grad.cpu().numpy(), want) <= 1e-5
+    assert loss.item() == oracle.popcount_bitset(np.bitwise_xor(bits, target)).sum()
 
 
 def test_curve_api(dev):

@@ -476,6 +476,80 @@ def test_antialias_word_planes(dev, mult):
         normwise_close(d_records.cpu().numpy(), want)
 
 
+@pytest.mark.parametrize("mult,shuffle,persp", [(1, False, False), (1, True, False),
+                                                (1, False, True), (17, False, False),
+                                                (17, True, True)])
+def test_antialias_marked_edges(dev, mult, shuffle, persp):
+    """The gradient from the uncompacted marked edges (DDA walk inside the
+    backward kernel, no fragment list) against the oracle's mark -> DDA ->
+    backward composition: two views of the small workload, S = 2 (records)
+    and 34 (word planes), card edges in file order (staged windows) and
+    shuffled (waves mixing clusters), orthographic and perspective views
+    (w in [0.5, 2], so the DDA divides and the depth test is live)."""
+    report(f"antialias backward from marked edges mult={mult} shuffle={shuffle} persp={persp}")
+    w = synth.small_workload()
+    c = w.cards
+    res = (w.H, w.W)
+    shift = np.array([0.05, -0.03, 0.0, 0.0], np.float32)
+    pos2 = np.concatenate([w.pos, w.pos + shift * w.pos[..., 3:4]], 0)
+    cpos2 = np.concatenate([w.card_pos, w.card_pos + shift * w.card_pos[..., 3:4]], 0)
+    if persp:
+        ww = np.random.default_rng(8).uniform(0.5, 2.0, cpos2.shape[:2] + (1,)).astype(np.float32)
+        cpos2 = (cpos2 * ww).astype(np.float32)
+    edges, e2t, e2c = c.edges, c.edge2tri, c.edge2cluster
+    if shuffle:
+        perm = np.random.default_rng(4).permutation(edges.shape[0])
+        edges, e2t, e2c = edges[perm].copy(), e2t[perm].copy(), e2c[perm].copy()
+    K = 63 * mult + 2
+    S = (K + 31) // 32
+    target = oracle.dda_cluster_mask(res, pos2, w.edges, (w.edge2cluster * mult + 1).astype(np.int32), K)
+    cards = oracle.triangle_cluster_mask(res, cpos2, c.tri, (c.tri2cluster * mult + 1).astype(np.int32), K)
+    e2c = (e2c * mult + 1).astype(np.int32)
+    B, V, _ = cpos2.shape
+    E = edges.shape[0]
+    tp, te = T(cpos2, dev), T(edges, dev)
+    marked = torch.full((B * E,), -1, dtype=torch.int32, device=dev)
+    normals = torch.zeros((B * E, 2), device=dev)
+    call("s2c_mark_discontinuity_edges", res[0], res[1], B, V, tp, T(c.tri, dev), E, te,
+         T(e2t, dev), marked, normals)
+    prim, _ = oracle.mark_discontinuity_edges(res, cpos2, c.tri, edges, e2t)
+    fp, fa = oracle.dda_compute_fragments(res, cpos2, edges, prim)
+    for rho in (1.0, 0.5):
+        want = oracle.antialiased_cluster_bitset_grad(cards, target, cpos2, c.tri, edges, e2t,
+                                                      e2c, 7.0, rho)
+        assert np.abs(want[1]).max() > 0
+        d_edges = torch.zeros((B, V, 4), device=dev)
+        call("s2c_backward_antialiased_cluster_bitset_edges", B, res[0], res[1], S, T(cards, dev),
+             T(target, dev), V, tp, E, te, marked, normals, T(e2c, dev), d_edges, 7.0, rho)
+        normwise_close(d_edges.cpu().numpy(), want)
+        d_list = torch.zeros((B, V, 4), device=dev)
+        call("s2c_backward_antialiased_cluster_bitset_views", B, fp.shape[0], T(fp, dev),
+             T(fa, dev), res[0], res[1], S, T(cards, dev), T(target, dev), V, tp, E, te, normals,
+             T(e2c, dev), d_list, 7.0, rho)
+        normwise_close(d_edges.cpu().numpy(), d_list.cpu().numpy(), 1e-6)
+
+
+def test_antialias_marked_edges_empty(dev):
+    """No marked edge, no edges, no views: d_pos is left untouched."""
+    report("antialias backward from marked edges: empty inputs")
+    H, W, S, V, E = 16, 16, 2, 4, 3
+    bits = torch.zeros((1, H, W, S), dtype=torch.int32, device=dev)
+    pos = torch.zeros((1, V, 4), device=dev)
+    edges = torch.tensor([[0, 1], [1, 2], [2, 3]], dtype=torch.int32, device=dev)
+    marked = torch.full((E,), -1, dtype=torch.int32, device=dev)
+    normals = torch.zeros((E, 2), device=dev)
+    e2c = torch.zeros(E, dtype=torch.int32, device=dev)
+    d = torch.full((1, V, 4), 3.0, device=dev)
+    call("s2c_backward_antialiased_cluster_bitset_edges", 1, H, W, S, bits, bits, V, pos, E, edges,
+         marked, normals, e2c, d, 7.0, 1.0)
+    call("s2c_backward_antialiased_cluster_bitset_edges", 1, H, W, S, bits, bits, V, pos, 0, edges,
+         marked, normals, e2c, d, 7.0, 1.0)
+    call("s2c_backward_antialiased_cluster_bitset_edges", 0, H, W, S, bits, bits, V, pos, E, edges,
+         marked, normals, e2c, d, 7.0, 1.0)
+    torch.cuda.synchronize()
+    assert (d == 3.0).all()
+
+
 # ---------------------------------------------------------------- curves
 
 
