@@ -503,19 +503,27 @@ def texture_section(dev, reps=5):
     return out
 
 
-def joint_fit_section(dev, iters=50):
+def joint_fit_section(dev, iters=50, fused_only=False):
     """3_fitting.py's loop at its defaults (13 views at 200 x 200, 300
     clusters, about 15,000 card quads in 600 crossed 2-row strips, large
     steps lam 100 / w 1/100, lr 0.1) on synthetic strands (10,000 x 64
     points): ms per iteration through the drop-in API as the script calls
-    it, and with `fused=True` (triangle_cluster_mask, the grid kernels and the
-    antialias gradient from the marked edges, no host sync), and the XOR count over all views before and after `iters`
-    iterations."""
+    it, and with `fused=True` (triangle_cluster_mask, the grid kernels and
+    the antialias gradient from the marked edges, no host sync), and the XOR
+    count over all views before and after `iters` iterations.  `fused_only`
+    skips the script-shaped loop (for kernel traces of the fused one)."""
     from strns2cards_amd import fitting
 
     t0 = time.perf_counter()
     inp = fitting.joint_fit_inputs(device=dev)
     prep = time.perf_counter() - t0
+    if fused_only:
+        fitting.joint_fit(**inp, num_iters=3, fused=True)
+        rep_f = fitting.joint_fit(**inp, num_iters=iters, fused=True)
+        out = {"iters": iters, "fused_ms_per_iter": round(rep_f.ms_per_iter, 3),
+               "xor_first": rep_f.xor_first, "fused_xor_last": rep_f.xor_last}
+        print(f"[bench] joint fit (fused only): {out}", file=sys.stderr, flush=True)
+        return out
     fitting.joint_fit(**inp, num_iters=3)  # warm-up
     rep = fitting.joint_fit(**inp, num_iters=iters)
     fitting.joint_fit(**inp, num_iters=3, fused=True)
@@ -560,6 +568,8 @@ def main():
                     help="skip the large-steps, card-mesh and texture sections")
     ap.add_argument("--largesteps-only", action="store_true",
                     help="only the large-steps, card-mesh, texture and joint-fit sections")
+    ap.add_argument("--joint-fit-only", action="store_true",
+                    help="only the fused joint-fit loop (for kernel traces)")
     ap.add_argument("--dda-only", action="store_true",
                     help="time the DDA->mask step only (for PMC runs)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -580,6 +590,9 @@ def main():
     from strns2cards_amd import _core, synth
 
     dev = torch.device("cuda", 0)
+    if args.joint_fit_only:
+        print(json.dumps({"joint_fit": joint_fit_section(dev, fused_only=True)}), flush=True)
+        return
     if args.largesteps_only:
         print(json.dumps({"largesteps": largesteps_section(dev),
                           "card_mesh": card_mesh_section(dev),

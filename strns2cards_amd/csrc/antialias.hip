@@ -154,11 +154,22 @@ __device__ __forceinline__ void wave_add_runs(float* d_pos, bool live, long long
 constexpr int kAaWaves = kBlock / kWave;
 constexpr int kAaBox = 8192;
 
-// One sample per lane, every lane of the wave calling (the body contains
-// __syncthreads(): every thread of the workgroup must call it as often).
-// `has`: the lane holds a fragment (b, h, w) of edge e with the DDA axis and
-// t1 of frag_attrs_dda (dda.cu:218-224).
-template <bool PLANES>
+// Orders a wave's own LDS accesses: the LDS executes one wave's operations
+// in issue order, so a buffer private to the wave needs only a compiler
+// fence at wavefront scope between its writes and reads.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One sample per lane, every lane of the wave calling.  `box` is the wave's
+// own LDS box; with WAVE_SYNC false the body synchronizes with
+// __syncthreads() (every thread of the workgroup must then call it as
+// often), with WAVE_SYNC true only the wave.  `has`: the lane holds a
+// fragment (b, h, w) of edge e with the DDA axis and t1 of frag_attrs_dda
+// (dda.cu:218-224).
+template <bool PLANES, bool WAVE_SYNC>
 __device__ __forceinline__ void aa_wave_fragment(
     bool has, int b, int h, int w, float axis, float t1, int e, int H, int W, int S,
     const int* __restrict__ bitset, const int* __restrict__ target, int V,
@@ -223,7 +234,8 @@ __device__ __forceinline__ void aa_wave_fragment(
             }
         }
     }
-    __syncthreads();
+    if (WAVE_SYNC) wave_lds_sync();
+    else __syncthreads();
 
     // the sample's contributions to its two vertices (antialias.cu:272-277)
     bool adds = false;
@@ -239,7 +251,9 @@ __device__ __forceinline__ void aa_wave_fragment(
     const long long vb = static_cast<long long>(b) * V;
     wave_add_runs(d_pos, adds, vb + v0, dpx * smp.t0, dpy * smp.t0, dpw * smp.t0);
     wave_add_runs(d_pos, adds, vb + v1, dpx * smp.t1, dpy * smp.t1, dpw * smp.t1);
-    __syncthreads();  // the box is reused by the wave's next sample
+    // the box is reused by the wave's next sample
+    if (WAVE_SYNC) wave_lds_sync();
+    else __syncthreads();
 }
 
 template <bool PLANES>
@@ -264,22 +278,26 @@ antialias_bwd_kernel(int N, const int* __restrict__ frag_pix, const float* __res
         axis = a.x;
         t1 = a.y;
     }
-    aa_wave_fragment<PLANES>(i < N, b, h, w, axis, t1, e, H, W, S, bitset, target, V, pos, E,
-                             edges, normals, edge2cluster, d_pos, R, rho, B, planes, box[wv]);
+    aa_wave_fragment<PLANES, false>(i < N, b, h, w, axis, t1, e, H, W, S, bitset, target, V, pos,
+                                    E, edges, normals, edge2cluster, d_pos, R, rho, B, planes,
+                                    box[wv]);
 }
 
-// The same samples without a fragment list: the silhouette edges marked in
-// prim_ids (mark_edges_kernel: b*E + e, or -1) are DDA-walked in the
-// workgroup that holds them (a block scan spreads their candidates over the
-// workgroup, in (edge, candidate) order, as the stable fragment list orders
-// them), and every candidate that dda_compute_fragments would keep
-// (dda.cu:185-216) becomes the lane's sample.  No count leaves the device.
-struct AaEdgeItem {
-    DdaSpan sp;
-    int b, e;
-};
+// The same samples without a fragment list.  Each wave owns 64*M
+// consecutive (view, edge) items of the marked-edge array (mark_edges_kernel:
+// b*E + e, or -1), M consecutive ones per lane, and DDA-walks the marked ones
+// itself: a wave scan of their candidate counts (dda_span, dda.cu:9-87)
+// spreads the candidates over the lanes in (edge, candidate) order, as the
+// stable fragment list orders them, and every candidate that
+// dda_compute_fragments would keep (dda.cu:185-216) becomes a lane's sample.
+// A lane finds its item by a binary search over the wave's exclusive prefix
+// and a scan of the owning lane's M partial sums (shuffles), and recomputes
+// that item's span.  No count leaves the device and no wave waits for
+// another.  M > 1 fills the wave's last pass better where few edges are
+// marked or their spans are short.
+constexpr int kAaEdgeItems = 2;
 
-template <bool PLANES>
+template <bool PLANES, int M>
 __global__ void __launch_bounds__(kBlock)
 antialias_edges_bwd_kernel(long long BE, int H, int W, int S, const int* __restrict__ bitset,
                            const int* __restrict__ target, int V, const float* __restrict__ pos,
@@ -288,56 +306,78 @@ antialias_edges_bwd_kernel(long long BE, int H, int W, int S, const int* __restr
                            const int* __restrict__ edge2cluster, float* __restrict__ d_pos,
                            float R, float rho, int B, const int2* __restrict__ planes) {
     __shared__ unsigned char box[kAaWaves][kAaBox];
-    __shared__ AaEdgeItem item[kBlock];
-    __shared__ int excl[kBlock];
-    __shared__ int sh[kBlock / kWave + 1];
     const int wv = threadIdx.x / kWave;
-    const long long q = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
-    int cnt = 0;
-    if (q < BE && prim_ids[q] >= 0) {
-        const int b = static_cast<int>(q / E);
-        const int e = static_cast<int>(q - static_cast<long long>(b) * E);
-        const float4* p = reinterpret_cast<const float4*>(pos + static_cast<long long>(b) * V * 4);
-        AaEdgeItem it;
-        bool wrote;
-        cnt = dda_span(H, W, p[edges[e * 2 + 0]], p[edges[e * 2 + 1]], it.sp, wrote);
-        it.b = b;
-        it.e = e;
-        item[threadIdx.x] = it;
+    const int lane = lane_id();
+    const long long q0 = (static_cast<long long>(blockIdx.x) * kAaWaves + wv) * kWave * M;
+    int sub[M];  // sub[m]: candidates of the lane's items before item m
+    int lsum = 0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        sub[m] = lsum;
+        const long long q = q0 + static_cast<long long>(lane) * M + m;
+        if (q < BE && prim_ids[q] >= 0) {
+            const int b = static_cast<int>(q / E);
+            const int e = static_cast<int>(q - static_cast<long long>(b) * E);
+            const float4* p =
+                reinterpret_cast<const float4*>(pos + static_cast<long long>(b) * V * 4);
+            DdaSpan sp;
+            bool wrote;
+            lsum += dda_span(H, W, p[edges[e * 2 + 0]], p[edges[e * 2 + 1]], sp, wrote);
+        }
     }
-    int total;
-    excl[threadIdx.x] = block_excl_scan(cnt, sh, &total);
-    __syncthreads();
+    const int incl = wave_incl_scan(lsum);
+    const int excl = incl - lsum;
+    const int total = __shfl(incl, kWave - 1, kWave);
 #pragma unroll 1
-    for (int base = 0; base < total; base += kBlock) {  // uniform trip count
-        const int j = base + static_cast<int>(threadIdx.x);
+    for (int base = 0; base < total; base += kWave) {  // wave-uniform trip count
+        const int j = base + lane;
+        // last lane with excl <= j; excl is non-decreasing and excl[0] = 0
+        int lo = 0;
+#pragma unroll
+        for (int step = kWave / 2; step > 0; step >>= 1) {
+            const int v = __shfl(excl, lo + step, kWave);
+            if (v <= j) lo += step;
+        }
+        int r = j - __shfl(excl, lo, kWave);
+        // last item of lane lo with sub <= r (it has candidates left)
+        int m_sel = 0, off = 0;
+#pragma unroll
+        for (int m = 1; m < M; ++m) {
+            const int v = __shfl(sub[m], lo, kWave);
+            if (v <= r) {
+                m_sel = m;
+                off = v;
+            }
+        }
+        const int k = r - off;
         bool has = false;
         int b = 0, h = 0, w = 0, e = -1;
         float axis = 0.f, t1 = 0.f;
         if (j < total) {
-            int lo = 0, hi = kBlock;  // last item with excl <= j
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (excl[mid] <= j) lo = mid;
-                else hi = mid;
-            }
-            const AaEdgeItem it = item[lo];
+            // the item has more than k >= 0 candidates: a marked edge in [0, BE)
+            const long long qi = q0 + static_cast<long long>(lo) * M + m_sel;
+            const int bi = static_cast<int>(qi / E);
+            const int ei = static_cast<int>(qi - static_cast<long long>(bi) * E);
             const float4* p =
-                reinterpret_cast<const float4*>(pos + static_cast<long long>(it.b) * V * 4);
+                reinterpret_cast<const float4*>(pos + static_cast<long long>(bi) * V * 4);
+            const float4 p0 = p[edges[ei * 2 + 0]], p1 = p[edges[ei * 2 + 1]];
+            DdaSpan sp;
+            bool wrote;
+            dda_span(H, W, p0, p1, sp, wrote);
             DdaFrag f;
-            if (dda_frag(H, W, it.sp, j - excl[lo], p[edges[it.e * 2 + 0]],
-                         p[edges[it.e * 2 + 1]], f)) {
+            if (dda_frag(H, W, sp, k, p0, p1, f)) {
                 has = true;
-                b = it.b;
+                b = bi;
                 h = f.h;
                 w = f.w;
-                e = it.e;
-                axis = it.sp.axis;
+                e = ei;
+                axis = sp.axis;
                 t1 = f.t1;
             }
         }
-        aa_wave_fragment<PLANES>(has, b, h, w, axis, t1, e, H, W, S, bitset, target, V, pos, E,
-                                 edges, normals, edge2cluster, d_pos, R, rho, B, planes, box[wv]);
+        aa_wave_fragment<PLANES, true>(has, b, h, w, axis, t1, e, H, W, S, bitset, target, V, pos,
+                                       E, edges, normals, edge2cluster, d_pos, R, rho, B, planes,
+                                       box[wv]);
     }
 }
 
@@ -383,7 +423,8 @@ extern "C" int s2c_backward_antialiased_cluster_bitset_edges(
     if (BE == 0 || np == 0) return S2C_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (num_slots < kPlaneMinSlots) {
-        antialias_edges_bwd_kernel<false><<<grid_for(BE, kBlock), kBlock, 0, s>>>(
+        antialias_edges_bwd_kernel<false, kAaEdgeItems>
+            <<<grid_for(BE, kBlock * kAaEdgeItems), kBlock, 0, s>>>(
             BE, H, W, num_slots, bitset, target_bitset, V, pos, E, edges, prim_ids, normals,
             edge2cluster, d_pos, kernel_radius, rho, num_views, nullptr);
         S2C_LAUNCH_CHECK();
@@ -394,7 +435,8 @@ extern "C" int s2c_backward_antialiased_cluster_bitset_edges(
     residual_planes_kernel<<<grid_for(np, kPlanePix), kBlock, 0, s>>>(
         np, static_cast<long long>(H) * W, num_slots, bitset, target_bitset, planes.as<int2>());
     S2C_LAUNCH_CHECK();
-    antialias_edges_bwd_kernel<true><<<grid_for(BE, kBlock), kBlock, 0, s>>>(
+    antialias_edges_bwd_kernel<true, kAaEdgeItems>
+        <<<grid_for(BE, kBlock * kAaEdgeItems), kBlock, 0, s>>>(
         BE, H, W, num_slots, bitset, target_bitset, V, pos, E, edges, prim_ids, normals,
         edge2cluster, d_pos, kernel_radius, rho, num_views, planes.as<int2>());
     S2C_LAUNCH_CHECK();
