@@ -164,12 +164,10 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // One sample per lane, every lane of the wave calling.  `box` is the wave's
-// own LDS box; with WAVE_SYNC false the body synchronizes with
-// __syncthreads() (every thread of the workgroup must then call it as
-// often), with WAVE_SYNC true only the wave.  `has`: the lane holds a
-// fragment (b, h, w) of edge e with the DDA axis and t1 of frag_attrs_dda
-// (dda.cu:218-224).
-template <bool PLANES, bool WAVE_SYNC>
+// own LDS box, so only the wave synchronizes (waves of a workgroup never
+// wait for each other).  `has`: the lane holds a fragment (b, h, w) of edge
+// e with the DDA axis and t1 of frag_attrs_dda (dda.cu:218-224).
+template <bool PLANES>
 __device__ __forceinline__ void aa_wave_fragment(
     bool has, int b, int h, int w, float axis, float t1, int e, int H, int W, int S,
     const int* __restrict__ bitset, const int* __restrict__ target, int V,
@@ -234,8 +232,7 @@ __device__ __forceinline__ void aa_wave_fragment(
             }
         }
     }
-    if (WAVE_SYNC) wave_lds_sync();
-    else __syncthreads();
+    wave_lds_sync();
 
     // the sample's contributions to its two vertices (antialias.cu:272-277)
     bool adds = false;
@@ -251,9 +248,7 @@ __device__ __forceinline__ void aa_wave_fragment(
     const long long vb = static_cast<long long>(b) * V;
     wave_add_runs(d_pos, adds, vb + v0, dpx * smp.t0, dpy * smp.t0, dpw * smp.t0);
     wave_add_runs(d_pos, adds, vb + v1, dpx * smp.t1, dpy * smp.t1, dpw * smp.t1);
-    // the box is reused by the wave's next sample
-    if (WAVE_SYNC) wave_lds_sync();
-    else __syncthreads();
+    wave_lds_sync();  // the box is reused by the wave's next sample
 }
 
 template <bool PLANES>
@@ -278,7 +273,7 @@ antialias_bwd_kernel(int N, const int* __restrict__ frag_pix, const float* __res
         axis = a.x;
         t1 = a.y;
     }
-    aa_wave_fragment<PLANES, false>(i < N, b, h, w, axis, t1, e, H, W, S, bitset, target, V, pos,
+    aa_wave_fragment<PLANES>(i < N, b, h, w, axis, t1, e, H, W, S, bitset, target, V, pos,
                                     E, edges, normals, edge2cluster, d_pos, R, rho, B, planes,
                                     box[wv]);
 }
@@ -375,7 +370,7 @@ antialias_edges_bwd_kernel(long long BE, int H, int W, int S, const int* __restr
                 t1 = f.t1;
             }
         }
-        aa_wave_fragment<PLANES, true>(has, b, h, w, axis, t1, e, H, W, S, bitset, target, V, pos,
+        aa_wave_fragment<PLANES>(has, b, h, w, axis, t1, e, H, W, S, bitset, target, V, pos,
                                        E, edges, normals, edge2cluster, d_pos, R, rho, B, planes,
                                        box[wv]);
     }
