@@ -345,6 +345,18 @@ int s2c_grid_quad_diagonals(int num_grids, const int* rows, const int* cols, con
 int s2c_grid_quad_diagonals_backward(int num_grids, const int* rows, const int* cols,
                                      const int* voff, const int* qoff, const float* dD,
                                      float* dpos, void* stream);
+
+/* optimizer.py:33-68 (UniformAdam.step) for one parameter of n points x dim
+ * channels, as a device step with no host synchronization: g1 (n, dim) and
+ * g2 (n) are its state, updated in place; `step` is the step count after the
+ * increment (>= 1); lr, b1, b2 are the optimizer group's values as the
+ * reference's Python floats, rounded to float32 once (b1 ** step included).
+ * param <- param - lr * (g1 / (1 - b1^step)) / (max_i sqrt(g2_i /
+ * (1 - b2^step)) + 1e-8); a NaN in any g2 makes every update NaN, as
+ * torch's max() does. */
+int s2c_uniform_adam_step(int n, int dim, double lr, double b1, double b2,
+                          int step, float* param, const float* grad,
+                          float* g1, float* g2, void* stream);
 int s2c_grid_system_factor(int num_grids, const int* rows, const int* cols, const int* fac_off,
                            float lam, float w_weight, float* factor, void* stream);
 int s2c_grid_system_solve(int num_grids, const int* rows, const int* cols, const int* voff,

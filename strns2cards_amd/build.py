@@ -27,7 +27,7 @@ ARCH = "gfx950"
 
 HIP_SOURCES = ["scan.hip", "binning.hip", "dda.hip", "dda_mask.hip", "raster.hip", "bitset.hip",
                "antialias.hip", "curve.hip", "fit.hip", "coverage.hip", "gridsolve.hip",
-               "cardmesh.hip", "texture.hip"]
+               "cardmesh.hip", "texture.hip", "adam.hip"]
 
 # Contraction off and correctly rounded divide/sqrt are the numerics contract
 # with the CPU oracle; packed fp32 is disabled for the device code.

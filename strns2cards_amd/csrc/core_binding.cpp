@@ -475,6 +475,23 @@ PYBIND11_MODULE(_core, m) {
                     fn);
           });
 
+    // optimizer.py:33-68 (UniformAdam.step) for a (n, dim) parameter.
+    m.def("uniform_adam_step", [](Tensor param, Tensor grad, Tensor g1, Tensor g2, double lr,
+                                  double b1, double b2, int step) {
+        const char* fn = "uniform_adam_step";
+        need(param, fn, "param", at::kFloat, {-1, -1});
+        const int64_t n = param.size(0), dim = param.size(1);
+        need(grad, fn, "grad", at::kFloat, {n, dim});
+        need(g1, fn, "g1", at::kFloat, {n, dim});
+        need(g2, fn, "g2", at::kFloat, {n});
+        same_device(fn, {&param, &grad, &g1, &g2});
+        DeviceGuard g(param.device());
+        check(s2c_uniform_adam_step(I(n, fn), I(dim, fn), lr, b1, b2, step, P<float>(param),
+                                    P<float>(grad), P<float>(g1), P<float>(g2),
+                                    stream_of(param)),
+              fn);
+    });
+
     // pystrns2cards.cpp:774-786
     m.def("compact_valid_ints", [](Tensor in, Tensor out) -> int {
         const char* fn = "compact_valid_ints";

@@ -491,7 +491,8 @@ def joint_fit(points: torch.Tensor, edges: torch.Tensor, edge2cluster: torch.Ten
     whose backward scatters sort their indices, and takes the silhouette
     gradient from the marked edges in one launch sequence
     (antialiased_cluster_bitset_loss(fused=True)), so the iteration makes
-    no host synchronization.  `quads` must then be the
+    no host synchronization, and takes the UniformAdam step in two launches
+    (UniformAdam(fused=True)).  `quads` must then be the
     grids' quads in wisp_to_grid_mesh order (as combine_wisps_with_cluster_
     labels writes them)."""
     from . import strand_io
@@ -515,7 +516,7 @@ def joint_fit(points: torch.Tensor, edges: torch.Tensor, edge2cluster: torch.Ten
     M = GridSystem.from_edges(V, h_edges, w_edges, lam, w_weight, device=dev)
     L_uniform = None if fused else laplacian_edges(card_pos, torch.cat([h_edges, w_edges], 0))
     param = to_differential(M, card_pos.float()).detach().requires_grad_(True)
-    opt = UniformAdam([param], lr=lr, betas=(0.9, 0.999))
+    opt = UniformAdam([param], lr=lr, betas=(0.9, 0.999), fused=fused)
     sched = torch.optim.lr_scheduler.ExponentialLR(opt, gamma=1e-2 ** (1 / num_iters))
     mt = mvp.transpose(1, 2)
 

@@ -2,8 +2,9 @@
 
 Mirror of the reference module pystrns2cards/optimizer.py: ``UniformAdam``
 (optimizer.py:3-68) and ``laplacian_edges`` (optimizer.py:70-99).  These run
-as plain torch ops; the batched fitting kernel re-implements the same step
-on the GPU for many jobs at once.
+as plain torch ops, or ``UniformAdam(fused=True)`` as one HIP step; the
+batched fitting kernel re-implements the same step on the GPU for many jobs
+at once.
 
 ``GridSystem``, ``to_differential`` and ``from_differential`` stand in for
 largesteps.parameterize on the grid meshes the fitting optimizes
@@ -23,10 +24,17 @@ class UniformAdam(torch.optim.Optimizer):
         g1 <- b1 g1 + (1 - b1) g
         g2 <- b2 g2 + (1 - b2) sum_c g_c^2
         p  <- p - lr * (g1 / (1 - b1^t)) / (max sqrt(g2 / (1 - b2^t)) + 1e-8)
+
+    ``fused=True`` (no reference counterpart) takes each step on the GPU in
+    two launches (``s2c_uniform_adam_step``: moments with per-workgroup
+    maxima, then the shared denominator and the update) instead of a dozen
+    tensor operations; parameters must then be float32 contiguous GPU
+    tensors.  The state tensors are the same.
     """
 
-    def __init__(self, params, lr=1e-3, betas=(0.9, 0.999)):
+    def __init__(self, params, lr=1e-3, betas=(0.9, 0.999), fused: bool = False):
         super().__init__(params, dict(lr=lr, betas=betas))
+        self.fused = fused
 
     @torch.no_grad()
     def step(self):
@@ -44,6 +52,16 @@ class UniformAdam(torch.optim.Optimizer):
                     st["g2"] = torch.zeros_like(p[..., :1])
                 st["step"] += 1
                 t = st["step"]
+                if self.fused:
+                    if not (p.is_cuda and p.dtype == torch.float32 and p.is_contiguous()
+                            and g.is_contiguous() and p.dim() >= 1):
+                        raise ValueError("UniformAdam(fused=True) needs float32 contiguous "
+                                         "GPU parameters and gradients")
+                    dim = p.shape[-1]  # g2 holds one entry per row of the last axis
+                    _core.uniform_adam_step(p.view(-1, dim), g.view(-1, dim),
+                                            st["g1"].view(-1, dim), st["g2"].view(-1),
+                                            float(lr), float(b1), float(b2), int(t))
+                    continue
                 st["g1"].mul_(b1).add_(g, alpha=1 - b1)
                 st["g2"].mul_(b2).add_(g.square().sum(dim=-1, keepdim=True), alpha=1 - b2)
                 m1 = st["g1"] / (1 - b1 ** t)

@@ -174,3 +174,53 @@ def test_grid_quad_diagonals(dev):
     for k in range(4):
         np.add.at(bound, quads[:, k], np.abs(g.astype(np.float64)))
     assert np.all(np.abs(pt.grad.cpu().numpy() - adj) <= 1e-6 * bound + 1e-30)
+
+
+@pytest.mark.parametrize("shape", [(31200, 3), (7, 3), (5000, 2), (40,)])
+def test_uniform_adam_fused_matches_torch_ops(dev, shape):
+    """UniformAdam(fused=True) (two HIP launches) against the torch-op step
+    that restates optimizer.py:33-68 op for op, over 6 steps with a changing
+    learning rate: parameters and both moments within 2e-6 normwise (the
+    kernel rounds each product separately, torch may contract them); a 1-D
+    parameter is one row, as the reference's p[..., :1] state makes it."""
+    print(f"[uniform adam] shape={shape}", flush=True)
+    from strns2cards_amd.optimizer import UniformAdam
+    g = torch.Generator(device="cpu").manual_seed(sum(shape))
+    x0 = torch.randn(shape, generator=g)
+    pa = x0.clone().to(dev).requires_grad_(True)
+    pb = x0.clone().to(dev).requires_grad_(True)
+    oa = UniformAdam([pa], lr=0.1, betas=(0.9, 0.999))
+    ob = UniformAdam([pb], lr=0.1, betas=(0.9, 0.999), fused=True)
+    for it in range(6):
+        grad = (torch.randn(shape, generator=g) * (1 + it)).to(dev)
+        pa.grad = grad.clone()
+        pb.grad = grad.clone()
+        oa.step()
+        ob.step()
+        for o in (oa, ob):
+            o.param_groups[0]["lr"] *= 0.9
+    for a, b in ((pa, pb), (oa.state[pa]["g1"], ob.state[pb]["g1"]),
+                 (oa.state[pa]["g2"], ob.state[pb]["g2"])):
+        assert a.shape == b.shape
+        err = ((a - b).abs().max() / a.abs().max()).item()
+        assert err <= 2e-6, err
+    assert ob.state[pb]["step"] == 6
+
+
+def test_uniform_adam_fused_nan_and_errors(dev):
+    """A NaN gradient entry makes the shared denominator NaN (torch's max
+    propagates it), so every parameter turns NaN in both forms; CPU
+    parameters are refused by the fused form."""
+    from strns2cards_amd.optimizer import UniformAdam
+    for fused in (False, True):
+        p = torch.ones((10, 3), device=dev, requires_grad=True)
+        opt = UniformAdam([p], lr=0.1, fused=fused)
+        grad = torch.ones((10, 3), device=dev)
+        grad[4, 1] = float("nan")
+        p.grad = grad
+        opt.step()
+        assert torch.isnan(p).all(), fused
+    q = torch.ones((4, 3), requires_grad=True)
+    q.grad = torch.ones((4, 3))
+    with pytest.raises(ValueError):
+        UniformAdam([q], fused=True).step()
