@@ -508,8 +508,9 @@ def joint_fit_section(dev, iters=50, fused_only=False):
     clusters, about 15,000 card quads in 600 crossed 2-row strips, large
     steps lam 100 / w 1/100, lr 0.1) on synthetic strands (10,000 x 64
     points): ms per iteration through the drop-in API as the script calls
-    it, and with `fused=True` (triangle_cluster_mask, the grid kernels and
-    the antialias gradient from the marked edges, no host sync), and the XOR
+    it, and with `fused=True` (triangle_cluster_mask, the grid kernels, the
+    antialias gradient from the marked edges and the two-launch UniformAdam
+    step, no host sync), and the XOR
     count over all views before and after `iters` iterations.  `fused_only`
     skips the script-shaped loop (for kernel traces of the fused one)."""
     from strns2cards_amd import fitting
