@@ -103,3 +103,24 @@ def test_sparse_solve_agrees_with_cholesky():
     u = np.random.default_rng(2).standard_normal((V, 3))
     np.testing.assert_allclose(oracle.from_differential_sparse(V, h, w, 100.0, 0.01, u),
                                oracle.from_differential(M, u), rtol=1e-9, atol=1e-11)
+
+
+def test_fused_uniform_adam_refuses_host_tensors():
+    """UniformAdam(fused=True) checks its parameters before any library
+    call (host tensors raise, the parameter is untouched); the torch-op
+    form's first step matches its closed form."""
+    from strns2cards_amd.optimizer import UniformAdam
+    p = torch.ones((4, 3), requires_grad=True)
+    p.grad = torch.ones((4, 3))
+    opt = UniformAdam([p], lr=0.1, fused=True)
+    with pytest.raises(ValueError, match="GPU"):
+        opt.step()
+    assert torch.equal(p.detach(), torch.ones((4, 3)))
+    q = torch.ones((4, 3), requires_grad=True)
+    q.grad = torch.ones((4, 3))
+    UniformAdam([q], lr=0.1).step()  # the torch-op form runs anywhere
+    # first step, unit gradient: m1 = 1 and m2 = 3 (the isotropic moment sums
+    # the three channels), so each coordinate moves by lr / sqrt(3); a
+    # per-channel Adam would move it by lr
+    want = 1.0 - 0.1 / (np.sqrt(3.0) + 1e-8)
+    assert torch.allclose(q.detach(), torch.full((4, 3), want), atol=1e-6)
