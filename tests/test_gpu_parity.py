@@ -529,6 +529,44 @@ def test_antialias_marked_edges(dev, mult, shuffle, persp):
         normwise_close(d_edges.cpu().numpy(), d_list.cpu().numpy(), 1e-6)
 
 
+def test_antialias_marked_edges_full_workload(dev):
+    """Both gradient forms at full size: 4 views at 1024^2, 1,024 clusters
+    (S = 32, the word-plane path), 1,024 cards x 20 quads, masks rendered by
+    the library's fused DDA and triangle masks.  The oracle is pinned to the
+    three-call composition at the small sizes above; here the marked-edges
+    form must match the list form within 1e-6 normwise (summation order of
+    the float atomics only)."""
+    report("antialias backward from marked edges, full workload")
+    w = synth.full_workload()
+    c = w.cards
+    res = (w.H, w.W)
+    B, V, _ = w.card_pos.shape
+    E = c.edges.shape[0]
+    target = T(fused_dda_mask(dev, res, w.pos, w.edges, w.edge2cluster, 1024), dev)
+    tp, te, tt = T(w.card_pos, dev), T(c.edges, dev), T(c.tri, dev)
+    cards = torch.zeros_like(target)
+    call("s2c_triangle_cluster_mask", B, res[0], res[1], V, tp, c.tri.shape[0], tt,
+         T(c.tri2cluster, dev), target.shape[3], cards)
+    marked = torch.full((B * E,), -1, dtype=torch.int32, device=dev)
+    normals = torch.zeros((B * E, 2), device=dev)
+    call("s2c_mark_discontinuity_edges", res[0], res[1], B, V, tp, tt, E, te, T(c.edge2tri, dev),
+         marked, normals)
+    prim = torch.empty_like(marked)
+    m = call_count("s2c_compact_valid_ints", B * E, marked, prim)
+    _, _, _, _, fp, fa = dda_gpu(dev, res, w.card_pos, c.edges, prim[:m].cpu().numpy())
+    S = target.shape[3]
+    e2c = T(c.edge2cluster, dev)
+    d_list = torch.zeros((B, V, 4), device=dev)
+    call("s2c_backward_antialiased_cluster_bitset_views", B, fp.shape[0], T(fp, dev), T(fa, dev),
+         res[0], res[1], S, cards, target, V, tp, E, te, normals, e2c, d_list, 7.0, 0.5)
+    d_edges = torch.zeros((B, V, 4), device=dev)
+    call("s2c_backward_antialiased_cluster_bitset_edges", B, res[0], res[1], S, cards, target, V,
+         tp, E, te, marked, normals, e2c, d_edges, 7.0, 0.5)
+    want = d_list.cpu().numpy()
+    assert np.abs(want).max() > 0 and fp.shape[0] > 100000
+    normwise_close(d_edges.cpu().numpy(), want, 1e-6)
+
+
 def test_antialias_marked_edges_empty(dev):
     """No marked edge, no edges, no views: d_pos is left untouched."""
     report("antialias backward from marked edges: empty inputs")
