@@ -456,3 +456,197 @@ def test_span_prefix_is_inclusive_cumsum(n):
     counts = [oracle.dda_compute_span((64, 64), pos, edges[i:i + 1])[3] for i in range(min(n, 50))]
     np.testing.assert_array_equal(prefix[:len(counts)], np.cumsum(counts))
     assert total == (prefix[-1] if n else 0)
+
+
+# ------------------------------------------------- attribute interpolation
+# dda.cu:258-342 and rasterize.cu:367-472: image = sum of the primitive's
+# vertex attributes weighted by (t0, t1) or (b0, b1, 1 - b0 - b1), with the
+# primitive decoded as int(rast[..., 3]) - 1; the backward scatters the same
+# weights.  Hand-built rasters with asymmetric attributes fix the weight
+# order, the vertex order and the id decode; rasters from the DDA and the
+# triangle raster fix that the weights reconstruct the sample point; float64
+# adjoint identities fix the backwards against the forwards.
+
+
+def _asym_attr(V, C, seed):
+    """Attributes with no two vertices or channels alike."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(-3.0, 3.0, size=(V, C)).astype(np.float32)
+    a += np.arange(V, dtype=np.float32)[:, None] * 10.0
+    a += np.arange(C, dtype=np.float32)[None, :] * 0.125
+    return a
+
+
+def test_dda_interpolate_hand_built_raster():
+    edges = np.array([[0, 1], [3, 5], [4, 2]], np.int32)
+    attr = _asym_attr(6, 3, 10)
+    rast = np.zeros((1, 2, 3, 4), np.float32)
+    rast[0, 0, 0] = [0, 0.25, 0.1, 2.0]  # edge 1 = (3, 5), t1 = 0.25
+    rast[0, 1, 2] = [1, 0.75, 0.3, 3.0]  # edge 2 = (4, 2), t1 = 0.75
+    rast[0, 0, 1] = [0, 0.5, 0.0, 1.0]   # edge 0 = (0, 1), t1 = 0.5
+    img = oracle.dda_interpolate_attributes(rast, attr, edges).astype(np.float64)
+    a = attr.astype(np.float64)
+    np.testing.assert_allclose(img[0, 0, 0], 0.75 * a[3] + 0.25 * a[5], rtol=1e-6)
+    np.testing.assert_allclose(img[0, 1, 2], 0.25 * a[4] + 0.75 * a[2], rtol=1e-6)
+    np.testing.assert_allclose(img[0, 0, 1], 0.5 * a[0] + 0.5 * a[1], rtol=1e-6)
+    empty = np.ones((1, 2, 3), bool)
+    empty[0, 0, 0] = empty[0, 1, 2] = empty[0, 0, 1] = False
+    assert (img[empty] == 0).all()  # rast[..., 3] == 0: nothing written
+
+
+def test_dda_interpolate_reconstructs_sample_point():
+    """Interpolating the screen coordinates of the vertices over a DDA raster
+    gives the fragment's sample point: on the pixel centre's major-axis
+    coordinate for samples strictly inside the segment, on the segment, and
+    inside the pixel."""
+    rng = np.random.default_rng(11)
+    H, W = 40, 48
+    P = rng.uniform(1.0, 47.0, size=(12, 2))
+    P[:, 1] = np.clip(P[:, 1], 1.0, 39.0)
+    pos = clip(P, H, W, z=rng.uniform(-0.5, 0.5, size=12))
+    edges = np.array([[2 * i, 2 * i + 1] for i in range(6)], np.int32)
+    fp, fa = oracle.dda_compute_fragments((H, W), pos, edges)
+    rast = oracle.depth_test(1, (H, W), fp, fa)
+    img = oracle.dda_interpolate_attributes(rast, P.astype(np.float32), edges).astype(np.float64)
+    hit = np.argwhere(rast[0, :, :, 3] > 0)
+    assert hit.shape[0] > 60
+    interior = 0
+    for y, x in hit:
+        e = int(rast[0, y, x, 3]) - 1
+        axis = int(rast[0, y, x, 0])
+        q = img[0, y, x]
+        p0, p1 = P[edges[e, 0]], P[edges[e, 1]]
+        # on the segment (collinear, between the endpoints)
+        d = p1 - p0
+        cross = d[0] * (q[1] - p0[1]) - d[1] * (q[0] - p0[0])
+        assert abs(cross) <= 1e-3 * np.hypot(*d)
+        s = np.dot(q - p0, d) / np.dot(d, d)
+        assert -1e-5 <= s <= 1 + 1e-5
+        # inside the fragment's pixel (closed square)
+        assert x - 1e-3 <= q[0] <= x + 1 + 1e-3 and y - 1e-3 <= q[1] <= y + 1 + 1e-3
+        c = (x + 0.5) if axis == 0 else (y + 0.5)
+        lo, hi = sorted((p0[axis], p1[axis]))
+        if lo < c < hi:
+            interior += 1
+            assert abs(q[axis] - c) < 1e-4
+    assert interior > 40
+
+
+def test_dda_interpolate_backward_hand_built_and_adjoint():
+    edges = np.array([[0, 1], [3, 5], [4, 2]], np.int32)
+    rast = np.zeros((1, 2, 2, 4), np.float32)
+    rast[0, 0, 0] = [0, 0.25, 0.0, 2.0]
+    rast[0, 1, 1] = [0, 0.5, 0.0, 2.0]  # the same edge again: contributions add
+    g = np.zeros((1, 2, 2, 2), np.float32)
+    g[0, 0, 0] = [1.0, -2.0]
+    g[0, 1, 1] = [4.0, 8.0]
+    d = oracle.backward_dda_interpolate_attributes(rast, g, edges, 6).astype(np.float64)
+    want = np.zeros((6, 2))
+    want[3] = 0.75 * np.array([1.0, -2.0]) + 0.5 * np.array([4.0, 8.0])
+    want[5] = 0.25 * np.array([1.0, -2.0]) + 0.5 * np.array([4.0, 8.0])
+    np.testing.assert_allclose(d, want, rtol=1e-6, atol=0)
+    # <interp(a), g> == <a, interp_bwd(g)> on a raster from the DDA
+    rng = np.random.default_rng(12)
+    H, W = 32, 32
+    P = rng.uniform(1.0, 31.0, size=(16, 2))
+    pos = clip(P, H, W)
+    edges = np.array([[i, i + 1] for i in range(15)], np.int32)
+    fp, fa = oracle.dda_compute_fragments((H, W), pos, edges)
+    rast = oracle.depth_test(1, (H, W), fp, fa)
+    a = rng.normal(size=(16, 3)).astype(np.float32)
+    g = rng.normal(size=(1, H, W, 3)).astype(np.float32)
+    lhs = (oracle.dda_interpolate_attributes(rast, a, edges).astype(np.float64) * g).sum()
+    rhs = (a.astype(np.float64) *
+           oracle.backward_dda_interpolate_attributes(rast, g, edges, 16)).sum()
+    scale = np.abs(g).sum() * np.abs(a).max()
+    assert abs(lhs - rhs) <= 1e-6 * scale
+
+
+def test_triangle_interpolate_hand_built_raster():
+    tri = np.array([[0, 1, 2], [5, 3, 4]], np.int32)
+    attr = _asym_attr(6, 4, 13)
+    rast = np.zeros((2, 2, 2, 4), np.float32)
+    rast[0, 1, 0] = [0.5, 0.25, 0.3, 2.0]    # triangle 1 = (5, 3, 4)
+    rast[1, 0, 1] = [0.125, 0.625, 0.1, 1.0]  # triangle 0 = (0, 1, 2)
+    img = oracle.interpolate_triangle_attributes(rast, attr, tri).astype(np.float64)
+    a = attr.astype(np.float64)
+    np.testing.assert_allclose(img[0, 1, 0], 0.5 * a[5] + 0.25 * a[3] + 0.25 * a[4], rtol=1e-6)
+    np.testing.assert_allclose(img[1, 0, 1], 0.125 * a[0] + 0.625 * a[1] + 0.25 * a[2], rtol=1e-6)
+    live = np.zeros((2, 2, 2), bool)
+    live[0, 1, 0] = live[1, 0, 1] = True
+    assert (img[~live] == 0).all()
+
+
+def test_triangle_interpolate_reprojects_pixel_centre():
+    """Interpolating the clip-space vertices over a perspective raster and
+    dividing by the interpolated w gives the NDC of each pixel centre."""
+    rng = np.random.default_rng(14)
+    H, W = 24, 30
+    P = np.zeros((1, 6, 4), np.float32)
+    w = rng.uniform(0.6, 2.0, 6)
+    ndc = np.array([[-0.8, -0.7], [0.9, -0.5], [0.1, 0.9], [-0.9, 0.8], [0.7, 0.85], [0.0, -0.9]])
+    P[0, :, 0] = ndc[:, 0] * w
+    P[0, :, 1] = ndc[:, 1] * w
+    P[0, :, 2] = np.array([0.1, 0.1, 0.1, 0.4, 0.4, 0.4]) * w
+    P[0, :, 3] = w
+    tri = np.array([[0, 1, 2], [3, 5, 4]], np.int32)
+    rast = oracle.rasterize((H, W), P, tri)
+    img = oracle.interpolate_triangle_attributes(rast, P[0], tri).astype(np.float64)
+    hit = np.argwhere(rast[0, :, :, 3] > 0)
+    assert hit.shape[0] > 100
+    for y, x in hit:
+        q = img[0, y, x]
+        assert abs(q[0] / q[3] - (-1 + 2 * (x + 0.5) / W)) < 2e-5
+        assert abs(q[1] / q[3] - (1 - 2 * (y + 0.5) / H)) < 2e-5
+        t = int(rast[0, y, x, 3]) - 1
+        assert abs(q[2] / q[3] - (0.1 if t == 0 else 0.4)) < 2e-5
+
+
+def test_triangle_interpolate_backward_hand_built_and_adjoint():
+    tri = np.array([[0, 1, 2], [5, 3, 4]], np.int32)
+    rast = np.zeros((1, 1, 2, 4), np.float32)
+    rast[0, 0, 0] = [0.5, 0.25, 0.0, 2.0]
+    rast[0, 0, 1] = [0.125, 0.375, 0.0, 2.0]
+    g = np.zeros((1, 1, 2, 1), np.float32)
+    g[0, 0, 0] = 2.0
+    g[0, 0, 1] = -8.0
+    d = oracle.backward_interpolate_triangle_attributes(rast, g, tri, 6)[:, 0].astype(np.float64)
+    want = np.zeros(6)
+    want[5] = 0.5 * 2.0 + 0.125 * -8.0
+    want[3] = 0.25 * 2.0 + 0.375 * -8.0
+    want[4] = 0.25 * 2.0 + 0.5 * -8.0
+    np.testing.assert_allclose(d, want, rtol=1e-6, atol=0)
+    rng = np.random.default_rng(15)
+    H, W = 28, 28
+    V = 30
+    P = clip(rng.uniform(0.0, 28.0, size=(V, 2)), H, W, z=rng.uniform(-0.5, 0.5, size=V))
+    tri = rng.integers(0, V, size=(20, 3)).astype(np.int32)
+    tri = tri[(tri[:, 0] != tri[:, 1]) & (tri[:, 1] != tri[:, 2]) & (tri[:, 0] != tri[:, 2])]
+    rast = oracle.rasterize((H, W), P, tri)
+    assert (rast[..., 3] > 0).sum() > 50
+    a = rng.normal(size=(V, 2)).astype(np.float32)
+    g = rng.normal(size=(1, H, W, 2)).astype(np.float32)
+    lhs = (oracle.interpolate_triangle_attributes(rast, a, tri).astype(np.float64) * g).sum()
+    rhs = (a.astype(np.float64) *
+           oracle.backward_interpolate_triangle_attributes(rast, g, tri, V)).sum()
+    scale = np.abs(g).sum() * np.abs(a).max()
+    assert abs(lhs - rhs) <= 1e-6 * scale
+
+
+# --------------------------------------------------------------- arclength
+
+
+def test_arclength_closed_form():
+    """curve_utils.py:167-180: 0 then the running sum of segment lengths;
+    integer Pythagorean segments (one of zero length) are exact in fp32."""
+    s = np.array([[[0, 0, 0], [3, 4, 0], [3, 4, 0], [3, 4, 12], [0, 0, 0]],
+                  [[1, 1, 1], [1, 1, 2], [1, 3, 2], [1, 3, 2], [1, 3, 2]]], np.float32)
+    got = oracle.compute_arclength(s)
+    np.testing.assert_array_equal(got, np.array([[0, 5, 5, 17, 30], [0, 1, 3, 3, 3]], np.float32))
+    rng = np.random.default_rng(16)
+    x = rng.normal(size=(7, 33, 3)).astype(np.float32)
+    want = np.concatenate(
+        [np.zeros((7, 1)),
+         np.cumsum(np.linalg.norm(np.diff(x.astype(np.float64), axis=1), axis=2), axis=1)], 1)
+    np.testing.assert_allclose(got := oracle.compute_arclength(x), want, rtol=2e-6, atol=1e-6)
+    assert got.dtype == np.float32 and (np.diff(got, axis=1) >= 0).all()
