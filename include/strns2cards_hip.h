@@ -58,6 +58,17 @@ int s2c_compute_fragments(int H, int W, int V, const float* pos, int T,
 int s2c_depth_test(int B, int H, int W, int num_fragments, const int* frag_pix,
                    const float* frag_attrs, float* rast_out, void* stream);
 
+/* s2c_depth_test without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * result and errors as s2c_depth_test; its key buffer is a static device
+ * array of 4,194,304 int64 (32 MiB) per GPU, held until the process ends, so
+ * B*H*W above 4,194,304 returns S2C_ERR_ARG.  Calls on one GPU share that
+ * buffer: calls that overlap on one GPU (other streams or threads) race. */
+int s2c_depth_test_noalloc(int B, int H, int W, int num_fragments,
+                           const int* frag_pix, const float* frag_attrs,
+                           float* rast_out, void* stream);
+
 /* rasterize.cu:336-365 (filter_valid_fragments), stable order. */
 int s2c_filter_valid_fragments(int num_frags, const int* frag_pix,
                                const float* frag_attrs, int* frag_pix_out,

@@ -1,6 +1,8 @@
 // Triangle rasterization, depth test, barycentric interpolation and cluster
 // masks (reference: pystrns2cards/src/rasterize.cu), plus the fused
 // triangle -> cluster-bitset kernel used by card fitting.
+#include <mutex>
+
 #include "raster_math.hpp"
 
 namespace s2c {
@@ -239,10 +241,10 @@ extern "C" int s2c_compute_triangle_rects(int H, int W, int B, int V, const floa
     tri_rects_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(H, W, B, V, pos, T, tri,
                                                             triangle_rects, frag_prefix_sum);
     S2C_LAUNCH_CHECK();
-    Temp scratch;
-    if (int r = scratch.alloc(sizeof(int) * scan_scratch_ints(n), s)) return r;
-    if (int r = inclusive_scan_inplace(frag_prefix_sum, n, scratch.as<int>(), s)) return r;
-    return read_scan_total(scratch.as<int>(), n, s, num_frags_out, "compute_triangle_rects");
+    Temp sums;
+    if (int r = sums.alloc(sizeof(int) * scan_scratch_ints(n), s)) return r;
+    if (int r = inclusive_scan_inplace(frag_prefix_sum, n, sums.as<int>(), s)) return r;
+    return read_scan_total(sums.as<int>(), n, s, num_frags_out, "compute_triangle_rects");
 }
 
 extern "C" int s2c_compute_fragments(int H, int W, int V, const float* pos, int T,
@@ -260,6 +262,60 @@ extern "C" int s2c_compute_fragments(int H, int W, int V, const float* pos, int 
     return S2C_OK;
 }
 
+namespace s2c {
+
+// The depth test's three launches over a key buffer of npix int64: a fill
+// kernel (no memset node, so the sequence can be captured in a graph), the
+// per-fragment atomicMin of the keys, and the gather of the winners.
+static int launch_depth_test(long long npix, int H, int W, int num_fragments,
+                             const int* frag_pix, const float* frag_attrs, long long* keys,
+                             float* rast_out, hipStream_t s) {
+    fill_i64_kernel<<<grid_for(dmin<long long>(npix, 1ll << 22), kBlock), kBlock, 0, s>>>(
+        keys, LLONG_MAX, npix);
+    S2C_LAUNCH_CHECK();
+    if (num_fragments > 0) {
+        depth_key_kernel<<<grid_for(num_fragments, kBlock), kBlock, 0, s>>>(
+            num_fragments, frag_pix, frag_attrs, H, W, keys);
+        S2C_LAUNCH_CHECK();
+    }
+    depth_gather_kernel<<<grid_for(npix, kBlock), kBlock, 0, s>>>(npix, keys, frag_attrs,
+                                                                   rast_out);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
+
+// Key buffer of s2c_depth_test_noalloc: one array per GPU, loaded with the
+// code object and held until the process ends (32 MiB: 4,194,304 pixels,
+// e.g. 4 views at 1024 x 1024 or 16 at 512 x 512).  Host code never uses
+// this name as an address; it uses the device address that
+// hipGetSymbolAddress returns for the current device.
+constexpr long long kDepthScratchPixels = 1ll << 22;
+static __device__ __align__(256) long long scratch[kDepthScratchPixels];
+
+constexpr int kMaxDevices = 64;
+
+// Device address of `scratch` on the current device, looked up on the first
+// call on that device (with it current) and kept per device.
+static int depth_scratch(long long** out) {
+    static std::mutex mu;
+    static long long* ptr[kMaxDevices] = {};
+    int dev = 0;
+    S2C_HIP(hipGetDevice(&dev));
+    S2C_CHECK(dev >= 0 && dev < kMaxDevices, "depth_test_noalloc: device %d beyond %d", dev,
+              kMaxDevices);
+    std::lock_guard<std::mutex> held(mu);
+    if (ptr[dev] == nullptr) {
+        void* p = nullptr;
+        S2C_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        S2C_CHECK(p != nullptr, "depth_test_noalloc: no device address for the key buffer");
+        ptr[dev] = static_cast<long long*>(p);
+    }
+    *out = ptr[dev];
+    return S2C_OK;
+}
+
+}  // namespace s2c
+
 extern "C" int s2c_depth_test(int B, int H, int W, int num_fragments, const int* frag_pix,
                               const float* frag_attrs, float* rast_out, void* stream) {
     S2C_CHECK(B >= 0 && H >= 0 && W >= 0 && num_fragments >= 0, "depth_test: negative size");
@@ -268,18 +324,24 @@ extern "C" int s2c_depth_test(int B, int H, int W, int num_fragments, const int*
     hipStream_t s = static_cast<hipStream_t>(stream);
     Temp keys;
     if (int r = keys.alloc(sizeof(long long) * npix, s)) return r;
-    fill_i64_kernel<<<grid_for(dmin<long long>(npix, 1ll << 22), kBlock), kBlock, 0, s>>>(
-        keys.as<long long>(), LLONG_MAX, npix);
-    S2C_LAUNCH_CHECK();
-    if (num_fragments > 0) {
-        depth_key_kernel<<<grid_for(num_fragments, kBlock), kBlock, 0, s>>>(
-            num_fragments, frag_pix, frag_attrs, H, W, keys.as<long long>());
-        S2C_LAUNCH_CHECK();
-    }
-    depth_gather_kernel<<<grid_for(npix, kBlock), kBlock, 0, s>>>(npix, keys.as<long long>(),
-                                                                   frag_attrs, rast_out);
-    S2C_LAUNCH_CHECK();
-    return S2C_OK;
+    return launch_depth_test(npix, H, W, num_fragments, frag_pix, frag_attrs,
+                             keys.as<long long>(), rast_out, s);
+}
+
+extern "C" int s2c_depth_test_noalloc(int B, int H, int W, int num_fragments,
+                                      const int* frag_pix, const float* frag_attrs,
+                                      float* rast_out, void* stream) {
+    S2C_CHECK(B >= 0 && H >= 0 && W >= 0 && num_fragments >= 0,
+              "depth_test_noalloc: negative size");
+    const long long npix = static_cast<long long>(B) * H * W;
+    S2C_CHECK(npix <= kDepthScratchPixels,
+              "depth_test_noalloc: B*H*W = %lld pixels exceeds the key buffer's %lld", npix,
+              kDepthScratchPixels);
+    if (npix == 0) return S2C_OK;
+    long long* keys = nullptr;
+    if (int r = depth_scratch(&keys)) return r;
+    return launch_depth_test(npix, H, W, num_fragments, frag_pix, frag_attrs, keys, rast_out,
+                             static_cast<hipStream_t>(stream));
 }
 
 extern "C" int s2c_interpolate_triangle_attributes(int B, int H, int W, int attr_dim,
