@@ -209,7 +209,10 @@ int s2c_skinning(int num_strands, int num_points, const float* strands,
 /* dda.py:5-76 then rasterize.py:170-214, as used at 3_fitting.py:254-261:
  * all B*E edges DDA-rasterized and OR-ed into bitset (B,H,W,num_slots),
  * which this call clears first.  Equal to the reference's mask on every view
- * (views without fragments are all zero, where the reference shrinks B). */
+ * (views without fragments are all zero, where the reference shrinks B);
+ * cluster ids outside [0, 32*num_slots) set no bit (the reference writes
+ * them out of bounds).  Its binning buffers are kept per (device, stream)
+ * until s2c_dda_cluster_mask_release. */
 int s2c_dda_cluster_mask(int B, int H, int W, int V, const float* pos, int E,
                          const int* edges, const int* edge2cluster,
                          int num_slots, int* bitset, void* stream);
@@ -220,6 +223,9 @@ int s2c_dda_cluster_mask(int B, int H, int W, int V, const float* pos, int E,
 int s2c_dda_cluster_mask_tiled(int B, int H, int W, int V, const float* pos, int E,
                                const int* edges, const int* edge2cluster, int num_slots,
                                int tile_log2, int* bitset, void* stream);
+/* Frees the binning buffers the two calls above keep for `stream` on the
+ * current device (after waiting for the stream). */
+int s2c_dda_cluster_mask_release(void* stream);
 
 /* rasterize.py:6-53 then rasterize.py:170-214, as used at 3_fitting.py:154-163
  * (render_bitset_from_triangles); clears bitset first. */

@@ -348,7 +348,11 @@ int BinWorkspace::get(hipStream_t s, Slot slot, size_t bytes, void** out) {
     S2C_HIP(hipGetDevice(&dev));
     WorkspaceBuffers& w = workspaces()[{dev, s}];
     if (w.n[slot] < bytes) {
-        if (w.p[slot]) S2C_HIP(hipFreeAsync(w.p[slot], s));  // ordered after its last use
+        if (w.p[slot]) {
+            // earlier calls on this stream may still read the old block
+            S2C_HIP(hipStreamSynchronize(s));
+            S2C_HIP(hipFree(w.p[slot]));
+        }
         w.p[slot] = nullptr;
         w.n[slot] = 0;
         const size_t grown = bytes + bytes / 8;  // headroom for the next, larger call
@@ -356,6 +360,22 @@ int BinWorkspace::get(hipStream_t s, Slot slot, size_t bytes, void** out) {
         w.n[slot] = grown;
     }
     *out = w.p[slot];
+    return S2C_OK;
+}
+
+int BinWorkspace::release(hipStream_t s) {
+    int dev = 0;
+    S2C_HIP(hipGetDevice(&dev));
+    auto& m = workspaces();
+    auto it = m.find({dev, s});
+    if (it == m.end()) return S2C_OK;
+    S2C_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < kSlots; ++i) {
+        if (it->second.p[i]) S2C_HIP(hipFree(it->second.p[i]));
+        it->second.p[i] = nullptr;
+        it->second.n[i] = 0;
+    }
+    m.erase(it);
     return S2C_OK;
 }
 

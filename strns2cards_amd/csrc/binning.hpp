@@ -191,20 +191,22 @@ int bin_check_overflow(const int* host, long long* entries, long long* runs);
 int bin_resolve(const BinPlan& p, int H, int W, int S, int* bitset, hipStream_t s);
 
 // Device buffers of the binned pass kept per (device, stream) across calls,
-// so that a call allocates nothing once its sizes have been seen (a
-// stream-ordered allocation behind the previous call's pending frees can
-// hold the host until they complete).  Calls on one stream are ordered, so
-// they can share the buffers; `lock` must be held from the first get() to
-// the last launch of a call, so that two threads enqueueing on one stream
-// cannot interleave.  Buffers only grow and are kept until the process
-// ends (for the 50k-strand, 4-view 1024^2 workload about 0.6 GB per
-// stream).
+// so that a call allocates nothing once its sizes have been seen.  Calls on
+// one stream are ordered, so they can share the buffers; `lock` must be
+// held from the first get() to the last launch of a call, so that two
+// threads enqueueing on one stream cannot interleave.  Buffers only grow: a
+// larger size waits for the stream, frees the old block and allocates a new
+// one.  They are kept until the process ends or release() (for the
+// 50k-strand, 4-view 1024^2 workload about 0.6 GB per stream).
 struct BinWorkspace {
     enum Slot { kPlanInts = 0, kEntries = 1, kRecords = 2, kRuns = 3, kSlots = 4 };
     static std::mutex& lock();
     // Device memory of at least `bytes` for `slot`, for calls on stream s of
     // the current device.
     static int get(hipStream_t s, Slot slot, size_t bytes, void** out);
+    // Frees the buffers of stream s on the current device, after waiting for
+    // the stream.
+    static int release(hipStream_t s);
 };
 
 // Running estimates of the overflow-area sizes, per process.

@@ -136,6 +136,8 @@ dda_mask_direct_kernel(long long P, int H, int W, int V, const float* __restrict
     __syncthreads();
     walk_edges(P, H, W, V, pos, E, edges, edge2cluster, deferred, &ndeferred,
                [&](int b, int h, int w, int c) {
+                   // ids beyond the bitset set no bit, as in the binned form
+                   if (static_cast<unsigned>(c) >= static_cast<unsigned>(S) * 32u) return;
                    const long long idx = ((static_cast<long long>(b) * H + h) * W + w) * S + (c >> 5);
                    atomicOr(&bitset[idx], static_cast<int>(1u << (c & 31)));
                });
@@ -439,4 +441,9 @@ extern "C" int s2c_dda_cluster_mask_tiled(int B, int H, int W, int V, const floa
         want_r = bin_run_hint(nr + nr / 8 + 4 * kHash);
     }
     return S2C_OK;
+}
+
+extern "C" int s2c_dda_cluster_mask_release(void* stream) {
+    std::lock_guard<std::mutex> held(BinWorkspace::lock());
+    return BinWorkspace::release(static_cast<hipStream_t>(stream));
 }

@@ -140,9 +140,17 @@ __device__ __forceinline__ float4 load_pos(const float* pos, long long row) {
 // lim2 = 1 - 2e-5 implies the rounded quotient |fl(z / w)| <= 1 - 1e-5
 // (relative error of the product and quotient below 2^-23), so this test
 // only ever answers "passes" where the divided test above would.
+// t1 is in [0, 1] only while dda.cu:124-144's |p.x - q p.w| terms are
+// finite: a sample that reaches the depth test is on screen, so q is in
+// [-1, 1], and |x|, |y| <= 1e30 with w <= 1e30 keeps them finite (an
+// infinite coordinate makes t1 = inf / inf = NaN, which fails the test).
 __device__ __forceinline__ bool dda_depth_always_passes(float4 p0, float4 p1) {
     const float lo = fminf(p0.w, p1.w), hi = fmaxf(p0.w, p1.w);
     if (!(lo >= 1e-30f && hi <= 1e30f && hi <= 16.f * lo)) return false;
+    constexpr float cmax = 1e30f;  // each comparison is false for NaN
+    if (!(fabsf(p0.x) <= cmax && fabsf(p0.y) <= cmax && fabsf(p1.x) <= cmax &&
+          fabsf(p1.y) <= cmax))
+        return false;
     constexpr float lim2 = 1.f - 2e-5f;
     return fabsf(p0.z) <= lim2 * p0.w && fabsf(p1.z) <= lim2 * p1.w;  // false for NaN
 }

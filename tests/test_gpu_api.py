@@ -121,6 +121,15 @@ def test_dda_cluster_mask_two_streams(dev):
         torch.cuda.synchronize()
         for out, ci in zip(outs, order):
             np.testing.assert_array_equal(out.cpu().numpy(), cases[ci][4])
+    # releasing a stream's buffers: the next call on it allocates them again
+    from capi import call
+    for st in streams:
+        call("s2c_dda_cluster_mask_release", stream=st.cuda_stream)
+    res, pos, edges, e2c, want = cases[1]
+    with torch.cuda.stream(streams[0]):
+        out = s2c.dda_cluster_mask(res, T(pos, dev), T(edges, dev), T(e2c, dev), cluster_count=200)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), want)
 
 
 def test_bitset_api(dev, small):

@@ -209,6 +209,25 @@ def test_dda_mask_workgroup_paths(dev, case):
     np.testing.assert_array_equal(got, want)
 
 
+def test_dda_mask_nonfinite_and_huge_vertices(dev):
+    """Vertices at +-inf, NaN, 1e30 and w = 0 beside ordinary ones.  An
+    infinite x or y makes dda.cu:124-144's t1 NaN, so those fragments fail
+    the depth test even where z and w alone would pass it; edges reaching far
+    off screen are long, and workgroups mix them with ordinary edges."""
+    report("dda->mask non-finite and huge vertices")
+    rng = np.random.default_rng(41)
+    pos, edges = random_lines(rng, 2, 400, 3000, 96, 80, persp=True)
+    bad = rng.choice(400, size=60, replace=False)
+    vals = np.array([np.inf, -np.inf, np.nan, 1e30, -1e30, 3e4], np.float32)
+    for j, v in enumerate(bad):
+        pos[j % 2, v, rng.integers(0, 2)] = vals[j % len(vals)]
+    pos[0, bad[:6], 3] = 0.0  # w = 0: the NDC divide overflows
+    e2c = rng.integers(0, 50, size=edges.shape[0]).astype(np.int32)
+    want = oracle.dda_cluster_mask((96, 80), pos, edges, e2c, 50)
+    got = fused_dda_mask(dev, (96, 80), pos, edges, e2c, 50)
+    np.testing.assert_array_equal(got, want)
+
+
 def test_dda_mask_depth_boundaries(dev):
     """Edges whose endpoint depths sit at / around the fast-path threshold
     (|z_ndc| = 1 - 1e-5) and at the clip limit, with w = 1 and with w
