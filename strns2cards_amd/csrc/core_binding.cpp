@@ -11,8 +11,14 @@
 #include <ATen/hip/impl/HIPGuardImplMasqueradingAsCUDA.h>
 #include <ATen/hip/impl/HIPStreamMasqueradingAsCUDA.h>
 
-#include <string>
+#include <pybind11/numpy.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
 #include <optional>
+#include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/strns2cards_hip.h"
@@ -73,6 +79,133 @@ int I(int64_t v, const char* fn) {
 }
 
 template <class T> T* P(const Tensor& t) { return static_cast<T*>(t.data_ptr()); }
+
+// ---- host-side strand files (hair_loader.cpp:15-256) ------------------------
+
+using Strand = std::vector<float>;  // x, y, z per point
+
+// Little-endian reader over a whole file; a read past the end returns false
+// and leaves the value untouched (the reference's fread on a short file).
+struct Bytes {
+    std::vector<unsigned char> b;
+    size_t at = 0;
+    template <class T> bool get(T& v) {
+        if (at + sizeof(T) > b.size()) return false;
+        std::memcpy(&v, b.data() + at, sizeof(T));
+        at += sizeof(T);
+        return true;
+    }
+};
+
+bool read_file(const std::string& path, Bytes& out) {
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) {
+        std::fprintf(stderr, "Couldn't open %s\n", path.c_str());
+        return false;
+    }
+    unsigned char buf[1 << 16];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) out.b.insert(out.b.end(), buf, buf + n);
+    std::fclose(f);
+    return true;
+}
+
+// hair_loader.cpp:15-58: count, then per strand a count and 7 floats per
+// point (position, normal, label); a strand with a segment shorter than
+// 1e-4 is dropped.
+std::vector<Strand> strands_bin(Bytes& r) {
+    std::vector<Strand> out;
+    int n = 0;
+    r.get(n);
+    for (int i = 0; i < n; ++i) {
+        int m = 0;
+        if (!r.get(m) || m < 0) break;
+        Strand s(static_cast<size_t>(m) * 3, 0.f);
+        for (int j = 0; j < m; ++j) {
+            float rec[7] = {0, 0, 0, 0, 0, 0, 0};
+            for (float& v : rec) r.get(v);
+            s[3 * j] = rec[0];
+            s[3 * j + 1] = rec[1];
+            s[3 * j + 2] = rec[2];
+        }
+        bool valid = true;
+        for (int j = 0; j + 1 < m && valid; ++j) {
+            const float dx = s[3 * j] - s[3 * j + 3], dy = s[3 * j + 1] - s[3 * j + 4],
+                        dz = s[3 * j + 2] - s[3 * j + 5];
+            valid = std::sqrt(dx * dx + dy * dy + dz * dz) >= 1e-4f;
+        }
+        if (valid) out.push_back(std::move(s));
+    }
+    return out;
+}
+
+// hair_loader.cpp:61-93: count, then per strand a count and its points; a
+// one-point strand is skipped.
+std::vector<Strand> strands_usc(Bytes& r) {
+    std::vector<Strand> out;
+    int n = 0;
+    r.get(n);
+    for (int i = 0; i < n; ++i) {
+        int m = 0;
+        if (!r.get(m) || m < 0) break;
+        Strand s(static_cast<size_t>(m) * 3, 0.f);
+        for (float& v : s) r.get(v);
+        if (m != 1) out.push_back(std::move(s));
+    }
+    return out;
+}
+
+// hair_loader.cpp:96-217: the "HAIR" header layout (segments stored as
+// count - 1, or one default count; the header is skipped as the reference
+// skips it) or the headerless ZJU layout (counts, then points).
+std::vector<Strand> strands_hair(Bytes& r) {
+    std::vector<Strand> out;
+    if (r.b.size() < 4) {
+        std::fprintf(stderr, "Failed to read header.\n");
+        return out;
+    }
+    const bool hair = std::memcmp(r.b.data(), "HAIR", 4) == 0;
+    unsigned n = 0, count = 0;
+    std::vector<unsigned> seg;
+    if (hair) {
+        r.at = 4;
+        unsigned bits = 0, def = 0;
+        r.get(n);
+        r.get(count);
+        r.get(bits);
+        if (!(bits & 2u)) {
+            std::fprintf(stderr, "Error: HAIR file missing points section.\n");
+            return out;
+        }
+        if (!(bits & 1u)) r.get(def);
+        r.at += 4u * (128u - (def ? 3u : 4u));
+        seg.resize(n);
+        for (unsigned i = 0; i < n; ++i) {
+            unsigned short v = static_cast<unsigned short>(def);
+            if (def == 0) r.get(v);
+            seg[i] = static_cast<unsigned>(v) + 1u;
+        }
+    } else {
+        r.at = 0;
+        r.get(n);
+        r.get(count);
+        seg.resize(n);
+        for (unsigned i = 0; i < n; ++i) {
+            unsigned short v = 0;
+            r.get(v);
+            seg[i] = v;
+        }
+    }
+    std::vector<float> pts(static_cast<size_t>(count) * 3, 0.f);
+    for (float& v : pts) r.get(v);
+    size_t at = 0;
+    for (unsigned i = 0; i < n; ++i) {
+        if ((at + seg[i]) * 3 > pts.size()) break;  // the file's points run out
+        out.emplace_back(pts.begin() + at * 3, pts.begin() + (at + seg[i]) * 3);
+        at += seg[i];
+    }
+    return out;
+}
 
 }  // namespace
 
@@ -651,6 +784,134 @@ PYBIND11_MODULE(_core, m) {
                                P<float>(T), P<float>(N), P<float>(B), stream_of(strands)),
               fn);
     });
+
+    // ---- host-side compat exports (pystrns2cards.cpp:208-243, 788-839) ----
+
+    m.def("build_edges_from_triangles", [](py::array tri_in) {
+        // mesh_utils.cpp:15-79: the undirected edges of the triangles, each
+        // with its first two triangles (-1 when it has one).  Edges come in
+        // first-appearance order over the sides (0,1), (1,2), (2,0) of each
+        // triangle (the reference's order comes from a hash map and is
+        // implementation-defined).
+        if (tri_in.dtype().kind() != 'i' || tri_in.itemsize() != 4)
+            throw py::type_error("build_edges_from_triangles: `tri` must be int32");
+        if (tri_in.ndim() != 2 || tri_in.shape(1) != 3)
+            throw py::type_error("build_edges_from_triangles: `tri` must have shape (T, 3)");
+        py::array_t<int32_t, py::array::c_style | py::array::forcecast> tri(tri_in);
+        const int64_t T = tri.shape(0);
+        const int32_t* t = tri.data();
+        std::unordered_map<uint64_t, int> id;
+        id.reserve(static_cast<size_t>(T) * 2);
+        std::vector<int32_t> edges, e2t;
+        for (int64_t k = 0; k < T; ++k) {
+            for (int side = 0; side < 3; ++side) {
+                int32_t a = t[3 * k + side], b = t[3 * k + (side + 1) % 3];
+                if (a > b) std::swap(a, b);
+                const uint64_t key = (static_cast<uint64_t>(static_cast<uint32_t>(a)) << 32) |
+                                     static_cast<uint32_t>(b);
+                auto it = id.find(key);
+                if (it == id.end()) {
+                    id.emplace(key, static_cast<int>(edges.size() / 2));
+                    edges.push_back(a);
+                    edges.push_back(b);
+                    e2t.push_back(static_cast<int32_t>(k));
+                    e2t.push_back(-1);
+                } else if (e2t[2 * it->second + 1] < 0) {
+                    e2t[2 * it->second + 1] = static_cast<int32_t>(k);
+                }
+            }
+        }
+        const py::ssize_t E = static_cast<py::ssize_t>(edges.size() / 2);
+        py::array_t<int32_t> out_e({E, static_cast<py::ssize_t>(2)});
+        py::array_t<int32_t> out_t({E, static_cast<py::ssize_t>(2)});
+        if (E) {
+            std::memcpy(out_e.mutable_data(), edges.data(), sizeof(int32_t) * edges.size());
+            std::memcpy(out_t.mutable_data(), e2t.data(), sizeof(int32_t) * e2t.size());
+        }
+        return py::make_tuple(out_e, out_t);
+    }, "Build unique edge list and triangle mapping from face indices");
+
+    m.def("load_strands", [](const std::string& path) {
+        // (num_samples (N, 1) int32, points (sum, 3) float32) of the strands
+        // with at least 2 points (hair_loader.cpp:220-258), by extension
+        auto ends = [&](const char* sfx) {
+            const size_t n = std::strlen(sfx);
+            return path.size() >= n && path.compare(path.size() - n, n, sfx) == 0;
+        };
+        std::vector<Strand> strands;
+        if (!ends(".bin") && !ends(".data") && !ends(".hair"))
+            throw std::invalid_argument("load_strands: unknown file extension.");
+        Bytes r;
+        if (read_file(path, r)) {
+            strands = ends(".bin") ? strands_bin(r) : ends(".data") ? strands_usc(r)
+                                                                    : strands_hair(r);
+        }
+        std::vector<int32_t> ns;
+        size_t npts = 0;
+        for (const Strand& s : strands)
+            if (s.size() >= 6) {
+                ns.push_back(static_cast<int32_t>(s.size() / 3));
+                npts += s.size() / 3;
+            }
+        py::array_t<int32_t> num({static_cast<py::ssize_t>(ns.size()), static_cast<py::ssize_t>(1)});
+        py::array_t<float> pts({static_cast<py::ssize_t>(npts), static_cast<py::ssize_t>(3)});
+        if (!ns.empty()) std::memcpy(num.mutable_data(), ns.data(), sizeof(int32_t) * ns.size());
+        float* dst = pts.mutable_data();
+        for (const Strand& s : strands)
+            if (s.size() >= 6) {
+                std::memcpy(dst, s.data(), sizeof(float) * s.size());
+                dst += s.size();
+            }
+        return py::make_tuple(num, pts);
+    }, "Load hair strands and return (num_samples, points) pair.");
+
+    m.def("resample_strands_by_arclength", [](py::array ns_in, py::array pts_in, int target) {
+        // (N, target, 3) float32 on the host, computed on the current GPU
+        // with the kernels of resample_strands_by_arclength_device
+        const char* fn = "resample_strands_by_arclength";
+        if (ns_in.dtype().kind() != 'i' || ns_in.itemsize() != 4 || ns_in.ndim() != 1)
+            throw py::type_error("resample_strands_by_arclength: `num_samples` must be int32 (N,)");
+        if (pts_in.dtype().kind() != 'f' || pts_in.itemsize() != 4 || pts_in.ndim() != 2 ||
+            pts_in.shape(1) != 3)
+            throw py::type_error("resample_strands_by_arclength: `points` must be float32 (P, 3)");
+        if (target < 2) throw std::invalid_argument("resample_strands_by_arclength: target < 2");
+        py::array_t<int32_t, py::array::c_style | py::array::forcecast> ns(ns_in);
+        py::array_t<float, py::array::c_style | py::array::forcecast> pts(pts_in);
+        const int64_t N = ns.shape(0), Pn = pts.shape(0);
+        std::vector<int64_t> off(static_cast<size_t>(N));
+        int64_t sum = 0;
+        for (int64_t i = 0; i < N; ++i) {
+            if (ns.data()[i] < 2)
+                throw std::invalid_argument(
+                    "resample_strands_by_arclength: every strand needs at least 2 points");
+            off[static_cast<size_t>(i)] = sum;
+            sum += ns.data()[i];
+        }
+        if (sum != Pn)
+            throw std::invalid_argument(
+                "resample_strands_by_arclength: sum(num_samples) must equal the number of points");
+        py::array_t<float> out({static_cast<py::ssize_t>(N), static_cast<py::ssize_t>(target),
+                                static_cast<py::ssize_t>(3)});
+        if (N == 0) return out;
+        const auto dev = at::Device(at::kCUDA, c10::hip::current_device());
+        auto f32 = at::TensorOptions().dtype(at::kFloat).device(dev);
+        Tensor t_ns = at::from_blob(const_cast<int32_t*>(ns.data()), {N}, at::kInt).to(dev);
+        Tensor t_off = at::from_blob(off.data(), {N}, at::kLong).to(dev);
+        Tensor t_pts = at::from_blob(const_cast<float*>(pts.data()), {Pn, 3}, at::kFloat).to(dev);
+        Tensor lens = at::empty({Pn}, f32);
+        Tensor res = at::empty({N, target, 3}, f32);
+        {
+            DeviceGuard g(dev);
+            py::gil_scoped_release nogil;
+            check(s2c_resample_strands_by_arclength(I(N, fn), P<int>(t_ns), P<long long>(t_off),
+                                                    P<float>(t_pts), target, P<float>(lens),
+                                                    P<float>(res), stream_of(t_pts)),
+                  fn);
+        }
+        Tensor host = res.cpu();
+        std::memcpy(out.mutable_data(), host.data_ptr<float>(), sizeof(float) * N * target * 3);
+        return out;
+    }, "Resample strands uniformly by arc length.");
 
     m.def("resample_strands_by_arclength_device", [](Tensor num_samples, Tensor offsets,
                                                       Tensor points, int target, Tensor lengths,

@@ -59,3 +59,21 @@ def test_python_api_surface():
         "print_version",
     ]:
         assert hasattr(s2c, name), name
+
+
+def test_core_exports_every_reference_binding_name():
+    """The names NB_MODULE(_core) defines at pystrns2cards.cpp:204-924."""
+    from strns2cards_amd import _core
+
+    reference = [
+        "print_version", "build_edges_from_triangles", "compute_triangle_rects",
+        "compute_fragments", "depth_test", "filter_valid_fragments",
+        "interpolate_triangle_attributes", "backward_interpolate_triangle_attributes",
+        "cluster_mask_from_fragments", "accumulate_bitset_rgb", "popcount_bitset",
+        "dda_compute_span", "dda_compute_fragments", "dda_interpolate_attributes",
+        "backward_dda_interpolate_attributes", "mark_discontinuity_edges",
+        "backward_antialiased_cluster_bitset", "compact_valid_ints", "load_strands",
+        "resample_strands_by_arclength", "smooth_strands", "skinning",
+    ]
+    missing = [n for n in reference if not hasattr(_core, n)]
+    assert not missing, missing

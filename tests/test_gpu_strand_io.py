@@ -36,6 +36,25 @@ def test_resample_workload_strands(dev):
     assert empty.shape == (0, 5, 3)
 
 
+def test_core_resample_host_arrays(dev):
+    """_core.resample_strands_by_arclength takes and returns host arrays, as
+    the reference's binding does (pystrns2cards.cpp:818-839)."""
+    from strns2cards_amd import _core
+
+    report("strand io: _core.resample_strands_by_arclength on host arrays")
+    rng = np.random.default_rng(4)
+    counts = rng.integers(2, 90, size=700).astype(np.int32)
+    pts = np.cumsum(rng.normal(size=(int(counts.sum()), 3)).astype(np.float32) * 0.02, 0)
+    got = _core.resample_strands_by_arclength(counts, pts, 33)
+    assert isinstance(got, np.ndarray) and got.shape == (700, 33, 3) and got.dtype == np.float32
+    np.testing.assert_array_equal(got, oracle.resample_strands_by_arclength(counts, pts, 33))
+    with pytest.raises(ValueError):
+        _core.resample_strands_by_arclength(np.array([1, 3], np.int32),
+                                           np.zeros((4, 3), np.float32), 5)
+    with pytest.raises(TypeError):
+        _core.resample_strands_by_arclength(counts.astype(np.int64), pts, 5)
+
+
 def test_npz_arc_length(dev, tmp_path):
     rng = np.random.default_rng(1)
     pts = rng.normal(size=(10, 3)).astype(np.float32)

@@ -59,6 +59,46 @@ def test_edge_indices_from_wisp():
     np.testing.assert_array_equal(e, [[0, 1], [1, 2], [2, 3], [4, 5], [5, 6], [6, 7]])
 
 
+def _core_load_matches(path):
+    """_core.load_strands (pystrns2cards.cpp:788-816: (num_samples (N, 1),
+    points (sum, 3))) reads what the Python loader reads."""
+    from strns2cards_amd import _core
+
+    _, ns, pts = sio.load_strands(str(path), return_raw=True)
+    cns, cpts = _core.load_strands(str(path))
+    assert cns.dtype == np.int32 and cns.shape == (ns.shape[0], 1)
+    assert cpts.dtype == np.float32 and cpts.shape == pts.shape
+    np.testing.assert_array_equal(cns[:, 0], ns)
+    np.testing.assert_array_equal(cpts, pts)
+
+
+def test_core_build_edges_matches_python():
+    from strns2cards_amd import _core
+
+    rng = np.random.default_rng(9)
+    for V, T in ((6, 4), (40, 120), (300, 900)):
+        tri = rng.integers(0, V, size=(T, 3)).astype(np.int32)
+        tri[: T // 10, 2] = tri[: T // 10, 1]  # degenerate triangles repeat a side
+        want = sio.build_edges_from_triangles(tri)
+        got = _core.build_edges_from_triangles(tri)
+        for g, w in zip(got, want):
+            assert g.dtype == np.int32
+            np.testing.assert_array_equal(g, w)
+    e, t = _core.build_edges_from_triangles(np.zeros((0, 3), np.int32))
+    assert e.shape == (0, 2) and t.shape == (0, 2)
+    with pytest.raises(TypeError):
+        _core.build_edges_from_triangles(np.zeros((2, 3), np.int64))
+
+
+def test_core_load_strands_errors(tmp_path):
+    from strns2cards_amd import _core
+
+    with pytest.raises(ValueError):
+        _core.load_strands(str(tmp_path / "x.obj"))
+    ns, pts = _core.load_strands(str(tmp_path / "missing.bin"))  # the reference prints and returns none
+    assert ns.shape == (0, 1) and pts.shape == (0, 3)
+
+
 def test_load_bin(tmp_path):
     rng = np.random.default_rng(1)
     ss = _strands(rng, [5, 1, 7, 3])
@@ -76,6 +116,7 @@ def test_load_bin(tmp_path):
     np.testing.assert_array_equal(arr[0], ss[0])
     np.testing.assert_array_equal(arr[1], ss[3])
     np.testing.assert_array_equal(pts, np.concatenate([ss[0], ss[3]]))
+    _core_load_matches(p)
 
 
 def test_load_usc_data(tmp_path):
@@ -90,6 +131,7 @@ def test_load_usc_data(tmp_path):
     arr = sio.load_strands(str(p))
     assert [a.shape[0] for a in arr] == [4, 6, 2]
     np.testing.assert_array_equal(arr[1], ss[2])
+    _core_load_matches(p)
 
 
 def test_load_hair_variants(tmp_path):
@@ -107,6 +149,7 @@ def test_load_hair_variants(tmp_path):
     arr = sio.load_strands(str(p))
     assert [a.shape[0] for a in arr] == [3, 5, 4]
     np.testing.assert_array_equal(arr[2], ss[2])
+    _core_load_matches(p)
     # HAIR header with a default segment count (bit 0 clear)
     same = _strands(rng, [4, 4])
     q = tmp_path / "b.hair"
@@ -116,6 +159,7 @@ def test_load_hair_variants(tmp_path):
         f.write(np.concatenate(same).tobytes())
     arr = sio.load_strands(str(q))
     np.testing.assert_array_equal(arr[1], same[1])
+    _core_load_matches(q)
     # ZJU: no header; per-strand point counts
     z = tmp_path / "c.hair"
     with open(z, "wb") as f:
@@ -124,6 +168,7 @@ def test_load_hair_variants(tmp_path):
         f.write(pts.tobytes())
     arr = sio.load_strands(str(z))
     np.testing.assert_array_equal(arr[1], ss[1])
+    _core_load_matches(z)
     with pytest.raises(ValueError):
         sio.load_strands(str(tmp_path / "x.obj"))
 
