@@ -37,6 +37,7 @@ _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _opt_i32p = ctypes.c_void_p  # an int32 array's .ctypes.data, or None
+_opt_f32p = ctypes.c_void_p  # a float32 array's .ctypes.data, or None
 _I = ctypes.c_int
 _F = ctypes.c_float
 
@@ -45,7 +46,8 @@ _SIGS = {
     "orc_dda_compute_fragments": (None, [_I, _I, _i32p, _i32p, _f32p, _f32p, _I, _I, _I, _f32p,
                                          _I, _i32p, _i32p, _f32p]),
     "orc_dda_interpolate_attributes": (None, [_I, _I, _I, _I, _f32p, _f32p, _i32p, _f32p]),
-    "orc_backward_dda_interpolate_attributes": (None, [_I, _I, _I, _f32p, _I, _f32p, _i32p, _f32p]),
+    "orc_backward_dda_interpolate_attributes": (None, [_I, _I, _I, _f32p, _I, _f32p, _i32p, _f32p,
+                                                       _opt_f32p]),
     "orc_compute_triangle_rects": (_I, [_I, _I, _I, _I, _f32p, _I, _i32p, _i32p, _i32p]),
     "orc_compute_fragments": (None, [_I, _I, _I, _f32p, _I, _i32p, _I, _I, _i32p, _i32p, _i32p,
                                      _f32p]),
@@ -53,7 +55,7 @@ _SIGS = {
     "orc_filter_valid_fragments": (_I, [_I, _i32p, _f32p, _i32p, _f32p]),
     "orc_interpolate_triangle_attributes": (None, [_I, _I, _I, _I, _f32p, _f32p, _i32p, _f32p]),
     "orc_backward_interpolate_triangle_attributes": (None, [_I, _I, _I, _I, _f32p, _f32p, _i32p,
-                                                            _f32p]),
+                                                            _f32p, _opt_f32p]),
     "orc_cluster_mask_from_fragments": (None, [_I, _i32p, _f32p, _i32p, _I, _I, _I, _i32p]),
     "orc_accumulate_bitset_rgb": (None, [_I, _I, _I, _I, _i32p, _f32p, _f32p]),
     "orc_popcount_bitset": (None, [_I, _I, _I, _I, _i32p, _f32p]),
@@ -62,9 +64,9 @@ _SIGS = {
     "orc_compact_valid_ints": (_I, [_I, _i32p, _i32p]),
     "orc_backward_antialiased_cluster_bitset": (None, [_I, _i32p, _f32p, _I, _I, _I, _i32p, _i32p,
                                                        _I, _f32p, _I, _i32p, _f32p, _i32p, _f32p,
-                                                       _F, _F]),
-    "orc_smooth_strands": (None, [_I, _I, _f32p, _f32p, _f32p, _F]),
-    "orc_skinning": (None, [_I, _I, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _F]),
+                                                       _F, _F, _opt_f32p]),
+    "orc_smooth_strands": (None, [_I, _I, _f32p, _f32p, _f32p, _F, _opt_f32p]),
+    "orc_skinning": (None, [_I, _I, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _F, _opt_f32p]),
     "orc_dda_cluster_mask": (None, [_I, _I, _I, _I, _f32p, _I, _i32p, _i32p, _I, _i32p, _I]),
     "orc_triangle_cluster_mask": (None, [_I, _I, _I, _I, _f32p, _I, _i32p, _i32p, _I, _i32p]),
     "orc_card_coverage": (None, [_I, _I, _I, _I, _f64p, _I, _i32p, _I, _opt_i32p, _opt_i32p,
@@ -147,13 +149,23 @@ def dda_interpolate_attributes(rast, attr, edges):
     return img
 
 
-def backward_dda_interpolate_attributes(rast, d_image, edges, num_vertices):
+def _abs_out(like, with_abs):
+    """(array, pointer) of a zero-filled Σ|term| output, or (None, None)."""
+    if not with_abs:
+        return None, None
+    a = np.zeros_like(like)
+    return a, a.ctypes.data
+
+
+def backward_dda_interpolate_attributes(rast, d_image, edges, num_vertices, with_abs=False):
+    """d_attr; with ``with_abs`` also each element's sum of |term|."""
     rast, d_image, edges = _f(rast), _f(d_image), _i(edges)
     B, H, W, _ = rast.shape
     C = d_image.shape[3]
     d_attr = np.zeros((num_vertices, C), np.float32)
-    lib().orc_backward_dda_interpolate_attributes(B, H, W, rast, C, d_image, edges, d_attr)
-    return d_attr
+    ab, abp = _abs_out(d_attr, with_abs)
+    lib().orc_backward_dda_interpolate_attributes(B, H, W, rast, C, d_image, edges, d_attr, abp)
+    return (d_attr, ab) if with_abs else d_attr
 
 
 def dda_cluster_mask(resolution, pos, edges, edge2cluster, cluster_count, threads=1):
@@ -224,13 +236,15 @@ def interpolate_triangle_attributes(rast, attr, tri):
     return img
 
 
-def backward_interpolate_triangle_attributes(rast, d_image, tri, num_vertices):
+def backward_interpolate_triangle_attributes(rast, d_image, tri, num_vertices, with_abs=False):
+    """d_attr; with ``with_abs`` also each element's sum of |term|."""
     rast, d_image, tri = _f(rast), _f(d_image), _i(tri)
     B, H, W, _ = rast.shape
     C = d_image.shape[3]
     d_attr = np.zeros((num_vertices, C), np.float32)
-    lib().orc_backward_interpolate_triangle_attributes(B, H, W, C, rast, d_image, tri, d_attr)
-    return d_attr
+    ab, abp = _abs_out(d_attr, with_abs)
+    lib().orc_backward_interpolate_triangle_attributes(B, H, W, C, rast, d_image, tri, d_attr, abp)
+    return (d_attr, ab) if with_abs else d_attr
 
 
 def cluster_mask_from_fragments(resolution, frag_pix, frag_attrs, prim2cluster, cluster_count=None,
@@ -298,27 +312,30 @@ def compact_valid_ints(a):
 
 
 def backward_antialiased_cluster_bitset(frag_pix, frag_attrs, bitset, target, pos, edges, normals,
-                                        edge2cluster, kernel_radius=7.0, rho=1.0):
+                                        edge2cluster, kernel_radius=7.0, rho=1.0, with_abs=False):
+    """d_pos; with ``with_abs`` also each element's sum of |per-sample term|."""
     fp, fa = _i(frag_pix), _f(frag_attrs)
     bits, tgt = _i(bitset), _i(target)
     pos, edges, normals, e2c = _f(pos), _i(edges), _f(normals), _i(edge2cluster)
     B, H, W, S = bits.shape
     V = pos.shape[1]
     d_pos = np.zeros_like(pos)
+    ab, abp = _abs_out(d_pos, with_abs)
     lib().orc_backward_antialiased_cluster_bitset(fp.shape[0], fp, fa, H, W, S, bits, tgt, V, pos,
                                                   edges.shape[0], edges, normals, e2c, d_pos,
-                                                  float(kernel_radius), float(rho))
-    return d_pos
+                                                  float(kernel_radius), float(rho), abp)
+    return (d_pos, ab) if with_abs else d_pos
 
 
 def antialiased_cluster_bitset_grad(bitset, target, pos, tri, edges, edge2tri, edge2cluster,
-                                    kernel_radius=7.0, rho=1.0):
-    """d_pos of antialias.py:66-130 (the gradient its loss carries)."""
+                                    kernel_radius=7.0, rho=1.0, with_abs=False):
+    """d_pos of antialias.py:66-130 (the gradient its loss carries); with
+    ``with_abs`` also each element's sum of |per-sample term|."""
     H, W = np.asarray(target).shape[1:3]
     prim, normals = mark_discontinuity_edges((H, W), pos, tri, edges, edge2tri)
     fp, fa = dda_compute_fragments((H, W), pos, edges, prim)
     return backward_antialiased_cluster_bitset(fp, fa, bitset, target, pos, edges, normals,
-                                               edge2cluster, kernel_radius, rho)
+                                               edge2cluster, kernel_radius, rho, with_abs)
 
 
 # ------------------------------------------------------------------ curves
@@ -331,12 +348,14 @@ def compute_arclength(strands):
                           1).astype(np.float32)
 
 
-def smooth_strands(strands, arclength, beta):
+def smooth_strands(strands, arclength, beta, with_abs=False):
+    """Smoothed strands; with ``with_abs`` also Σ w|x| / Σ w per element."""
     x, s = _f(strands), _f(arclength)
     N, P, _ = x.shape
     out = np.zeros_like(x)
-    lib().orc_smooth_strands(N, P, x, s, out, float(beta))
-    return out
+    ab, abp = _abs_out(out, with_abs)
+    lib().orc_smooth_strands(N, P, x, s, out, float(beta), abp)
+    return (out, ab) if with_abs else out
 
 
 def bishop_frame(strands):
@@ -350,12 +369,15 @@ def bishop_frame(strands):
     return T, Nr, Bn
 
 
-def skinning(strands, arclength, handle, canonical, R, beta):
+def skinning(strands, arclength, handle, canonical, R, beta, with_abs=False):
+    """Skinned strands; with ``with_abs`` also the weighted sum of |term|
+    per element."""
     x, s, h, c, r = _f(strands), _f(arclength), _f(handle), _f(canonical), _f(R)
     N, P, _ = x.shape
     out = np.zeros_like(x)
-    lib().orc_skinning(N, P, x, s, h, c, r, out, float(beta))
-    return out
+    ab, abp = _abs_out(out, with_abs)
+    lib().orc_skinning(N, P, x, s, h, c, r, out, float(beta), abp)
+    return (out, ab) if with_abs else out
 
 
 # ------------------------------------------------------ card coverage (adjoint)

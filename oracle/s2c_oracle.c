@@ -196,11 +196,14 @@ void orc_dda_interpolate_attributes(int B, int H, int W, int C,
     }
 }
 
-/* dda.cu:300-327 (serial accumulation in pixel order). */
+/* dda.cu:300-327 (serial accumulation in pixel order).  `absum` (may be
+ * NULL) receives each d_attr element's sum of |term|, the scale of the
+ * rounding of any other summation order (the tests' elementwise bound). */
 void orc_backward_dda_interpolate_attributes(int B, int H, int W,
                                              const float *rast, int C,
                                              const float *d_image,
-                                             const int *edges, float *d_attr) {
+                                             const int *edges, float *d_attr,
+                                             float *absum) {
     const long long n = (long long)B * H * W;
     for (long long i = 0; i < n; ++i) {
         const float t1 = rast[i * 4 + 1];
@@ -210,8 +213,13 @@ void orc_backward_dda_interpolate_attributes(int B, int H, int W,
         float *d0 = d_attr + (size_t)edges[e * 2 + 0] * C;
         float *d1 = d_attr + (size_t)edges[e * 2 + 1] * C;
         for (int c = 0; c < C; ++c) {
-            d0[c] += d_image[i * C + c] * t0;
-            d1[c] += d_image[i * C + c] * t1;
+            const float g0 = d_image[i * C + c] * t0, g1 = d_image[i * C + c] * t1;
+            d0[c] += g0;
+            d1[c] += g1;
+            if (absum) {
+                absum[(size_t)edges[e * 2 + 0] * C + c] += fabsf(g0);
+                absum[(size_t)edges[e * 2 + 1] * C + c] += fabsf(g1);
+            }
         }
     }
 }
@@ -365,11 +373,13 @@ void orc_interpolate_triangle_attributes(int B, int H, int W, int C,
     }
 }
 
-/* rasterize.cu:402-438 (serial accumulation in pixel order). */
+/* rasterize.cu:402-438 (serial accumulation in pixel order); `absum` as in
+ * orc_backward_dda_interpolate_attributes. */
 void orc_backward_interpolate_triangle_attributes(int B, int H, int W, int C,
                                                   const float *rast,
                                                   const float *d_image,
-                                                  const int *tri, float *d_attr) {
+                                                  const int *tri, float *d_attr,
+                                                  float *absum) {
     const long long n = (long long)B * H * W;
     for (long long i = 0; i < n; ++i) {
         const float b0 = rast[i * 4 + 0], b1 = rast[i * 4 + 1];
@@ -382,6 +392,11 @@ void orc_backward_interpolate_triangle_attributes(int B, int H, int W, int C,
         for (int c = 0; c < C; ++c) {
             const float g = d_image[i * C + c];
             g0[c] += g * b0; g1[c] += g * b1; g2[c] += g * b2;
+            if (absum) {
+                absum[(size_t)tri[t * 3 + 0] * C + c] += fabsf(g * b0);
+                absum[(size_t)tri[t * 3 + 1] * C + c] += fabsf(g * b1);
+                absum[(size_t)tri[t * 3 + 2] * C + c] += fabsf(g * b2);
+            }
         }
     }
 }
@@ -540,7 +555,7 @@ void orc_backward_antialiased_cluster_bitset(
     int N, const int *frag_pix, const float *fa, int H, int W, int S,
     const int *bitset, const int *target, int V, const float *pos, int E,
     const int *edges, const float *normals, const int *edge2cluster,
-    float *d_pos, float R, float rho) {
+    float *d_pos, float R, float rho, float *absum) {
     const float fH = (float)H, fW = (float)W;
     for (int i = 0; i < N; ++i) {
         const int b = frag_pix[i * 3 + 0];
@@ -592,6 +607,16 @@ void orc_backward_antialiased_cluster_bitset(
         if (isfinite(dpx) && isfinite(dpy) && isfinite(dpw)) {
             dp[v0 * 4 + 0] += dpx * t0; dp[v0 * 4 + 1] += dpy * t0; dp[v0 * 4 + 3] += dpw * t0;
             dp[v1 * 4 + 0] += dpx * t1; dp[v1 * 4 + 1] += dpy * t1; dp[v1 * 4 + 3] += dpw * t1;
+            if (absum) {
+                /* dpw's own numerator cancels: its rounding scales with
+                 * |dx px| + |dy py|, not with |dpw| */
+                const float mw = (fabsf(dx * px) + fabsf(dy * py)) / (pw * pw);
+                float *ab = absum + (size_t)b * V * 4;
+                ab[v0 * 4 + 0] += fabsf(dpx * t0); ab[v0 * 4 + 1] += fabsf(dpy * t0);
+                ab[v0 * 4 + 3] += mw * t0;
+                ab[v1 * 4 + 0] += fabsf(dpx * t1); ab[v1 * 4 + 1] += fabsf(dpy * t1);
+                ab[v1 * 4 + 3] += mw * t1;
+            }
         }
     }
 }
@@ -603,7 +628,7 @@ void orc_backward_antialiased_cluster_bitset(
 /* curve_utils.cu:24-68.  Output entries with total weight <= 1e-6 are left
  * as the caller filled them (zeros, curve_utils.py:203). */
 void orc_smooth_strands(int N, int P, const float *x, const float *s,
-                        float *out, float beta) {
+                        float *out, float beta, float *absum) {
     const float inv_b2 = 1.0f / (beta * beta);
     const float cut = 3.0f * beta;
     for (int n = 0; n < N; ++n) {
@@ -611,16 +636,24 @@ void orc_smooth_strands(int N, int P, const float *x, const float *s,
         const float *ss = s + (size_t)n * P;
         for (int i = 0; i < P; ++i) {
             float a0 = 0.f, a1 = 0.f, a2 = 0.f, sw = 0.f;
+            double m0 = 0., m1 = 0., m2 = 0.;  /* sums of |w x| for `absum` */
             for (int j = 0; j < P; ++j) {
                 const float dist = fabsf(ss[i] - ss[j]);
                 if (dist > cut) continue;
                 const float w = expf(-(dist * dist) * inv_b2);
                 a0 += w * xs[j * 3 + 0]; a1 += w * xs[j * 3 + 1]; a2 += w * xs[j * 3 + 2];
                 sw += w;
+                m0 += fabs((double)w * xs[j * 3 + 0]);
+                m1 += fabs((double)w * xs[j * 3 + 1]);
+                m2 += fabs((double)w * xs[j * 3 + 2]);
             }
             if (sw > 1e-6f) {
                 float *o = out + ((size_t)n * P + i) * 3;
                 o[0] = a0 / sw; o[1] = a1 / sw; o[2] = a2 / sw;
+                if (absum) {
+                    float *ab = absum + ((size_t)n * P + i) * 3;
+                    ab[0] = (float)(m0 / sw); ab[1] = (float)(m1 / sw); ab[2] = (float)(m2 / sw);
+                }
             }
         }
     }
@@ -629,7 +662,7 @@ void orc_smooth_strands(int N, int P, const float *x, const float *s,
 /* curve_utils.cu:70-131. */
 void orc_skinning(int N, int P, const float *x, const float *s,
                   const float *hnd, const float *can, const float *R,
-                  float *out, float beta) {
+                  float *out, float beta, float *absum) {
     const float inv_b2 = 1.0f / (beta * beta);
     const float cut = 3.0f * beta;
     for (int n = 0; n < N; ++n) {
@@ -637,6 +670,7 @@ void orc_skinning(int N, int P, const float *x, const float *s,
         for (int i = 0; i < P; ++i) {
             const float si = s[(size_t)n * P + i];
             float a0 = 0.f, a1 = 0.f, a2 = 0.f, sw = 0.f;
+            double m[3] = {0., 0., 0.};  /* weighted sums of |term| for `absum` */
             const float *xi = x + o3 + (size_t)i * 3;
             for (int j = 0; j < P; ++j) {
                 const float dist = fabsf(si - s[(size_t)n * P + j]);
@@ -651,10 +685,20 @@ void orc_skinning(int N, int P, const float *x, const float *s,
                 const float r2 = Rj[6] * l0 + Rj[7] * l1 + Rj[8] * l2;
                 a0 += w * (r0 + cj[0]); a1 += w * (r1 + cj[1]); a2 += w * (r2 + cj[2]);
                 sw += w;
+                for (int d = 0; d < 3; ++d)
+                    m[d] += (double)w * (fabs((double)Rj[3 * d] * l0) + fabs((double)Rj[3 * d + 1] * l1) +
+                                         fabs((double)Rj[3 * d + 2] * l2) + fabs((double)cj[d]) +
+                                         fabs((double)l0) + fabs((double)l1) + fabs((double)l2));
             }
             float *o = out + o3 + (size_t)i * 3;
-            if (sw > 1e-8f) { o[0] = a0 / sw; o[1] = a1 / sw; o[2] = a2 / sw; }
-            else { o[0] = xi[0]; o[1] = xi[1]; o[2] = xi[2]; }
+            float *ab = absum ? absum + o3 + (size_t)i * 3 : NULL;
+            if (sw > 1e-8f) {
+                o[0] = a0 / sw; o[1] = a1 / sw; o[2] = a2 / sw;
+                if (ab) { ab[0] = (float)(m[0] / sw); ab[1] = (float)(m[1] / sw); ab[2] = (float)(m[2] / sw); }
+            } else {
+                o[0] = xi[0]; o[1] = xi[1]; o[2] = xi[2];
+                if (ab) ab[0] = ab[1] = ab[2] = 0.f;
+            }
         }
     }
 }

@@ -51,3 +51,32 @@ def dev():
 def report(msg: str) -> None:
     """Progress line that names the case before a GPU step."""
     print(msg, flush=True)
+
+
+# fp32 unit roundoff
+U32 = 2.0 ** -24
+
+
+def elementwise_close(got, want, absum, rtol=1e-5, k=64, what=""):
+    """Asserts, for every element, |got - want| <= rtol * |want| + k * u *
+    absum, with u the fp32 unit roundoff and absum the element's sum of
+    |term| (from the oracle): the relative bar, plus the rounding that a
+    different summation order (float atomics) or an exponential differing in
+    its last ulp can add to a sum of those terms.  Prints the largest excess
+    over the relative bar in units of u * absum."""
+    import numpy as np
+
+    got = np.asarray(got, np.float64)
+    want = np.asarray(want, np.float64)
+    absum = np.asarray(absum, np.float64)
+    assert got.shape == want.shape == absum.shape
+    err = np.abs(got - want)
+    tol = rtol * np.abs(want) + k * U32 * absum
+    excess = (err - rtol * np.abs(want)) / np.maximum(U32 * absum, 1e-300)
+    worst = float(excess.max(initial=0.0))
+    print(f"{what}: elementwise |err| <= {rtol:g}|want| + {k} u sum|term|; "
+          f"largest excess {worst:.3g} u sum|term| over {got.size} elements", flush=True)
+    bad = np.argwhere(err > tol)
+    assert bad.size == 0, (f"{what}: {len(bad)} elements outside the bound, first {bad[0].tolist()}: "
+                           f"got {got[tuple(bad[0])]!r} want {want[tuple(bad[0])]!r} "
+                           f"absum {absum[tuple(bad[0])]!r}")

@@ -540,11 +540,16 @@ def test_dda_interpolate_backward_hand_built_and_adjoint():
     g = np.zeros((1, 2, 2, 2), np.float32)
     g[0, 0, 0] = [1.0, -2.0]
     g[0, 1, 1] = [4.0, 8.0]
-    d = oracle.backward_dda_interpolate_attributes(rast, g, edges, 6).astype(np.float64)
+    d, ab = oracle.backward_dda_interpolate_attributes(rast, g, edges, 6, with_abs=True)
     want = np.zeros((6, 2))
     want[3] = 0.75 * np.array([1.0, -2.0]) + 0.5 * np.array([4.0, 8.0])
     want[5] = 0.25 * np.array([1.0, -2.0]) + 0.5 * np.array([4.0, 8.0])
-    np.testing.assert_allclose(d, want, rtol=1e-6, atol=0)
+    np.testing.assert_allclose(d.astype(np.float64), want, rtol=1e-6, atol=0)
+    # the sum of |term| each element carries (the scale of its rounding)
+    want_abs = np.zeros((6, 2))
+    want_abs[3] = 0.75 * np.array([1.0, 2.0]) + 0.5 * np.array([4.0, 8.0])
+    want_abs[5] = 0.25 * np.array([1.0, 2.0]) + 0.5 * np.array([4.0, 8.0])
+    np.testing.assert_allclose(ab.astype(np.float64), want_abs, rtol=1e-6, atol=0)
     # <interp(a), g> == <a, interp_bwd(g)> on a raster from the DDA
     rng = np.random.default_rng(12)
     H, W = 32, 32
@@ -650,3 +655,27 @@ def test_arclength_closed_form():
          np.cumsum(np.linalg.norm(np.diff(x.astype(np.float64), axis=1), axis=2), axis=1)], 1)
     np.testing.assert_allclose(got := oracle.compute_arclength(x), want, rtol=2e-6, atol=1e-6)
     assert got.dtype == np.float32 and (np.diff(got, axis=1) >= 0).all()
+
+
+def test_absum_bounds_the_value():
+    """Every with_abs output is at least the magnitude of its value and zero
+    where nothing contributes: smoothing (weights are positive), skinning
+    and the antialias gradient."""
+    x = np.random.default_rng(17).normal(size=(5, 12, 3)).astype(np.float32)
+    s = oracle.compute_arclength(x)
+    out, ab = oracle.smooth_strands(x, s, 0.3, with_abs=True)
+    assert (ab >= np.abs(out) * (1 - 1e-6)).all() and (ab > 0).all()
+    R = np.broadcast_to(np.eye(3, dtype=np.float32), (5, 12, 3, 3)).copy()
+    out, ab = oracle.skinning(x, s, x, x, R, 0.3, with_abs=True)
+    assert (ab >= np.abs(out) * (1 - 1e-6)).all()
+    H = W = 32
+    pos = clip([(8.3, 8.3), (24.6, 9.1), (14.2, 25.7)], H, W)
+    tri = np.array([[0, 1, 2]], np.int32)
+    edges, e2t = np.array([[0, 1], [1, 2], [0, 2]], np.int32), np.array([[0, -1]] * 3, np.int32)
+    e2c = np.zeros(3, np.int32)
+    cards = oracle.triangle_cluster_mask((H, W), pos, tri, np.zeros(1, np.int32), 1)
+    target = np.zeros_like(cards)
+    g, ab = oracle.antialiased_cluster_bitset_grad(cards, target, pos, tri, edges, e2t, e2c, 3.0,
+                                                   1.0, with_abs=True)
+    assert np.abs(g).max() > 0 and (ab >= np.abs(g) * (1 - 1e-6)).all()
+    assert (ab[..., 2] == 0).all()  # z carries no gradient
