@@ -265,7 +265,9 @@ int s2c_fit_targets(int num_jobs, const int* res, const int* bits_off,
 /* Runs `iters` fitting iterations for the jobs slots[0..num_slots) (a
  * negative entry is an idle slot and is skipped), then
  * renders the updated card and writes popcount(render XOR target) to
- * loss_out[job]; verts, params, Adam state and steps are updated in place.  Jobs
+ * loss_out[job]; verts, params, Adam state and steps are updated in place
+ * (with iters = 0 only loss_out is written, so measuring a job changes
+ * nothing).  Jobs
  * never share state, and each job's arithmetic order is fixed, so a job's
  * result does not depend on which other jobs run in the same call, on
  * their order in `slots`, or on `chunk`: the silhouette samples a

@@ -433,7 +433,10 @@ fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict
         loss_out[job] = static_cast<float>(total);
         steps[job] = step;
     }
-    if (tid < V) {
+    // iters == 0 only measures the current state: with large steps at step 0
+    // the render used M^{-1} (M pos), which differs from pos in the last
+    // bits, so nothing but the loss is written back
+    if (iters > 0 && tid < V) {
         for (int k = 0; k < 3; ++k) verts[(vo + tid) * 3 + k] = P[tid][k];
         if (ls)
             for (int k = 0; k < 3; ++k) params[(vo + tid) * 3 + k] = U[tid][k];

@@ -41,6 +41,8 @@ MAX_RES = 256
 MAX_QUADS = 32
 MAX_VERTS = 2 * (MAX_QUADS + 1)
 MAX_WORDS = MAX_RES * MAX_RES // 32
+# per-slot scalars (res, quads, offsets, steps, loss, slot ids) and the 4 x 4 clip matrix
+SLOT_SCALAR_BYTES = 8 * 4 + 16 * 4
 
 
 @dataclass
@@ -82,11 +84,28 @@ class FitJob:
     arrival: float = 0.0
 
     @property
+    def words(self) -> int:
+        """32-bit words of the packed res x res target mask."""
+        return (self.res * self.res + 31) // 32
+
+    @property
     def bytes(self) -> int:
-        """HBM footprint while resident: target mask, vertices, optimizer
-        state and strand vertices (for the target)."""
-        nv = self.verts.shape[0]
-        return (self.res * self.res // 8) + nv * 7 * 4 + self.strand_clip.nbytes + 64 + 16 * 4
+        """HBM footprint while resident: the target mask, the vertices, the
+        large-steps parameter and the Adam moments (40 B per vertex), and
+        the slot's per-job scalars and clip matrix.  The strand vertices are
+        uploaded for the target computation at admission and released."""
+        return 4 * self.words + 40 * self.verts.shape[0] + SLOT_SCALAR_BYTES
+
+    def validate(self) -> None:
+        """Raises ValueError for a job the fitting kernel cannot hold."""
+        if not (1 <= self.res <= MAX_RES):
+            raise ValueError(f"job {self.jid}: resolution {self.res} outside [1, {MAX_RES}]")
+        if not (1 <= self.quads <= MAX_QUADS):
+            raise ValueError(f"job {self.jid}: {self.quads} quads outside [1, {MAX_QUADS}]")
+        if self.verts.shape != (2 * (self.quads + 1), 3):
+            raise ValueError(f"job {self.jid}: the strip needs {2 * (self.quads + 1)} vertices")
+        if self.strands.ndim != 3 or self.strands.shape[1] < 2:
+            raise ValueError(f"job {self.jid}: strands must be (n, nv >= 2, 3)")
 
 
 def make_jobs(count: int, seed: int = 0, nv: int = 64, cap_angle: float = 0.15) -> list[FitJob]:
@@ -243,34 +262,91 @@ class SlotState:
     done_iters: int = 0
 
 
+class WordPool:
+    """Allocator of target words in one device buffer: blocks have
+    power-of-two sizes, a freed block returns to the free list of its size,
+    and a bump pointer serves a size with no free block.  ``alloc`` returns
+    None when the buffer cannot hold the block."""
+
+    def __init__(self, words: int):
+        self.words = int(words)
+        self.top = 0
+        self.free: dict[int, list[int]] = {}
+
+    @staticmethod
+    def block(words: int) -> int:
+        return 1 << max(0, int(words - 1).bit_length())
+
+    def can_alloc(self, words: int) -> bool:
+        b = self.block(words)
+        return bool(self.free.get(b)) or self.top + b <= self.words
+
+    def alloc(self, words: int):
+        b = self.block(words)
+        if self.free.get(b):
+            return self.free[b].pop()
+        if self.top + b > self.words:
+            return None
+        off = self.top
+        self.top += b
+        return off
+
+    def release(self, off: int, words: int) -> None:
+        self.free.setdefault(self.block(words), []).append(off)
+
+
+def _upload(a, dev, dtype=None) -> torch.Tensor:
+    """Host array -> device tensor through pinned memory, queued on the
+    current stream without waiting for the work already queued on it."""
+    t = torch.from_numpy(np.ascontiguousarray(a if dtype is None else np.asarray(a, dtype)))
+    return t.pin_memory().to(dev, non_blocking=True)
+
+
 class CardFitServer:
-    """Device-resident slot table for fitting jobs.  Slot s owns vertices
-    [s*MAX_VERTS, ...) and target words [s*MAX_WORDS, ...); ``run(iters)``
-    advances every active slot with one ``fit_cards`` launch whose grid is
-    the whole table (idle slots exit at once), so the launch can be captured
-    once and replayed while jobs come and go."""
+    """Device-resident state of fitting jobs, admitted by HBM budget.
+
+    Slot s (one workgroup of the ``fit_cards`` launch) holds a job's per-job
+    scalars, clip matrix and its vertex state at rows [s*MAX_VERTS, ...);
+    the job's target mask takes a block of a word pool sized from the
+    budget.  The budget pays for the slot tables first; what is left (up
+    to what ``capacity`` masks at the largest resolution need) is the pool.
+    A job is admitted when a slot is free and the pool can hold its mask;
+    otherwise it waits.  ``run(iters)`` advances every active slot
+    with one ``fit_cards`` launch whose grid is the whole table (idle slots
+    exit at once), so the launch can be captured once and replayed while
+    jobs come and go."""
 
     def __init__(self, capacity: int, hyper: FitHyper, dev, budget_bytes: float = 200e9,
                  chunk: int = 0, order: str = "admission"):
-        """``chunk`` (silhouette samples per workgroup pass: 0 = 512, 256,
+        """``capacity``: the most jobs resident at once (the launch grid).
+        ``budget_bytes``: HBM for the jobs' resident state; the slot tables
+        take their part first, and the target pool gets the rest, up to
+        what ``capacity`` jobs at the largest resolution would need.
+        ``chunk`` (silhouette samples per workgroup pass: 0 = 512, 256,
         512, 1024) and ``order`` ("admission": slot order; "cost": jobs
         with the most work first) are launch knobs chosen by
         ``autotune.tune_fit``; no job's result depends on them."""
         if order not in ("admission", "cost"):
             raise ValueError("order must be 'admission' or 'cost'")
+        table = capacity * (40 * MAX_VERTS + SLOT_SCALAR_BYTES)
+        if budget_bytes < table + 4 * MAX_WORDS:
+            raise ValueError(f"budget of {budget_bytes:.0f} B cannot hold {capacity} slots "
+                             f"({table} B) and one {MAX_RES}^2 mask")
         self.cap = capacity
         self.chunk = int(chunk)
         self.order = order
         self.hyper = hyper
         self.dev = dev
-        self.budget = budget_bytes
+        self.budget = float(budget_bytes)
         self.used_bytes = 0
+        pool_words = int(min((budget_bytes - table) // 4, capacity * MAX_WORDS))
+        self.pool = WordPool(pool_words)
         i32 = dict(dtype=torch.int32, device=dev)
         f32 = dict(dtype=torch.float32, device=dev)
         self.res = torch.zeros(capacity, **i32)
         self.quads = torch.ones(capacity, **i32)
         self.vert_off = torch.arange(capacity, **i32) * MAX_VERTS
-        self.bits_off = torch.arange(capacity, **i32) * MAX_WORDS
+        self.bits_off = torch.zeros(capacity, **i32)
         self.mvp = torch.zeros((capacity, 4, 4), **f32)
         self.steps = torch.zeros(capacity, **i32)
         self.loss = torch.zeros(capacity, **f32)
@@ -278,50 +354,80 @@ class CardFitServer:
         self.params = torch.zeros((capacity * MAX_VERTS, 3), **f32)
         self.adam_m = torch.zeros((capacity * MAX_VERTS, 3), **f32)
         self.adam_v = torch.zeros(capacity * MAX_VERTS, **f32)
-        self.target = torch.zeros(capacity * MAX_WORDS, **i32)
+        self.target = torch.zeros(max(pool_words, 1), **i32)
         self.slots = torch.full((capacity,), -1, **i32)
         self.launch = torch.full((capacity,), -1, **i32)  # workgroup -> slot ("cost" order)
         self.free = deque(range(capacity))
         self.state: dict[int, SlotState] = {}
+        self.word_off: dict[int, int] = {}
         self.graph = None
         self.graph_iters = None
 
     # -- admission -----------------------------------------------------
-    def can_admit(self, job: FitJob) -> bool:
-        return bool(self.free) and self.used_bytes + job.bytes <= self.budget
+    def can_admit(self, job: FitJob, pending_words=()) -> bool:
+        """True when `job` fits now: a free slot and room in the target pool
+        (beyond the masks in `pending_words`, promised to jobs not yet
+        admitted).  Raises ValueError for a job the kernel cannot hold."""
+        job.validate()
+        if len(self.free) <= len(pending_words):
+            return False
+        if not pending_words:
+            return self.pool.can_alloc(job.words)
+        trial = WordPool(self.pool.words)
+        trial.top, trial.free = self.pool.top, {k: list(v) for k, v in self.pool.free.items()}
+        for w in pending_words:
+            if trial.alloc(w) is None:
+                return False
+        return trial.can_alloc(job.words)
 
     def admit(self, jobs: list[FitJob], now: float = 0.0) -> list[int]:
         """Uploads the jobs into free slots, computes their target masks on
         the GPU and activates the slots (all asynchronous on the current
-        stream)."""
+        stream).  Raises ValueError if one does not fit (``can_admit``)."""
         if not jobs:
             return []
+        for j in jobs:
+            j.validate()
+        if len(jobs) > len(self.free):
+            raise ValueError(f"{len(jobs)} jobs for {len(self.free)} free slots")
+        offs_w = []
+        for j in jobs:
+            o = self.pool.alloc(j.words)
+            if o is None:
+                for k, jj in zip(offs_w, jobs):
+                    self.pool.release(k, jj.words)
+                raise ValueError(f"job {j.jid} does not fit the HBM budget")
+            offs_w.append(o)
+        self.used_bytes += sum(j.bytes for j in jobs)
         slots = [self.free.popleft() for _ in jobs]
         dev = self.dev
-        idx = torch.tensor(slots, dtype=torch.long, device=dev)
-        self.res[idx] = torch.tensor([j.res for j in jobs], dtype=torch.int32, device=dev)
-        self.quads[idx] = torch.tensor([j.quads for j in jobs], dtype=torch.int32, device=dev)
-        self.mvp[idx] = torch.from_numpy(np.stack([j.mvp for j in jobs])).to(dev)
+        idx = _upload(slots, dev, np.int64)
+        res_t = _upload([j.res for j in jobs], dev, np.int32)
+        offw_t = _upload(offs_w, dev, np.int32)
+        self.res[idx] = res_t
+        self.quads[idx] = _upload([j.quads for j in jobs], dev, np.int32)
+        self.bits_off[idx] = offw_t
+        self.mvp[idx] = _upload(np.stack([j.mvp for j in jobs]), dev, np.float32)
         self.steps[idx] = 0
         vrows = np.concatenate([s * MAX_VERTS + np.arange(j.verts.shape[0]) for s, j in zip(slots, jobs)])
-        vidx = torch.from_numpy(vrows).to(dev)
-        self.verts[vidx] = torch.from_numpy(np.concatenate([j.verts for j in jobs])).to(dev)
+        vidx = _upload(vrows, dev, np.int64)
+        self.verts[vidx] = _upload(np.concatenate([j.verts for j in jobs]), dev, np.float32)
         self.adam_m[vidx] = 0.0
         self.adam_v[vidx] = 0.0
-        # target masks
+        # target masks: every strand edge DDA-rasterized into the job's block
         nv = jobs[0].strands.shape[1]
+        if any(j.strands.shape[1] != nv for j in jobs):
+            raise ValueError("jobs admitted together need the same points per strand")
         clip = np.concatenate([j.strand_clip for j in jobs]).reshape(-1, nv, 4)
         counts = [j.strands.shape[0] for j in jobs]
         offs = np.concatenate([[0], np.cumsum(counts)[:-1]]) * nv
-        _core.fit_targets(torch.tensor([j.res for j in jobs], dtype=torch.int32, device=dev),
-                          torch.tensor([s * MAX_WORDS for s in slots], dtype=torch.int32, device=dev),
-                          torch.from_numpy(offs.astype(np.int32)).to(dev),
-                          torch.tensor(counts, dtype=torch.int32, device=dev),
-                          torch.from_numpy(clip).to(dev), self.target)
+        _core.fit_targets(res_t, offw_t, _upload(offs, dev, np.int32),
+                          _upload(counts, dev, np.int32), _upload(clip, dev, np.float32),
+                          self.target)
         self.slots[idx] = idx.to(torch.int32)
-        for s, j in zip(slots, jobs):
+        for s, j, o in zip(slots, jobs, offs_w):
             self.state[s] = SlotState(j, now)
-            self.used_bytes += j.bytes
+            self.word_off[s] = o
         self._refresh_launch()
         return slots
 
@@ -337,14 +443,18 @@ class CardFitServer:
         act = sorted(self.state, key=lambda s: -self.job_cost(self.state[s].job))
         arr = np.full(self.cap, -1, np.int32)
         arr[: len(act)] = act
-        self.launch.copy_(torch.from_numpy(arr))
+        self.launch.copy_(_upload(arr, self.dev))
 
     def release(self, slots: list[int]) -> None:
+        """Frees the slots and their budget (stream-ordered: launches
+        already queued still see them)."""
         if not slots:
             return
-        self.slots[torch.tensor(slots, dtype=torch.long, device=self.dev)] = -1
+        self.slots[_upload(slots, self.dev, np.int64)] = -1
         for s in slots:
-            self.used_bytes -= self.state.pop(s).job.bytes
+            job = self.state.pop(s).job
+            self.used_bytes -= job.bytes
+            self.pool.release(self.word_off.pop(s), job.words)
             self.free.append(s)
         self._refresh_launch()
 
@@ -366,7 +476,7 @@ class CardFitServer:
                 s = torch.cuda.Stream()
                 s.wait_stream(torch.cuda.current_stream())
                 with torch.cuda.stream(s):
-                    self._launch(0)  # warm-up outside capture (no state change)
+                    self._launch(0)  # warm-up outside capture (iters = 0 writes only the loss)
                 torch.cuda.current_stream().wait_stream(s)
                 g = torch.cuda.CUDAGraph()
                 with torch.cuda.graph(g):
@@ -382,8 +492,9 @@ class CardFitServer:
 
     def target_mask(self, slot: int) -> np.ndarray:
         """(res, res) bool target of a slot, unpacked from its bit words."""
-        res = self.state[slot].job.res
-        words = self.target[slot * MAX_WORDS: slot * MAX_WORDS + res * res // 32].cpu().numpy()
+        job = self.state[slot].job
+        res, o = job.res, self.word_off[slot]
+        words = self.target[o: o + job.words].cpu().numpy()
         bits = np.unpackbits(words.view(np.uint8), bitorder="little")
         return bits[: res * res].reshape(res, res).astype(bool)
 
@@ -395,7 +506,7 @@ class ServeReport:
     seconds: float
     latencies_ms: list[float] = field(default_factory=list)
     losses: dict[int, float] = field(default_factory=dict)
-    target_pixels: dict[int, int] = field(default_factory=dict)
+    vertices: dict[int, np.ndarray] = field(default_factory=dict)
 
     @property
     def iters_per_s(self) -> float:
@@ -406,41 +517,77 @@ class ServeReport:
 
 
 def serve(jobs: list[FitJob], server: CardFitServer, chunk: int = 10, graph: bool = True,
-          clock=time.perf_counter) -> ServeReport:
+          clock=time.perf_counter, keep_vertices: bool = False, ahead: int = 2) -> ServeReport:
     """Continuous batching: jobs (with `arrival` seconds) are admitted as
-    slots and budget free up, all resident jobs advance together `chunk`
-    iterations per replay, finished jobs are retired and their latency
-    (arrival -> result on the host) recorded."""
+    slots and budget free up, and all resident jobs advance together
+    `chunk` iterations per replay.  The host does not wait for a replay
+    before queueing the next (at most `ahead` are queued beyond the one
+    running): the losses (and, with `keep_vertices`, the vertices) of the
+    jobs a replay finishes are copied to pinned host memory behind it, and
+    are collected, with the job's latency (arrival -> result on the host),
+    once its event has completed, while arrivals are admitted meanwhile."""
     total = server.hyper.iters
     if total % chunk:
         raise ValueError("chunk must divide the per-job iteration count")
+    dev = server.dev
     pending = deque(sorted(jobs, key=lambda j: j.arrival))
     rep = ServeReport(0, 0, 0.0)
+    results = deque()  # (event, [(jid, arrival, n_verts)], pinned loss, pinned vertices)
+    queued = deque()   # one event per queued replay
+    stream = torch.cuda.current_stream(dev)
     t0 = clock()
-    while pending or server.state:
+
+    def collect(wait: bool) -> None:
+        while results and (wait or results[0][0].query()):
+            ev, done, lh, vh = results.popleft()
+            ev.synchronize()
+            t_done = clock() - t0
+            for k, (jid, arrival, n) in enumerate(done):
+                rep.latencies_ms.append((t_done - arrival) * 1e3)
+                rep.losses[jid] = float(lh[k])
+                if vh is not None:
+                    rep.vertices[jid] = vh[k, :n].numpy().copy()
+                rep.completed += 1
+                rep.iterations += total
+            wait = False
+
+    while pending or server.state or results:
         now = clock() - t0
-        batch = []
-        while pending and pending[0].arrival <= now and server.can_admit(pending[0]):
-            batch.append(pending.popleft())
+        batch, pw = [], []
+        while pending and pending[0].arrival <= now and server.can_admit(pending[0], pw):
+            j = pending.popleft()
+            batch.append(j)
+            pw.append(j.words)
         if batch:
             server.admit(batch, now)
         if not server.state:
-            if pending:
+            if results:
+                collect(wait=True)
+            elif pending:
                 time.sleep(max(0.0, pending[0].arrival - (clock() - t0)))
             continue
         server.run(chunk, graph=graph)
-        torch.cuda.current_stream().synchronize()
-        done = [s for s, st in server.state.items() if st.done_iters >= total]
+        ev = torch.cuda.Event()
+        ev.record(stream)
+        queued.append(ev)
+        done = [sl for sl, st in server.state.items() if st.done_iters >= total]
         if done:
-            t_done = clock() - t0
-            loss = server.loss.cpu().numpy()
-            for s in done:
-                st = server.state[s]
-                rep.latencies_ms.append((t_done - st.job.arrival) * 1e3)
-                rep.losses[st.job.jid] = float(loss[s])
-                rep.completed += 1
-                rep.iterations += total
+            idx = _upload(done, dev, np.int64)
+            lh = torch.empty(len(done), dtype=torch.float32, pin_memory=True)
+            lh.copy_(server.loss[idx], non_blocking=True)
+            vh = None
+            if keep_vertices:
+                rows = (idx[:, None] * MAX_VERTS + torch.arange(MAX_VERTS, device=dev)).reshape(-1)
+                vh = torch.empty((len(done), MAX_VERTS, 3), dtype=torch.float32, pin_memory=True)
+                vh.copy_(server.verts[rows].reshape(len(done), MAX_VERTS, 3), non_blocking=True)
+            ev_done = torch.cuda.Event()
+            ev_done.record(stream)
+            results.append((ev_done, [(server.state[sl].job.jid, server.state[sl].job.arrival,
+                                       server.state[sl].job.verts.shape[0]) for sl in done], lh, vh))
             server.release(done)
+        while len(queued) > ahead:
+            queued.popleft().synchronize()
+        collect(wait=False)
     rep.seconds = clock() - t0
     return rep
 

@@ -64,6 +64,27 @@ def test_alone_and_mixed_are_identical(dev, jobs):
     assert a.loss[sa].item() == b.loss[sb].item()
 
 
+def test_measuring_changes_nothing(dev, jobs):
+    """run(0) (also the warm-up before a graph capture) writes only the loss:
+    a job measured first, then run in graph-replayed chunks, ends
+    bit-identical to the same job run in one launch."""
+    report("fit: run(0) and graph replays leave the trajectory unchanged")
+    hyper = fitting.FitHyper(iters=40)
+    a = fitting.CardFitServer(2, hyper, dev)
+    sa = a.admit([jobs[2]])[0]
+    a.run(40)
+    b = fitting.CardFitServer(2, hyper, dev)
+    sb = b.admit([jobs[2]])[0]
+    b.run(0)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(b.job_vertices(sb), jobs[2].verts)
+    for _ in range(4):
+        b.run(10, graph=True)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(a.job_vertices(sa), b.job_vertices(sb))
+    assert a.loss[sa].item() == b.loss[sb].item()
+
+
 def test_final_loss_matches_plain_loop(dev, jobs):
     """100 iterations: the coverage loss (XOR pixels / target pixels) of the
     served fit is within 1% of the plain loop's, and fitting reduces it."""
@@ -88,20 +109,117 @@ def test_final_loss_matches_plain_loop(dev, jobs):
     assert rel <= 0.01, (served, plain, tpix)
 
 
-def test_serve_continuous_batching(dev):
-    report("fit: serve() with Poisson arrivals")
-    hyper = fitting.FitHyper(iters=20)
-    js = fitting.poisson_arrivals(fitting.make_jobs(24, seed=9), rate=400.0, seed=1)
-    srv = fitting.CardFitServer(8, hyper, dev)
-    rep = fitting.serve(js, srv, chunk=10)
-    assert rep.completed == 24 and rep.iterations == 24 * 20
-    assert len(rep.latencies_ms) == 24 and min(rep.latencies_ms) > 0
-    # each served result equals the job run alone
-    alone = fitting.CardFitServer(2, hyper, dev)
-    s = alone.admit([js[5]])[0]
-    alone.run(20)
+def run_alone(job, hyper, dev):
+    """The job by itself: (final vertices, final loss)."""
+    srv = fitting.CardFitServer(1, hyper, dev)
+    s = srv.admit([job])[0]
+    srv.run(hyper.iters)
     torch.cuda.synchronize()
-    assert alone.loss[s].item() == rep.losses[js[5].jid]
+    return srv.job_vertices(s), srv.loss[s].item()
+
+
+def stratified(jobs, per_class):
+    """Up to `per_class` jobs of every (resolution, quads) class."""
+    seen, out = {}, []
+    for j in jobs:
+        k = (j.res, j.quads)
+        if seen.get(k, 0) < per_class:
+            seen[k] = seen.get(k, 0) + 1
+            out.append(j)
+    return out
+
+
+@pytest.mark.parametrize("lam", [200.0, 1000.0, 5000.0])
+def test_serve_poisson_rates(dev, lam):
+    """Config 4: Poisson arrivals at lambda jobs/s, 100 iterations per job,
+    continuous batching with graph replays and no per-chunk host wait;
+    every completed job's vertices and loss equal the job run alone."""
+    report(f"fit: serve() at lambda={lam:.0f}/s, 100 iterations per job")
+    hyper = fitting.FitHyper(iters=100)
+    js = fitting.poisson_arrivals(fitting.make_jobs(160, seed=int(lam) + 3), rate=lam, seed=2)
+    srv = fitting.CardFitServer(256, hyper, dev)
+    rep = fitting.serve(js, srv, chunk=10, keep_vertices=True)
+    assert rep.completed == len(js) and rep.iterations == len(js) * 100
+    assert len(rep.latencies_ms) == len(js) and min(rep.latencies_ms) > 0
+    for j in js:
+        v, loss = run_alone(j, hyper, dev)
+        np.testing.assert_array_equal(rep.vertices[j.jid], v)
+        assert rep.losses[j.jid] == loss, j.jid
+    report(f"  p50 {rep.percentile(50):.2f} ms, p99 {rep.percentile(99):.2f} ms, "
+           f"{rep.iters_per_s:.0f} it/s")
+
+
+def test_serve_waits_on_the_budget(dev):
+    """A budget with room for three 256^2 masks: arriving jobs wait for the
+    pool, every job completes, and results equal the jobs run alone."""
+    report("fit: serve() under a tight HBM budget")
+    hyper = fitting.FitHyper(iters=20)
+    js = fitting.make_jobs(40, seed=11)
+    for j in js:
+        j.arrival = 0.0  # all at once: admission is bounded by the pool alone
+    cap = 32
+    table = cap * (40 * fitting.MAX_VERTS + fitting.SLOT_SCALAR_BYTES)
+    srv = fitting.CardFitServer(cap, hyper, dev, budget_bytes=table + 3 * 4 * fitting.MAX_WORDS)
+    first = [j for j in js if srv.can_admit(j, [])]
+    assert len(first) == len(js)  # each fits on its own ...
+    pw, taken = [], 0
+    for j in js:  # ... but not all at once
+        if srv.can_admit(j, pw):
+            pw.append(j.words)
+            taken += 1
+    assert taken < len(js)
+    rep = fitting.serve(js, srv, chunk=10, keep_vertices=True)
+    assert rep.completed == len(js)
+    assert srv.pool.top <= srv.pool.words and not srv.state and srv.used_bytes == 0
+    for j in js[:8]:
+        v, loss = run_alone(j, hyper, dev)
+        np.testing.assert_array_equal(rep.vertices[j.jid], v)
+        assert rep.losses[j.jid] == loss
+
+
+@pytest.mark.slow
+def test_batched_4096_jobs(dev):
+    """Config 3: 4,096 jobs in one persistent launch of 100 iterations.
+    Every target mask is bit-exact with the oracle's DDA mask; 32 jobs
+    spread over every (resolution, quads) class end bit-identical to the
+    same job run alone; and on 24 such jobs the coverage loss (XOR pixels /
+    target pixels, summed) is within 1% of the plain per-job loop's."""
+    from strns2cards_amd import synth
+
+    report("fit: 4,096 jobs x 100 iterations in one launch (config 3)")
+    hyper = fitting.FitHyper()
+    jobs = fitting.make_jobs(4096, seed=0)
+    srv = fitting.CardFitServer(4096, hyper, dev)
+    slots = srv.admit(jobs)
+    srv.run(0)
+    torch.cuda.synchronize()
+    init = srv.loss.cpu().numpy()[slots]
+    for s, j in zip(slots, jobs):
+        n, nv = j.strands.shape[:2]
+        want = oracle.dda_cluster_mask((j.res, j.res), j.strand_clip[None],
+                                       synth.strand_edges(n, nv),
+                                       np.zeros(n * (nv - 1), np.int32), 1)[0, :, :, 0] != 0
+        np.testing.assert_array_equal(srv.target_mask(s), want, err_msg=f"job {j.jid}")
+    srv.run(hyper.iters)
+    torch.cuda.synchronize()
+    final = srv.loss.cpu().numpy()[slots]
+    assert final.sum() < init.sum()
+    strat = stratified(jobs, 3)
+    assert len({(j.res, j.quads) for j in strat}) == 15
+    for j in strat[:32]:
+        v, loss = run_alone(j, hyper, dev)
+        np.testing.assert_array_equal(srv.job_vertices(slots[j.jid]), v, err_msg=f"job {j.jid}")
+        assert final[j.jid] == loss
+    report("  plain per-job loop on 24 stratified jobs")
+    plain, served, tpix = 0, 0.0, 0
+    for j in stratified(jobs, 2)[:24]:
+        _, xor, tp = fitting.plain_fit(j, hyper, dev)
+        plain += xor
+        tpix += tp
+        served += float(final[j.jid])
+    rel = abs(served - plain) / max(plain, 1)
+    report(f"  coverage loss served {served / tpix:.5f} plain {plain / tpix:.5f} (rel diff {rel:.4f})")
+    assert rel <= 0.01, (served, plain, tpix)
 
 
 def test_joint_fit_reduces_the_mask_error(dev):
