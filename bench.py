@@ -13,7 +13,9 @@ B*V*16 + E*8 + E*4 + B*H*W*S*4 (vertex reads per view, edges and
 edge->cluster once, bitset written once) per step.  The card raster -> mask
 pass and the antialias edge gradient at the same size are timed in the same
 run and reported as extra fields.  ``cpu_baseline`` times the CPU oracle
-(a C restatement of the reference kernels, OpenMP) on one of the 4 views.
+(a C restatement of the reference kernels, OpenMP) on the same step, all 4
+views, at 1 thread and at the host's cores (capped at 16); ``card_fit``
+carries its own CPU baseline (the oracle's per-job fitting loop).
 """
 import argparse
 import json
@@ -45,35 +47,134 @@ def traffic_from_profiles():
         return None
 
 
+# Worker processes and OpenMP threads for the CPU legs: the host cores this
+# process may run on, capped at the 16 cores a one-GPU box gives a job.
+CPU_SHARE = 16
+
+
+def host_cores():
+    visible = len(os.sched_getaffinity(0))
+    return visible, max(1, min(CPU_SHARE, visible))
+
+
 def cpu_baseline(w, seconds):
-    """Oracle DDA->mask on view 0 of the workload, repeated for ~`seconds`."""
+    """Oracle DDA->mask over all 4 views of the workload: one timed step at 1
+    thread, and steps repeated for ~`seconds` on the host's cores."""
     import numpy as np
 
     import oracle
 
     oracle.lib()
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    pos0 = np.ascontiguousarray(w.pos[:1])
+    visible, threads = host_cores()
+    S = (w.num_clusters + 31) // 32
+    B, V = w.pos.shape[:2]
+    nbytes = dda_mask_bytes(B, V, w.edges.shape[0], w.H, w.W, S)
+    t0 = time.perf_counter()
+    oracle.dda_cluster_mask((w.H, w.W), w.pos, w.edges, w.edge2cluster, w.num_clusters, threads=1)
+    dt1 = time.perf_counter() - t0
     reps, t0 = 0, time.perf_counter()
     while True:
-        oracle.dda_cluster_mask((w.H, w.W), pos0, w.edges, w.edge2cluster, w.num_clusters,
+        oracle.dda_cluster_mask((w.H, w.W), w.pos, w.edges, w.edge2cluster, w.num_clusters,
                                 threads=threads)
         reps += 1
         if time.perf_counter() - t0 >= seconds:
             break
-    dt = (time.perf_counter() - t0) / reps
-    S = (w.num_clusters + 31) // 32
-    nbytes = dda_mask_bytes(1, pos0.shape[1], w.edges.shape[0], w.H, w.W, S)
-    return {"value": round(nbytes / dt / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"view 0 of 4 (all 50k strands, 1024^2, S=32), {reps} rep(s), "
-                      f"{dt * 1e3:.1f} ms/rep"}
+    dtn = (time.perf_counter() - t0) / reps
+    return {"value": round(nbytes / dtn / 1e9, 3), "unit": "GB/s", "cores": threads,
+            "kind": "port", "host_cores_visible": visible,
+            "value_1_thread": round(nbytes / dt1 / 1e9, 3),
+            "sample": f"the whole step (4 views, 50k strands, 1024^2, S=32): 1 rep at 1 thread "
+                      f"({dt1 * 1e3:.0f} ms), {reps} rep(s) at {threads} threads "
+                      f"({dtn * 1e3:.1f} ms/rep; OpenMP over views x edge ranges)"}
 
 
-def fit_section(dev, num_jobs=4096, plain_jobs=6, rates=(200.0, 1000.0, 5000.0), tune=True):
+def fit_job_dict(job):
+    """Host arrays of one fitting job for the CPU fitting loop."""
+    from oracle import cardfit
+
+    n, nv = job.strands.shape[:2]
+    return cardfit.job_dict(job.res, job.quads, job.strand_clip.reshape(n, nv, 4), job.mvp,
+                            job.verts)
+
+
+def fit_cpu_baseline(jobs, hyper, one_thread_jobs=320, pool_jobs=2048):
+    """Card fitting on the host: the oracle's per-job loop (oracle/cardfit.py,
+    the iteration of 3_fitting.py:381-407 on the C oracle and numpy), all
+    100 iterations per job, first in this process (1 thread) on
+    `one_thread_jobs` jobs, then one single-threaded process per core on
+    `pool_jobs` jobs (spawned children that only import the oracle)."""
+    import multiprocessing as mp
+
+    from oracle import cardfit
+
+    hyp = cardfit.hyper_dict(hyper)
+    visible, procs = host_cores()
+    it1, s1, _, _ = cardfit.fit_jobs(([fit_job_dict(j) for j in jobs[:one_thread_jobs]], hyp,
+                                      hyper.iters))
+    sel = [fit_job_dict(j) for j in jobs[:pool_jobs]]
+    parts = [(sel[i:i + 8], hyp, hyper.iters) for i in range(0, len(sel), 8)]  # balanced
+    env = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS")}
+    os.environ["OMP_NUM_THREADS"] = os.environ["OPENBLAS_NUM_THREADS"] = "1"
+    try:
+        with mp.get_context("spawn").Pool(procs) as pool:
+            pool.map(cardfit.fit_jobs, [([], hyp, 1)] * procs)  # start-up outside the timing
+            t0 = time.perf_counter()
+            res = pool.map(cardfit.fit_jobs, parts)
+            wall = time.perf_counter() - t0
+    finally:
+        for k, v in env.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    its = sum(r[0] for r in res)
+    return {"value": round(its / wall, 1), "unit": "card-fit iterations/s", "cores": procs,
+            "kind": "port", "host_cores_visible": visible,
+            "value_1_thread": round(it1 / s1, 1),
+            "sample": f"jobs 0..{one_thread_jobs - 1} x {hyper.iters} iterations in one process "
+                      f"({s1:.1f} s); jobs 0..{pool_jobs - 1} x {hyper.iters} iterations over "
+                      f"{procs} single-threaded processes ({wall:.1f} s wall), target masks "
+                      f"included, as in the plain loop"}
+
+
+def stratified(jobs, per_class):
+    """Up to `per_class` jobs of every (resolution, quads) class, in order."""
+    seen, out = {}, []
+    for j in jobs:
+        k = (j.res, j.quads)
+        if seen.get(k, 0) < per_class:
+            seen[k] = seen.get(k, 0) + 1
+            out.append(j)
+    return out
+
+
+def fit_roofline_profile():
+    """VALU instructions of one fit_cards_kernel launch (4,096 jobs x 100
+    iterations) from the committed rocprofv3 SQ counter summary (None when
+    absent)."""
+    p = os.path.join(ROOT, "profiles", "fit_cards_sq_counters.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+# Wave64 VALU issue peak of MI355X: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64
+# VALU instruction per SIMD every 4 cycles.
+VALU_PEAK_GWIS = 256 * 4 * 2.4 / 4
+
+
+def fit_section(dev, num_jobs=4096, plain_per_class=2, rates=(200.0, 1000.0, 5000.0), tune=True,
+                batched_only=False, cpu=True):
     """Card fitting: batched persistent launch, plain per-job loop, served
     continuous batching under Poisson arrivals (all 100 iterations per
-    job).  With tune, the launch shape is autotuned on the first 1024 jobs
-    first; otherwise the library defaults are used."""
+    job), and the CPU loop of the oracle.  With tune, the launch shape is
+    autotuned on the first 1024 jobs first; otherwise the library defaults
+    are used.  batched_only stops after the persistent launch (profiler
+    runs)."""
     import numpy as np
     import torch
 
@@ -101,60 +202,83 @@ def fit_section(dev, num_jobs=4096, plain_jobs=6, rates=(200.0, 1000.0, 5000.0),
     srv.run(0)
     torch.cuda.synchronize()
     init_loss = srv.loss.cpu().numpy()[slots]
+    stream = torch.cuda.current_stream()
     a = torch.cuda.Event(enable_timing=True)
     b = torch.cuda.Event(enable_timing=True)
-    a.record()
+    a.record(stream)
     srv.run(hyper.iters)
-    b.record()
+    b.record(stream)
     torch.cuda.synchronize()
     batched_s = a.elapsed_time(b) / 1e3
     final_loss = srv.loss.cpu().numpy()[slots]
-    tpix = {}
     out["batched_jobs"] = num_jobs
     out["batched_launch_ms"] = round(batched_s * 1e3, 3)
     out["batched_iters_per_s"] = round(num_jobs * hyper.iters / batched_s, 1)
     print(f"[bench] batched: {num_jobs} jobs x {hyper.iters} its in {batched_s * 1e3:.1f} ms",
           file=sys.stderr, flush=True)
+    prof = fit_roofline_profile()
+    if prof and prof.get("jobs") == num_jobs and prof.get("iters") == hyper.iters:
+        achieved = prof["valu_insts_per_launch"] / batched_s / 1e9
+        out["roofline"] = {
+            "bound": "valu", "achieved": round(achieved, 2), "peak": VALU_PEAK_GWIS,
+            "unit": "G wave64 VALU instructions/s", "frac": round(achieved / VALU_PEAK_GWIS, 4),
+            "traffic": prof.get("hbm_bytes_per_launch"), "kernel": "fit_cards_kernel",
+            "avg_launch_ms": round(batched_s * 1e3, 3),
+            "valu_insts_per_launch": prof["valu_insts_per_launch"],
+            "source": "VALU count: profiles/fit_cards_sq_counters.md; time: this launch"}
+    if batched_only:
+        return out
 
-    # plain per-job loop through the drop-in API, on the first jobs
-    plain_xor, served_xor, plain_tp = 0, 0.0, 0
+    # plain per-job loop through the drop-in API, on every (resolution,
+    # quads) class of the first 200 jobs, which every served rate completes
+    sample = stratified(jobs[:200], plain_per_class)
+    plain = {}
     fitting.plain_fit(jobs[0], hyper, dev, iters=2)  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for j in jobs[:plain_jobs]:
+    plain_tp = 0
+    for j in sample:
         _, xor, tp = fitting.plain_fit(j, hyper, dev)
-        plain_xor += xor
+        plain[j.jid] = xor
         plain_tp += tp
-        served_xor += float(final_loss[j.jid])
-        tpix[j.jid] = tp
     torch.cuda.synchronize()
     plain_s = time.perf_counter() - t0
-    plain_ips = plain_jobs * hyper.iters / plain_s
-    out["plain_jobs"] = plain_jobs
+    plain_ips = len(sample) * hyper.iters / plain_s
+    plain_xor = sum(plain.values())
+    batched_xor = float(sum(final_loss[j.jid] for j in sample))
+    out["plain_jobs"] = len(sample)
     out["plain_iters_per_s"] = round(plain_ips, 2)
     out["plain_final_coverage_loss"] = round(plain_xor / max(plain_tp, 1), 5)
-    out["served_final_coverage_loss"] = round(served_xor / max(plain_tp, 1), 5)
-    out["final_loss_rel_diff"] = round(abs(served_xor - plain_xor) / max(plain_xor, 1), 5)
+    out["batched_final_coverage_loss"] = round(batched_xor / max(plain_tp, 1), 5)
+    out["batched_final_loss_rel_diff"] = round(abs(batched_xor - plain_xor) / max(plain_xor, 1), 6)
     out["initial_coverage_loss_all_jobs"] = float(init_loss.sum())
     out["final_coverage_loss_all_jobs"] = float(final_loss.sum())
-    print(f"[bench] plain loop {plain_ips:.1f} it/s, loss plain {plain_xor} served {served_xor}",
-          file=sys.stderr, flush=True)
+    print(f"[bench] plain loop {plain_ips:.1f} it/s over {len(sample)} jobs, loss plain "
+          f"{plain_xor} batched {batched_xor}", file=sys.stderr, flush=True)
 
     # served: Poisson arrivals, continuous batching, graph replays
     served = {}
     for lam in rates:
-        n = min(num_jobs, max(64, int(lam * 1.0)))
+        n = min(num_jobs, max(200, int(lam * 1.0)))
         js = fitting.poisson_arrivals(jobs[:n], lam, seed=int(lam))
         srv = fitting.CardFitServer(num_jobs, hyper, dev, chunk=tun.chunk, order=tun.order)
         rep = fitting.serve(js, srv, chunk=10)
+        sxor = sum(rep.losses[j.jid] for j in sample)
         served[str(int(lam))] = {
-            "jobs": n, "iters_per_s": round(rep.iters_per_s, 1),
+            "jobs": n, "completed": rep.completed, "iters_per_s": round(rep.iters_per_s, 1),
             "p50_ms": round(rep.percentile(50), 3), "p99_ms": round(rep.percentile(99), 3),
             "makespan_s": round(rep.seconds, 3),
-            "vs_plain": round(rep.iters_per_s / plain_ips, 1)}
+            "vs_plain": round(rep.iters_per_s / plain_ips, 1),
+            "final_coverage_loss_plain_sample": round(sxor / max(plain_tp, 1), 5),
+            "final_loss_rel_diff_vs_plain": round(abs(sxor - plain_xor) / max(plain_xor, 1), 6)}
         print(f"[bench] serve lambda={lam:.0f}: {served[str(int(lam))]}", file=sys.stderr, flush=True)
     out["served"] = served
     out["batched_vs_plain"] = round(out["batched_iters_per_s"] / plain_ips, 1)
+    if cpu:
+        c = fit_cpu_baseline(jobs, hyper)
+        c["gpu_batched_vs_cpu"] = round(out["batched_iters_per_s"] / c["value"], 1)
+        out["cpu_baseline"] = c
+        print(f"[bench] card-fit cpu baseline {c}", file=sys.stderr, flush=True)
     return out
 
 
@@ -571,6 +695,10 @@ def main():
                     help="only the large-steps, card-mesh, texture and joint-fit sections")
     ap.add_argument("--joint-fit-only", action="store_true",
                     help="only the fused joint-fit loop (for kernel traces)")
+    ap.add_argument("--fit-only", action="store_true",
+                    help="only the persistent card-fitting launch (for PMC runs)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the coverage-adjoint fit section (for PMC runs)")
     ap.add_argument("--dda-only", action="store_true",
                     help="time the DDA->mask step only (for PMC runs)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -591,6 +719,16 @@ def main():
     from strns2cards_amd import _core, synth
 
     dev = torch.device("cuda", 0)
+    if args.fit_only or args.adjoint_only:
+        if args.fit_only:
+            res = {"card_fit": fit_section(dev, num_jobs=args.fit_jobs, tune=not args.no_autotune,
+                                           batched_only=True)}
+        else:
+            res = {"adjoint_fit": adjoint_section(dev)}
+        if args.copy_calibration:
+            copy_calibration(dev)
+        print(json.dumps(res), flush=True)
+        return
     if args.joint_fit_only:
         print(json.dumps({"joint_fit": joint_fit_section(dev, fused_only=True)}), flush=True)
         return
@@ -693,7 +831,7 @@ def main():
           f"(from the marked edges, no host sync: {ms_aa_f:.3f} ms)", file=sys.stderr, flush=True)
 
     fit = None if args.no_fit else fit_section(dev, num_jobs=args.fit_jobs,
-                                                   tune=not args.no_autotune)
+                                                   tune=not args.no_autotune, cpu=not args.no_cpu)
     adj = None if args.no_adjoint else adjoint_section(dev)
     lsteps = None if args.no_largesteps else largesteps_section(dev)
     cmesh = None if args.no_largesteps else card_mesh_section(dev)

@@ -47,6 +47,72 @@ def test_one_iteration_matches_plain_loop(dev, jobs):
         assert err <= 1e-3 * moved + 1e-6, (j.jid, err, moved)
 
 
+def cpu_job(j):
+    from oracle import cardfit
+
+    n, nv = j.strands.shape[:2]
+    return cardfit.job_dict(j.res, j.quads, j.strand_clip.reshape(n, nv, 4), j.mvp, j.verts)
+
+
+def test_iterations_match_the_cpu_loop(dev, jobs):
+    """The batched kernel against the oracle's fitting loop (3_fitting.py:
+    381-407 restated on the CPU in float64 with a Cholesky solve of the
+    largesteps system): after 1 and after 3 iterations every vertex agrees
+    to 1e-3 of how far the job has moved plus the float32 rounding of the
+    solve, cond(M) * 2^-24 * max|vertex| (positions are recovered from
+    parameters M x whose magnitude is |M| |x|; the masks are the same)."""
+    from oracle import cardfit
+
+    report("fit: 1 and 3 iterations vs the CPU fitting loop")
+    hyper = fitting.FitHyper()
+    hyp = cardfit.hyper_dict(hyper)
+    for iters in (1, 3):
+        srv = fitting.CardFitServer(16, hyper, dev)
+        slots = srv.admit(jobs[:8])
+        srv.run(iters)
+        torch.cuda.synchronize()
+        for s, j in zip(slots, jobs[:8]):
+            job = cpu_job(j)
+            want, _, _ = cardfit.fit_job(job, hyp, iters)
+            got = srv.job_vertices(s)
+            M = oracle.largesteps_matrix(want.shape[0], job["h_edges"], job["w_edges"], hyper.lam,
+                                         hyper.w_weight)
+            solve_eps = np.linalg.cond(M) * 2.0 ** -24 * np.abs(j.verts).max()
+            moved = np.abs(want - j.verts).max()
+            err = np.abs(got - want).max()
+            report(f"  job {j.jid} res {j.res} Q {j.quads} iters {iters}: moved {moved:.3e} "
+                   f"err {err:.3e} (solve rounding {solve_eps:.2e})")
+            assert moved > 10 * solve_eps
+            assert err <= 1e-3 * moved + solve_eps, (j.jid, iters, err, moved, solve_eps)
+
+
+def test_final_loss_matches_the_cpu_loop(dev):
+    """All 100 iterations of 30 jobs (two of every resolution x quads class):
+    the summed final coverage loss of the batched kernel is within 1% of the
+    oracle's CPU fitting loop (the north-star's 'equal final loss')."""
+    from oracle import cardfit
+
+    report("fit: 100 iterations, 30 stratified jobs vs the CPU fitting loop")
+    hyper = fitting.FitHyper()
+    hyp = cardfit.hyper_dict(hyper)
+    sample = stratified(fitting.make_jobs(400, seed=11), 2)
+    assert len(sample) == 30
+    srv = fitting.CardFitServer(32, hyper, dev)
+    slots = srv.admit(sample)
+    srv.run(hyper.iters)
+    torch.cuda.synchronize()
+    loss = srv.loss.cpu().numpy()
+    gpu = cpu = tpix = 0
+    for s, j in zip(slots, sample):
+        _, xor, tp = cardfit.fit_job(cpu_job(j), hyp, hyper.iters)
+        gpu += float(loss[s])
+        cpu += xor
+        tpix += tp
+    rel = abs(gpu - cpu) / max(cpu, 1)
+    report(f"  coverage loss GPU {gpu / tpix:.5f} CPU {cpu / tpix:.5f} (rel diff {rel:.4f})")
+    assert rel <= 0.01, (gpu, cpu, tpix)
+
+
 def test_alone_and_mixed_are_identical(dev, jobs):
     """A job's result does not depend on the other jobs in the batch."""
     report("fit: alone vs in a mix (bit-identical)")

@@ -679,3 +679,94 @@ def test_absum_bounds_the_value():
                                                    1.0, with_abs=True)
     assert np.abs(g).max() > 0 and (ab >= np.abs(g) * (1 - 1e-6)).all()
     assert (ab[..., 2] == 0).all()  # z carries no gradient
+
+
+# ------------------------------------------------------------ card fitting loop
+
+
+def test_cardfit_strip_mesh_counts_and_adjacency():
+    """The strip of Q quads: 2Q triangles covering each quad once, 4Q + 1
+    unique edges of which the 2Q row edges and the two end w-edges are
+    boundary (one triangle) and the Q diagonals and Q - 1 inner w-edges are
+    shared by the two triangles on either side; quads are CCW cells."""
+    from oracle import cardfit
+
+    for Q in (1, 2, 8, 32):
+        N = Q + 1
+        tri, quads, edges, e2t, h, w = cardfit.strip_mesh(Q)
+        assert tri.shape == (2 * Q, 3) and quads.shape == (Q, 4)
+        assert edges.shape == (4 * Q + 1, 2) and (edges[:, 0] < edges[:, 1]).all()
+        boundary = {tuple(e) for e, t in zip(edges.tolist(), e2t.tolist()) if t[1] == -1}
+        want = {(r * N + j, r * N + j + 1) for r in range(2) for j in range(Q)} | {(0, N), (Q, 2 * Q + 1)}
+        assert boundary == want
+        for e, (t0, t1) in zip(edges.tolist(), e2t.tolist()):
+            assert set(e) <= set(tri[t0].tolist())
+            if t1 >= 0:
+                assert t1 != t0 and set(e) <= set(tri[t1].tolist())
+        for q in range(Q):  # the two triangles of a cell cover its 4 corners
+            assert set(tri[2 * q].tolist()) | set(tri[2 * q + 1].tolist()) == set(quads[q].tolist())
+        assert {tuple(x) for x in h.tolist()} | {tuple(x) for x in w.tolist()} == \
+            {tuple(e) for e in edges.tolist()} - {(q, N + q + 1) for q in range(Q)}
+
+
+def test_cardfit_regularizer_gradient_is_the_energy_gradient():
+    """Central differences of w_pln * sum |q0 + q2 - q1 - q3|^2 +
+    w_lap * |L pos|^2, computed directly from the definitions."""
+    from oracle import cardfit
+
+    rng = np.random.default_rng(23)
+    Q = 5
+    _, quads, _, _, h, w = cardfit.strip_mesh(Q)
+    V = 2 * (Q + 1)
+    L = oracle.laplacian_edges(V, np.concatenate([h, w]))
+    pos = rng.normal(size=(V, 3))
+
+    def energy(p):
+        pl = sum(np.sum((p[a] + p[c] - p[b] - p[d]) ** 2) for a, b, c, d in quads)
+        return 10.0 * pl + 0.01 * np.sum((L @ p) ** 2)
+
+    g = cardfit.regularizer_grad(pos, quads, L, 10.0, 0.01)
+    fd = np.zeros_like(pos)
+    for i in range(V):
+        for k in range(3):
+            e = np.zeros_like(pos)
+            e[i, k] = 1e-6
+            fd[i, k] = (energy(pos + e) - energy(pos - e)) / 2e-6
+    np.testing.assert_allclose(g, fd, rtol=1e-6, atol=1e-6)
+
+
+def test_cardfit_clip_chain_rule():
+    """d/dpos of sum(gc * clip) with clip = mvp [pos, 1] for a
+    non-symmetric mvp, z component of gc dropped."""
+    from oracle import cardfit
+
+    rng = np.random.default_rng(29)
+    mvp = rng.normal(size=(4, 4))
+    gc = rng.normal(size=(6, 4))
+    gz = gc.copy()
+    gz[:, 2] = 0.0
+    pos = rng.normal(size=(6, 3))
+    f = lambda p: np.sum(gz * (np.concatenate([p, np.ones((6, 1))], 1) @ mvp.T))  # noqa: E731
+    fd = np.zeros_like(pos)
+    for i in range(6):
+        for k in range(3):
+            e = np.zeros_like(pos)
+            e[i, k] = 1e-6
+            fd[i, k] = (f(pos + e) - f(pos - e)) / 2e-6
+    np.testing.assert_allclose(cardfit.clip_grad_to_positions(gc, mvp), fd, rtol=1e-6, atol=1e-7)
+
+
+def test_cardfit_loop_lowers_the_xor():
+    """Twenty iterations of the CPU loop on a small job lower its XOR pixel
+    count and keep the vertices finite."""
+    from oracle import cardfit
+    from strns2cards_amd import fitting
+
+    j = [x for x in fitting.make_jobs(40, seed=3) if x.res == 64][0]
+    n, nv = j.strands.shape[:2]
+    job = cardfit.job_dict(j.res, j.quads, j.strand_clip.reshape(n, nv, 4), j.mvp, j.verts)
+    hyp = cardfit.hyper_dict(fitting.FitHyper())
+    _, xor0, tp = cardfit.fit_job(job, hyp, 0)
+    pos, xor, tp2 = cardfit.fit_job(job, hyp, 20)
+    assert tp == tp2 > 0 and np.isfinite(pos).all()
+    assert xor < xor0, (xor0, xor)
