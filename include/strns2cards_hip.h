@@ -262,12 +262,31 @@ int s2c_fit_targets(int num_jobs, const int* res, const int* bits_off,
                     int verts_per_strand, const float* strand_clip,
                     unsigned* target_bits, void* stream);
 
+/* Admits num_jobs jobs into slot tables of `capacity` slots (the tables
+ * of s2c_fit_cards; vertex rows of slot s start at s * max_verts).  Job i
+ * is records[8i .. 8i+8) = (slot, res, quads, word offset of its target
+ * mask, vertex count <= 66, iteration limit, 0, 0) and the 16 + 3 *
+ * max_verts floats from frecords[i * (16 + 3 * max_verts)] = its 4 x 4
+ * clip matrix (row-major) then its vertices (x, y, z); max_verts >= 66.
+ * Writes res, quads, bits_off, mvp, the vertices and limit (skipped when
+ * null), zeroes steps and the Adam moments and sets slots[slot] = slot.
+ * One launch for a whole admission batch, so a serving loop admits jobs
+ * with one host-to-device copy.  No counterpart in the reference: it is
+ * the admission step of the serving loop around 3_fitting.py:381-407. */
+int s2c_fit_admit(int num_jobs, const int* records, const float* frecords, int capacity,
+                  int max_verts, int* slots, int* res, int* quads, int* bits_off, int* limit,
+                  float* mvp, int* steps, float* verts, float* adam_m, float* adam_v,
+                  void* stream);
+
 /* Runs `iters` fitting iterations for the jobs slots[0..num_slots) (a
  * negative entry is an idle slot and is skipped), then
  * renders the updated card and writes popcount(render XOR target) to
  * loss_out[job]; verts, params, Adam state and steps are updated in place
  * (with iters = 0 only loss_out is written, so measuring a job changes
- * nothing).  Jobs
+ * nothing).  With a non-null `limit`, a job runs at most limit[job] -
+ * steps[job] of the iterations, and a job with none left is not touched
+ * by a call with iters > 0 (its results stay readable while later calls
+ * run).  Jobs
  * never share state, and each job's arithmetic order is fixed, so a job's
  * result does not depend on which other jobs run in the same call, on
  * their order in `slots`, or on `chunk`: the silhouette samples a
@@ -276,8 +295,8 @@ int s2c_fit_targets(int num_jobs, const int* res, const int* bits_off,
 int s2c_fit_cards(int num_slots, const int* slots, int iters, const int* res,
                   const int* quads, const int* vert_off, const int* bits_off,
                   const float* mvp, float* verts, float* params, float* adam_m,
-                  float* adam_v, int* steps, const unsigned* target_bits, const float* hyper,
-                  float* loss_out, int chunk, void* stream);
+                  float* adam_v, int* steps, const int* limit, const unsigned* target_bits,
+                  const float* hyper, float* loss_out, int chunk, void* stream);
 
 /* compute_bishop_frame (curve_utils.py:104-165, a Python loop over the
  * vertices in the reference): rotation-minimizing frames of num_strands

@@ -739,10 +739,47 @@ PYBIND11_MODULE(_core, m) {
               fn);
     });
 
+    m.def("fit_admit", [](Tensor records, Tensor frecords, Tensor slots, Tensor res, Tensor quads,
+                          Tensor bits_off, std::optional<Tensor> limit, Tensor mvp, Tensor steps,
+                          Tensor verts, Tensor adam_m, Tensor adam_v) {
+        const char* fn = "fit_admit";
+        need(records, fn, "records", at::kInt, {-1, 8});
+        const int64_t n = records.size(0);
+        need(frecords, fn, "frecords", at::kFloat, {n, -1});
+        need(slots, fn, "slots", at::kInt, {-1});
+        const int64_t J = slots.size(0);
+        need(res, fn, "res", at::kInt, {J});
+        need(quads, fn, "quads", at::kInt, {J});
+        need(bits_off, fn, "bits_off", at::kInt, {J});
+        need(mvp, fn, "mvp", at::kFloat, {J, 4, 4});
+        need(steps, fn, "steps", at::kInt, {J});
+        need(verts, fn, "verts", at::kFloat, {-1, 3});
+        need(adam_m, fn, "adam_m", at::kFloat, {verts.size(0), 3});
+        need(adam_v, fn, "adam_v", at::kFloat, {verts.size(0)});
+        same_device(fn, {&records, &frecords, &slots, &res, &quads, &bits_off, &mvp, &steps,
+                         &verts, &adam_m, &adam_v});
+        if (limit) {
+            need(*limit, fn, "limit", at::kInt, {J});
+            same_device(fn, {&res, &*limit});
+        }
+        if (J == 0 || verts.size(0) % J != 0)
+            throw std::runtime_error("fit_admit: verts must hold max_verts rows per slot");
+        if (frecords.size(1) != 16 + 3 * (verts.size(0) / J) && n > 0)
+            throw std::runtime_error("fit_admit: frecords rows must hold 16 + 3 * max_verts floats");
+        DeviceGuard g(res.device());
+        check(s2c_fit_admit(I(n, fn), P<int>(records), P<float>(frecords), I(J, fn),
+                            I(verts.size(0) / J, fn), P<int>(slots), P<int>(res), P<int>(quads),
+                            P<int>(bits_off), limit ? P<int>(*limit) : nullptr, P<float>(mvp),
+                            P<int>(steps), P<float>(verts), P<float>(adam_m), P<float>(adam_v),
+                            stream_of(res)),
+              fn);
+    });
+
     m.def("fit_cards", [](Tensor slots, int iters, Tensor res, Tensor quads, Tensor vert_off,
                           Tensor bits_off, Tensor mvp, Tensor verts, Tensor params, Tensor adam_m,
-                          Tensor adam_v, Tensor steps, Tensor target_bits, std::vector<float> hyper,
-                          Tensor loss_out, int chunk) {
+                          Tensor adam_v, Tensor steps, std::optional<Tensor> limit,
+                          Tensor target_bits, std::vector<float> hyper, Tensor loss_out,
+                          int chunk) {
         const char* fn = "fit_cards";
         need(slots, fn, "slots", at::kInt, {-1});
         need(res, fn, "res", at::kInt, {-1});
@@ -760,6 +797,10 @@ PYBIND11_MODULE(_core, m) {
         need(loss_out, fn, "loss_out", at::kFloat, {J});
         same_device(fn, {&slots, &res, &quads, &vert_off, &bits_off, &mvp, &verts, &params,
                          &adam_m, &adam_v, &steps, &target_bits, &loss_out});
+        if (limit) {
+            need(*limit, fn, "limit", at::kInt, {J});
+            same_device(fn, {&res, &*limit});
+        }
         if (hyper.size() != 10)
             throw std::runtime_error("fit_cards: hyper must hold 10 values");
         if (iters < 0) throw std::runtime_error("fit_cards: iters must be >= 0");
@@ -767,8 +808,8 @@ PYBIND11_MODULE(_core, m) {
         check(s2c_fit_cards(I(slots.size(0), fn), P<int>(slots), iters, P<int>(res), P<int>(quads),
                             P<int>(vert_off), P<int>(bits_off), P<float>(mvp), P<float>(verts),
                             P<float>(params), P<float>(adam_m), P<float>(adam_v), P<int>(steps),
-                            P<unsigned>(target_bits), hyper.data(), P<float>(loss_out), chunk,
-                            stream_of(res)),
+                            limit ? P<int>(*limit) : nullptr, P<unsigned>(target_bits),
+                            hyper.data(), P<float>(loss_out), chunk, stream_of(res)),
               fn);
     });
 

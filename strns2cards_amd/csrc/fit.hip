@@ -157,7 +157,7 @@ fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict
                  const int* __restrict__ bits_off, const float* __restrict__ mvp_arr,
                  float* __restrict__ verts, float* __restrict__ params,
                  float* __restrict__ adam_m, float* __restrict__ adam_v,
-                 int* __restrict__ steps,
+                 int* __restrict__ steps, const int* __restrict__ limit,
                  const unsigned* __restrict__ target_bits, FitHyper hp,
                  float* __restrict__ loss_out) {
     __shared__ unsigned render[kFitMaxWords];
@@ -183,6 +183,14 @@ fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict
 
     const int job = slots[blockIdx.x];
     if (job < 0) return;  // inactive slot (workgroup-uniform)
+    // A job with an iteration limit runs only the iterations it has left;
+    // once it has none, an advancing launch leaves it (and its loss) as is.
+    int n_it = iters;
+    if (limit != nullptr && iters > 0) {
+        const int left = limit[job] - steps[job];
+        if (left <= 0) return;  // workgroup-uniform
+        n_it = dmin(iters, left);
+    }
     const int res = res_arr[job], Q = quad_arr[job];
     if (res < 1 || res > kFitMaxRes || Q < 1 || Q > kFitMaxQuads) {
         if (threadIdx.x == 0) loss_out[job] = -1.f;
@@ -230,7 +238,7 @@ fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict
         __syncthreads();
     };
 
-    for (int it = 0; it <= iters; ++it) {
+    for (int it = 0; it <= n_it; ++it) {
         // ---- pos = from_differential(M, param) (3_fitting.py:384)
         if (ls) {
             for (int q = tid; q < 3 * V; q += kBlock) (&P[0][0])[q] = (&U[0][0])[q];
@@ -251,7 +259,7 @@ fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict
         // ---- raster (render_bitset_from_triangles)
         fit_raster(res, Q, clip, render, rect, excl, sh);
         __syncthreads();
-        if (it == iters) break;  // final render of the updated card
+        if (it == n_it) break;  // final render of the updated card
 
         // ---- silhouette edges and their antialiased gradient.  Each edge's
         // DDA samples are spread over the whole workgroup (kFitChunk at a
@@ -436,7 +444,7 @@ fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict
     // iters == 0 only measures the current state: with large steps at step 0
     // the render used M^{-1} (M pos), which differs from pos in the last
     // bits, so nothing but the loss is written back
-    if (iters > 0 && tid < V) {
+    if (n_it > 0 && tid < V) {
         for (int k = 0; k < 3; ++k) verts[(vo + tid) * 3 + k] = P[tid][k];
         if (ls)
             for (int k = 0; k < 3; ++k) params[(vo + tid) * 3 + k] = U[tid][k];
@@ -445,6 +453,42 @@ fit_cards_kernel(const int* __restrict__ slots, int iters, const int* __restrict
         adam_m[(vo + tid) * 3 + 2] = g1z;
         adam_v[vo + tid] = g2;
     }
+}
+
+// Admission: one workgroup per admitted job writes its slot's tables from
+// one packed record (kFitAdmitInts ints: slot, res, quads, word offset,
+// vertex count, iteration limit; 16 + 3 * max_verts floats: the 4 x 4
+// clip matrix, then the vertices), zeroes its optimizer state and
+// activates the slot.
+constexpr int kFitAdmitInts = 8;
+
+__global__ void __launch_bounds__(kBlock)
+fit_admit_kernel(const int* __restrict__ rec, const float* __restrict__ frec,
+                 int* __restrict__ slots, int* __restrict__ res, int* __restrict__ quads,
+                 int* __restrict__ bits_off, int* __restrict__ limit, float* __restrict__ mvp,
+                 int* __restrict__ steps, float* __restrict__ verts,
+                 float* __restrict__ adam_m, float* __restrict__ adam_v, int capacity,
+                 int max_verts) {
+    const int* r = rec + static_cast<long long>(blockIdx.x) * kFitAdmitInts;
+    const float* f = frec + static_cast<long long>(blockIdx.x) * (16 + 3 * max_verts);
+    const int s = r[0], nv = r[4];
+    if (s < 0 || s >= capacity || nv < 0 || nv > kFitMaxVerts) return;  // workgroup-uniform
+    const long long v0 = static_cast<long long>(s) * max_verts;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        res[s] = r[1];
+        quads[s] = r[2];
+        bits_off[s] = r[3];
+        if (limit != nullptr) limit[s] = r[5];
+        steps[s] = 0;
+        slots[s] = s;
+    }
+    if (tid < 16) mvp[static_cast<long long>(s) * 16 + tid] = f[tid];
+    for (int q = tid; q < 3 * nv; q += kBlock) {
+        verts[v0 * 3 + q] = f[16 + q];
+        adam_m[v0 * 3 + q] = 0.f;
+    }
+    for (int v = tid; v < nv; v += kBlock) adam_v[v0 + v] = 0.f;
 }
 
 // Target masks: every edge of a job's strands (consecutive vertices of
@@ -502,12 +546,27 @@ extern "C" int s2c_fit_targets(int num_jobs, const int* res, const int* bits_off
     return S2C_OK;
 }
 
+extern "C" int s2c_fit_admit(int num_jobs, const int* records, const float* frecords,
+                             int capacity, int max_verts, int* slots, int* res, int* quads,
+                             int* bits_off, int* limit, float* mvp, int* steps, float* verts,
+                             float* adam_m, float* adam_v, void* stream) {
+    S2C_CHECK(num_jobs >= 0 && capacity >= 0, "fit_admit: negative size");
+    S2C_CHECK(max_verts >= kFitMaxVerts, "fit_admit: max_verts below the strip's vertex count");
+    if (num_jobs == 0) return S2C_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    fit_admit_kernel<<<num_jobs, kBlock, 0, s>>>(records, frecords, slots, res, quads, bits_off,
+                                                 limit, mvp, steps, verts, adam_m, adam_v,
+                                                 capacity, max_verts);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
+
 extern "C" int s2c_fit_cards(int num_slots, const int* slots, int iters, const int* res,
                              const int* quads, const int* vert_off, const int* bits_off,
                              const float* mvp, float* verts, float* params, float* adam_m,
                              float* adam_v,
-                             int* steps, const unsigned* target_bits, const float* hyper,
-                             float* loss_out, int chunk, void* stream) {
+                             int* steps, const int* limit, const unsigned* target_bits,
+                             const float* hyper, float* loss_out, int chunk, void* stream) {
     S2C_CHECK(num_slots >= 0 && iters >= 0, "fit_cards: negative size");
     S2C_CHECK(hyper != nullptr, "fit_cards: null hyper-parameters");
     if (chunk == 0) chunk = kFitChunkDefault;
@@ -523,8 +582,8 @@ extern "C" int s2c_fit_cards(int num_slots, const int* slots, int iters, const i
 #define S2C_FIT_LAUNCH(C)                                                                      \
     fit_cards_kernel<C><<<num_slots, kBlock, 0, s>>>(slots, iters, res, quads, vert_off,     \
                                                      bits_off, mvp, verts, params, adam_m,   \
-                                                     adam_v,                                 \
-                                                     steps, target_bits, hp, loss_out)
+                                                     adam_v, steps, limit,                   \
+                                                     target_bits, hp, loss_out)
     if (chunk == 256) S2C_FIT_LAUNCH(256);
     else if (chunk == 512) S2C_FIT_LAUNCH(512);
     else S2C_FIT_LAUNCH(1024);

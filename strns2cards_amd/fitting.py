@@ -41,8 +41,10 @@ MAX_RES = 256
 MAX_QUADS = 32
 MAX_VERTS = 2 * (MAX_QUADS + 1)
 MAX_WORDS = MAX_RES * MAX_RES // 32
-# per-slot scalars (res, quads, offsets, steps, loss, slot ids) and the 4 x 4 clip matrix
-SLOT_SCALAR_BYTES = 8 * 4 + 16 * 4
+# per-slot scalars (res, quads, offsets, steps, iteration limit, loss, slot
+# ids) and the 4 x 4 clip matrix
+SLOT_SCALAR_BYTES = 9 * 4 + 16 * 4
+NO_LIMIT = 2**31 - 1
 
 
 @dataclass
@@ -297,9 +299,15 @@ class WordPool:
 
 def _upload(a, dev, dtype=None) -> torch.Tensor:
     """Host array -> device tensor through pinned memory, queued on the
-    current stream without waiting for the work already queued on it."""
-    t = torch.from_numpy(np.ascontiguousarray(a if dtype is None else np.asarray(a, dtype)))
-    return t.pin_memory().to(dev, non_blocking=True)
+    current stream without waiting for the work already queued on it.  The
+    pinned block comes from torch's caching host allocator (reused once
+    the copy has completed) and is filled by numpy: a torch host copy
+    would start its thread pool, whose spinning threads cost a serving
+    loop its CPU share."""
+    a = np.asarray(a if dtype is None else np.asarray(a, dtype))
+    buf = torch.empty(a.shape, dtype=torch.from_numpy(a[:0].reshape(-1)).dtype, pin_memory=True)
+    buf.numpy()[...] = a
+    return buf.to(dev, non_blocking=True)
 
 
 class CardFitServer:
@@ -355,13 +363,14 @@ class CardFitServer:
         self.adam_m = torch.zeros((capacity * MAX_VERTS, 3), **f32)
         self.adam_v = torch.zeros(capacity * MAX_VERTS, **f32)
         self.target = torch.zeros(max(pool_words, 1), **i32)
+        self.limit = torch.full((capacity,), NO_LIMIT, **i32)
         self.slots = torch.full((capacity,), -1, **i32)
         self.launch = torch.full((capacity,), -1, **i32)  # workgroup -> slot ("cost" order)
+        self.launch_dirty = False
         self.free = deque(range(capacity))
         self.state: dict[int, SlotState] = {}
         self.word_off: dict[int, int] = {}
-        self.graph = None
-        self.graph_iters = None
+        self.graphs: dict[tuple[int, bool], torch.cuda.CUDAGraph] = {}
 
     # -- admission -----------------------------------------------------
     def can_admit(self, job: FitJob, pending_words=()) -> bool:
@@ -380,16 +389,24 @@ class CardFitServer:
                 return False
         return trial.can_alloc(job.words)
 
-    def admit(self, jobs: list[FitJob], now: float = 0.0) -> list[int]:
+    def admit(self, jobs: list[FitJob], now: float = 0.0, limit: int | None = None) -> list[int]:
         """Uploads the jobs into free slots, computes their target masks on
         the GPU and activates the slots (all asynchronous on the current
-        stream).  Raises ValueError if one does not fit (``can_admit``)."""
+        stream: one packed host-to-device copy of integers and one of
+        floats, then the admission and target kernels).  With `limit`, each
+        job stops after that many iterations however many later launches
+        ask for.  Raises ValueError if one does not fit (``can_admit``)."""
         if not jobs:
             return []
         for j in jobs:
             j.validate()
         if len(jobs) > len(self.free):
             raise ValueError(f"{len(jobs)} jobs for {len(self.free)} free slots")
+        nv = jobs[0].strands.shape[1]
+        if any(j.strands.shape[1] != nv for j in jobs):
+            raise ValueError("jobs admitted together need the same points per strand")
+        if limit is not None and not 0 <= limit <= NO_LIMIT:
+            raise ValueError("limit must be in [0, 2^31 - 1]")
         offs_w = []
         for j in jobs:
             o = self.pool.alloc(j.words)
@@ -400,35 +417,40 @@ class CardFitServer:
             offs_w.append(o)
         self.used_bytes += sum(j.bytes for j in jobs)
         slots = [self.free.popleft() for _ in jobs]
-        dev = self.dev
-        idx = _upload(slots, dev, np.int64)
-        res_t = _upload([j.res for j in jobs], dev, np.int32)
-        offw_t = _upload(offs_w, dev, np.int32)
-        self.res[idx] = res_t
-        self.quads[idx] = _upload([j.quads for j in jobs], dev, np.int32)
-        self.bits_off[idx] = offw_t
-        self.mvp[idx] = _upload(np.stack([j.mvp for j in jobs]), dev, np.float32)
-        self.steps[idx] = 0
-        vrows = np.concatenate([s * MAX_VERTS + np.arange(j.verts.shape[0]) for s, j in zip(slots, jobs)])
-        vidx = _upload(vrows, dev, np.int64)
-        self.verts[vidx] = _upload(np.concatenate([j.verts for j in jobs]), dev, np.float32)
-        self.adam_m[vidx] = 0.0
-        self.adam_v[vidx] = 0.0
-        # target masks: every strand edge DDA-rasterized into the job's block
-        nv = jobs[0].strands.shape[1]
-        if any(j.strands.shape[1] != nv for j in jobs):
-            raise ValueError("jobs admitted together need the same points per strand")
-        clip = np.concatenate([j.strand_clip for j in jobs]).reshape(-1, nv, 4)
-        counts = [j.strands.shape[0] for j in jobs]
+        n = len(jobs)
+        counts = np.array([j.strands.shape[0] for j in jobs], np.int64)
         offs = np.concatenate([[0], np.cumsum(counts)[:-1]]) * nv
-        _core.fit_targets(res_t, offw_t, _upload(offs, dev, np.int32),
-                          _upload(counts, dev, np.int32), _upload(clip, dev, np.float32),
-                          self.target)
-        self.slots[idx] = idx.to(torch.int32)
+        # integers: n admission records (slot, res, quads, word offset,
+        # vertex count, limit, 0, 0), then the target kernel's per-job
+        # res, word offset, strand offset and strand count
+        ib = np.zeros(12 * n, np.int32)
+        rec = ib[: 8 * n].reshape(n, 8)
+        rec[:, 0] = slots
+        rec[:, 1] = [j.res for j in jobs]
+        rec[:, 2] = [j.quads for j in jobs]
+        rec[:, 3] = offs_w
+        rec[:, 4] = [j.verts.shape[0] for j in jobs]
+        rec[:, 5] = NO_LIMIT if limit is None else limit
+        tail = ib[8 * n:].reshape(4, n)
+        tail[0], tail[1], tail[2], tail[3] = rec[:, 1], rec[:, 3], offs, counts
+        # floats: n records (clip matrix, vertices), then the strand vertices
+        F = 16 + 3 * MAX_VERTS
+        fb = np.zeros(n * F + int(counts.sum()) * nv * 4, np.float32)
+        fr = fb[: n * F].reshape(n, F)
+        for k, j in enumerate(jobs):
+            fr[k, :16] = j.mvp.reshape(-1)
+            fr[k, 16: 16 + j.verts.size] = j.verts.reshape(-1)
+        fb[n * F:] = np.concatenate([j.strand_clip.reshape(-1) for j in jobs])
+        idev, fdev = _upload(ib, self.dev), _upload(fb, self.dev)
+        _core.fit_admit(idev[: 8 * n].view(n, 8), fdev[: n * F].view(n, F), self.slots, self.res,
+                        self.quads, self.bits_off, self.limit, self.mvp, self.steps, self.verts,
+                        self.adam_m, self.adam_v)
+        t = idev[8 * n:].view(4, n)
+        _core.fit_targets(t[0], t[1], t[2], t[3], fdev[n * F:].view(-1, nv, 4), self.target)
         for s, j, o in zip(slots, jobs, offs_w):
             self.state[s] = SlotState(j, now)
             self.word_off[s] = o
-        self._refresh_launch()
+        self.launch_dirty = True
         return slots
 
     @staticmethod
@@ -438,12 +460,23 @@ class CardFitServer:
         return job.res * job.res / 32.0 + 4.0 * job.res * job.quads
 
     def _refresh_launch(self) -> None:
-        if self.order != "cost":
-            return
         act = sorted(self.state, key=lambda s: -self.job_cost(self.state[s].job))
         arr = np.full(self.cap, -1, np.int32)
         arr[: len(act)] = act
         self.launch.copy_(_upload(arr, self.dev))
+        self.launch_dirty = False
+
+    def _forget(self, slots: list[int]) -> None:
+        """Host side of a release: the slots and their budget return to the
+        free lists.  The device tables are left as they are (a job that has
+        reached its iteration limit is not advanced, and the next admission
+        into the slot overwrites them)."""
+        for s in slots:
+            job = self.state.pop(s).job
+            self.used_bytes -= job.bytes
+            self.pool.release(self.word_off.pop(s), job.words)
+            self.free.append(s)
+        self.launch_dirty = True
 
     def release(self, slots: list[int]) -> None:
         """Frees the slots and their budget (stream-ordered: launches
@@ -451,38 +484,40 @@ class CardFitServer:
         if not slots:
             return
         self.slots[_upload(slots, self.dev, np.int64)] = -1
-        for s in slots:
-            job = self.state.pop(s).job
-            self.used_bytes -= job.bytes
-            self.pool.release(self.word_off.pop(s), job.words)
-            self.free.append(s)
-        self._refresh_launch()
+        self._forget(slots)
 
     # -- iterations ----------------------------------------------------
-    def _launch(self, iters: int) -> None:
-        _core.fit_cards(self.launch if self.order == "cost" else self.slots, iters, self.res,
-                        self.quads, self.vert_off, self.bits_off, self.mvp, self.verts,
-                        self.params, self.adam_m, self.adam_v, self.steps, self.target,
-                        self.hyper.vector(), self.loss, self.chunk)
+    def _launch(self, iters: int, order: torch.Tensor) -> None:
+        _core.fit_cards(order, iters, self.res, self.quads, self.vert_off, self.bits_off,
+                        self.mvp, self.verts, self.params, self.adam_m, self.adam_v, self.steps,
+                        self.limit, self.target, self.hyper.vector(), self.loss, self.chunk)
 
-    def run(self, iters: int, graph: bool = False) -> None:
-        """Advances every active slot by `iters` iterations (and refreshes
-        its loss).  With graph=True the launch is captured once per
-        iteration count and replayed."""
+    def run(self, iters: int, graph: bool = False, ordered: bool = True) -> None:
+        """Advances every active slot by `iters` iterations, or by what is
+        left of its limit (and refreshes its loss).  With graph=True the
+        launch is captured once per iteration count and replayed.  With
+        ordered=False the launch walks the slot table instead of the
+        "cost" order (no host work when jobs come and go; what a serving
+        loop with few resident jobs wants)."""
+        cost = ordered and self.order == "cost"
+        if cost and self.launch_dirty:
+            self._refresh_launch()
+        order = self.launch if cost else self.slots
         if not graph:
-            self._launch(iters)
+            self._launch(iters, order)
         else:
-            if self.graph is None or self.graph_iters != iters:
+            g = self.graphs.get((iters, cost))
+            if g is None:
                 s = torch.cuda.Stream()
                 s.wait_stream(torch.cuda.current_stream())
                 with torch.cuda.stream(s):
-                    self._launch(0)  # warm-up outside capture (iters = 0 writes only the loss)
+                    self._launch(0, order)  # warm-up outside capture (iters = 0 writes only the loss)
                 torch.cuda.current_stream().wait_stream(s)
                 g = torch.cuda.CUDAGraph()
                 with torch.cuda.graph(g):
-                    self._launch(iters)
-                self.graph, self.graph_iters = g, iters
-            self.graph.replay()
+                    self._launch(iters, order)
+                self.graphs[(iters, cost)] = g
+            g.replay()
         for st in self.state.values():
             st.done_iters += iters
 
@@ -517,24 +552,33 @@ class ServeReport:
 
 
 def serve(jobs: list[FitJob], server: CardFitServer, chunk: int = 10, graph: bool = True,
-          clock=time.perf_counter, keep_vertices: bool = False, ahead: int = 2) -> ServeReport:
+          clock=time.perf_counter, keep_vertices: bool = False, ahead: int = 1) -> ServeReport:
     """Continuous batching: jobs (with `arrival` seconds) are admitted as
     slots and budget free up, and all resident jobs advance together
-    `chunk` iterations per replay.  The host does not wait for a replay
-    before queueing the next (at most `ahead` are queued beyond the one
-    running): the losses (and, with `keep_vertices`, the vertices) of the
-    jobs a replay finishes are copied to pinned host memory behind it, and
-    are collected, with the job's latency (arrival -> result on the host),
-    once its event has completed, while arrivals are admitted meanwhile."""
+    `chunk` iterations per replay (in slot order; each job is admitted with
+    its iteration count as its limit, so a finished job is not advanced and
+    its slot is reused without any device work).  The host does not wait
+    for a replay before queueing the next (at most `ahead` are queued
+    beyond the one running): after a replay that finishes jobs, the loss
+    table (and, with `keep_vertices`, the finished jobs' vertices) is
+    copied to pinned host memory behind it, and the results are collected,
+    with each job's latency (arrival -> result on the host), once that
+    copy's event has completed, while arrivals are admitted meanwhile."""
     total = server.hyper.iters
     if total % chunk:
         raise ValueError("chunk must divide the per-job iteration count")
     dev = server.dev
     pending = deque(sorted(jobs, key=lambda j: j.arrival))
     rep = ServeReport(0, 0, 0.0)
-    results = deque()  # (event, [(jid, arrival, n_verts)], pinned loss, pinned vertices)
+    results = deque()  # (event, [(slot, jid, arrival, n_verts)], pinned loss table, vertices)
     queued = deque()   # one event per queued replay
+    # pinned loss tables free for reuse (more are allocated if results pile up)
+    tables = [torch.empty(server.cap, dtype=torch.float32, pin_memory=True)
+              for _ in range(ahead + 2)]
     stream = torch.cuda.current_stream(dev)
+    if graph and not server.state:
+        server.run(chunk, graph=True, ordered=False)  # capture before the clock starts
+        stream.synchronize()
     t0 = clock()
 
     def collect(wait: bool) -> None:
@@ -542,13 +586,14 @@ def serve(jobs: list[FitJob], server: CardFitServer, chunk: int = 10, graph: boo
             ev, done, lh, vh = results.popleft()
             ev.synchronize()
             t_done = clock() - t0
-            for k, (jid, arrival, n) in enumerate(done):
+            for k, (slot, jid, arrival, n) in enumerate(done):
                 rep.latencies_ms.append((t_done - arrival) * 1e3)
-                rep.losses[jid] = float(lh[k])
+                rep.losses[jid] = float(lh[slot])
                 if vh is not None:
                     rep.vertices[jid] = vh[k, :n].numpy().copy()
                 rep.completed += 1
                 rep.iterations += total
+            tables.append(lh)
             wait = False
 
     while pending or server.state or results:
@@ -559,32 +604,34 @@ def serve(jobs: list[FitJob], server: CardFitServer, chunk: int = 10, graph: boo
             batch.append(j)
             pw.append(j.words)
         if batch:
-            server.admit(batch, now)
+            server.admit(batch, now, limit=total)
         if not server.state:
             if results:
                 collect(wait=True)
             elif pending:
                 time.sleep(max(0.0, pending[0].arrival - (clock() - t0)))
             continue
-        server.run(chunk, graph=graph)
+        server.run(chunk, graph=graph, ordered=False)
         ev = torch.cuda.Event()
         ev.record(stream)
         queued.append(ev)
         done = [sl for sl, st in server.state.items() if st.done_iters >= total]
         if done:
-            idx = _upload(done, dev, np.int64)
-            lh = torch.empty(len(done), dtype=torch.float32, pin_memory=True)
-            lh.copy_(server.loss[idx], non_blocking=True)
+            lh = tables.pop() if tables else torch.empty(server.cap, dtype=torch.float32,
+                                                         pin_memory=True)
+            lh.copy_(server.loss, non_blocking=True)
             vh = None
             if keep_vertices:
+                idx = _upload(done, dev, np.int64)
                 rows = (idx[:, None] * MAX_VERTS + torch.arange(MAX_VERTS, device=dev)).reshape(-1)
                 vh = torch.empty((len(done), MAX_VERTS, 3), dtype=torch.float32, pin_memory=True)
                 vh.copy_(server.verts[rows].reshape(len(done), MAX_VERTS, 3), non_blocking=True)
             ev_done = torch.cuda.Event()
             ev_done.record(stream)
-            results.append((ev_done, [(server.state[sl].job.jid, server.state[sl].job.arrival,
-                                       server.state[sl].job.verts.shape[0]) for sl in done], lh, vh))
-            server.release(done)
+            results.append((ev_done, [(sl, server.state[sl].job.jid, server.state[sl].job.arrival,
+                                       server.state[sl].job.verts.shape[0]) for sl in done],
+                            lh, vh))
+            server._forget(done)
         while len(queued) > ahead:
             queued.popleft().synchronize()
         collect(wait=False)

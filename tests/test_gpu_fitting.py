@@ -151,6 +151,33 @@ def test_measuring_changes_nothing(dev, jobs):
     assert a.loss[sa].item() == b.loss[sb].item()
 
 
+def test_iteration_limit(dev, jobs):
+    """A job admitted with a limit of 30 iterations: two 20-iteration
+    launches run 20 + 10 of them (the same result as one 30-iteration
+    launch), and a later launch changes neither its vertices nor its loss;
+    a job admitted beside it without a limit keeps advancing."""
+    report("fit: per-job iteration limit")
+    hyper = fitting.FitHyper()
+    a = fitting.CardFitServer(4, hyper, dev)
+    sa = a.admit([jobs[1]])[0]
+    a.run(30)
+    b = fitting.CardFitServer(4, hyper, dev)
+    sb = b.admit([jobs[1]], limit=30)[0]
+    sc = b.admit([jobs[2]])[0]
+    b.run(20)
+    b.run(20)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(a.job_vertices(sa), b.job_vertices(sb))
+    assert a.loss[sa].item() == b.loss[sb].item()
+    assert b.steps[sb].item() == 30 and b.steps[sc].item() == 40
+    v, l, vc = b.job_vertices(sb), b.loss[sb].item(), b.job_vertices(sc)
+    b.run(10, graph=True)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(b.job_vertices(sb), v)
+    assert b.loss[sb].item() == l and b.steps[sb].item() == 30
+    assert b.steps[sc].item() == 50 and not np.array_equal(b.job_vertices(sc), vc)
+
+
 def test_final_loss_matches_plain_loop(dev, jobs):
     """100 iterations: the coverage loss (XOR pixels / target pixels) of the
     served fit is within 1% of the plain loop's, and fitting reduces it."""
