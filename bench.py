@@ -699,6 +699,10 @@ def main():
                     help="only the persistent card-fitting launch (for PMC runs)")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the coverage-adjoint fit section (for PMC runs)")
+    ap.add_argument("--cards-only", action="store_true",
+                    help="stop after the card raster and antialias timings (for PMC runs)")
+    ap.add_argument("--card-reps", type=int, default=10,
+                    help="timed repetitions of each card stage")
     ap.add_argument("--dda-only", action="store_true",
                     help="time the DDA->mask step only (for PMC runs)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -794,7 +798,7 @@ def main():
     achieved = nbytes / (ms_launch / 1e3) / 1e9
     print(f"[bench] dda->mask {ms_step:.3f} ms/step, {value:.1f} GB/s", file=sys.stderr, flush=True)
 
-    if args.copy_calibration:
+    if args.copy_calibration and not args.cards_only:
         copy_calibration(dev)
     if args.dda_only:
         print(json.dumps({"metric": METRIC, "value": round(value, 3),
@@ -813,7 +817,7 @@ def main():
     ce2c = torch.from_numpy(c.edge2cluster).to(dev)
     cbits = torch.empty_like(bitset)
 
-    def timed(fn, reps=10):
+    def timed(fn, reps=args.card_reps):
         fn()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -829,6 +833,13 @@ def main():
         cbits, bitset, cpos, tri, cedges, e2t, ce2c, 7.0, 0.5, False, fused=True))
     print(f"[bench] card raster->mask {ms_tri:.3f} ms, antialias grad {ms_aa:.3f} ms "
           f"(from the marked edges, no host sync: {ms_aa_f:.3f} ms)", file=sys.stderr, flush=True)
+    if args.cards_only:
+        if args.copy_calibration:
+            copy_calibration(dev)
+        print(json.dumps({"card_raster_mask_ms": round(ms_tri, 4),
+                          "antialias_grad_ms": round(ms_aa, 4),
+                          "antialias_grad_fused_ms": round(ms_aa_f, 4)}), flush=True)
+        return
 
     fit = None if args.no_fit else fit_section(dev, num_jobs=args.fit_jobs,
                                                    tune=not args.no_autotune, cpu=not args.no_cpu)

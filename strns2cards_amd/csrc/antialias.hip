@@ -1,5 +1,7 @@
 // Silhouette-edge marking and the antialiased cluster-bitset edge gradient
 // (reference: pystrns2cards/src/antialias.cu).
+#include <cmath>
+
 #include "antialias_math.hpp"
 #include "dda_math.hpp"
 
@@ -48,30 +50,126 @@ __device__ __forceinline__ unsigned bit_u(const int* bs, long long pix, int S, i
 // (bitset[b,h,w,j], target[b,h,w,j]).  With S words per pixel a window
 // read from the (B,H,W,S) layout takes one word of each S*4-byte pixel
 // record, so a window row touches a line per pixel; in word planes the
-// row is contiguous.  One workgroup transposes kPlanePix consecutive
-// pixels: coalesced reads of their records, coalesced plane-row writes.
-constexpr int kPlanePix = 64;
+// row is contiguous.  Only the pixels the samples read are copied (below).
 // Below 16 words per pixel the records win: the transpose costs more than the
-// planes save (measured on 13 views at 200^2 and 512^2: records 4-10% faster
-// at 7 and 10 words, planes 3-8% faster at 16 and 20, 25% at 32).
+// planes save (measured on 13 views at 200^2 and 512^2 with a transpose of
+// every pixel: records 4-10% faster at 7 and 10 words, planes 3-8% faster
+// at 16 and 20, 25% at 32).
 constexpr int kPlaneMinSlots = 16;
 
+// Only the planes' pixels that some sample reads are needed: the window of
+// a sample (floor(R) pixels around its point on the edge) and the pixel
+// that gates it (the fragment's or its neighbour's), all within
+// floor(R) + 2 pixels of the fragment.  The silhouette fragments mark the
+// kPlaneTile x kPlaneTile tiles within that reach (plain stores of 1, so
+// no atomics), and the transpose skips the pixels of unmarked tiles (a
+// wave's staged box may cover such pixels; no sample reads them).
+// Measured and not kept: marking the words of each tile as well (one bit
+// per word, atomicOr): the marks cost 0.13-0.85 ms in atomics and the
+// transpose, still reading whole records, gained 14 us.
+constexpr int kPlaneTileLog2 = 4;
+constexpr int kPlaneTile = 1 << kPlaneTileLog2;
+
+struct PlaneTiles {
+    int tx, ty;       // tiles per row / column of a view
+    unsigned* mark;   // mark[(b * ty + y) * tx + x] != 0: transpose the tile
+};
+
+__device__ __forceinline__ void mark_plane_box(const PlaneTiles& t, int b, int h0, int h1, int w0,
+                                               int w1, int H, int W) {
+    h0 = dmax(h0, 0);
+    w0 = dmax(w0, 0);
+    h1 = dmin(h1, H - 1);
+    w1 = dmin(w1, W - 1);
+    if (h0 > h1 || w0 > w1) return;
+    for (int y = h0 >> kPlaneTileLog2; y <= (h1 >> kPlaneTileLog2); ++y)
+        for (int x = w0 >> kPlaneTileLog2; x <= (w1 >> kPlaneTileLog2); ++x)
+            t.mark[(static_cast<long long>(b) * t.ty + y) * t.tx + x] = 1u;
+}
+
+// Marks from a fragment list (frag_pix rows (b, h, w)); fragments of views
+// >= B are dropped by the backward and mark nothing.
 __global__ void __launch_bounds__(kBlock)
-residual_planes_kernel(long long NP, long long HW, int S, const int* __restrict__ bitset,
-                       const int* __restrict__ target, int2* __restrict__ planes) {
+plane_tiles_frags_kernel(int N, const int* __restrict__ frag_pix, int H, int W, int B, int reach,
+                         PlaneTiles t) {
+    const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const int b = frag_pix[i * 3 + 0], h = frag_pix[i * 3 + 1], w = frag_pix[i * 3 + 2];
+    if (b < 0 || b >= B) return;
+    mark_plane_box(t, b, h - reach, h + reach, w - reach, w + reach, H, W);
+}
+
+// Screen coordinate 0.5 (1 + c / cw) * n of the sample formula (aa_sample),
+// clamped into [-1, n] so that it converts to int safely.
+__device__ __forceinline__ int screen_clamped(float c, float cw, float n, bool flip) {
+    const float v = 0.5f * (1.f + (flip ? -c : c) / cw) * n;
+    return f2i(floorf(fminf(fmaxf(v, -1.f), n)));
+}
+
+// Marks from the marked-edge array (b*E + e or -1): every sample of a
+// marked edge lies on its screen segment, so the segment's bounding box
+// grown by the reach covers its fragments' reads.  With w <= 0 at an end
+// (or a non-finite one) the projected samples are not bounded by the ends'
+// projections, and the whole view is marked.
+__global__ void __launch_bounds__(kBlock)
+plane_tiles_edges_kernel(long long BE, const int* __restrict__ prim_ids, int E,
+                         const int* __restrict__ edges, int V, const float* __restrict__ pos,
+                         int H, int W, int reach, PlaneTiles t) {
+    const long long q = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (q >= BE || prim_ids[q] < 0) return;
+    const int b = static_cast<int>(q / E);
+    const int e = static_cast<int>(q - static_cast<long long>(b) * E);
+    const float4* p = reinterpret_cast<const float4*>(pos + static_cast<long long>(b) * V * 4);
+    const float4 p0 = p[edges[e * 2 + 0]], p1 = p[edges[e * 2 + 1]];
+    const bool finite = isfinite(p0.x) && isfinite(p0.y) && isfinite(p1.x) && isfinite(p1.y);
+    if (!(p0.w > 0.f && p1.w > 0.f && finite)) {
+        mark_plane_box(t, b, 0, H - 1, 0, W - 1, H, W);
+        return;
+    }
+    const float fW = static_cast<float>(W), fH = static_cast<float>(H);
+    const int xa = screen_clamped(p0.x, p0.w, fW, false), xb = screen_clamped(p1.x, p1.w, fW, false);
+    const int ya = screen_clamped(p0.y, p0.w, fH, true), yb = screen_clamped(p1.y, p1.w, fH, true);
+    mark_plane_box(t, b, dmin(ya, yb) - reach, dmax(ya, yb) + reach, dmin(xa, xb) - reach,
+                   dmax(xa, xb) + reach, H, W);
+}
+
+// One workgroup per kPlanePix consecutive pixels: coalesced reads of the
+// records of the pixels in marked tiles, coalesced plane-row writes.
+constexpr int kPlanePix = 64;
+
+__global__ void __launch_bounds__(kBlock)
+residual_planes_kernel(long long NP, long long HW, int W, int S, const int* __restrict__ bitset,
+                       const int* __restrict__ target, int2* __restrict__ planes,
+                       PlaneTiles t) {
     __shared__ int2 tile[kPlanePix][33];
+    __shared__ unsigned char take[kPlanePix];
+    __shared__ int any;
     const long long p0 = static_cast<long long>(blockIdx.x) * kPlanePix;
     const int np = static_cast<int>(dmin(static_cast<long long>(kPlanePix), NP - p0));
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    if (threadIdx.x < np) {
+        const long long p = p0 + threadIdx.x;
+        const long long b = p / HW, hw = p - b * HW;
+        const int h = static_cast<int>(hw / W), w = static_cast<int>(hw - static_cast<long long>(h) * W);
+        const bool m = t.mark[(b * t.ty + (h >> kPlaneTileLog2)) * t.tx + (w >> kPlaneTileLog2)] != 0u;
+        take[threadIdx.x] = m;
+        if (m) any = 1;
+    }
+    __syncthreads();
+    if (!any) return;  // workgroup-uniform
     for (int j0 = 0; j0 < S; j0 += 32) {
         const int nj = dmin(32, S - j0);
         for (int i = threadIdx.x; i < np * nj; i += kBlock) {
             const int pl = i / nj, j = i - pl * nj;
+            if (!take[pl]) continue;
             const long long src = (p0 + pl) * S + j0 + j;
             tile[pl][j] = make_int2(bitset[src], target[src]);
         }
         __syncthreads();
         for (int i = threadIdx.x; i < np * nj; i += kBlock) {
             const int j = i / np, pl = i - j * np;
+            if (!take[pl]) continue;
             const long long p = p0 + pl;
             const long long b = p / HW, hw = p - b * HW;
             planes[(b * S + j0 + j) * HW + hw] = tile[pl][j];
@@ -79,6 +177,33 @@ residual_planes_kernel(long long NP, long long HW, int S, const int* __restrict_
         __syncthreads();
     }
 }
+
+// The marks' reach, floor(R) + 2 pixels, from the host: the whole image for
+// a radius beyond it or NaN, 2 pixels for a negative one.
+inline int plane_reach(float R, int H, int W) {
+    const int cap = H + W + 2;
+    if (!(R < static_cast<float>(cap))) return cap;  // also NaN
+    return (R > 0.f ? static_cast<int>(std::floor(R)) : 0) + 2;
+}
+
+// Word planes and tile marks of one call in one temporary: the planes, then
+// the marks at a 256-byte boundary.
+struct PlaneBuffers {
+    Temp mem;
+    int2* planes = nullptr;
+    PlaneTiles tiles{0, 0, nullptr};
+    int alloc(int B, int H, int W, int S, hipStream_t s) {
+        const size_t pb = sizeof(int2) * static_cast<size_t>(B) * H * W * S;
+        const size_t po = (pb + 255) / 256 * 256;
+        tiles.tx = (W + kPlaneTile - 1) >> kPlaneTileLog2;
+        tiles.ty = (H + kPlaneTile - 1) >> kPlaneTileLog2;
+        const long long nm = static_cast<long long>(B) * tiles.tx * tiles.ty;
+        if (int r = mem.alloc(po + 4 * static_cast<size_t>(nm), s)) return r;
+        planes = mem.as<int2>();
+        tiles.mark = reinterpret_cast<unsigned*>(mem.as<char>() + po);
+        return fill_u32(tiles.mark, 0u, nm, s);
+    }
+};
 
 // Residual bits (bit 0 = rendered, bit 1 = target) of cluster k at a pixel
 // of the plane starting at `base` (plane of word k / 32).
@@ -425,15 +550,20 @@ extern "C" int s2c_backward_antialiased_cluster_bitset_edges(
         S2C_LAUNCH_CHECK();
         return S2C_OK;
     }
-    Temp planes;
-    if (int r = planes.alloc(sizeof(int2) * static_cast<size_t>(np) * num_slots, s)) return r;
+    PlaneBuffers pl;
+    if (int r = pl.alloc(num_views, H, W, num_slots, s)) return r;
+    const int reach = plane_reach(kernel_radius, H, W);
+    plane_tiles_edges_kernel<<<grid_for(BE, kBlock), kBlock, 0, s>>>(
+        BE, prim_ids, E, edges, V, pos, H, W, reach, pl.tiles);
+    S2C_LAUNCH_CHECK();
     residual_planes_kernel<<<grid_for(np, kPlanePix), kBlock, 0, s>>>(
-        np, static_cast<long long>(H) * W, num_slots, bitset, target_bitset, planes.as<int2>());
+        np, static_cast<long long>(H) * W, W, num_slots, bitset, target_bitset, pl.planes,
+        pl.tiles);
     S2C_LAUNCH_CHECK();
     antialias_edges_bwd_kernel<true, kAaEdgeItems>
         <<<grid_for(BE, kBlock * kAaEdgeItems), kBlock, 0, s>>>(
         BE, H, W, num_slots, bitset, target_bitset, V, pos, E, edges, prim_ids, normals,
-        edge2cluster, d_pos, kernel_radius, rho, num_views, planes.as<int2>());
+        edge2cluster, d_pos, kernel_radius, rho, num_views, pl.planes);
     S2C_LAUNCH_CHECK();
     return S2C_OK;
 }
@@ -456,14 +586,19 @@ extern "C" int s2c_backward_antialiased_cluster_bitset_views(
         S2C_LAUNCH_CHECK();
         return S2C_OK;
     }
-    Temp planes;
-    if (int r = planes.alloc(sizeof(int2) * static_cast<size_t>(np) * num_slots, s)) return r;
+    PlaneBuffers pl;
+    if (int r = pl.alloc(num_views, H, W, num_slots, s)) return r;
+    const int reach = plane_reach(kernel_radius, H, W);
+    plane_tiles_frags_kernel<<<grid_for(num_frags, kBlock), kBlock, 0, s>>>(
+        num_frags, frag_pix, H, W, num_views, reach, pl.tiles);
+    S2C_LAUNCH_CHECK();
     residual_planes_kernel<<<grid_for(np, kPlanePix), kBlock, 0, s>>>(
-        np, static_cast<long long>(H) * W, num_slots, bitset, target_bitset, planes.as<int2>());
+        np, static_cast<long long>(H) * W, W, num_slots, bitset, target_bitset, pl.planes,
+        pl.tiles);
     S2C_LAUNCH_CHECK();
     antialias_bwd_kernel<true><<<grid_for(num_frags, kBlock), kBlock, 0, s>>>(
         num_frags, frag_pix, frag_attrs_dda, H, W, num_slots, bitset, target_bitset, V, pos, E,
-        edges, normals, edge2cluster, d_pos, kernel_radius, rho, num_views, planes.as<int2>());
+        edges, normals, edge2cluster, d_pos, kernel_radius, rho, num_views, pl.planes);
     S2C_LAUNCH_CHECK();
     return S2C_OK;
 }
