@@ -221,8 +221,12 @@ def align_wisp(strands: torch.Tensor, beta: float) -> Tuple[torch.Tensor, torch.
     dt, dev = strands.dtype, strands.device
     x = strands - strands[:, 0, :].mean(0, keepdim=True)
     roots = x[:, 0, :]
-    axes = _sym_axes(_root_gram(roots, torch.zeros(n, dtype=torch.long, device=dev), 1))[0]
-    pa = axes[0] if np.all(np.isfinite(axes[0])) else np.array([1.0, 0.0, 0.0])
+    gram = _root_gram(roots, torch.zeros(n, dtype=torch.long, device=dev), 1)
+    axes = _sym_axes(gram)[0]
+    # all roots equal (one strand): the reference's SVD of a zero matrix
+    # returns the identity, so its first axis is x; eigh's order would give z
+    rank0 = not np.any(gram)
+    pa = axes[0] if np.all(np.isfinite(axes[0])) and not rank0 else np.array([1.0, 0.0, 0.0])
     pa = torch.as_tensor(pa, dtype=dt, device=dev)
     lengths = (x[:, 1:] - x[:, :-1]).norm(dim=2).sum(1)
     longest = torch.argmax(lengths)
