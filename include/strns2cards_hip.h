@@ -34,6 +34,10 @@ extern "C" {
 int s2c_version(void);
 /* Message of the last failing call on this host thread ("" if none). */
 const char* s2c_last_error(void);
+/* Hash of the sources the library was built from (build.py:source_hash):
+ * the package refuses a library whose hash is not that of the sources
+ * beside it.  No counterpart in the reference. */
+const char* s2c_build_id(void);
 
 /* ---- triangle rasterization: rasterize.cu -------------------------------- */
 

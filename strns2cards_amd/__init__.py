@@ -22,6 +22,29 @@ except ImportError as _e:  # pragma: no cover - exercised only on broken install
         "strns2cards_amd: native extension not built or not loadable "
         f"({_e}); run `python strns2cards_amd/build.py`") from _e
 
+
+def _check_build() -> None:
+    """Refuses libraries built from other sources than those beside them
+    (when the sources are present): a stale build would run silently."""
+    import ctypes
+
+    from .build import CSRC, source_hash
+
+    if not _os.path.isdir(CSRC):
+        return
+    want = source_hash()
+    lib = ctypes.CDLL(LIBRARY_PATH)
+    lib.s2c_build_id.restype = ctypes.c_char_p
+    got = {"libstrns2cards_hip.so": lib.s2c_build_id().decode(), "_core": _core.build_id}
+    stale = {k: v for k, v in got.items() if v != want}
+    if stale:
+        raise ImportError(f"strns2cards_amd: {', '.join(stale)} built from other sources "
+                          f"(build id {stale}, sources {want}); run "
+                          "`python strns2cards_amd/build.py`")
+
+
+_check_build()
+
 from .adjoint import *  # noqa: E402,F401,F403
 from .antialias import *  # noqa: E402,F401,F403
 from .bitset_utils import *  # noqa: E402,F401,F403

@@ -10,6 +10,8 @@ libraries exist.
 """
 from __future__ import annotations
 
+import glob
+import hashlib
 import os
 import subprocess
 import sys
@@ -40,6 +42,23 @@ HIP_FLAGS = [
 ]
 
 
+def source_hash() -> str:
+    """Hash of what the libraries are built from: every file of csrc/ and
+    include/ and the compiler flags (16 hex digits)."""
+    h = hashlib.sha256(" ".join(HIP_FLAGS).encode())
+    files = sorted(glob.glob(os.path.join(CSRC, "*")) + glob.glob(os.path.join(ROOT, "include", "*")))
+    for f in files:
+        if os.path.isfile(f):
+            h.update(os.path.relpath(f, ROOT).encode())
+            with open(f, "rb") as fh:
+                h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def _id_flag() -> str:
+    return f'-DS2C_BUILD_ID="{source_hash()}"'
+
+
 def _run(cmd: list[str]) -> None:
     proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if proc.returncode != 0:
@@ -52,7 +71,7 @@ def _run(cmd: list[str]) -> None:
 
 def _compile_hip(src: str) -> str:
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", out])
+    _run([HIPCC, *HIP_FLAGS, _id_flag(), "-c", os.path.join(CSRC, src), "-o", out])
     return out
 
 
@@ -84,7 +103,7 @@ def build_core() -> str:
         os.path.join(CSRC, "core_binding.cpp"), "-o", out + ".tmp",
         *incs, "-I", sysconfig.get_paths()["include"],
         "-D__HIP_PLATFORM_AMD__=1", "-DUSE_ROCM=1",
-        "-DTORCH_EXTENSION_NAME=_core", "-DTORCH_API_INCLUDE_EXTENSION_H",
+        "-DTORCH_EXTENSION_NAME=_core", "-DTORCH_API_INCLUDE_EXTENSION_H", _id_flag(),
         f"-D_GLIBCXX_USE_CXX11_ABI={abi}",
         *libs, "-L", HERE,
         "-lstrns2cards_hip", "-lc10", "-lc10_hip", "-ltorch", "-ltorch_cpu",

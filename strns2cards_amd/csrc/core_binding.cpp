@@ -211,6 +211,11 @@ std::vector<Strand> strands_hair(Bytes& r) {
 
 PYBIND11_MODULE(_core, m) {
     m.doc() = "strns2cards_amd native core (gfx950 HIP)";
+#ifndef S2C_BUILD_ID
+#define S2C_BUILD_ID "unknown"
+#endif
+    // hash of the sources this binding was built from (build.py:source_hash)
+    m.attr("build_id") = S2C_BUILD_ID;
 
     m.def("print_version", []() {
         const int v = s2c_version();

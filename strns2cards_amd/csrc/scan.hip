@@ -280,7 +280,12 @@ compact_scatter(const int* __restrict__ a, const float* __restrict__ b, long lon
 
 using namespace s2c;
 
+#ifndef S2C_BUILD_ID
+#define S2C_BUILD_ID "unknown"
+#endif
 extern "C" int s2c_version(void) { return 100; }
+
+extern "C" const char* s2c_build_id(void) { return S2C_BUILD_ID; }
 
 extern "C" const char* s2c_last_error(void) { return g_err.c_str(); }
 

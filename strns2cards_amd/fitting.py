@@ -33,7 +33,7 @@ from . import _core
 from . import synth
 from .antialias import antialiased_cluster_bitset_loss
 from .bitset_utils import popcount_bitset
-from .optimizer import (GridSystem, UniformAdam, from_differential, grid_laplacian,
+from .optimizer import (GridSystem, UniformAdam, from_differential, grid_laplacian, grid_quads,
                         grid_quad_diagonals, laplacian_edges, to_differential)
 from .rasterize import cluster_mask_from_fragments, compute_fragments, triangle_cluster_mask
 
@@ -686,9 +686,10 @@ def joint_fit(points: torch.Tensor, edges: torch.Tensor, edge2cluster: torch.Ten
     gradient from the marked edges in one launch sequence
     (antialiased_cluster_bitset_loss(fused=True)), so the iteration makes
     no host synchronization, and takes the UniformAdam step in two launches
-    (UniformAdam(fused=True)).  `quads` must then be the
-    grids' quads in wisp_to_grid_mesh order (as combine_wisps_with_cluster_
-    labels writes them)."""
+    (UniformAdam(fused=True)).  The fused planarity term differentiates
+    the grids' quads in wisp_to_grid_mesh order (as combine_wisps_with_
+    cluster_labels writes them); if `quads` is anything else it falls back
+    to planarity_loss(pos, quads), so the loss stays 3_fitting.py's."""
     from . import strand_io
     from .dda import dda_compute_fragments
 
@@ -709,6 +710,8 @@ def joint_fit(points: torch.Tensor, edges: torch.Tensor, edge2cluster: torch.Ten
     V = card_pos.shape[0]
     M = GridSystem.from_edges(V, h_edges, w_edges, lam, w_weight, device=dev)
     L_uniform = None if fused else laplacian_edges(card_pos, torch.cat([h_edges, w_edges], 0))
+    grid_pln = fused and np.array_equal(quads.detach().cpu().numpy().astype(np.int64),
+                                        grid_quads(M.rows_host, M.cols_host))
     param = to_differential(M, card_pos.float()).detach().requires_grad_(True)
     opt = UniformAdam([param], lr=lr, betas=(0.9, 0.999), fused=fused)
     sched = torch.optim.lr_scheduler.ExponentialLR(opt, gamma=1e-2 ** (1 / num_iters))
@@ -738,7 +741,7 @@ def joint_fit(points: torch.Tensor, edges: torch.Tensor, edge2cluster: torch.Ten
         pos_ndc_h, bits = render(pos)
         loss = antialiased_cluster_bitset_loss(bits, target, pos_ndc_h, tri, cedges, e2t, ce2c,
                                                kernel_radius=7.0, rho=alpha, fused=fused)
-        if fused:
+        if grid_pln:
             loss = loss + w_pln * torch.sum(grid_quad_diagonals(M, pos) ** 2)
         else:
             loss = loss + w_pln * planarity_loss(pos, quads)

@@ -288,6 +288,20 @@ class _GridQuadDiagonals(torch.autograd.Function):
         return dpos, None
 
 
+def grid_quads(rows, cols) -> np.ndarray:
+    """(Q, 4) quads [i0, i1, i2, i3] of concatenated grid meshes, in the order
+    of wisp_to_grid_mesh (texture_utils.py:557-565) with the vertex offsets
+    of combine_wisps_with_cluster_labels (texture_utils.py:634-655): the
+    quads grid_quad_diagonals differentiates."""
+    qs = []
+    off = 0
+    for S, N in zip(np.asarray(rows).tolist(), np.asarray(cols).tolist()):
+        idx = off + np.arange(S * N, dtype=np.int64).reshape(S, N)
+        qs.append(np.stack([idx[:-1, :-1], idx[1:, :-1], idx[1:, 1:], idx[:-1, 1:]], -1).reshape(-1, 4))
+        off += S * N
+    return np.concatenate(qs, 0) if qs else np.zeros((0, 4), np.int64)
+
+
 def grid_edges(rows, cols):
     """h_edges and w_edges of concatenated grid meshes, in the order of
     wisp_to_grid_mesh (texture_utils.py:566-584) with the vertex offsets of

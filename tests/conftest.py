@@ -16,23 +16,35 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: full-size workloads")
 
 
+def _built_id(lib):
+    import ctypes
+
+    try:
+        f = ctypes.CDLL(lib).s2c_build_id
+    except (OSError, AttributeError):
+        return None
+    f.restype = ctypes.c_char_p
+    return f().decode()
+
+
 def _ensure_built():
-    """Builds the oracle and the native package only when nothing is built."""
+    """Builds the oracle when it is missing and the native package when it is
+    missing or was built from other sources (the package refuses such a
+    build at import)."""
     import oracle
 
     if not os.path.exists(oracle.LIB_PATH):
         oracle.build()
     lib = os.path.join(ROOT, "strns2cards_amd", "libstrns2cards_hip.so")
     import glob
+    import importlib.util
 
+    spec = importlib.util.spec_from_file_location(
+        "_s2c_build", os.path.join(ROOT, "strns2cards_amd", "build.py"))
+    build = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(build)
     core = glob.glob(os.path.join(ROOT, "strns2cards_amd", "_core*.so"))
-    if not os.path.exists(lib) or not core:
-        import importlib.util
-
-        spec = importlib.util.spec_from_file_location(
-            "_s2c_build", os.path.join(ROOT, "strns2cards_amd", "build.py"))
-        build = importlib.util.module_from_spec(spec)
-        spec.loader.exec_module(build)
+    if not os.path.exists(lib) or not core or _built_id(lib) != build.source_hash():
         build.build_all()
 
 

@@ -77,3 +77,26 @@ def test_core_exports_every_reference_binding_name():
     ]
     missing = [n for n in reference if not hasattr(_core, n)]
     assert not missing, missing
+
+
+def test_build_id_matches_the_sources():
+    """Both libraries carry the hash of the sources they were built from, and
+    it is that of the sources in the tree (the package refuses a stale
+    build at import)."""
+    import ctypes
+    import importlib.util
+    import os
+
+    import strns2cards_amd as s2c
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location(
+        "_s2c_build_check", os.path.join(root, "strns2cards_amd", "build.py"))
+    build = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(build)
+    lib = ctypes.CDLL(s2c.LIBRARY_PATH)
+    lib.s2c_build_id.restype = ctypes.c_char_p
+    want = build.source_hash()
+    assert len(want) == 16
+    assert lib.s2c_build_id().decode() == want
+    assert s2c._core.build_id == want

@@ -336,3 +336,16 @@ def test_joint_fit_fused_follows_the_same_path(dev):
     assert a.xor_first == b.xor_first
     assert torch.allclose(a.pos, b.pos, rtol=0, atol=1e-3)
 
+
+
+def test_joint_fit_fused_with_reordered_quads(dev):
+    """joint_fit(fused=True) given quads in another order than the grids'
+    falls back to planarity_loss(pos, quads): the same loss, so the same
+    path up to the rounding of the sums."""
+    inp = fitting.joint_fit_inputs(800, 24, 12, 600, 0.1, seed=5, device=dev)
+    a = fitting.joint_fit(**inp, resolution=48, num_iters=3, fused=True)
+    perm = torch.randperm(inp["quads"].shape[0], generator=torch.Generator().manual_seed(1))
+    inp["quads"] = inp["quads"][perm.to(inp["quads"].device)].contiguous()
+    b = fitting.joint_fit(**inp, resolution=48, num_iters=3, fused=True)
+    assert abs(a.xor_last - b.xor_last) <= 2
+    assert torch.allclose(a.pos, b.pos, atol=1e-4)

@@ -101,3 +101,15 @@ def test_clustered_strand_edges_by_hand():
     assert e.tolist() == [[0, 1], [1, 2], [4, 5]]
     assert c.tolist() == [4, 4, 2]
     assert k == 5
+
+
+def test_grid_quads_match_the_combined_meshes():
+    """optimizer.grid_quads (the quads the fused planarity term
+    differentiates) equals the quads of combine_wisps_with_cluster_labels for
+    mixed grid shapes, including a one-row grid (no quads)."""
+    from strns2cards_amd.optimizer import grid_quads
+
+    shapes = [(2, 5), (1, 4), (3, 3), (2, 2), (4, 6)]
+    _, _, _, quads, _, _ = oracle.combine_grid_meshes(shapes, [(0, 0, 1, 1)] * 5, [0] * 5)
+    got = grid_quads([s for s, _ in shapes], [n for _, n in shapes])
+    np.testing.assert_array_equal(got, quads)
