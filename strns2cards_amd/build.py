@@ -44,8 +44,9 @@ HIP_FLAGS = [
 
 def source_hash() -> str:
     """Hash of what the libraries are built from: every file of csrc/ and
-    include/ and the compiler flags (16 hex digits)."""
-    h = hashlib.sha256(" ".join(HIP_FLAGS).encode())
+    include/ and the compiler flags (16 hex digits), independent of where
+    the tree lies."""
+    h = hashlib.sha256(" ".join(HIP_FLAGS).replace(ROOT, "<root>").encode())
     files = sorted(glob.glob(os.path.join(CSRC, "*")) + glob.glob(os.path.join(ROOT, "include", "*")))
     for f in files:
         if os.path.isfile(f):

@@ -16,15 +16,14 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: full-size workloads")
 
 
-def _built_id(lib):
-    import ctypes
-
+def _carries(path, build_id):
+    """True when the built file embeds `build_id` (read as bytes: loading it
+    here would keep this copy in the process even after a rebuild)."""
     try:
-        f = ctypes.CDLL(lib).s2c_build_id
-    except (OSError, AttributeError):
-        return None
-    f.restype = ctypes.c_char_p
-    return f().decode()
+        with open(path, "rb") as f:
+            return build_id.encode() in f.read()
+    except OSError:
+        return False
 
 
 def _ensure_built():
@@ -44,7 +43,8 @@ def _ensure_built():
     build = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(build)
     core = glob.glob(os.path.join(ROOT, "strns2cards_amd", "_core*.so"))
-    if not os.path.exists(lib) or not core or _built_id(lib) != build.source_hash():
+    want = build.source_hash()
+    if not (_carries(lib, want) and core and _carries(core[0], want)):
         build.build_all()
 
 
