@@ -260,8 +260,11 @@ def fit_section(dev, num_jobs=4096, plain_per_class=2, rates=(200.0, 1000.0, 500
     served = {}
     for lam in rates:
         n = min(num_jobs, max(200, int(lam * 1.0)))
-        js = fitting.poisson_arrivals(jobs[:n], lam, seed=int(lam))
         srv = fitting.CardFitServer(num_jobs, hyper, dev, chunk=tun.chunk, order=tun.order)
+        # untimed warm-up on the same server (its graph, the allocators'
+        # caches), as the DDA step gets warm-up steps
+        fitting.serve(fitting.poisson_arrivals(jobs[-128:], lam, seed=1), srv, chunk=10)
+        js = fitting.poisson_arrivals(jobs[:n], lam, seed=int(lam))
         rep = fitting.serve(js, srv, chunk=10)
         sxor = sum(rep.losses[j.jid] for j in sample)
         served[str(int(lam))] = {
