@@ -843,6 +843,19 @@ def main():
                           "antialias_grad_ms": round(ms_aa, 4),
                           "antialias_grad_fused_ms": round(ms_aa_f, 4)}), flush=True)
         return
+    # the drop-in API form at the same size: fragment lists (dda.py:5-76),
+    # then the mask from them (rasterize.py:170-214)
+    res2 = (w.H, w.W)
+    fp, fa = s2c.dda_compute_fragments(res2, pos, edges)
+    nf = int(fp.shape[0])
+    del fp, fa
+    ms_api = timed(lambda: s2c.dda_compute_fragments(res2, pos, edges), reps=5)
+    ms_api_mask = timed(lambda: s2c.cluster_mask_from_fragments(
+        res2, *s2c.dda_compute_fragments(res2, pos, edges), e2c, w.num_clusters), reps=5)
+    api_bytes = B * V * 16 + E * 8 + nf * 28
+    print(f"[bench] API form: dda_compute_fragments {ms_api:.3f} ms ({nf} fragments, "
+          f"{api_bytes / ms_api / 1e6:.0f} GB/s), + cluster_mask_from_fragments {ms_api_mask:.3f} ms",
+          file=sys.stderr, flush=True)
 
     fit = None if args.no_fit else fit_section(dev, num_jobs=args.fit_jobs,
                                                    tune=not args.no_autotune, cpu=not args.no_cpu)
@@ -887,6 +900,10 @@ def main():
         "cpu_baseline": cpu,
         "extra": {"card_raster_mask_ms": round(ms_tri, 4), "antialias_grad_ms": round(ms_aa, 4),
                   "antialias_grad_fused_ms": round(ms_aa_f, 4),
+                  "dda_api_fragments_ms": round(ms_api, 4), "dda_api_fragments": nf,
+                  "dda_api_algorithmic_bytes": api_bytes,
+                  "dda_api_GBps": round(api_bytes / ms_api / 1e6, 1),
+                  "dda_api_fragments_plus_mask_ms": round(ms_api_mask, 4),
                   "cards": int(c.tri.shape[0] // 40), "card_triangles": int(c.tri.shape[0])},
         "card_fit": fit,
         "adjoint_fit": adj,

@@ -177,16 +177,22 @@ struct TriPrim {
     int b, cluster;
 };
 
+// kMaskTris triangles per workgroup of kBlock threads: a wave sets them up
+// and scans their candidate counts, then all threads share the candidates.
+// Few triangles per workgroup keep many workgroups on each CU and bound the
+// tail of the largest triangles.
+constexpr int kMaskTris = 32;
+
 __global__ void __launch_bounds__(kBlock)
 tri_mask_kernel(long long BT, int H, int W, int V, const float* __restrict__ pos, int T,
                 const int* __restrict__ tri, const int* __restrict__ tri2cluster, int S,
                 int* __restrict__ bitset) {
-    __shared__ TriPrim prim[kBlock];
-    __shared__ int excl[kBlock];
-    __shared__ int sh[kBlock / kWave + 1];
-    const long long q = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    __shared__ TriPrim prim[kMaskTris];
+    __shared__ int excl[kMaskTris];
+    __shared__ int sum;
+    const long long q = static_cast<long long>(blockIdx.x) * kMaskTris + threadIdx.x;
     int cnt = 0;
-    if (q < BT) {
+    if (threadIdx.x < kMaskTris && q < BT) {
         const int b = static_cast<int>(q / T);
         const int t = static_cast<int>(q - static_cast<long long>(b) * T);
         const long long vb = static_cast<long long>(b) * V;
@@ -199,12 +205,16 @@ tri_mask_kernel(long long BT, int H, int W, int V, const float* __restrict__ pos
         m.cluster = cnt > 0 ? tri2cluster[t] : 0;
         prim[threadIdx.x] = m;
     }
-    int total;
-    excl[threadIdx.x] = block_excl_scan(cnt, sh, &total);
+    if (threadIdx.x < kWave) {  // wave 0 (lanes past kMaskTris count 0)
+        const int incl = wave_incl_scan(cnt);
+        if (threadIdx.x < kMaskTris) excl[threadIdx.x] = incl - cnt;
+        if (threadIdx.x == kWave - 1) sum = incl;
+    }
     __syncthreads();
+    const int total = sum;
 #pragma unroll 1
     for (int j = threadIdx.x; j < total; j += kBlock) {
-        int lo = 0, hi = kBlock;
+        int lo = 0, hi = kMaskTris;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
             if (excl[mid] <= j) lo = mid;
@@ -397,8 +407,8 @@ extern "C" int s2c_triangle_cluster_mask(int B, int H, int W, int V, const float
     if (int r = fill_u32(bitset, 0u, words, s)) return r;
     const long long BT = static_cast<long long>(B) * T;
     if (BT == 0 || words == 0) return S2C_OK;
-    tri_mask_kernel<<<grid_for(BT, kBlock), kBlock, 0, s>>>(BT, H, W, V, pos, T, tri, tri2cluster,
-                                                            num_slots, bitset);
+    tri_mask_kernel<<<grid_for(BT, kMaskTris), kBlock, 0, s>>>(BT, H, W, V, pos, T, tri,
+                                                               tri2cluster, num_slots, bitset);
     S2C_LAUNCH_CHECK();
     return S2C_OK;
 }
