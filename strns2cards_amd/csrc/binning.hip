@@ -276,28 +276,17 @@ int bin_clear(const BinPlan& p, hipStream_t s) {
     return fill_u32(p.counters, 0u, kBinCounters + p.slots() + 1, s);
 }
 
-int* bin_pinned_counters() {
-    thread_local int* p = nullptr;
-    if (!p && hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(int) * kBinCounters,
-                            hipHostMallocPortable) != hipSuccess)
-        p = nullptr;
-    return p;
-}
-
 int bin_counters_async(const BinPlan& p, int* host, hipStream_t s) {
     S2C_HIP(hipMemcpyAsync(host, p.counters, sizeof(int) * kBinCounters, hipMemcpyDeviceToHost, s));
     return S2C_OK;
 }
 
-int bin_check_overflow(const int* host, long long* entries, long long* runs) {
+bool bin_overflow_use(const int* host, long long* entries, long long* runs) {
     // int32 atomics: a wrapped (negative) value means more than 2^31
-    if (host[kEntriesOvf] < 0 || host[kRunsOvf] < 0) {
-        set_error("dda_cluster_mask: overflow area beyond 2^31 entries");
-        return S2C_ERR_OVERFLOW;
-    }
+    if (host[kEntriesOvf] < 0 || host[kRunsOvf] < 0) return false;
     *entries = host[kEntriesOvf];
     *runs = host[kRunsOvf];
-    return S2C_OK;
+    return true;
 }
 
 int bin_resolve(const BinPlan& p, int H, int W, int S, int* bitset, hipStream_t s) {
@@ -331,6 +320,7 @@ namespace {
 struct WorkspaceBuffers {
     void* p[BinWorkspace::kSlots] = {};
     size_t n[BinWorkspace::kSlots] = {};
+    BinStreamCounters counters;
 };
 std::map<std::pair<int, hipStream_t>, WorkspaceBuffers>& workspaces() {
     static std::map<std::pair<int, hipStream_t>, WorkspaceBuffers> m;
@@ -363,6 +353,26 @@ int BinWorkspace::get(hipStream_t s, Slot slot, size_t bytes, void** out) {
     return S2C_OK;
 }
 
+int BinWorkspace::counters(hipStream_t s, BinStreamCounters** out) {
+    int dev = 0;
+    S2C_HIP(hipGetDevice(&dev));
+    BinStreamCounters& c = workspaces()[{dev, s}].counters;
+    if (!c.host) {
+        S2C_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.host), sizeof(int) * kBinCounters,
+                              hipHostMallocPortable));
+        if (hipEventCreateWithFlags(&c.done, hipEventDisableTiming) != hipSuccess) {
+            (void)hipHostFree(c.host);
+            c.host = nullptr;
+            S2C_HIP(hipGetLastError());
+            set_error("dda_cluster_mask: event creation failed");
+            return S2C_ERR_ARG;
+        }
+        c.pending = false;
+    }
+    *out = &c;
+    return S2C_OK;
+}
+
 int BinWorkspace::release(hipStream_t s) {
     int dev = 0;
     S2C_HIP(hipGetDevice(&dev));
@@ -375,6 +385,9 @@ int BinWorkspace::release(hipStream_t s) {
         it->second.p[i] = nullptr;
         it->second.n[i] = 0;
     }
+    BinStreamCounters& c = it->second.counters;
+    if (c.done) S2C_HIP(hipEventDestroy(c.done));
+    if (c.host) S2C_HIP(hipHostFree(c.host));
     m.erase(it);
     return S2C_OK;
 }
@@ -392,5 +405,11 @@ long long raise_hint(std::atomic<long long>& h, long long want) {
 
 long long bin_entry_hint(long long want) { return raise_hint(g_entry_hint, want); }
 long long bin_run_hint(long long want) { return raise_hint(g_run_hint, want); }
+void bin_reset_hints() {
+    g_entry_hint.store(0, std::memory_order_relaxed);
+    g_run_hint.store(0, std::memory_order_relaxed);
+    // counts not read yet would raise the estimates again at the next call
+    for (auto& kv : workspaces()) kv.second.counters.pending = false;
+}
 
 }  // namespace s2c

@@ -14,9 +14,12 @@
 // bitset, no global OR atomics).
 //
 // A workgroup that needs more than its fixed regions takes space from
-// overflow areas; their sizes come from a running estimate, the use is
-// counted even past the capacity, and the host relaunches with exact sizes
-// when it did not fit.
+// overflow areas; their sizes come from a running estimate, and the use is
+// counted even past the capacity.  A pass whose areas were too small
+// resolves to zeros and the same call finishes with one global OR per
+// fragment (a kernel that exits at once when the areas sufficed); the host
+// reads the counts of a call when they have arrived, without waiting, and
+// grows the estimate for the calls after it.
 #pragma once
 
 #include <mutex>
@@ -178,23 +181,30 @@ size_t bin_plan_ints(const TileGrid& g, long long blocks);
 void bin_carve(BinPlan& p, int* ints);
 // Zeroes the run counters and the global counters.
 int bin_clear(const BinPlan& p, hipStream_t s);
-// Pinned host words for the stage pass's counters (one block per thread,
-// kept for the process); nullptr if it cannot be allocated.
-int* bin_pinned_counters();
-// Queues the copy of the counters into `host`.
+// Queues the copy of the counters into `host` (pinned).
 int bin_counters_async(const BinPlan& p, int* host, hipStream_t s);
-// The overflow-area use of the stage pass from the copied counters.
-int bin_check_overflow(const int* host, long long* entries, long long* runs);
+// The overflow-area use of a stage pass from its copied counters; false
+// when a count wrapped past 2^31.
+bool bin_overflow_use(const int* host, long long* entries, long long* runs);
 // Run offsets per tile, runs scattered into tile order, then one workgroup
 // per tile ORs its entries into LDS and writes the tile bitset.  If the
 // stage pass overflowed its areas, every tile is written as zeros instead.
 int bin_resolve(const BinPlan& p, int H, int W, int S, int* bitset, hipStream_t s);
 
+// Pinned copy of the last call's counters on a stream, the event after that
+// copy, and whether the copy has not been read yet.
+struct BinStreamCounters {
+    int* host = nullptr;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+};
+
 // Device buffers of the binned pass kept per (device, stream) across calls,
 // so that a call allocates nothing once its sizes have been seen.  Calls on
 // one stream are ordered, so they can share the buffers; `lock` must be
 // held from the first get() to the last launch of a call, so that two
-// threads enqueueing on one stream cannot interleave.  Buffers only grow: a
+// threads enqueueing on one stream cannot interleave (the lock is never held
+// across a wait for the device, except to free a block that grows).  Buffers only grow: a
 // larger size waits for the stream, frees the old block and allocates a new
 // one.  They are kept until the process ends or release() (for the
 // 50k-strand, 4-view 1024^2 workload about 0.6 GB per stream).
@@ -204,6 +214,9 @@ struct BinWorkspace {
     // Device memory of at least `bytes` for `slot`, for calls on stream s of
     // the current device.
     static int get(hipStream_t s, Slot slot, size_t bytes, void** out);
+    // The pinned counter copy of stream s on the current device (allocated
+    // on first use).
+    static int counters(hipStream_t s, BinStreamCounters** out);
     // Frees the buffers of stream s on the current device, after waiting for
     // the stream.
     static int release(hipStream_t s);
@@ -212,5 +225,8 @@ struct BinWorkspace {
 // Running estimates of the overflow-area sizes, per process.
 long long bin_entry_hint(long long want);
 long long bin_run_hint(long long want);
+// Back to the smallest estimates, forgetting counts not read yet (hold
+// BinWorkspace::lock()).
+void bin_reset_hints();
 
 }  // namespace s2c

@@ -51,8 +51,9 @@ __device__ __forceinline__ void walk_edges(long long P, int H, int W, int V,
                                            const float* __restrict__ pos, int E,
                                            const int* __restrict__ edges,
                                            const int* __restrict__ edge2cluster, int* deferred,
-                                           int* ndeferred, Emit&& emit) {
-    const long long q0 = static_cast<long long>(blockIdx.x) * kPrimsPerBlock +
+                                           int* ndeferred, Emit&& emit,
+                                           long long blk = blockIdx.x) {
+    const long long q0 = blk * kPrimsPerBlock +
                          static_cast<long long>(threadIdx.x) * kPrimsPerThread;
     // All loads of the thread's edges are issued before any is used (one
     // memory round trip for kPrimsPerThread edges instead of one per edge).
@@ -100,8 +101,7 @@ __device__ __forceinline__ void walk_edges(long long P, int H, int W, int V,
         if (cnt > kSerialFrags) {
             const int d = atomicAdd(ndeferred, 1);
             if (d < kDeferred) {
-                deferred[d] =
-                    static_cast<int>(q0 + i - static_cast<long long>(blockIdx.x) * kPrimsPerBlock);
+                deferred[d] = static_cast<int>(q0 + i - blk * kPrimsPerBlock);
             } else {  // list full: the thread walks the rest of its edge itself
                 for (int k = kSerialFrags; k < cnt; ++k) {
                     int h, w;
@@ -113,7 +113,7 @@ __device__ __forceinline__ void walk_edges(long long P, int H, int W, int V,
     }
     __syncthreads();
     const int nd = dmin(*ndeferred, kDeferred);
-    const long long qb = static_cast<long long>(blockIdx.x) * kPrimsPerBlock;
+    const long long qb = blk * kPrimsPerBlock;
 #pragma unroll 1
     for (int d = 0; d < nd; ++d) {
         const EdgeView v = load_edge(qb + deferred[d], H, W, V, pos, E, edges, edge2cluster);
@@ -125,22 +125,51 @@ __device__ __forceinline__ void walk_edges(long long P, int H, int W, int V,
 }
 
 // Direct form: one global atomicOr per fragment into a zeroed bitset, for
-// tile bitsets that do not fit LDS.
-__global__ void __launch_bounds__(kBlock)
-dda_mask_direct_kernel(long long P, int H, int W, int V, const float* __restrict__ pos, int E,
-                       const int* __restrict__ edges, const int* __restrict__ edge2cluster, int S,
-                       int* __restrict__ bitset) {
+// the edges of block index `blk`.
+__device__ __forceinline__ void dda_mask_direct(long long P, int H, int W, int V,
+                                                const float* __restrict__ pos, int E,
+                                                const int* __restrict__ edges,
+                                                const int* __restrict__ edge2cluster, int S,
+                                                int* __restrict__ bitset, long long blk) {
     __shared__ int deferred[kDeferred];
     __shared__ int ndeferred;
     if (threadIdx.x == 0) ndeferred = 0;
     __syncthreads();
-    walk_edges(P, H, W, V, pos, E, edges, edge2cluster, deferred, &ndeferred,
-               [&](int b, int h, int w, int c) {
-                   // ids beyond the bitset set no bit, as in the binned form
-                   if (static_cast<unsigned>(c) >= static_cast<unsigned>(S) * 32u) return;
-                   const long long idx = ((static_cast<long long>(b) * H + h) * W + w) * S + (c >> 5);
-                   atomicOr(&bitset[idx], static_cast<int>(1u << (c & 31)));
-               });
+    walk_edges(
+        P, H, W, V, pos, E, edges, edge2cluster, deferred, &ndeferred,
+        [&](int b, int h, int w, int c) {
+            // ids beyond the bitset set no bit, as in the binned form
+            if (static_cast<unsigned>(c) >= static_cast<unsigned>(S) * 32u) return;
+            const long long idx = ((static_cast<long long>(b) * H + h) * W + w) * S + (c >> 5);
+            atomicOr(&bitset[idx], static_cast<int>(1u << (c & 31)));
+        },
+        blk);
+    __syncthreads();  // `deferred` is reused by the caller's next block
+}
+
+// For tile bitsets that do not fit LDS.
+__global__ void __launch_bounds__(kBlock)
+dda_mask_direct_kernel(long long P, int H, int W, int V, const float* __restrict__ pos, int E,
+                       const int* __restrict__ edges, const int* __restrict__ edge2cluster, int S,
+                       int* __restrict__ bitset) {
+    dda_mask_direct(P, H, W, V, pos, E, edges, edge2cluster, S, bitset, blockIdx.x);
+}
+
+// Behind the binned pass: when its overflow areas were too small (the
+// resolve then wrote every tile as zeros), OR every fragment directly, the
+// grid striding over the walk's blocks; otherwise every workgroup exits at
+// once (a small grid, so the common case costs a few microseconds).
+constexpr int kFallbackGrid = 1024;
+
+__global__ void __launch_bounds__(kBlock)
+dda_mask_fallback_kernel(long long P, int H, int W, int V, const float* __restrict__ pos, int E,
+                         const int* __restrict__ edges, const int* __restrict__ edge2cluster,
+                         int S, int* __restrict__ bitset, const int* __restrict__ counters,
+                         int ovf_entry_cap, int ovf_cap, long long blocks) {
+    const int ce = counters[kEntriesOvf], cr = counters[kRunsOvf];
+    if (ce >= 0 && ce <= ovf_entry_cap && cr >= 0 && cr <= ovf_cap) return;  // workgroup-uniform
+    for (long long blk = blockIdx.x; blk < blocks; blk += gridDim.x)
+        dda_mask_direct(P, H, W, V, pos, E, edges, edge2cluster, S, bitset, blk);
 }
 
 // Binned form: one walk.  Each thread stages its fragments in its own
@@ -386,60 +415,65 @@ extern "C" int s2c_dda_cluster_mask_tiled(int B, int H, int W, int V, const floa
                                   sizeof(int) * bin_plan_ints(plan.g, plan.blocks), &work))
         return r;
     bin_carve(plan, static_cast<int*>(work));
-    long long want_e = bin_entry_hint(4 * kStage), want_r = bin_run_hint(4 * kHash);
-    // The overflow counters are checked without stalling the device: the
-    // run ordering and the resolve are queued right behind the stage pass
-    // (an attempt that overflowed resolves to zeros), and the host waits for
-    // the stage pass and the counter copy alone, relaunching in the rare
-    // case the areas were too small.
-    int* host = bin_pinned_counters();
-    S2C_CHECK(host != nullptr, "dda_cluster_mask: no pinned host memory for the counters");
-    struct Event {
-        hipEvent_t e = nullptr;
-        ~Event() {
-            if (e) (void)hipEventDestroy(e);
+    // No host wait: the overflow-area use of the last call on this stream
+    // grows the running estimates once its copy has arrived, and a call
+    // whose areas are too small finishes with direct ORs on the device.
+    BinStreamCounters* cnt = nullptr;
+    if (int r = BinWorkspace::counters(s, &cnt)) return r;
+    if (cnt->pending) {
+        const hipError_t q = hipEventQuery(cnt->done);
+        if (q == hipSuccess) {
+            long long ne = 0, nr = 0;
+            if (bin_overflow_use(cnt->host, &ne, &nr)) {
+                bin_entry_hint(ne + ne / 8 + 4 * kStage);
+                bin_run_hint(nr + nr / 8 + 4 * kHash);
+            }
+            cnt->pending = false;
+        } else if (q != hipErrorNotReady) {
+            S2C_HIP(q);
         }
-    } ev;
-    S2C_HIP(hipEventCreateWithFlags(&ev.e, hipEventDisableTiming));
-    for (int attempt = 0;; ++attempt) {
-        want_e = dmin(want_e, static_cast<long long>(INT_MAX / 2));
-        want_r = dmin(want_r, static_cast<long long>(INT_MAX / 2));
-        plan.ovf_entries = static_cast<int>(want_e);
-        plan.ovf_runs = static_cast<int>(want_r);
-        void *ebuf = nullptr, *rbuf = nullptr;
-        if (int r = BinWorkspace::get(
-                s, BinWorkspace::kEntries,
-                sizeof(unsigned) * static_cast<size_t>(plan.fixed_entries() + want_e), &ebuf))
-            return r;
-        if (int r = BinWorkspace::get(s, BinWorkspace::kRecords,
-                                      sizeof(BinRecord) *
-                                          static_cast<size_t>(plan.fixed_runs() + want_r),
-                                      &rbuf))
-            return r;
-        plan.entries = static_cast<unsigned*>(ebuf);
-        plan.records = static_cast<BinRecord*>(rbuf);
-        if (int r = bin_clear(plan, s)) return r;
-        dda_bin_stage_kernel<<<static_cast<unsigned>(plan.blocks), kBlock, 0, s>>>(
-            P, H, W, V, pos, E, edges, edge2cluster, plan.g, plan.runs_per_slot, plan.counters,
-            plan.block_runs, plan.entries, plan.ovf_entries, plan.records, plan.ovf_runs);
-        S2C_LAUNCH_CHECK();
-        if (int r = bin_counters_async(plan, host, s)) return r;
-        S2C_HIP(hipEventRecord(ev.e, s));
-        void* obuf = nullptr;
-        if (int r = BinWorkspace::get(
-                s, BinWorkspace::kRuns,
-                sizeof(BinRun) * static_cast<size_t>(plan.fixed_runs() + plan.ovf_runs), &obuf))
-            return r;
-        plan.runs = static_cast<BinRun*>(obuf);
-        if (int r = bin_resolve(plan, H, W, num_slots, bitset, s)) return r;
-        S2C_HIP(hipEventSynchronize(ev.e));
-        long long ne = 0, nr = 0;
-        if (int r = bin_check_overflow(host, &ne, &nr)) return r;
-        if (ne <= want_e && nr <= want_r) break;
-        S2C_CHECK(attempt == 0, "dda_cluster_mask: buffer sizing did not converge");
-        want_e = bin_entry_hint(ne + ne / 8 + 4 * kStage);
-        want_r = bin_run_hint(nr + nr / 8 + 4 * kHash);
     }
+    const long long want_e = dmin(bin_entry_hint(4 * kStage), static_cast<long long>(INT_MAX / 2));
+    const long long want_r = dmin(bin_run_hint(4 * kHash), static_cast<long long>(INT_MAX / 2));
+    plan.ovf_entries = static_cast<int>(want_e);
+    plan.ovf_runs = static_cast<int>(want_r);
+    void *ebuf = nullptr, *rbuf = nullptr, *obuf = nullptr;
+    if (int r = BinWorkspace::get(
+            s, BinWorkspace::kEntries,
+            sizeof(unsigned) * static_cast<size_t>(plan.fixed_entries() + want_e), &ebuf))
+        return r;
+    if (int r = BinWorkspace::get(
+            s, BinWorkspace::kRecords,
+            sizeof(BinRecord) * static_cast<size_t>(plan.fixed_runs() + want_r), &rbuf))
+        return r;
+    if (int r = BinWorkspace::get(
+            s, BinWorkspace::kRuns,
+            sizeof(BinRun) * static_cast<size_t>(plan.fixed_runs() + want_r), &obuf))
+        return r;
+    plan.entries = static_cast<unsigned*>(ebuf);
+    plan.records = static_cast<BinRecord*>(rbuf);
+    plan.runs = static_cast<BinRun*>(obuf);
+    if (int r = bin_clear(plan, s)) return r;
+    dda_bin_stage_kernel<<<static_cast<unsigned>(plan.blocks), kBlock, 0, s>>>(
+        P, H, W, V, pos, E, edges, edge2cluster, plan.g, plan.runs_per_slot, plan.counters,
+        plan.block_runs, plan.entries, plan.ovf_entries, plan.records, plan.ovf_runs);
+    S2C_LAUNCH_CHECK();
+    if (int r = bin_counters_async(plan, cnt->host, s)) return r;
+    S2C_HIP(hipEventRecord(cnt->done, s));
+    cnt->pending = true;
+    if (int r = bin_resolve(plan, H, W, num_slots, bitset, s)) return r;
+    dda_mask_fallback_kernel<<<static_cast<unsigned>(dmin(plan.blocks,
+                                                          static_cast<long long>(kFallbackGrid))),
+                               kBlock, 0, s>>>(P, H, W, V, pos, E, edges, edge2cluster,
+                                               num_slots, bitset, plan.counters, plan.ovf_entries,
+                                               plan.ovf_runs, plan.blocks);
+    S2C_LAUNCH_CHECK();
+    return S2C_OK;
+}
+
+extern "C" int s2c_dda_cluster_mask_reset_estimates(void) {
+    std::lock_guard<std::mutex> held(BinWorkspace::lock());
+    bin_reset_hints();
     return S2C_OK;
 }
 

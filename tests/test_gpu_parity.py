@@ -172,6 +172,28 @@ def test_dda_mask_long_scattered_edges(dev):
     np.testing.assert_array_equal(got, want)
 
 
+def test_dda_mask_overflow_fallback_then_sized(dev):
+    """With the overflow-area estimate reset, a call whose workgroups spill
+    far past the smallest areas finishes with direct ORs on the device (the
+    call never waits for the device); once its use has been read back, the
+    next call is sized for it and takes the binned path.  Both masks equal
+    the oracle's."""
+    import capi
+
+    report("dda->mask overflow areas too small, then sized from the first call")
+    rng = np.random.default_rng(37)
+    pos, edges = random_lines(rng, 2, 6000, 5000, 1024, 1024, persp=False)
+    e2c = rng.integers(0, 100, size=edges.shape[0]).astype(np.int32)
+    want = oracle.dda_cluster_mask((1024, 1024), pos, edges, e2c, 100, threads=8)
+    torch.cuda.synchronize()
+    assert capi.lib().s2c_dda_cluster_mask_reset_estimates() == 0
+    got = fused_dda_mask(dev, (1024, 1024), pos, edges, e2c, 100)  # areas of 16K entries
+    np.testing.assert_array_equal(got, want)
+    torch.cuda.synchronize()
+    got = fused_dda_mask(dev, (1024, 1024), pos, edges, e2c, 100)  # areas sized from the use
+    np.testing.assert_array_equal(got, want)
+
+
 def _segments(rng, n, lo, hi, region, H, W):
     """n segments of length in [lo, hi] px with random starts and directions
     inside the top-left region x region pixels of an H x W image (one view,
