@@ -144,7 +144,7 @@ tile_resolve_kernel(TileGrid g, const int4* __restrict__ order,
                     const BinRun* __restrict__ runs,
                     const unsigned* __restrict__ entries, int H, int W, int S,
                     int* __restrict__ bitset) {
-    __shared__ unsigned tile[kLds / 4];
+    __shared__ __align__(16) unsigned tile[kLds / 4];
     __shared__ int pre[kResolveThreads];
     __shared__ int src[kResolveThreads];
     __shared__ int sh[kResolveThreads / kWave + 1];
@@ -154,8 +154,9 @@ tile_resolve_kernel(TileGrid g, const int4* __restrict__ order,
     BinRun next{0, 0};
     if (r0 + static_cast<int>(threadIdx.x) < r1) next = runs[r0 + threadIdx.x];
     const int side = 1 << g.ts_log2;
-    const int words = side * side * S;
-    for (int i = threadIdx.x; i < words; i += kResolveThreads) tile[i] = 0u;
+    const int words = side * side * S;  // a multiple of 4 (side >= 4)
+    for (int i = threadIdx.x; i < words / 4; i += kResolveThreads)
+        reinterpret_cast<uint4*>(tile)[i] = make_uint4(0u, 0u, 0u, 0u);
     __syncthreads();  // a tile without runs goes straight to the write-out
     const unsigned lmask = (1u << g.local_bits) - 1u;
     const unsigned kmax = static_cast<unsigned>(S) * 32u;
@@ -224,7 +225,25 @@ tile_resolve_kernel(TileGrid g, const int4* __restrict__ order,
     const int x0 = (rr % g.tx) * side;
     const int row_w = dmin(side, W - x0);  // pixels of this tile inside the image
     const int rows = dmin(side, H - y0);
-    if (VEC4) {
+    if (VEC4 && row_w == side && rows == side && (S & (S - 1)) == 0) {
+        // whole tile, S a power of two: the index math is shifts and masks
+        // (the general form below divides twice per store)
+        const int lq = __ffs(S / 4) - 1;  // log2 of uint4 per pixel
+        const int lr = g.ts_log2 + lq;    // log2 of uint4 per tile row
+        const long long row0 = (static_cast<long long>(b) * H + y0) * W + x0;
+        for (int i = threadIdx.x; i < (side << lr); i += kResolveThreads) {
+            const int ly = i >> lr;
+            const int rem = i & ((1 << lr) - 1);
+            const unsigned l = static_cast<unsigned>((ly << g.ts_log2) | (rem >> lq));
+            const int j0 = (rem & ((1 << lq) - 1)) << 2;
+            const unsigned* lw = tile + l * S;
+            const unsigned sw = l & smask;
+            const uint4 v = make_uint4(lw[(j0 + 0) ^ sw], lw[(j0 + 1) ^ sw], lw[(j0 + 2) ^ sw],
+                                       lw[(j0 + 3) ^ sw]);
+            const long long dst = ((row0 + static_cast<long long>(ly) * W) << lq) + rem;
+            reinterpret_cast<uint4*>(bitset)[dst] = v;
+        }
+    } else if (VEC4) {
         const int q = S / 4;
         const int per_row = row_w * q;
         for (int i = threadIdx.x; i < rows * per_row; i += kResolveThreads) {

@@ -111,12 +111,16 @@ def test_dda_empty(dev):
 # --------------------------------------------------------- DDA -> mask
 
 
-def fused_dda_mask(dev, res, pos, edges, e2c, K):
+def fused_dda_mask(dev, res, pos, edges, e2c, K, tile_log2=0):
     S = (K + 31) // 32
     B, V, _ = pos.shape
     bits = torch.full((B, res[0], res[1], S), 0x5A5A5A5A, dtype=torch.int32, device=dev)
-    call("s2c_dda_cluster_mask", B, res[0], res[1], V, T(pos, dev), edges.shape[0], T(edges, dev),
-         T(e2c, dev), S, bits)
+    if tile_log2:
+        call("s2c_dda_cluster_mask_tiled", B, res[0], res[1], V, T(pos, dev), edges.shape[0],
+             T(edges, dev), T(e2c, dev), S, tile_log2, bits)
+    else:
+        call("s2c_dda_cluster_mask", B, res[0], res[1], V, T(pos, dev), edges.shape[0],
+             T(edges, dev), T(e2c, dev), S, bits)
     return bits.cpu().numpy()
 
 
@@ -210,18 +214,25 @@ def _segments(rng, n, lo, hi, region, H, W):
     return pos, np.arange(2 * n, dtype=np.int32).reshape(n, 2)
 
 
-@pytest.mark.parametrize("case", ["records_overflow", "stage_spill"])
+@pytest.mark.parametrize("case", ["records_overflow", "stage_spill", "thread_runs", "long_runs"])
 def test_dda_mask_workgroup_paths(dev, case):
     """Workgroups (1,024 edges each) touching 513-1,024 distinct tiles (their
-    run records go to the overflow area) and workgroups whose threads have
-    more fragments than their LDS stage share (placed by a second walk)."""
+    run records go to the overflow area); workgroups whose threads have more
+    fragments than their LDS stage share (placed by a second walk); threads
+    whose four edges make more than four runs of same-tile fragments (the
+    runs held in registers enter the LDS hash in the middle of the walk);
+    and threads whose runs reach the 127-fragment cap of a register run
+    (64-pixel tiles at 64 clusters, edges of 40-60 pixels inside one
+    tile)."""
     report(f"dda->mask workgroup path: {case}")
     rng = np.random.default_rng(5)
-    lo, hi, region = (1.5, 3.0, 448) if case == "records_overflow" else (10.0, 14.0, 320)
+    lo, hi, region = {"records_overflow": (1.5, 3.0, 448), "stage_spill": (10.0, 14.0, 320),
+                      "thread_runs": (30.0, 60.0, 512), "long_runs": (40.0, 60.0, 63)}[case]
     pos, edges = _segments(rng, 4096, lo, hi, region, 1024, 1024)
     e2c = rng.integers(0, 64, size=edges.shape[0]).astype(np.int32)
     want = oracle.dda_cluster_mask((1024, 1024), pos, edges, e2c, 64, threads=8)
-    got = fused_dda_mask(dev, (1024, 1024), pos, edges, e2c, 64)
+    got = fused_dda_mask(dev, (1024, 1024), pos, edges, e2c, 64,
+                         tile_log2=6 if case == "long_runs" else 0)
     np.testing.assert_array_equal(got, want)
 
 
