@@ -127,6 +127,24 @@ int s2c_dda_compute_fragments(int num_prims, int num_frags,
                               const int* edges, int* frag_pix,
                               float* frag_attrs, void* stream);
 
+/* dda.py:5-76 (dda_compute_fragments: span, candidate expansion and
+ * filter_valid_fragments) as two calls in the space of surviving fragments,
+ * with no candidate list.  s2c_dda_count_fragments writes the inclusive
+ * prefix of each primitive's surviving fragments to valid_prefix
+ * (num_prims ints) and their total; s2c_dda_emit_fragments then writes the
+ * num_frags survivors, in the reference's compacted order, to frag_pix
+ * (num_frags, 3) and frag_attrs (num_frags, 4).  edge_ids may be null
+ * (primitive p is edge p, the reference's default arange). */
+int s2c_dda_count_fragments(int num_prims, const int* edge_ids, int H, int W,
+                            int V, const float* pos, int E, const int* edges,
+                            int* valid_prefix, int* num_frags_out,
+                            void* stream);
+int s2c_dda_emit_fragments(int num_prims, int num_frags,
+                           const int* valid_prefix, const int* edge_ids, int H,
+                           int W, int V, const float* pos, int E,
+                           const int* edges, int* frag_pix, float* frag_attrs,
+                           void* stream);
+
 /* dda.cu:286-298 (dda_interpolate_attributes). */
 int s2c_dda_interpolate_attributes(int B, int H, int W, int C,
                                    const float* rast_dda, const float* attr,

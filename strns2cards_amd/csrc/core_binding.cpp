@@ -462,6 +462,61 @@ PYBIND11_MODULE(_core, m) {
               fn);
     });
 
+    // dda.py:5-76 in the space of surviving fragments: per-primitive counts
+    // (scanned into valid_prefix), then the fragments themselves.  edge_ids
+    // None = every edge of every view.
+    m.def("dda_count_fragments", [](std::optional<Tensor> edge_ids, int H, int W, Tensor pos,
+                                    Tensor edges, Tensor valid_prefix) -> int {
+        const char* fn = "dda_count_fragments";
+        need(pos, fn, "pos", at::kFloat, {-1, -1, 4});
+        need(edges, fn, "edges", at::kInt, {-1, 2});
+        need(valid_prefix, fn, "valid_prefix", at::kInt, {-1});
+        same_device(fn, {&pos, &edges, &valid_prefix});
+        const int P_ = I(valid_prefix.size(0), fn);
+        if (edge_ids) {
+            need(*edge_ids, fn, "edge_ids", at::kInt, {P_});
+            same_device(fn, {&pos, &*edge_ids});
+        } else if (P_ != pos.size(0) * edges.size(0)) {
+            throw std::runtime_error(std::string(fn) + ": valid_prefix must have B * E entries");
+        }
+        DeviceGuard g(pos.device());
+        int n = 0;
+        {
+            py::gil_scoped_release nogil;
+            check(s2c_dda_count_fragments(P_, edge_ids ? P<int>(*edge_ids) : nullptr, H, W,
+                                          I(pos.size(1), fn), P<float>(pos), I(edges.size(0), fn),
+                                          P<int>(edges), P<int>(valid_prefix), &n, stream_of(pos)),
+                  fn);
+        }
+        return n;
+    });
+
+    m.def("dda_emit_fragments", [](Tensor valid_prefix, std::optional<Tensor> edge_ids, int H,
+                                   int W, Tensor pos, Tensor edges, Tensor frag_pix,
+                                   Tensor frag_attrs) {
+        const char* fn = "dda_emit_fragments";
+        need(valid_prefix, fn, "valid_prefix", at::kInt, {-1});
+        need(pos, fn, "pos", at::kFloat, {-1, -1, 4});
+        need(edges, fn, "edges", at::kInt, {-1, 2});
+        need(frag_pix, fn, "frag_pix", at::kInt, {-1, 3});
+        need(frag_attrs, fn, "frag_attrs", at::kFloat, {frag_pix.size(0), 4});
+        same_device(fn, {&valid_prefix, &pos, &edges, &frag_pix, &frag_attrs});
+        const int P_ = I(valid_prefix.size(0), fn);
+        if (edge_ids) {
+            need(*edge_ids, fn, "edge_ids", at::kInt, {P_});
+            same_device(fn, {&pos, &*edge_ids});
+        } else if (P_ != pos.size(0) * edges.size(0)) {
+            throw std::runtime_error(std::string(fn) + ": valid_prefix must have B * E entries");
+        }
+        DeviceGuard g(pos.device());
+        check(s2c_dda_emit_fragments(P_, I(frag_pix.size(0), fn), P<int>(valid_prefix),
+                                     edge_ids ? P<int>(*edge_ids) : nullptr, H, W,
+                                     I(pos.size(1), fn), P<float>(pos), I(edges.size(0), fn),
+                                     P<int>(edges), P<int>(frag_pix), P<float>(frag_attrs),
+                                     stream_of(pos)),
+              fn);
+    });
+
     // pystrns2cards.cpp:613-647
     m.def("dda_interpolate_attributes", [](Tensor rast, Tensor attr, Tensor edges, Tensor image) {
         const char* fn = "dda_interpolate_attributes";

@@ -45,26 +45,21 @@ def dda_compute_fragments(
     B = pos.shape[0]
     E = edges.shape[0]
     device = pos.device
-    if edge_ids is None:
-        edge_ids = torch.arange(B * E, dtype=torch.int32, device=device)
-    elif not edge_ids.is_cuda or edge_ids.dtype != torch.int32 or edge_ids.ndim != 1:
+    if edge_ids is not None and (not edge_ids.is_cuda or edge_ids.dtype != torch.int32
+                                 or edge_ids.ndim != 1):
         raise ValueError("`edge_ids` must be a 1D int32 CUDA tensor.")
 
-    P = edge_ids.shape[0]
-    prefix = torch.empty((P,), dtype=torch.int32, device=device)
-    slopes = torch.empty((P, 2), dtype=torch.float32, device=device)
-    spans = torch.empty((P, 4), dtype=torch.float32, device=device)
-    num_frags = _core.dda_compute_span(edge_ids, H, W, pos, edges, prefix, slopes, spans)
-
-    frag_pix = torch.full((num_frags, 3), -1, dtype=torch.int32, device=device)
-    frag_attrs = torch.empty((num_frags, 4), dtype=torch.float32, device=device)
-    _core.dda_compute_fragments(prefix, edge_ids, slopes, spans, H, W, pos, edges,
-                                frag_pix, frag_attrs)
-
-    pix_out = torch.empty_like(frag_pix)
-    attrs_out = torch.empty_like(frag_attrs)
-    n = _core.filter_valid_fragments(frag_pix, frag_attrs, pix_out, attrs_out)
-    return pix_out[:n], attrs_out[:n]
+    # The reference's span, -1-filled candidate list and compaction
+    # (dda.py:45-76) as two passes over the surviving fragments: the same
+    # fragments in the same order, with no candidate list (edge_ids None is
+    # the reference's arange over every edge of every view).
+    P = B * E if edge_ids is None else edge_ids.shape[0]
+    valid_prefix = torch.empty((P,), dtype=torch.int32, device=device)
+    n = _core.dda_count_fragments(edge_ids, H, W, pos, edges, valid_prefix)
+    frag_pix = torch.empty((n, 3), dtype=torch.int32, device=device)
+    frag_attrs = torch.empty((n, 4), dtype=torch.float32, device=device)
+    _core.dda_emit_fragments(valid_prefix, edge_ids, H, W, pos, edges, frag_pix, frag_attrs)
+    return frag_pix, frag_attrs
 
 
 class DDAInterpolateAttributes(torch.autograd.Function):

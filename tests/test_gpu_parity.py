@@ -59,6 +59,16 @@ def dda_gpu(dev, res, pos, edges, edge_ids=None):
     call("s2c_dda_compute_fragments", P, n, prefix, ti, slopes, spans, H, W, V, tp, E, te, fp, fa)
     fpo, fao = torch.empty_like(fp), torch.empty_like(fa)
     m = call_count("s2c_filter_valid_fragments", n, fp, fa, fpo, fao)
+    # the survivor-space form (count, then emit) gives the same list, with
+    # the ids given and, when they are every edge in order, with null ids
+    for ids_arg in ([ti] + ([None] if edge_ids is None else [])):
+        vp = torch.full((P,), -9, dtype=torch.int32, device=dev)
+        mc = call_count("s2c_dda_count_fragments", P, ids_arg, H, W, V, tp, E, te, vp)
+        assert mc == m
+        fpc = torch.full((m, 3), 7, dtype=torch.int32, device=dev)
+        fac = torch.full((m, 4), 7.0, device=dev)
+        call("s2c_dda_emit_fragments", P, m, vp, ids_arg, H, W, V, tp, E, te, fpc, fac)
+        assert torch.equal(fpc, fpo[:m]) and torch.equal(fac, fao[:m])
     torch.cuda.synchronize()
     return (prefix.cpu().numpy(), slopes.cpu().numpy(), spans.cpu().numpy(), n,
             fpo[:m].cpu().numpy(), fao[:m].cpu().numpy())
