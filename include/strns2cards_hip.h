@@ -100,6 +100,23 @@ int s2c_cluster_mask_from_fragments(int num_frags, const int* frag_pix,
                                     const int* prim2cluster, int H, int W,
                                     int num_slots, int* bitset, void* stream);
 
+/* rasterize.py:170-214 (a zeroed (B, H, W, num_slots) bitset, then
+ * cluster_mask_from_fragments) as one binned pass that writes every word
+ * of `bitset` (no clear, no global OR per fragment).  Fragments with a prim
+ * id <= 0, a pixel outside B x H x W or a cluster >= 32 * num_slots set no
+ * bit.  No host wait; the binning workspace is that of
+ * s2c_dda_cluster_mask on the same stream. */
+int s2c_fragment_cluster_mask(int B, int H, int W, int num_frags,
+                              const int* frag_pix, const float* frag_attrs,
+                              const int* prim2cluster, int num_slots,
+                              int* bitset, void* stream);
+
+/* max over i of frag_pix[i][0] (rasterize.py:197: the view count B is that
+ * plus one) into *max_out; num_frags > 0, frag_pix 16-byte aligned.
+ * Synchronizes the stream. */
+int s2c_max_frag_view(int num_frags, const int* frag_pix, int* max_out,
+                      void* stream);
+
 /* ---- bitset utilities: bitset_utils.cu ----------------------------------- */
 
 /* bitset_utils.cu:55-68 (accumulate_bitset_rgb). */

@@ -374,6 +374,40 @@ PYBIND11_MODULE(_core, m) {
                     fn);
           });
 
+    // rasterize.py:170-214 (zeros, then cluster_mask_from_fragments) as one
+    // binned pass writing every word of `bitset`
+    m.def("fragment_cluster_mask", [](Tensor fp, Tensor fa, Tensor prim2cluster, Tensor bitset) {
+        const char* fn = "fragment_cluster_mask";
+        need(fp, fn, "frag_pix", at::kInt, {-1, 3});
+        need(fa, fn, "frag_attrs", at::kFloat, {-1, 4});
+        need(prim2cluster, fn, "prim2cluster", at::kInt, {-1});
+        need(bitset, fn, "bitset", at::kInt, {-1, -1, -1, -1});
+        same_device(fn, {&fp, &fa, &prim2cluster, &bitset});
+        const int N = I(fp.size(0), fn);
+        if (fa.size(0) != N)
+            throw std::runtime_error(
+                "frag_pix and frag_attrs must have the same number of fragments.");
+        DeviceGuard g(bitset.device());
+        check(s2c_fragment_cluster_mask(I(bitset.size(0), fn), I(bitset.size(1), fn),
+                                        I(bitset.size(2), fn), N, P<int>(fp), P<float>(fa),
+                                        P<int>(prim2cluster), I(bitset.size(3), fn),
+                                        P<int>(bitset), stream_of(bitset)),
+              fn);
+    });
+
+    m.def("max_frag_view", [](Tensor fp) -> int {
+        const char* fn = "max_frag_view";
+        need(fp, fn, "frag_pix", at::kInt, {-1, 3});
+        if (fp.size(0) == 0) throw std::runtime_error("max_frag_view: no fragments");
+        DeviceGuard g(fp.device());
+        int m = 0;
+        {
+            py::gil_scoped_release nogil;
+            check(s2c_max_frag_view(I(fp.size(0), fn), P<int>(fp), &m, stream_of(fp)), fn);
+        }
+        return m;
+    });
+
     // pystrns2cards.cpp:480-505
     m.def("accumulate_bitset_rgb", [](Tensor bitset, Tensor rgb, Tensor out) {
         const char* fn = "accumulate_bitset_rgb";

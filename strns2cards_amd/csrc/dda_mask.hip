@@ -172,6 +172,48 @@ dda_mask_fallback_kernel(long long P, int H, int W, int V, const float* __restri
         dda_mask_direct(P, H, W, V, pos, E, edges, edge2cluster, S, bitset, blk);
 }
 
+// One global OR per listed fragment (rasterize.cu:474-498) into a zeroed
+// bitset, with ListSource's checks; grid-striding.
+__device__ __forceinline__ void list_mask_or(long long N, int B, int H, int W,
+                                             const int* __restrict__ frag_pix,
+                                             const float* __restrict__ frag_attrs,
+                                             const int* __restrict__ prim2cluster, unsigned kmax,
+                                             int S, int* __restrict__ bitset) {
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    for (long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; i < N;
+         i += stride) {
+        const int prim = f2i(frag_attrs[i * 4 + 3]) - 1;
+        if (prim < 0) continue;
+        const int c = prim2cluster[prim];
+        const int b = frag_pix[i * 3 + 0], h = frag_pix[i * 3 + 1], w = frag_pix[i * 3 + 2];
+        if (static_cast<unsigned>(c) >= kmax || static_cast<unsigned>(b) >= static_cast<unsigned>(B) ||
+            static_cast<unsigned>(h) >= static_cast<unsigned>(H) ||
+            static_cast<unsigned>(w) >= static_cast<unsigned>(W))
+            continue;
+        const long long idx = ((static_cast<long long>(b) * H + h) * W + w) * S + (c >> 5);
+        atomicOr(&bitset[idx], static_cast<int>(1u << (c & 31)));
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+list_mask_direct_kernel(long long N, int B, int H, int W, const int* __restrict__ frag_pix,
+                        const float* __restrict__ frag_attrs,
+                        const int* __restrict__ prim2cluster, unsigned kmax, int S,
+                        int* __restrict__ bitset) {
+    list_mask_or(N, B, H, W, frag_pix, frag_attrs, prim2cluster, kmax, S, bitset);
+}
+
+__global__ void __launch_bounds__(kBlock)
+list_mask_fallback_kernel(long long N, int B, int H, int W, const int* __restrict__ frag_pix,
+                          const float* __restrict__ frag_attrs,
+                          const int* __restrict__ prim2cluster, unsigned kmax, int S,
+                          int* __restrict__ bitset, const int* __restrict__ counters,
+                          int ovf_entry_cap, int ovf_cap) {
+    const int ce = counters[kEntriesOvf], cr = counters[kRunsOvf];
+    if (ce >= 0 && ce <= ovf_entry_cap && cr >= 0 && cr <= ovf_cap) return;  // workgroup-uniform
+    list_mask_or(N, B, H, W, frag_pix, frag_attrs, prim2cluster, kmax, S, bitset);
+}
+
 // Binned form: one walk.  Each thread stages its fragments in its own
 // kStage / kBlock words of LDS as slot << 22 | entry (the tile's LDS hash
 // slot and the packed entry), and the workgroup counts fragments per slot
@@ -190,20 +232,101 @@ dda_mask_fallback_kernel(long long P, int H, int W, int V, const float* __restri
 constexpr int kPerThread = kStage / kBlock;
 constexpr unsigned kEntryMask = (1u << 22) - 1u;
 
-// 80 VGPRs (6 waves per SIMD, which the 25.6 KB of LDS also allows)
-// instead of the 82 the compiler picks unconstrained (5 waves); no spills.
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8)))
-dda_bin_stage_kernel(long long P, int H, int W, int V, const float* __restrict__ pos, int E,
-                     const int* __restrict__ edges, const int* __restrict__ edge2cluster,
-                     TileGrid g, int* __restrict__ runs_per_slot, int* __restrict__ counters,
-                     int* __restrict__ block_runs, unsigned* __restrict__ entries, int ovf_entries,
-                     BinRecord* __restrict__ records, int ovf_runs) {
-    __shared__ int keys[kHash];   // tile of each slot, then per-slot cursors
-    __shared__ int vals[kHash];   // fragments per slot, then slot offsets in the region
-    __shared__ unsigned stage[kStage];
-    __shared__ int deferred[kDeferred];
-    __shared__ int ndeferred, spill, full, base_e, base_r;
-    __shared__ int sh[kBlock / kWave + 1];
+// The fragment sources of the binned pass.  walk(emit) calls
+// emit(b, h, w, cluster) for every fragment of the workgroup's share of the
+// work (every thread calls it: it may hold barriers); thread 0 calls
+// reset() before a second walk, with a barrier after it.
+
+// The DDA walk of kPrimsPerBlock consecutive edges (dda.cu:124-225).
+struct EdgeSource {
+    long long P;
+    int H, W, V;
+    const float* __restrict__ pos;
+    int E;
+    const int* __restrict__ edges;
+    const int* __restrict__ edge2cluster;
+    int* deferred;  // LDS, kDeferred ints
+    int* ndeferred;
+    template <class Emit>
+    __device__ __forceinline__ void walk(Emit&& emit) const {
+        walk_edges(P, H, W, V, pos, E, edges, edge2cluster, deferred, ndeferred, emit);
+    }
+    __device__ __forceinline__ void reset() const { *ndeferred = 0; }
+};
+
+// kStage consecutive fragments of a list (rasterize.cu:474-498) in
+// kListRounds rounds; in each, thread t takes 4 consecutive fragments, read
+// with 16-byte loads (a wave's loads cover 3 KB of frag_pix and 4 KB of
+// frag_attrs contiguously), so a thread's fragments of a round mostly share
+// a tile.  A fragment with prim id <= 0, a pixel outside the B x H x W
+// image or a cluster beyond the bitset sets no bit.
+constexpr int kListRounds = kPerThread / 4;
+
+struct ListSource {
+    long long N;
+    int B, H, W;
+    const int* __restrict__ frag_pix;
+    const float* __restrict__ frag_attrs;
+    const int* __restrict__ prim2cluster;
+    unsigned kmax;
+    template <class Emit>
+    __device__ __forceinline__ void walk(Emit&& emit) const {
+#pragma unroll 1
+        for (int r = 0; r < kListRounds; ++r) {
+            const long long i0 = static_cast<long long>(blockIdx.x) * kStage +
+                                 static_cast<long long>(r) * (kBlock * 4) +
+                                 static_cast<long long>(threadIdx.x) * 4;
+            int pix[12], prim[4], c[4];
+            if (i0 + 4 <= N) {  // 48 and 64 bytes from 16-byte aligned rows
+                const int4* pp = reinterpret_cast<const int4*>(frag_pix + i0 * 3);
+                const int4 a = pp[0], b = pp[1], d = pp[2];
+                pix[0] = a.x; pix[1] = a.y; pix[2] = a.z; pix[3] = a.w;
+                pix[4] = b.x; pix[5] = b.y; pix[6] = b.z; pix[7] = b.w;
+                pix[8] = d.x; pix[9] = d.y; pix[10] = d.z; pix[11] = d.w;
+                const float4* pa = reinterpret_cast<const float4*>(frag_attrs) + i0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) prim[u] = f2i(pa[u].w) - 1;
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const long long i = i0 + u;
+                    prim[u] = -1;
+                    pix[u * 3] = pix[u * 3 + 1] = pix[u * 3 + 2] = 0;
+                    if (i < N) {
+                        pix[u * 3 + 0] = frag_pix[i * 3 + 0];
+                        pix[u * 3 + 1] = frag_pix[i * 3 + 1];
+                        pix[u * 3 + 2] = frag_pix[i * 3 + 2];
+                        prim[u] = f2i(frag_attrs[i * 4 + 3]) - 1;
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) c[u] = prim[u] >= 0 ? prim2cluster[prim[u]] : -1;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (prim[u] < 0 || static_cast<unsigned>(c[u]) >= kmax) continue;
+                const int b = pix[u * 3], h = pix[u * 3 + 1], w = pix[u * 3 + 2];
+                if (static_cast<unsigned>(b) >= static_cast<unsigned>(B) ||
+                    static_cast<unsigned>(h) >= static_cast<unsigned>(H) ||
+                    static_cast<unsigned>(w) >= static_cast<unsigned>(W))
+                    continue;
+                emit(b, h, w, c[u]);
+            }
+        }
+    }
+    __device__ __forceinline__ void reset() const {}
+};
+
+// The stage pass over one workgroup's fragments from `src`.
+template <class Source>
+__device__ __forceinline__ void bin_stage(const Source& src, TileGrid g,
+                                          int* __restrict__ runs_per_slot,
+                                          int* __restrict__ counters,
+                                          int* __restrict__ block_runs,
+                                          unsigned* __restrict__ entries, int ovf_entries,
+                                          BinRecord* __restrict__ records, int ovf_runs,
+                                          int* keys, int* vals, unsigned* stage, int& spill,
+                                          int& full, int& base_e, int& base_r, int* sh) {
     const long long fixed_e = static_cast<long long>(gridDim.x) * kStage;
     const long long fixed_r = static_cast<long long>(gridDim.x) * kBinRecs;
     const int group = static_cast<int>(blockIdx.x % kBinGroups);
@@ -211,7 +334,10 @@ dda_bin_stage_kernel(long long P, int H, int W, int V, const float* __restrict__
         keys[s] = -1;
         vals[s] = 0;
     }
-    if (threadIdx.x == 0) ndeferred = spill = full = 0;
+    if (threadIdx.x == 0) {
+        src.reset();
+        spill = full = 0;
+    }
     __syncthreads();
     // thread t's i-th staged fragment lives at i * kBlock + t: consecutive
     // lanes touch consecutive LDS words
@@ -220,8 +346,7 @@ dda_bin_stage_kernel(long long P, int H, int W, int V, const float* __restrict__
         if (slot >= 0 && len > 0) atomicAdd(&vals[slot], len);
         len = 0;
     };
-    walk_edges(P, H, W, V, pos, E, edges, edge2cluster, deferred, &ndeferred,
-               [&](int b, int h, int w, int c) {
+    src.walk([&](int b, int h, int w, int c) {
                    const int key = tile_of(g, b, h, w);
                    if (key != cur) {
                        flush();
@@ -242,12 +367,11 @@ dda_bin_stage_kernel(long long P, int H, int W, int V, const float* __restrict__
     __syncthreads();
     if (full) {  // workgroup-uniform: one run per fragment, in the overflow areas
         if (threadIdx.x == 0) {
-            ndeferred = 0;
+            src.reset();
             block_runs[blockIdx.x] = 0;
         }
         __syncthreads();
-        walk_edges(P, H, W, V, pos, E, edges, edge2cluster, deferred, &ndeferred,
-                   [&](int b, int h, int w, int c) {
+        src.walk([&](int b, int h, int w, int c) {
                        const int ks = tile_of(g, b, h, w) * kBinGroups + group;
                        const int j = atomicAdd(&counters[kEntriesOvf], 1);
                        const int r = atomicAdd(&counters[kRunsOvf], 1);
@@ -352,12 +476,11 @@ dda_bin_stage_kernel(long long P, int H, int W, int V, const float* __restrict__
     // the stage words now serve as per-slot cursors
     __syncthreads();
     for (int s = threadIdx.x; s < kHash; s += kBlock) stage[s] = 0u;
-    if (threadIdx.x == 0) ndeferred = 0;
+    if (threadIdx.x == 0) src.reset();
     __syncthreads();
     cur = -1;
     slot = -1;
-    walk_edges(P, H, W, V, pos, E, edges, edge2cluster, deferred, &ndeferred,
-               [&](int b, int h, int w, int c) {
+    src.walk([&](int b, int h, int w, int c) {
                    const int key = tile_of(g, b, h, w);
                    if (key != cur) {
                        cur = key;
@@ -370,6 +493,45 @@ dda_bin_stage_kernel(long long P, int H, int W, int V, const float* __restrict__
                });
 }
 
+// LDS of a stage workgroup (25.6 KB with the walk's deferred list).
+#define S2C_STAGE_LDS                                                                  \
+    __shared__ int keys[kHash]; /* tile of each slot, then per-slot cursors */         \
+    __shared__ int vals[kHash]; /* fragments per slot, then slot offsets */            \
+    __shared__ unsigned stage[kStage];                                                 \
+    __shared__ int spill, full, base_e, base_r;                                        \
+    __shared__ int sh[kBlock / kWave + 1];
+
+// 80 VGPRs (6 waves per SIMD, which the 25.6 KB of LDS also allows)
+// instead of the 82 the compiler picks unconstrained (5 waves); no spills.
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8)))
+dda_bin_stage_kernel(long long P, int H, int W, int V, const float* __restrict__ pos, int E,
+                     const int* __restrict__ edges, const int* __restrict__ edge2cluster,
+                     TileGrid g, int* __restrict__ runs_per_slot, int* __restrict__ counters,
+                     int* __restrict__ block_runs, unsigned* __restrict__ entries, int ovf_entries,
+                     BinRecord* __restrict__ records, int ovf_runs) {
+    S2C_STAGE_LDS
+    __shared__ int deferred[kDeferred];
+    __shared__ int ndeferred;
+    const EdgeSource src{P, H, W, V, pos, E, edges, edge2cluster, deferred, &ndeferred};
+    bin_stage(src, g, runs_per_slot, counters, block_runs, entries, ovf_entries, records,
+              ovf_runs, keys, vals, stage, spill, full, base_e, base_r, sh);
+}
+
+// The stage pass over a fragment list: kStage fragments per workgroup.
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8)))
+list_bin_stage_kernel(long long N, int B, int H, int W, const int* __restrict__ frag_pix,
+                      const float* __restrict__ frag_attrs, const int* __restrict__ prim2cluster,
+                      unsigned kmax, TileGrid g, int* __restrict__ runs_per_slot,
+                      int* __restrict__ counters, int* __restrict__ block_runs,
+                      unsigned* __restrict__ entries, int ovf_entries,
+                      BinRecord* __restrict__ records, int ovf_runs) {
+    S2C_STAGE_LDS
+    const ListSource src{N, B, H, W, frag_pix, frag_attrs, prim2cluster, kmax};
+    bin_stage(src, g, runs_per_slot, counters, block_runs, entries, ovf_entries, records,
+              ovf_runs, keys, vals, stage, spill, full, base_e, base_r, sh);
+}
+#undef S2C_STAGE_LDS
+
 }  // namespace s2c
 
 using namespace s2c;
@@ -381,34 +543,16 @@ extern "C" int s2c_dda_cluster_mask(int B, int H, int W, int V, const float* pos
                                       bitset, stream);
 }
 
-extern "C" int s2c_dda_cluster_mask_tiled(int B, int H, int W, int V, const float* pos, int E,
-                                          const int* edges, const int* edge2cluster,
-                                          int num_slots, int tile_log2, int* bitset,
-                                          void* stream) {
-    S2C_CHECK(B >= 0 && H >= 0 && W >= 0 && V >= 0 && E >= 0 && num_slots >= 0,
-              "dda_cluster_mask: negative size");
-    S2C_CHECK(tile_log2 == 0 || (tile_log2 >= 2 && tile_log2 <= 6),
-              "dda_cluster_mask: tile_log2 must be 0 or in [2, 6]");
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const long long words = static_cast<long long>(B) * H * W * num_slots;
-    const long long P = static_cast<long long>(B) * E;
-    if (words == 0) return S2C_OK;
-    BinPlan plan;
-    plan.blocks = (P + kPrimsPerBlock - 1) / kPrimsPerBlock;
-    const bool tiled = P > 0 && choose_tiles(B, H, W, num_slots, 4096, plan.g, tile_log2);
-    S2C_CHECK(tiled || tile_log2 == 0 || P == 0,
-              "dda_cluster_mask: the requested tile bitset does not fit LDS");
-    if (!tiled) {
-        // Empty input, or a tile bitset too large for LDS: clear + direct ORs.
-        if (int r = fill_u32(bitset, 0u, words, s)) return r;
-        if (P == 0) return S2C_OK;
-        dda_mask_direct_kernel<<<static_cast<unsigned>(plan.blocks), kBlock, 0, s>>>(
-            P, H, W, V, pos, E, edges, edge2cluster, num_slots, bitset);
-        S2C_LAUNCH_CHECK();
-        return S2C_OK;
-    }
-    S2C_CHECK(plan.blocks * kStage < INT_MAX / 2 && plan.blocks * kBinRecs < INT_MAX / 2,
-              "dda_cluster_mask: too many primitives for 32-bit entry offsets");
+namespace {
+
+// The binned pass after its tile grid is chosen: workspace, counters,
+// stage (stage(plan) launches the stage kernel over plan.blocks
+// workgroups), resolve, and the device-side fallback (fallback(plan)
+// launches a kernel that ORs every fragment into the zeroed bitset when the
+// overflow areas were too small, and exits at once otherwise).
+template <class Stage, class Fallback>
+int binned_mask(BinPlan& plan, int H, int W, int num_slots, int* bitset, hipStream_t s,
+                Stage&& stage, Fallback&& fallback) {
     std::lock_guard<std::mutex> held(BinWorkspace::lock());
     void* work = nullptr;
     if (int r = BinWorkspace::get(s, BinWorkspace::kPlanInts,
@@ -454,21 +598,101 @@ extern "C" int s2c_dda_cluster_mask_tiled(int B, int H, int W, int V, const floa
     plan.records = static_cast<BinRecord*>(rbuf);
     plan.runs = static_cast<BinRun*>(obuf);
     if (int r = bin_clear(plan, s)) return r;
-    dda_bin_stage_kernel<<<static_cast<unsigned>(plan.blocks), kBlock, 0, s>>>(
-        P, H, W, V, pos, E, edges, edge2cluster, plan.g, plan.runs_per_slot, plan.counters,
-        plan.block_runs, plan.entries, plan.ovf_entries, plan.records, plan.ovf_runs);
+    stage(plan);
     S2C_LAUNCH_CHECK();
     if (int r = bin_counters_async(plan, cnt->host, s)) return r;
     S2C_HIP(hipEventRecord(cnt->done, s));
     cnt->pending = true;
     if (int r = bin_resolve(plan, H, W, num_slots, bitset, s)) return r;
-    dda_mask_fallback_kernel<<<static_cast<unsigned>(dmin(plan.blocks,
-                                                          static_cast<long long>(kFallbackGrid))),
-                               kBlock, 0, s>>>(P, H, W, V, pos, E, edges, edge2cluster,
-                                               num_slots, bitset, plan.counters, plan.ovf_entries,
-                                               plan.ovf_runs, plan.blocks);
+    fallback(plan);
     S2C_LAUNCH_CHECK();
     return S2C_OK;
+}
+
+unsigned fallback_grid(long long blocks) {
+    return static_cast<unsigned>(dmin(blocks, static_cast<long long>(kFallbackGrid)));
+}
+
+}  // namespace
+
+extern "C" int s2c_dda_cluster_mask_tiled(int B, int H, int W, int V, const float* pos, int E,
+                                          const int* edges, const int* edge2cluster,
+                                          int num_slots, int tile_log2, int* bitset,
+                                          void* stream) {
+    S2C_CHECK(B >= 0 && H >= 0 && W >= 0 && V >= 0 && E >= 0 && num_slots >= 0,
+              "dda_cluster_mask: negative size");
+    S2C_CHECK(tile_log2 == 0 || (tile_log2 >= 2 && tile_log2 <= 6),
+              "dda_cluster_mask: tile_log2 must be 0 or in [2, 6]");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long words = static_cast<long long>(B) * H * W * num_slots;
+    const long long P = static_cast<long long>(B) * E;
+    if (words == 0) return S2C_OK;
+    BinPlan plan;
+    plan.blocks = (P + kPrimsPerBlock - 1) / kPrimsPerBlock;
+    const bool tiled = P > 0 && choose_tiles(B, H, W, num_slots, 4096, plan.g, tile_log2);
+    S2C_CHECK(tiled || tile_log2 == 0 || P == 0,
+              "dda_cluster_mask: the requested tile bitset does not fit LDS");
+    if (!tiled) {
+        // Empty input, or a tile bitset too large for LDS: clear + direct ORs.
+        if (int r = fill_u32(bitset, 0u, words, s)) return r;
+        if (P == 0) return S2C_OK;
+        dda_mask_direct_kernel<<<static_cast<unsigned>(plan.blocks), kBlock, 0, s>>>(
+            P, H, W, V, pos, E, edges, edge2cluster, num_slots, bitset);
+        S2C_LAUNCH_CHECK();
+        return S2C_OK;
+    }
+    S2C_CHECK(plan.blocks * kStage < INT_MAX / 2 && plan.blocks * kBinRecs < INT_MAX / 2,
+              "dda_cluster_mask: too many primitives for 32-bit entry offsets");
+    return binned_mask(
+        plan, H, W, num_slots, bitset, s,
+        [&](const BinPlan& p) {
+            dda_bin_stage_kernel<<<static_cast<unsigned>(p.blocks), kBlock, 0, s>>>(
+                P, H, W, V, pos, E, edges, edge2cluster, p.g, p.runs_per_slot, p.counters,
+                p.block_runs, p.entries, p.ovf_entries, p.records, p.ovf_runs);
+        },
+        [&](const BinPlan& p) {
+            dda_mask_fallback_kernel<<<fallback_grid(p.blocks), kBlock, 0, s>>>(
+                P, H, W, V, pos, E, edges, edge2cluster, num_slots, bitset, p.counters,
+                p.ovf_entries, p.ovf_runs, p.blocks);
+        });
+}
+
+extern "C" int s2c_fragment_cluster_mask(int B, int H, int W, int num_frags, const int* frag_pix,
+                                         const float* frag_attrs, const int* prim2cluster,
+                                         int num_slots, int* bitset, void* stream) {
+    S2C_CHECK(B >= 0 && H >= 0 && W >= 0 && num_frags >= 0 && num_slots >= 0,
+              "fragment_cluster_mask: negative size");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long words = static_cast<long long>(B) * H * W * num_slots;
+    if (words == 0) return S2C_OK;
+    const long long N = num_frags;
+    const unsigned kmax = static_cast<unsigned>(dmin(static_cast<long long>(num_slots) * 32,
+                                                     static_cast<long long>(UINT_MAX)));
+    BinPlan plan;
+    plan.blocks = (N + kStage - 1) / kStage;
+    const bool tiled = N > 0 && choose_tiles(B, H, W, num_slots, 4096, plan.g);
+    if (!tiled) {
+        if (int r = fill_u32(bitset, 0u, words, s)) return r;
+        if (N == 0) return S2C_OK;
+        list_mask_direct_kernel<<<grid_for(N, kBlock), kBlock, 0, s>>>(
+            N, B, H, W, frag_pix, frag_attrs, prim2cluster, kmax, num_slots, bitset);
+        S2C_LAUNCH_CHECK();
+        return S2C_OK;
+    }
+    S2C_CHECK(plan.blocks * kStage < INT_MAX / 2 && plan.blocks * kBinRecs < INT_MAX / 2,
+              "fragment_cluster_mask: too many fragments for 32-bit entry offsets");
+    return binned_mask(
+        plan, H, W, num_slots, bitset, s,
+        [&](const BinPlan& p) {
+            list_bin_stage_kernel<<<static_cast<unsigned>(p.blocks), kBlock, 0, s>>>(
+                N, B, H, W, frag_pix, frag_attrs, prim2cluster, kmax, p.g, p.runs_per_slot,
+                p.counters, p.block_runs, p.entries, p.ovf_entries, p.records, p.ovf_runs);
+        },
+        [&](const BinPlan& p) {
+            list_mask_fallback_kernel<<<fallback_grid(p.blocks), kBlock, 0, s>>>(
+                N, B, H, W, frag_pix, frag_attrs, prim2cluster, kmax, num_slots, bitset,
+                p.counters, p.ovf_entries, p.ovf_runs);
+        });
 }
 
 extern "C" int s2c_dda_cluster_mask_reset_estimates(void) {

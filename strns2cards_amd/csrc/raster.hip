@@ -383,6 +383,34 @@ extern "C" int s2c_backward_interpolate_triangle_attributes(int B, int H, int W,
     return S2C_OK;
 }
 
+// Largest view index frag_pix[i][0] (rasterize.py:197, B = max + 1): four
+// rows (three 16-byte loads) per thread and step, a workgroup maximum and
+// one atomicMax per workgroup (one per wave, all on one address, took 0.26
+// ms for 16K waves).
+constexpr int kMaxViewGrid = 1024;
+
+__global__ void __launch_bounds__(kBlock)
+max_frag_view_kernel(long long n, const int* __restrict__ frag_pix, int* __restrict__ out) {
+    __shared__ int wm[kBlock / kWave];
+    int m = INT_MIN;
+    const long long quads = n / 4;
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    const long long t0 = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    const int4* p4 = reinterpret_cast<const int4*>(frag_pix);
+    for (long long q = t0; q < quads; q += stride) {
+        const int4 a = p4[q * 3], b = p4[q * 3 + 1], c = p4[q * 3 + 2];
+        m = dmax(dmax(m, a.x), dmax(a.w, dmax(b.z, c.y)));
+    }
+    for (long long i = quads * 4 + t0; i < n; i += stride) m = dmax(m, frag_pix[i * 3]);
+    m = wave_max(m);
+    if (lane_id() == 0) wm[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / kWave; ++k) m = dmax(m, wm[k]);
+        atomicMax(out, m);
+    }
+}
+
 extern "C" int s2c_cluster_mask_from_fragments(int num_frags, const int* frag_pix,
                                                const float* frag_attrs,
                                                const int* prim2cluster, int H, int W,
@@ -411,4 +439,22 @@ extern "C" int s2c_triangle_cluster_mask(int B, int H, int W, int V, const float
                                                                tri2cluster, num_slots, bitset);
     S2C_LAUNCH_CHECK();
     return S2C_OK;
+}
+
+extern "C" int s2c_max_frag_view(int num_frags, const int* frag_pix, int* max_out,
+                                 void* stream) {
+    S2C_CHECK(max_out != nullptr, "max_frag_view: null output pointer");
+    S2C_CHECK(num_frags > 0, "max_frag_view: no fragments");
+    S2C_CHECK((reinterpret_cast<uintptr_t>(frag_pix) & 15) == 0,
+              "max_frag_view: frag_pix not 16-byte aligned");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Temp t;
+    if (int r = t.alloc(sizeof(int), s)) return r;
+    if (int r = fill_u32(t.ptr, 0x80000000u, 1, s)) return r;
+    const long long quads = (static_cast<long long>(num_frags) + 3) / 4;
+    const long long grid = dmin((quads + kBlock - 1) / kBlock, static_cast<long long>(kMaxViewGrid));
+    max_frag_view_kernel<<<static_cast<unsigned>(grid), kBlock, 0, s>>>(num_frags, frag_pix,
+                                                                          t.as<int>());
+    S2C_LAUNCH_CHECK();
+    return read_device_int(t.as<int>(), s, max_out);
 }

@@ -142,13 +142,22 @@ def cluster_mask_from_fragments(
     if not (frag_pix.is_cuda and frag_attrs.is_cuda and tri2cluster.is_cuda):
         raise ValueError("All input tensors must be CUDA tensors.")
 
-    B = int(frag_pix[:, 0].max().item()) + 1
+    # the reference's frag_pix[:, 0].max() + 1, read with 16-byte loads over
+    # the rows (torch's strided column copy and reduction took 4x longer)
+    if frag_pix.shape[0] == 0 or not frag_pix.is_contiguous() or frag_pix.data_ptr() % 16:
+        B = int(frag_pix[:, 0].max().item()) + 1
+    else:
+        B = _core.max_frag_view(frag_pix) + 1
     if cluster_count is None:
         cluster_count = int(tri2cluster.max().item()) + 1
     S = (cluster_count + 31) // 32
-    bitset = torch.zeros((B, H, W, S), dtype=torch.int32, device=frag_pix.device)
-    _core.cluster_mask_from_fragments(frag_pix.contiguous(), frag_attrs.contiguous(),
-                                      tri2cluster.contiguous(), bitset)
+    # The reference's zeroed bitset and one atomic OR per fragment, as one
+    # binned pass that writes every word (the same bits for fragments the
+    # reference's kernel can address: pixels inside the image, cluster ids
+    # below 32 * S).
+    bitset = torch.empty((B, H, W, S), dtype=torch.int32, device=frag_pix.device)
+    _core.fragment_cluster_mask(frag_pix.contiguous(), frag_attrs.contiguous(),
+                                tri2cluster.contiguous(), bitset)
     return bitset
 
 

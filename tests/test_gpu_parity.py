@@ -286,6 +286,97 @@ def test_dda_mask_depth_boundaries(dev):
     np.testing.assert_array_equal(got, want)
 
 
+def list_mask(dev, B, res, fp, fa, p2c, K):
+    """s2c_fragment_cluster_mask over a fragment list (every word written)."""
+    S = (K + 31) // 32
+    bits = torch.full((B, res[0], res[1], S), 0x5A5A5A5A, dtype=torch.int32, device=dev)
+    call("s2c_fragment_cluster_mask", B, res[0], res[1], fp.shape[0], T(fp, dev), T(fa, dev),
+         T(p2c, dev), S, bits)
+    return bits.cpu().numpy()
+
+
+@pytest.mark.parametrize("K", [1, 33, 1024, 17000])  # 17000: tile bitset exceeds LDS -> direct ORs
+def test_fragment_mask_lists(dev, K):
+    """The binned mask of a fragment list (rasterize.py:170-214) against the
+    oracle's cluster_mask_from_fragments, for DDA and triangle fragments."""
+    report(f"fragment-list mask K={K}")
+    rng = np.random.default_rng(K + 5)
+    res = (61, 70)
+    pos, edges = random_lines(rng, 2, 300, 2500, *res, persp=True)
+    fp, fa = oracle.dda_compute_fragments(res, pos, edges)
+    e2c = rng.integers(0, K, size=edges.shape[0]).astype(np.int32)
+    want = oracle.cluster_mask_from_fragments(res, fp, fa, e2c, K, B=2)
+    np.testing.assert_array_equal(list_mask(dev, 2, res, fp, fa, e2c, K), want)
+    tpos = random_tris(rng, 2, 200, 300, persp=True)[0]
+    tri = rng.integers(0, 200, size=(300, 3)).astype(np.int32)
+    tfp, tfa = oracle.compute_fragments(res, tpos, tri)
+    t2c = rng.integers(0, K, size=300).astype(np.int32)
+    want = oracle.cluster_mask_from_fragments(res, tfp, tfa, t2c, K, B=2)
+    np.testing.assert_array_equal(list_mask(dev, 2, res, tfp, tfa, t2c, K), want)
+
+
+@pytest.mark.parametrize("n", [1, 3, 4, 5, 4097, 300001])
+def test_max_frag_view(dev, n):
+    """s2c_max_frag_view: the largest view index of a fragment list, for
+    lengths around its 4-row steps."""
+    rng = np.random.default_rng(n)
+    fp = np.stack([rng.integers(-5, 40, n), rng.integers(0, 9, n), rng.integers(0, 9, n)],
+                  1).astype(np.int32)
+    fp[rng.integers(0, n), 0] = 57
+    m = call_count("s2c_max_frag_view", n, T(fp, dev))
+    assert m == fp[:, 0].max() == 57
+
+
+def test_fragment_mask_invalid_entries(dev):
+    """Prim ids <= 0, pixels outside the image and clusters beyond the
+    bitset set no bit; the other fragments set theirs."""
+    report("fragment-list mask with invalid entries")
+    rng = np.random.default_rng(77)
+    res, B, K, n = (40, 52), 3, 70, 20000
+    fp = np.stack([rng.integers(0, B, n), rng.integers(0, res[0], n),
+                   rng.integers(0, res[1], n)], 1).astype(np.int32)
+    fa = rng.uniform(size=(n, 4)).astype(np.float32)
+    fa[:, 3] = rng.integers(1, 500, n)
+    p2c = rng.integers(0, K, size=500).astype(np.int32)
+    bad = rng.choice(n, size=3000, replace=False)
+    fa[bad[:600], 3] = 0.0
+    fa[bad[600:900], 3] = -3.0
+    fp[bad[900:1300], 0] = B
+    fp[bad[1300:1700], 1] = -1
+    fp[bad[1700:2100], 2] = res[1]
+    p2c_bad = p2c.copy()
+    p2c_bad[:20] = 96  # >= 32 * S (S = 3)
+    ok = np.ones(n, bool)
+    ok[bad[:2100]] = False
+    prim = fa[:, 3].astype(np.int64) - 1
+    ok &= ~np.isin(prim, np.arange(20))
+    want = oracle.cluster_mask_from_fragments(res, fp[ok], fa[ok], p2c_bad, K, B=B)
+    np.testing.assert_array_equal(list_mask(dev, B, res, fp, fa, p2c_bad, K), want)
+
+
+def test_fragment_mask_overflow_fallback_then_sized(dev):
+    """Fragments at random pixels of 4 views at 1024^2: every workgroup's
+    4,096 fragments touch more tiles than its LDS table holds, so all go to
+    the overflow areas; from the smallest estimate the pass falls back to
+    direct ORs on the device, then runs binned with the areas sized."""
+    import capi
+
+    report("fragment-list mask overflow fallback, then sized")
+    rng = np.random.default_rng(91)
+    res, B, K, n = (1024, 1024), 4, 256, 300000
+    fp = np.stack([rng.integers(0, B, n), rng.integers(0, res[0], n),
+                   rng.integers(0, res[1], n)], 1).astype(np.int32)
+    fa = np.zeros((n, 4), np.float32)
+    fa[:, 3] = rng.integers(1, 1000, n)
+    p2c = rng.integers(0, K, size=1000).astype(np.int32)
+    want = oracle.cluster_mask_from_fragments(res, fp, fa, p2c, K, B=B)
+    torch.cuda.synchronize()
+    assert capi.lib().s2c_dda_cluster_mask_reset_estimates() == 0
+    np.testing.assert_array_equal(list_mask(dev, B, res, fp, fa, p2c, K), want)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(list_mask(dev, B, res, fp, fa, p2c, K), want)
+
+
 @pytest.mark.slow
 def test_dda_mask_full_workload(dev):
     report("dda->mask full workload (50k strands x 100, 4 views 1024^2, K=1024)")
@@ -293,6 +384,12 @@ def test_dda_mask_full_workload(dev):
     want = oracle.dda_cluster_mask((w.H, w.W), w.pos, w.edges, w.edge2cluster, 1024, threads=16)
     got = fused_dda_mask(dev, (w.H, w.W), w.pos, w.edges, w.edge2cluster, 1024)
     np.testing.assert_array_equal(got, want)
+    # the drop-in pair (dda_compute_fragments, cluster_mask_from_fragments)
+    import strns2cards_amd as s2c
+    pos, edges = T(w.pos, dev), T(w.edges, dev)
+    fp, fa = s2c.dda_compute_fragments((w.H, w.W), pos, edges)
+    got = s2c.cluster_mask_from_fragments((w.H, w.W), fp, fa, T(w.edge2cluster, dev), 1024)
+    np.testing.assert_array_equal(got.cpu().numpy(), want)
 
 
 # ------------------------------------------------------------ triangles
