@@ -41,16 +41,16 @@ dda_span_kernel(int P, const int* __restrict__ edge_ids, int H, int W, int V,
 constexpr int kExpandItems = 4;
 constexpr int kExpand = kBlock * kExpandItems;
 
-// first[b] = the primitive holding candidate b * kExpand (prefix inclusive),
-// for every b <= last with b * kExpand < prefix[P - 1].
+// first[b] = the primitive holding item b * rows of an inclusive prefix,
+// for every b <= last with b * rows < prefix[P - 1].
 __global__ void __launch_bounds__(kBlock)
-dda_block_first_kernel(int P, const int* __restrict__ prefix, int last,
+dda_block_first_kernel(int P, const int* __restrict__ prefix, int rows, int last,
                        int* __restrict__ first) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= P) return;
     const long long beg = p == 0 ? 0 : prefix[p - 1];
     const long long end = prefix[p];
-    for (long long b = dmax((beg + kExpand - 1) / kExpand, 0ll); b <= last && b * kExpand < end; ++b)
+    for (long long b = dmax((beg + rows - 1) / rows, 0ll); b <= last && b * rows < end; ++b)
         first[b] = p;
 }
 
@@ -200,24 +200,27 @@ dda_survivors_kernel(int P, const int* __restrict__ edge_ids, int H, int W, int 
     counts[p] = n;
 }
 
-__global__ void __launch_bounds__(kBlock)
+constexpr int kEmitRows = 512;  // output rows per workgroup (1,024: 0.66 ms, 512: 0.56 ms)
+constexpr int kEmitThreads = 256;
+
+__global__ void __launch_bounds__(kEmitThreads)
 dda_emit_kernel(int P, int N, const int* __restrict__ prefix, const int* __restrict__ first,
                 const int* __restrict__ edge_ids, int H, int W, int V,
                 const float* __restrict__ pos, int E, const int* __restrict__ edges,
                 int* __restrict__ frag_pix, float* __restrict__ frag_attrs) {
-    __shared__ float4 stage_attr[kExpand];
-    __shared__ int stage_pix[kExpand * 3];
+    __shared__ float4 stage_attr[kEmitRows];
+    __shared__ __align__(16) int stage_pix[kEmitRows * 3];
     const int blk = blockIdx.x;
-    const long long f0 = static_cast<long long>(blk) * kExpand;
+    const long long f0 = static_cast<long long>(blk) * kEmitRows;
     const int total = prefix[P - 1];
     const int rows = static_cast<int>(dmin(static_cast<long long>(dmin(total, N)) - f0,
-                                           static_cast<long long>(kExpand)));
+                                           static_cast<long long>(kEmitRows)));
     if (rows <= 0) return;  // workgroup-uniform
     const int lo = dmin(dmax(first[blk], 0), P - 1);
-    const int hi = f0 + kExpand < total ? dmin(dmax(first[blk + 1], lo), P - 1) : P - 1;
-    for (int p = lo + static_cast<int>(threadIdx.x); p <= hi; p += kBlock) {
+    const int hi = f0 + kEmitRows < total ? dmin(dmax(first[blk + 1], lo), P - 1) : P - 1;
+    for (int p = lo + static_cast<int>(threadIdx.x); p <= hi; p += kEmitThreads) {
         const long long beg = p == 0 ? 0 : prefix[p - 1];
-        if (prefix[p] <= beg || beg >= f0 + kExpand || prefix[p] <= f0) continue;
+        if (prefix[p] <= beg || beg >= f0 + kEmitRows || prefix[p] <= f0) continue;
         const int id = prim_id(edge_ids, p);
         if (id < 0) continue;
         const int b = id / E;
@@ -230,7 +233,7 @@ dda_emit_kernel(int P, int N, const int* __restrict__ prefix, const int* __restr
         bool wrote;
         const int cnt = dda_span(H, W, p0, p1, sp, wrote);
         long long q = beg;  // output row of the next survivor
-        for (int k = 0; k < cnt && q < f0 + kExpand; ++k) {
+        for (int k = 0; k < cnt && q < f0 + kEmitRows; ++k) {
             DdaFrag fr;
             if (!dda_frag(H, W, sp, k, p0, p1, fr)) continue;
             if (q >= f0) {
@@ -244,9 +247,16 @@ dda_emit_kernel(int P, int N, const int* __restrict__ prefix, const int* __restr
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < rows * 3; i += kBlock) frag_pix[f0 * 3 + i] = stage_pix[i];
+    // rows start at f0 * 12 bytes, a multiple of 16: 16-byte stores, then
+    // the tail of the last workgroup's rows
+    const int quads = rows * 3 / 4;
+    int4* pout = reinterpret_cast<int4*>(frag_pix + f0 * 3);
+    const int4* pst = reinterpret_cast<const int4*>(stage_pix);
+    for (int i = threadIdx.x; i < quads; i += kEmitThreads) pout[i] = pst[i];
+    for (int i = quads * 4 + threadIdx.x; i < rows * 3; i += kEmitThreads)
+        frag_pix[f0 * 3 + i] = stage_pix[i];
     float4* out = reinterpret_cast<float4*>(frag_attrs) + f0;
-    for (int i = threadIdx.x; i < rows; i += kBlock) out[i] = stage_attr[i];
+    for (int i = threadIdx.x; i < rows; i += kEmitThreads) out[i] = stage_attr[i];
 }
 
 // ---------------------------------------------------- attribute interp
@@ -314,14 +324,15 @@ extern "C" int s2c_dda_compute_span(int num_prims, const int* edge_ids, int H, i
 }
 
 namespace {
-// first[] of the fragment workgroups (kExpand candidates each) in `t`,
-// followed by `extra` ints.
-int block_first(int P, int N, const int* prefix, hipStream_t s, Temp& t, int extra,
+// first[] of the workgroups of `rows` items each in `t`, followed by
+// `extra` ints.
+int block_first(int P, int N, int rows, const int* prefix, hipStream_t s, Temp& t, int extra,
                 int** first) {
-    const int blocks = static_cast<int>((static_cast<long long>(N) + kExpand - 1) / kExpand);
+    const int blocks = static_cast<int>((static_cast<long long>(N) + rows - 1) / rows);
     if (int r = t.alloc(sizeof(int) * (static_cast<size_t>(blocks) + 1 + extra), s)) return r;
     *first = t.as<int>();
-    dda_block_first_kernel<<<grid_for(P, kBlock), kBlock, 0, s>>>(P, prefix, blocks, *first);
+    dda_block_first_kernel<<<grid_for(P, kBlock), kBlock, 0, s>>>(P, prefix, rows, blocks,
+                                                                  *first);
     S2C_LAUNCH_CHECK();
     return S2C_OK;
 }
@@ -339,7 +350,8 @@ extern "C" int s2c_dda_compute_fragments(int num_prims, int num_frags,
     hipStream_t s = static_cast<hipStream_t>(stream);
     Temp t;
     int* first = nullptr;
-    if (int r = block_first(num_prims, num_frags, frag_prefix_sum, s, t, 0, &first)) return r;
+    if (int r = block_first(num_prims, num_frags, kExpand, frag_prefix_sum, s, t, 0, &first))
+        return r;
     dda_fragments_kernel<<<grid_for(num_frags, kExpand), kBlock, 0, s>>>(
         num_prims, num_frags, frag_prefix_sum, first, edge_ids, frag_slopes, frag_spans, H, W,
         V, pos, E, edges, frag_pix, frag_attrs);
@@ -376,8 +388,9 @@ extern "C" int s2c_dda_emit_fragments(int num_prims, int num_frags, const int* v
     hipStream_t s = static_cast<hipStream_t>(stream);
     Temp t;
     int* first = nullptr;
-    if (int r = block_first(num_prims, num_frags, valid_prefix, s, t, 0, &first)) return r;
-    dda_emit_kernel<<<grid_for(num_frags, kExpand), kBlock, 0, s>>>(
+    if (int r = block_first(num_prims, num_frags, kEmitRows, valid_prefix, s, t, 0, &first))
+        return r;
+    dda_emit_kernel<<<grid_for(num_frags, kEmitRows), kEmitThreads, 0, s>>>(
         num_prims, num_frags, valid_prefix, first, edge_ids, H, W, V, pos, E, edges, frag_pix,
         frag_attrs);
     S2C_LAUNCH_CHECK();
