@@ -682,6 +682,41 @@ def copy_calibration(dev, nbytes=1 << 29, reps=3):
     print(f"[bench] copy calibration: {reps} x {nbytes} bytes", file=sys.stderr, flush=True)
 
 
+def api_form_section(s2c, w, pos, edges, e2c, reps=5):
+    """The drop-in API form at the full size: fragment lists (dda.py:5-76),
+    then the mask from them (rasterize.py:170-214), as 3_fitting.py:254-261
+    calls them.  Wall-clock per call (each holds host synchronizations)."""
+    import torch
+
+    res = (w.H, w.W)
+    B, V, _ = w.pos.shape
+    E = w.edges.shape[0]
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    fp, fa = s2c.dda_compute_fragments(res, pos, edges)
+    nf = int(fp.shape[0])
+    del fp, fa
+    ms_api = timed(lambda: s2c.dda_compute_fragments(res, pos, edges))
+    ms_api_mask = timed(lambda: s2c.cluster_mask_from_fragments(
+        res, *s2c.dda_compute_fragments(res, pos, edges), e2c, w.num_clusters))
+    api_bytes = B * V * 16 + E * 8 + nf * 28
+    print(f"[bench] API form: dda_compute_fragments {ms_api:.3f} ms ({nf} fragments, "
+          f"{api_bytes / ms_api / 1e6:.0f} GB/s), + cluster_mask_from_fragments {ms_api_mask:.3f} ms",
+          file=sys.stderr, flush=True)
+    return {"dda_api_fragments_ms": round(ms_api, 4), "dda_api_fragments": nf,
+            "dda_api_algorithmic_bytes": api_bytes,
+            "dda_api_GBps": round(api_bytes / ms_api / 1e6, 1),
+            "dda_api_fragments_plus_mask_ms": round(ms_api_mask, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -706,6 +741,9 @@ def main():
                     help="stop after the card raster and antialias timings (for PMC runs)")
     ap.add_argument("--card-reps", type=int, default=10,
                     help="timed repetitions of each card stage")
+    ap.add_argument("--api-only", action="store_true",
+                    help="only the drop-in API form (dda_compute_fragments, "
+                         "cluster_mask_from_fragments) at the full size")
     ap.add_argument("--dda-only", action="store_true",
                     help="time the DDA->mask step only (for PMC runs)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -757,6 +795,10 @@ def main():
     edges = torch.from_numpy(w.edges).to(dev)
     e2c = torch.from_numpy(w.edge2cluster).to(dev)
     bitset = torch.empty((B, w.H, w.W, S), dtype=torch.int32, device=dev)
+
+    if args.api_only:
+        print(json.dumps(api_form_section(s2c, w, pos, edges, e2c)), flush=True)
+        return
 
     from strns2cards_amd import autotune
 
@@ -843,19 +885,7 @@ def main():
                           "antialias_grad_ms": round(ms_aa, 4),
                           "antialias_grad_fused_ms": round(ms_aa_f, 4)}), flush=True)
         return
-    # the drop-in API form at the same size: fragment lists (dda.py:5-76),
-    # then the mask from them (rasterize.py:170-214)
-    res2 = (w.H, w.W)
-    fp, fa = s2c.dda_compute_fragments(res2, pos, edges)
-    nf = int(fp.shape[0])
-    del fp, fa
-    ms_api = timed(lambda: s2c.dda_compute_fragments(res2, pos, edges), reps=5)
-    ms_api_mask = timed(lambda: s2c.cluster_mask_from_fragments(
-        res2, *s2c.dda_compute_fragments(res2, pos, edges), e2c, w.num_clusters), reps=5)
-    api_bytes = B * V * 16 + E * 8 + nf * 28
-    print(f"[bench] API form: dda_compute_fragments {ms_api:.3f} ms ({nf} fragments, "
-          f"{api_bytes / ms_api / 1e6:.0f} GB/s), + cluster_mask_from_fragments {ms_api_mask:.3f} ms",
-          file=sys.stderr, flush=True)
+    api = api_form_section(s2c, w, pos, edges, e2c)
 
     fit = None if args.no_fit else fit_section(dev, num_jobs=args.fit_jobs,
                                                    tune=not args.no_autotune, cpu=not args.no_cpu)
@@ -900,10 +930,7 @@ def main():
         "cpu_baseline": cpu,
         "extra": {"card_raster_mask_ms": round(ms_tri, 4), "antialias_grad_ms": round(ms_aa, 4),
                   "antialias_grad_fused_ms": round(ms_aa_f, 4),
-                  "dda_api_fragments_ms": round(ms_api, 4), "dda_api_fragments": nf,
-                  "dda_api_algorithmic_bytes": api_bytes,
-                  "dda_api_GBps": round(api_bytes / ms_api / 1e6, 1),
-                  "dda_api_fragments_plus_mask_ms": round(ms_api_mask, 4),
+                  **api,
                   "cards": int(c.tri.shape[0] // 40), "card_triangles": int(c.tri.shape[0])},
         "card_fit": fit,
         "adjoint_fit": adj,
