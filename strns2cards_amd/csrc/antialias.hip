@@ -215,21 +215,37 @@ __device__ __forceinline__ unsigned plane_code(const int2* planes, long long bas
            (((static_cast<unsigned>(v.y) >> kb) & 1u) << 1);
 }
 
-// Residual bits from the (B,H,W,S) bitsets or their word planes, or from a
-// per-wave LDS box.
+// Residual bits from the (B,H,W,S) bitsets or their word planes.
 struct ResidualSrc {
-    const unsigned char* box;  // staged bytes, or nullptr for global reads
-    int h0, w0, width;
     const int* bitset;
     const int* target;
     const int2* planes;  // word planes, or nullptr to read the records
     long long plane;     // first pixel of the view / first entry of the word plane
     int W, S, k;
     __device__ __forceinline__ unsigned operator()(int h, int w) const {
-        if (box) return box[(h - h0) * width + (w - w0)];
         if (planes) return plane_code(planes, plane, W, h, w, k);
         const long long pix = plane + w + static_cast<long long>(h) * W;
         return bit_u(bitset, pix, S, k) | (bit_u(target, pix, S, k) << 1);
+    }
+};
+
+// A wave's staged box of residual bits: rows h0 .. h0 + rows - 1 and
+// columns w0 .. w0 + wpr * 32 - 1, one bit per pixel in two planes
+// (rendered, target) of `wpr` words per row.
+struct BoxBits {
+    const unsigned* render;
+    const unsigned* target;
+    int h0, w0, wpr;
+    __device__ __forceinline__ unsigned operator()(int h, int w) const {  // as ResidualSrc
+        const int i = (h - h0) * wpr * 32 + (w - w0);
+        return ((render[i >> 5] >> (i & 31)) & 1u) | (((target[i >> 5] >> (i & 31)) & 1u) << 1);
+    }
+    // the window-row form (aa_adjoint_sum_rows)
+    __device__ __forceinline__ void operator()(int h, int w, int n, unsigned& diff,
+                                               unsigned& tgt) const {
+        const int r = (h - h0) * wpr;
+        tgt = bits_at(target + r, w - w0, n);
+        diff = bits_at(render + r, w - w0, n) ^ tgt;
     }
 };
 
@@ -271,13 +287,18 @@ __device__ __forceinline__ void wave_add_runs(float* d_pos, bool live, long long
 
 // antialias.cu:179-279.  A wave takes 64 consecutive fragments.  When all of
 // its live samples share one view and cluster (consecutive silhouette
-// fragments of one card) and the union of their windows fits kAaBox bytes,
-// the wave stages that box's residual bits in LDS once; otherwise every
-// window pixel is read from the bitsets directly.  PLANES: the bits come
-// from the word planes of B views (residual_planes_kernel); fragments of
-// views >= B are dropped.
+// fragments of one card) and the union of their windows fits the wave's
+// kAaBoxWords, the wave stages that box's residual bits in LDS once, as
+// bit rows (ballots of 64 pixels), and each sample reads its window a row
+// at a time (aa_adjoint_sum_rows: the same terms in the same order);
+// otherwise every window pixel is read from the bitsets directly.  PLANES:
+// the bits come from the word planes of B views (residual_planes_kernel);
+// fragments of views >= B are dropped.
 constexpr int kAaWaves = kBlock / kWave;
-constexpr int kAaBox = 8192;
+// Words per wave for the staged box's two bit planes (4 KB): up to 512
+// words of rows, e.g. 15 rows of 34 words (1,088 pixels).  The box was a
+// byte per pixel (8 KB for 8,192 pixels), read a pixel at a time.
+constexpr int kAaBoxWords = 1024;
 
 // Orders a wave's own LDS accesses: the LDS executes one wave's operations
 // in issue order, so a buffer private to the wave needs only a compiler
@@ -299,7 +320,7 @@ __device__ __forceinline__ void aa_wave_fragment(
     const float* __restrict__ pos, int E, const int* __restrict__ edges,
     const float* __restrict__ normals, const int* __restrict__ edge2cluster,
     float* __restrict__ d_pos, float R, float rho, int B, const int2* __restrict__ planes,
-    unsigned char* box) {
+    unsigned* box) {
     // ---- per-sample setup, as the reference (antialias.cu:198-256)
     bool live = false;
     int k = 0, v0 = 0, v1 = 0;
@@ -321,7 +342,7 @@ __device__ __forceinline__ void aa_wave_fragment(
 
     // ---- wave-uniform staging decision
     const unsigned long long lm = __ballot(live);
-    int bh0 = 0, bw0 = 0, bwid = 0;
+    int bh0 = 0, bw0 = 0, wpr = 0;
     bool staged = false;
     if (lm) {
         const int first = __ffsll(static_cast<long long>(lm)) - 1;
@@ -336,22 +357,39 @@ __device__ __forceinline__ void aa_wave_fragment(
             bw0 = wave_min(lo_w);
             const long long hh = static_cast<long long>(wave_max(hi_h)) - bh0 + 1;
             const long long ww = static_cast<long long>(wave_max(hi_w)) - bw0 + 1;
-            if (hh > 0 && ww > 0 && hh * ww <= kAaBox) {
+            const long long words = hh * ((ww + 31) / 32);
+            if (hh > 0 && ww > 0 && 2 * words <= kAaBoxWords) {
                 staged = true;
-                bwid = static_cast<int>(ww);
-                const int area = static_cast<int>(hh * ww);
+                wpr = static_cast<int>((ww + 31) / 32);
+                const int lane = lane_id();
                 const long long plane = static_cast<long long>(H) * W * b_ref;
                 const long long pbase = (static_cast<long long>(b_ref) * S + (k_ref >> 5)) *
                                         (static_cast<long long>(H) * W);
-                for (int q = lane_id(); q < area; q += kWave) {
-                    const int y = bh0 + q / bwid, x = bw0 + q % bwid;  // in [0,H) x [0,W)
-                    if (PLANES) {
-                        box[q] = static_cast<unsigned char>(
-                            plane_code(planes, pbase, W, y, x, k_ref));
-                    } else {
-                        const long long pix = plane + x + static_cast<long long>(y) * W;
-                        box[q] = static_cast<unsigned char>(
-                            bit_u(bitset, pix, S, k_ref) | (bit_u(target, pix, S, k_ref) << 1));
+                unsigned* rbits = box;
+                unsigned* tbits = box + kAaBoxWords / 2;
+                // 64 pixels of a row per step: ballots of the two bits,
+                // lanes 0 and 1 store the step's two words of each plane
+                for (int y = 0; y < static_cast<int>(hh); ++y) {
+                    for (int c0 = 0; c0 < wpr * 32; c0 += kWave) {
+                        const int x = bw0 + c0 + lane;
+                        unsigned code = 0u;
+                        if (c0 + lane < ww) {  // (bh0 + y, x) in [0,H) x [0,W)
+                            if (PLANES) {
+                                code = plane_code(planes, pbase, W, bh0 + y, x, k_ref);
+                            } else {
+                                const long long pix =
+                                    plane + x + static_cast<long long>(bh0 + y) * W;
+                                code = bit_u(bitset, pix, S, k_ref) |
+                                       (bit_u(target, pix, S, k_ref) << 1);
+                            }
+                        }
+                        const unsigned long long rb = __ballot(code & 1u);
+                        const unsigned long long tb = __ballot(code & 2u);
+                        const int wi = (c0 >> 5) + lane;
+                        if (lane < 2 && wi < wpr) {
+                            rbits[y * wpr + wi] = static_cast<unsigned>(lane ? rb >> 32 : rb);
+                            tbits[y * wpr + wi] = static_cast<unsigned>(lane ? tb >> 32 : tb);
+                        }
                     }
                 }
             }
@@ -363,12 +401,16 @@ __device__ __forceinline__ void aa_wave_fragment(
     bool adds = false;
     float dpx = 0.f, dpy = 0.f, dpw = 0.f;
     if (live) {
-        const long long hw = static_cast<long long>(H) * W;
-        const ResidualSrc src{staged ? box : nullptr, bh0, bw0, bwid, bitset, target,
-                              PLANES ? planes : nullptr,
-                              PLANES ? (static_cast<long long>(b) * S + (k >> 5)) * hw : hw * b, W,
-                              S, k};
-        adds = aa_sample_grad(smp, H, W, R, rho, n, src, dpx, dpy, dpw);
+        if (staged) {
+            const BoxBits bx{box, box + kAaBoxWords / 2, bh0, bw0, wpr};
+            adds = aa_sample_grad_rows<false>(smp, H, W, R, rho, n, bx, bx, dpx, dpy, dpw);
+        } else {
+            const long long hw = static_cast<long long>(H) * W;
+            const ResidualSrc src{bitset, target, PLANES ? planes : nullptr,
+                                  PLANES ? (static_cast<long long>(b) * S + (k >> 5)) * hw : hw * b,
+                                  W, S, k};
+            adds = aa_sample_grad(smp, H, W, R, rho, n, src, dpx, dpy, dpw);
+        }
     }
     const long long vb = static_cast<long long>(b) * V;
     wave_add_runs(d_pos, adds, vb + v0, dpx * smp.t0, dpy * smp.t0, dpw * smp.t0);
@@ -384,7 +426,7 @@ antialias_bwd_kernel(int N, const int* __restrict__ frag_pix, const float* __res
                      const int* __restrict__ edges, const float* __restrict__ normals,
                      const int* __restrict__ edge2cluster, float* __restrict__ d_pos, float R,
                      float rho, int B, const int2* __restrict__ planes) {
-    __shared__ unsigned char box[kAaWaves][kAaBox];
+    __shared__ unsigned box[kAaWaves][kAaBoxWords];
     const int wv = threadIdx.x / kWave;
     const long long i = (static_cast<long long>(blockIdx.x) * kAaWaves + wv) * kWave + lane_id();
     int b = 0, h = 0, w = 0, e = -1;
@@ -425,7 +467,7 @@ antialias_edges_bwd_kernel(long long BE, int H, int W, int S, const int* __restr
                            const int* __restrict__ prim_ids, const float* __restrict__ normals,
                            const int* __restrict__ edge2cluster, float* __restrict__ d_pos,
                            float R, float rho, int B, const int2* __restrict__ planes) {
-    __shared__ unsigned char box[kAaWaves][kAaBox];
+    __shared__ unsigned box[kAaWaves][kAaBoxWords];
     const int wv = threadIdx.x / kWave;
     const int lane = lane_id();
     const long long q0 = (static_cast<long long>(blockIdx.x) * kAaWaves + wv) * kWave * M;
