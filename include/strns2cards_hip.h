@@ -250,10 +250,11 @@ int s2c_skinning(int num_strands, int num_points, const float* strands,
  * (views without fragments are all zero, where the reference shrinks B);
  * cluster ids outside [0, 32*num_slots) set no bit (the reference writes
  * them out of bounds).  Its binning buffers are kept per (device, stream)
- * until s2c_dda_cluster_mask_release.  Their overflow areas are sized from
- * a running estimate of earlier calls' use, read back without waiting; a
- * call whose areas are too small still returns the exact mask (it finishes
- * with one global OR per fragment), and later calls size from its use. */
+ * until s2c_dda_cluster_mask_release; their size follows from the shape,
+ * so no call reads anything back.  A workgroup of the walk whose
+ * (cell, cluster) keys overflow its table has its fragments OR-ed
+ * directly into the bitset by a kernel behind the binned pass (same
+ * mask). */
 int s2c_dda_cluster_mask(int B, int H, int W, int V, const float* pos, int E,
                          const int* edges, const int* edge2cluster,
                          int num_slots, int* bitset, void* stream);
@@ -267,11 +268,6 @@ int s2c_dda_cluster_mask_tiled(int B, int H, int W, int V, const float* pos, int
 /* Frees the binning buffers the two calls above keep for `stream` on the
  * current device (after waiting for the stream). */
 int s2c_dda_cluster_mask_release(void* stream);
-/* Resets the running estimate of the overflow areas to its smallest value,
- * so the next call on a stream whose last use has been read takes the
- * direct-OR path whenever its workgroups spill (tests use it to exercise
- * that path).  No counterpart in the reference. */
-int s2c_dda_cluster_mask_reset_estimates(void);
 
 /* rasterize.py:6-53 then rasterize.py:170-214, as used at 3_fitting.py:154-163
  * (render_bitset_from_triangles); clears bitset first. */

@@ -1,223 +1,254 @@
-// Run ordering and the per-tile LDS resolve of the binned cluster-bitset
+// Record ordering and the per-tile LDS resolve of the binned cluster-bitset
 // pass (see binning.hpp).
 #include "binning.hpp"
 
-#include <atomic>
 #include <map>
 #include <utility>
 
 namespace s2c {
 
 // A tile's resolve is a chain of dependent global round trips (order,
-// offsets, runs, entries, then the stores), so the kernel is bound by how
+// offsets, records, masks, then the stores), so the kernel is bound by how
 // many tiles are in flight: 512-thread workgroups let four tiles share a CU
-// (LDS-bound at 32 KB of bitset each), twice as many as 1024-thread ones,
-// and each thread keeps kResolveUnroll entry loads in flight.
+// (LDS-bound at 32 KB of bitset each), and each thread keeps
+// kResolveUnroll mask loads in flight.
 constexpr int kResolveThreads = 512;
-constexpr int kResolveUnroll = 8;
+constexpr int kResolveUnroll = 4;
 
-// Resolve order: tiles by decreasing run count in classes of powers of 4,
-// so the heaviest tiles start first and the light and empty ones fill the
-// tail.  Each entry is (tile, first run, end of its runs), so a resolve
-// workgroup starts with one load.  One workgroup; each thread classifies kOrderPer consecutive tiles
-// from one round of loads, and counts / places them with wave scans and one
-// LDS atomic per wave and class.  The order within a class is arbitrary.
-constexpr int kOrderThreads = 1024;
-constexpr int kOrderClasses = 8;
-constexpr int kOrderPer = 16;
+// ------------------------------------------------ counting sort by tile
 
-__device__ __forceinline__ void order_classes(long long t0, long long nt, const int* offs,
-                                              int (&cls)[kOrderPer], int (&v)[kOrderPer + 1]) {
+// Records of sort block j's stage workgroups: wave w of the block takes
+// workgroups first + w, first + w + waves, ..., lane l records l, l + 64, ...
+// of each; f(r, (tile, info)) for every record r.
+template <class F>
+__device__ __forceinline__ void sort_records(long long blocks, long long j,
+                                             const int* __restrict__ block_recs,
+                                             const int2* __restrict__ rec_ti, F&& f) {
+    const long long b1 = dmin((j + 1) * kSortChunk, blocks);
+    for (long long b = j * kSortChunk + threadIdx.x / kWave; b < b1; b += kSortThreads / kWave) {
+        const int n = block_recs[b];
+        const long long r0 = b * kBinRecs;
+#pragma unroll 1
+        for (int i = lane_id(); i < n; i += 4 * kWave) {
+            int2 ti[4];
 #pragma unroll
-    for (int i = 0; i <= kOrderPer; ++i) v[i] = offs[dmin(t0 + i, nt) * kBinGroups];
+            for (int u = 0; u < 4; ++u)
+                ti[u] = i + u * kWave < n ? rec_ti[r0 + i + u * kWave] : make_int2(-1, 0);
 #pragma unroll
-    for (int i = 0; i < kOrderPer; ++i) {
-        const int w = v[i + 1] - v[i];
-        cls[i] = t0 + i >= nt ? -1
-                 : w <= 0     ? kOrderClasses - 1
-                              : dmax(0, kOrderClasses - 2 - (31 - __clz(w)) / 2);
+            for (int u = 0; u < 4; ++u)
+                if (ti[u].x >= 0) f(r0 + i + u * kWave, ti[u]);
+        }
     }
 }
 
-__device__ __forceinline__ void tile_order(long long nt, const int* __restrict__ offs,
-                                           int4* __restrict__ order) {
+// counts[j * nt + t] = records of sort block j in tile t (LDS counters,
+// kSortTiles tiles per round; `cnt` is dynamic LDS of min(nt, kSortTiles)
+// ints).
+__global__ void __launch_bounds__(kSortThreads)
+bin_count_kernel(long long nt, long long blocks, const int* __restrict__ block_recs,
+                 const int2* __restrict__ rec_ti, int* __restrict__ counts) {
+    extern __shared__ int cnt[];
+    for (long long t0 = 0; t0 < nt; t0 += kSortTiles) {
+        const int nr = static_cast<int>(dmin(static_cast<long long>(kSortTiles), nt - t0));
+        for (int i = threadIdx.x; i < nr; i += kSortThreads) cnt[i] = 0;
+        __syncthreads();
+        sort_records(blocks, blockIdx.x, block_recs, rec_ti, [&](long long, int2 ti) {
+            if (ti.x >= t0 && ti.x < t0 + nr) atomicAdd(&cnt[ti.x - t0], 1);
+        });
+        __syncthreads();
+        int* row = counts + blockIdx.x * nt + t0;
+        for (int i = threadIdx.x; i < nr; i += kSortThreads) row[i] = cnt[i];
+        __syncthreads();
+    }
+}
+
+// Rows [j0, j1) of the count matrix's segment `seg`.
+__device__ __forceinline__ void sort_segment(long long nb, int seg, long long& j0, long long& j1) {
+    const long long per = (nb + kSortSegs - 1) / kSortSegs;
+    j0 = dmin(seg * per, nb);
+    j1 = dmin(j0 + per, nb);
+}
+
+// segs[seg * nt + t] = sum of column t over the segment's rows.  Grid
+// (tiles / kBlock, kSortSegs).
+__global__ void __launch_bounds__(kBlock)
+bin_colsum_kernel(long long nt, long long nb, const int* __restrict__ counts,
+                  int* __restrict__ segs) {
+    const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= nt) return;
+    long long j0, j1;
+    sort_segment(nb, blockIdx.y, j0, j1);
+    int acc = 0;
+    for (long long j = j0; j < j1; ++j) acc += counts[j * nt + t];
+    segs[blockIdx.y * nt + t] = acc;
+}
+
+// Resolve order class of a tile with w records: powers of 4, heaviest
+// first, empty tiles last.
+__device__ __forceinline__ int order_class(int w) {
+    return w <= 0 ? kOrderClasses - 1 : dmax(0, kOrderClasses - 2 - (31 - __clz(w)) / 2);
+}
+
+// In place: counts[j * nt + t] becomes the number of records of tile t in
+// sort blocks before j.  The last segment writes the tile's total into
+// offs[t + 1] (offs is scanned into tile offsets next) and counts its
+// block's tiles per order class into classes[k * gridDim.x + blockIdx.x].
+__global__ void __launch_bounds__(kBlock)
+bin_colprefix_kernel(long long nt, long long nb, int* __restrict__ counts,
+                     const int* __restrict__ segs, int* __restrict__ offs,
+                     int* __restrict__ classes) {
     __shared__ int hist[kOrderClasses];
-    if (threadIdx.x < kOrderClasses) hist[threadIdx.x] = 0;
-    __syncthreads();
-    const int lane = lane_id();
-    constexpr long long step = static_cast<long long>(kOrderThreads) * kOrderPer;
-    for (long long c0 = 0; c0 < nt; c0 += step) {
-        int cls[kOrderPer], v[kOrderPer + 1];
-        order_classes(c0 + threadIdx.x * kOrderPer, nt, offs, cls, v);
-#pragma unroll
-        for (int k = 0; k < kOrderClasses; ++k) {
-            int n = 0;
-#pragma unroll
-            for (int i = 0; i < kOrderPer; ++i) n += cls[i] == k;
-            const int incl = wave_incl_scan(n);
-            if (lane == kWave - 1 && incl > 0) atomicAdd(&hist[k], incl);
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
+    const bool last = blockIdx.y == kSortSegs - 1;  // workgroup-uniform
+    if (last && threadIdx.x < kOrderClasses) hist[threadIdx.x] = 0;
+    if (last) __syncthreads();
+    const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (t < nt) {
+        long long j0, j1;
+        sort_segment(nb, blockIdx.y, j0, j1);
         int acc = 0;
-        for (int k = 0; k < kOrderClasses; ++k) {
-            const int v = hist[k];
-            hist[k] = acc;
-            acc += v;
+        for (int k = 0; k < static_cast<int>(blockIdx.y); ++k) acc += segs[k * nt + t];
+        for (long long j = j0; j < j1; ++j) {
+            const int c = counts[j * nt + t];
+            counts[j * nt + t] = acc;
+            acc += c;
+        }
+        if (last) {
+            offs[t + 1] = acc;
+            atomicAdd(&hist[order_class(acc)], 1);
         }
     }
+    if (!last) return;
     __syncthreads();
-    for (long long c0 = 0; c0 < nt; c0 += step) {
-        const long long t0 = c0 + threadIdx.x * kOrderPer;
-        int cls[kOrderPer], v[kOrderPer + 1];
-        order_classes(t0, nt, offs, cls, v);
-#pragma unroll
-        for (int k = 0; k < kOrderClasses; ++k) {
-            int n = 0;
-#pragma unroll
-            for (int i = 0; i < kOrderPer; ++i) n += cls[i] == k;
-            const int incl = wave_incl_scan(n);
-            int b = 0;
-            if (lane == kWave - 1 && incl > 0) b = atomicAdd(&hist[k], incl);
-            b = __shfl(b, kWave - 1, kWave) + incl - n;
-#pragma unroll
-            for (int i = 0; i < kOrderPer; ++i)
-                if (cls[i] == k) order[b++] = make_int4(static_cast<int>(t0 + i), v[i], v[i + 1], 0);
-        }
-    }
+    if (threadIdx.x < kOrderClasses) classes[threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
 }
 
-// Runs into tile order, and the resolve order, in one launch of
-// kOrderThreads-thread workgroups: workgroup 0 orders the tiles while the
-// others scatter the run records, each covering kOrderThreads / kBlock
-// stage workgroups' fixed record regions (one kBlock-thread group each) or
-// kOrderThreads records of the overflow area.
-constexpr int kStagePerOrder = kOrderThreads / kBlock;
+// Exclusive scan of the n class counts in place (class-major, so each
+// (class, block) gets the first order position of its tiles).  One
+// workgroup of kOrderThreads.
+constexpr int kOrderThreads = 1024;
 
 __global__ void __launch_bounds__(kOrderThreads)
-bin_order_kernel(long long nt, long long blocks, const int* __restrict__ block_runs, int ovf_cap,
-                 int ovf_entry_cap, const int* __restrict__ counters,
-                 const BinRecord* __restrict__ recs, const int* __restrict__ offs,
-                 BinRun* __restrict__ runs, int4* __restrict__ order) {
-    // An attempt whose overflow areas were too small (the host relaunches
-    // it) gets an order of empty tiles: its resolve writes zeros and reads
-    // no run or entry.
-    const int ce = counters[kEntriesOvf], cr = counters[kRunsOvf];
-    if (ce < 0 || ce > ovf_entry_cap || cr < 0 || cr > ovf_cap) {
-        if (blockIdx.x == 0)
-            for (long long t = threadIdx.x; t < nt; t += kOrderThreads)
-                order[t] = make_int4(static_cast<int>(t), 0, 0, 0);
-        return;
-    }
-    if (blockIdx.x == 0) {
-        tile_order(nt, offs, order);
-        return;
-    }
-    const long long fixed_groups = (blocks + kStagePerOrder - 1) / kStagePerOrder;
-    const long long w = blockIdx.x - 1;
-    long long first, count;
-    int tid = threadIdx.x, stride = kOrderThreads;
-    if (w < fixed_groups) {
-        const long long b = w * kStagePerOrder + threadIdx.x / kBlock;
-        if (b >= blocks) return;
-        first = b * kBinRecs;
-        count = block_runs[b];
-        tid = threadIdx.x % kBlock;
-        stride = kBlock;
-    } else {
-        const long long o = (w - fixed_groups) * kOrderThreads;
-        const long long used = dmin(static_cast<long long>(counters[kRunsOvf]),
-                                    static_cast<long long>(ovf_cap));
-        first = blocks * kBinRecs + o;
-        count = dmax(dmin(used - o, static_cast<long long>(kOrderThreads)), 0ll);
-    }
-    for (long long i = tid; i < count; i += stride) {
-        const BinRecord rec = recs[first + i];
-        runs[offs[rec.slot] + rec.k] = BinRun{rec.src, rec.n};
+bin_class_scan_kernel(long long n, int* __restrict__ classes) {
+    __shared__ int sh[kOrderThreads / kWave + 1];
+    int carry = 0;
+    for (long long c0 = 0; c0 < n; c0 += kOrderThreads) {
+        const long long i = c0 + threadIdx.x;
+        const int v = i < n ? classes[i] : 0;
+        const int incl = wave_incl_scan(v);
+        if (lane_id() == kWave - 1) sh[threadIdx.x / kWave] = incl;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int acc = 0;
+            for (int w = 0; w < kOrderThreads / kWave; ++w) {
+                const int x = sh[w];
+                sh[w] = acc;
+                acc += x;
+            }
+            sh[kOrderThreads / kWave] = acc;
+        }
+        __syncthreads();
+        if (i < n) classes[i] = carry + sh[threadIdx.x / kWave] + incl - v;
+        carry += sh[kOrderThreads / kWave];
+        __syncthreads();
     }
 }
 
+// The resolve order: (tile, first record, record end) of each tile at its
+// class's next position (order within a class arbitrary).  Same grid as
+// the class counts.
+__global__ void __launch_bounds__(kBlock)
+bin_order_kernel(long long nt, const int* __restrict__ offs, const int* __restrict__ classes,
+                 int4* __restrict__ order) {
+    __shared__ int next[kOrderClasses];
+    if (threadIdx.x < kOrderClasses) next[threadIdx.x] = classes[threadIdx.x * gridDim.x + blockIdx.x];
+    __syncthreads();
+    const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= nt) return;
+    const int a = offs[t], e = offs[t + 1];
+    const int pos = atomicAdd(&next[order_class(e - a)], 1);
+    order[pos] = make_int4(static_cast<int>(t), a, e, 0);
+}
+
+// Sort block j places its records at offs[t] + (its first position in tile
+// t) + (rank within the block, from an LDS counter), as (record, info).
+__global__ void __launch_bounds__(kSortThreads)
+bin_place_kernel(long long nt, long long blocks, const int* __restrict__ block_recs,
+                 const int2* __restrict__ rec_ti, const int* __restrict__ firsts,
+                 const int* __restrict__ offs, int2* __restrict__ runs) {
+    extern __shared__ int cnt[];
+    const int* row = firsts + blockIdx.x * nt;
+    for (long long t0 = 0; t0 < nt; t0 += kSortTiles) {
+        const int nr = static_cast<int>(dmin(static_cast<long long>(kSortTiles), nt - t0));
+        for (int i = threadIdx.x; i < nr; i += kSortThreads) cnt[i] = 0;
+        __syncthreads();
+        sort_records(blocks, blockIdx.x, block_recs, rec_ti, [&](long long r, int2 ti) {
+            if (ti.x < t0 || ti.x >= t0 + nr) return;
+            const int rank = atomicAdd(&cnt[ti.x - t0], 1);
+            runs[offs[ti.x] + row[ti.x] + rank] = make_int2(static_cast<int>(r), ti.y);
+        });
+        __syncthreads();
+    }
+}
+
+// One workgroup per tile, in the resolve order.  Thread i takes word
+// i % mask_words of record i / mask_words of each pass, so a record's mask
+// is read with consecutive loads, and ORs the cluster's bit into the LDS
+// bitset at every pixel the word sets.  The LDS word of (pixel l, word j)
+// is l*S + (j ^ (l & smask)): with S a power of two, the pixels of one
+// cluster's mask (same j) fall in different banks.
 template <bool VEC4, int kLds>
 __global__ void __launch_bounds__(kResolveThreads)
-tile_resolve_kernel(TileGrid g, const int4* __restrict__ order,
-                    const BinRun* __restrict__ runs,
-                    const unsigned* __restrict__ entries, int H, int W, int S,
+cell_resolve_kernel(TileGrid g, const int4* __restrict__ order, const int2* __restrict__ runs,
+                    const uint4* __restrict__ rec_mask, int H, int W, int S,
                     int* __restrict__ bitset) {
     __shared__ __align__(16) unsigned tile[kLds / 4];
-    __shared__ int pre[kResolveThreads];
-    __shared__ int src[kResolveThreads];
-    __shared__ int sh[kResolveThreads / kWave + 1];
     const int4 ord = order[blockIdx.x];
     const int t = ord.x, r0 = ord.y, r1 = ord.z;
-    // the first chunk's run is loaded before the clear of the LDS bitset
-    BinRun next{0, 0};
-    if (r0 + static_cast<int>(threadIdx.x) < r1) next = runs[r0 + threadIdx.x];
     const int side = 1 << g.ts_log2;
     const int words = side * side * S;  // a multiple of 4 (side >= 4)
     for (int i = threadIdx.x; i < words / 4; i += kResolveThreads)
         reinterpret_cast<uint4*>(tile)[i] = make_uint4(0u, 0u, 0u, 0u);
-    __syncthreads();  // a tile without runs goes straight to the write-out
-    const unsigned lmask = (1u << g.local_bits) - 1u;
-    const unsigned kmax = static_cast<unsigned>(S) * 32u;
-    // LDS word of (pixel l, word j) is l*S + (j ^ (l & smask)): with S a
-    // power of two, consecutive entries of one cluster (same j) at
-    // different pixels fall in different banks
+    __syncthreads();  // a tile without records goes straight to the write-out
     const unsigned smask = (S & (S - 1)) == 0 ? static_cast<unsigned>(S) - 1u : 0u;
-    const int w = threadIdx.x / kWave;
-    // Runs are taken kResolveThreads at a time: their lengths are scanned
-    // into `pre`, then the chunk's entries are read as one flat range (each
-    // lane finds its run by binary search), so consecutive lanes read
-    // consecutive entries of a run and kResolveUnroll loads are in flight.
-    for (int rb = r0; rb < r1; rb += kResolveThreads) {
-        const int r = rb + static_cast<int>(threadIdx.x);
-        BinRun run = next;
-        if (rb != r0) run = r < r1 ? runs[r] : BinRun{0, 0};
-        const int incl = wave_incl_scan(run.n);
-        if (lane_id() == kWave - 1) sh[w] = incl;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int acc = 0;
-            for (int i = 0; i < kResolveThreads / kWave; ++i) {
-                const int v = sh[i];
-                sh[i] = acc;
-                acc += v;
-            }
-            sh[kResolveThreads / kWave] = acc;
-        }
-        __syncthreads();
-        pre[threadIdx.x] = sh[w] + incl - run.n;
-        src[threadIdx.x] = run.src;
-        const int total = sh[kResolveThreads / kWave];
-        __syncthreads();
-        const int nr = dmin(r1 - rb, kResolveThreads);
-        constexpr int step = kResolveThreads * kResolveUnroll;
-        for (int e = threadIdx.x; e < total; e += step) {
-            unsigned v[kResolveUnroll];
+    const int mwl = g.mw_log2;
+    const int word = static_cast<int>(threadIdx.x) & ((1 << mwl) - 1);
+    const int per = kResolveThreads >> mwl;  // records per pass
+    const int cs = g.cs_log2;
+    const int cpl = g.ts_log2 - cs;  // log2 of cells per tile row
+    for (int rb = r0 + static_cast<int>(threadIdx.x >> mwl); rb < r1; rb += per * kResolveUnroll) {
+        unsigned m[kResolveUnroll];
+        int info[kResolveUnroll];
 #pragma unroll
-            for (int u = 0; u < kResolveUnroll; ++u) {
-                const int j = e + u * kResolveThreads;
-                v[u] = 0xFFFFFFFFu;
-                if (j < total) {
-                    int lo = 0, hi = nr;  // last run with pre <= j
-                    while (hi - lo > 1) {
-                        const int mid = (lo + hi) >> 1;
-                        if (pre[mid] <= j) lo = mid;
-                        else hi = mid;
-                    }
-                    v[u] = entries[src[lo] + (j - pre[lo])];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < kResolveUnroll; ++u) {
-                const unsigned local = v[u] & lmask;
-                const unsigned k = v[u] >> g.local_bits;
-                if (k < kmax)
-                    atomicOr(&tile[local * S + ((k >> 5) ^ (local & smask))], 1u << (k & 31));
+        for (int u = 0; u < kResolveUnroll; ++u) {
+            const int r = rb + u * per;
+            m[u] = 0u;
+            info[u] = 0;
+            if (r < r1) {
+                const int2 run = runs[r];
+                m[u] = reinterpret_cast<const unsigned*>(rec_mask + 2ll * run.x)[word];
+                info[u] = run.y;
             }
         }
-        __syncthreads();  // pre / src / sh are rewritten by the next chunk
+#pragma unroll
+        for (int u = 0; u < kResolveUnroll; ++u) {
+            unsigned v = m[u];
+            if (v == 0u) continue;
+            const unsigned c = static_cast<unsigned>(info[u]) & 0xFFFFu;
+            const int cell = info[u] >> 16;
+            const int oy = (cell >> cpl) << cs;
+            const int ox = (cell & ((1 << cpl) - 1)) << cs;
+            const unsigned bit = 1u << (c & 31u);
+            const unsigned j = c >> 5;
+            do {
+                const int p = (word << 5) + __ffs(v) - 1;  // pixel within the cell
+                v &= v - 1u;
+                const unsigned l = static_cast<unsigned>(((oy + (p >> cs)) << g.ts_log2) |
+                                                         (ox + (p & ((1 << cs) - 1))));
+                atomicOr(&tile[l * S + (j ^ (l & smask))], bit);
+            } while (v != 0u);
+        }
     }
+    __syncthreads();
     const int per_b = g.tx * g.ty;
     const int b = t / per_b;
     const int rr = t - b * per_b;
@@ -274,63 +305,79 @@ tile_resolve_kernel(TileGrid g, const int4* __restrict__ order,
 }
 
 size_t bin_plan_ints(const TileGrid& g, long long blocks) {
-    const long long n = g.nt * kBinGroups;
-    return static_cast<size_t>(scan_scratch_ints(n + 1) + 4 + 4 * g.nt + kBinCounters + (n + 2) +
-                               blocks + 4);
+    BinPlan p;
+    p.g = g;
+    p.blocks = blocks;
+    return static_cast<size_t>(scan_scratch_ints(g.nt + 1) + 4 + 4 * g.nt + kBinCounters +
+                               (g.nt + 2) + 2 * blocks + kSortSegs * g.nt +
+                               kOrderClasses * p.class_blocks() + p.sort_blocks() * g.nt +
+                               2 * p.records() + 8);
 }
 
 void bin_carve(BinPlan& p, int* ints) {
-    const long long n = p.slots();
+    const long long nt = p.g.nt;
     p.scratch = ints;  // int64 values first: 8-byte aligned
-    long long off = static_cast<long long>(scan_scratch_ints(n + 1));
+    long long off = static_cast<long long>(scan_scratch_ints(nt + 1));
     off = (off + 3) & ~3ll;
     p.order = reinterpret_cast<int4*>(ints + off);  // 16-byte aligned
-    off += 4 * p.g.nt;
-    p.counters = ints + off;  // counters then run counts: one contiguous clear
-    p.runs_per_slot = ints + off + kBinCounters;
-    p.block_runs = ints + off + kBinCounters + n + 2;
+    off += 4 * nt;
+    p.counters = ints + off;  // counters then tile totals: one contiguous clear
+    p.offs = p.counters + kBinCounters;
+    off += kBinCounters + nt + 2;
+    p.ovf_blocks = ints + off;
+    off += p.blocks;
+    p.block_recs = ints + off;
+    off += p.blocks;
+    p.segs = ints + off;
+    off += kSortSegs * nt;
+    p.classes = ints + off;
+    off += kOrderClasses * p.class_blocks();
+    p.counts = ints + off;
+    off += p.sort_blocks() * nt;
+    off = (off + 1) & ~1ll;
+    p.rec_ti = reinterpret_cast<int2*>(ints + off);  // 8-byte aligned
 }
 
 int bin_clear(const BinPlan& p, hipStream_t s) {
-    return fill_u32(p.counters, 0u, kBinCounters + p.slots() + 1, s);
-}
-
-int bin_counters_async(const BinPlan& p, int* host, hipStream_t s) {
-    S2C_HIP(hipMemcpyAsync(host, p.counters, sizeof(int) * kBinCounters, hipMemcpyDeviceToHost, s));
-    return S2C_OK;
-}
-
-bool bin_overflow_use(const int* host, long long* entries, long long* runs) {
-    // int32 atomics: a wrapped (negative) value means more than 2^31
-    if (host[kEntriesOvf] < 0 || host[kRunsOvf] < 0) return false;
-    *entries = host[kEntriesOvf];
-    *runs = host[kRunsOvf];
-    return true;
+    return fill_u32(p.counters, 0u, kBinCounters + p.g.nt + 2, s);
 }
 
 int bin_resolve(const BinPlan& p, int H, int W, int S, int* bitset, hipStream_t s) {
-    if (int r = inclusive_scan_inplace(p.runs_per_slot, p.slots() + 1, p.scratch, s)) return r;
-    const long long grid = 1 + (p.blocks + kStagePerOrder - 1) / kStagePerOrder +
-                           (p.ovf_runs + kOrderThreads - 1) / kOrderThreads;
-    bin_order_kernel<<<static_cast<unsigned>(grid), kOrderThreads, 0, s>>>(
-        p.g.nt, p.blocks, p.block_runs, p.ovf_runs, p.ovf_entries, p.counters, p.records,
-        p.runs_per_slot, p.runs, p.order);
+    const long long nt = p.g.nt;
+    const long long nb = p.sort_blocks();
+    const size_t lds = sizeof(int) * static_cast<size_t>(dmin(nt, static_cast<long long>(kSortTiles)));
+    bin_count_kernel<<<static_cast<unsigned>(nb), kSortThreads, lds, s>>>(
+        nt, p.blocks, p.block_recs, p.rec_ti, p.counts);
+    S2C_LAUNCH_CHECK();
+    const unsigned cb = static_cast<unsigned>(p.class_blocks());
+    bin_colsum_kernel<<<dim3(cb, kSortSegs), kBlock, 0, s>>>(nt, nb, p.counts, p.segs);
+    S2C_LAUNCH_CHECK();
+    bin_colprefix_kernel<<<dim3(cb, kSortSegs), kBlock, 0, s>>>(nt, nb, p.counts, p.segs, p.offs,
+                                                                p.classes);
+    S2C_LAUNCH_CHECK();
+    if (int r = inclusive_scan_inplace(p.offs, nt + 1, p.scratch, s)) return r;
+    bin_class_scan_kernel<<<1, kOrderThreads, 0, s>>>(kOrderClasses * p.class_blocks(), p.classes);
+    S2C_LAUNCH_CHECK();
+    bin_order_kernel<<<cb, kBlock, 0, s>>>(nt, p.offs, p.classes, p.order);
+    S2C_LAUNCH_CHECK();
+    bin_place_kernel<<<static_cast<unsigned>(nb), kSortThreads, lds, s>>>(
+        nt, p.blocks, p.block_recs, p.rec_ti, p.counts, p.offs, p.runs);
     S2C_LAUNCH_CHECK();
     const bool vec = (S % 4 == 0) && ((reinterpret_cast<uintptr_t>(bitset) & 15) == 0);
     const bool small = tile_bitset_bytes(p.g.ts_log2, S) <= kTileLdsBytes;
-    const unsigned grid_t = static_cast<unsigned>(p.g.nt);
+    const unsigned grid_t = static_cast<unsigned>(nt);
     if (vec && small)
-        tile_resolve_kernel<true, kTileLdsBytes><<<grid_t, kResolveThreads, 0, s>>>(
-            p.g, p.order, p.runs, p.entries, H, W, S, bitset);
+        cell_resolve_kernel<true, kTileLdsBytes><<<grid_t, kResolveThreads, 0, s>>>(
+            p.g, p.order, p.runs, p.rec_mask, H, W, S, bitset);
     else if (vec)
-        tile_resolve_kernel<true, kTileLdsMax><<<grid_t, kResolveThreads, 0, s>>>(
-            p.g, p.order, p.runs, p.entries, H, W, S, bitset);
+        cell_resolve_kernel<true, kTileLdsMax><<<grid_t, kResolveThreads, 0, s>>>(
+            p.g, p.order, p.runs, p.rec_mask, H, W, S, bitset);
     else if (small)
-        tile_resolve_kernel<false, kTileLdsBytes><<<grid_t, kResolveThreads, 0, s>>>(
-            p.g, p.order, p.runs, p.entries, H, W, S, bitset);
+        cell_resolve_kernel<false, kTileLdsBytes><<<grid_t, kResolveThreads, 0, s>>>(
+            p.g, p.order, p.runs, p.rec_mask, H, W, S, bitset);
     else
-        tile_resolve_kernel<false, kTileLdsMax><<<grid_t, kResolveThreads, 0, s>>>(
-            p.g, p.order, p.runs, p.entries, H, W, S, bitset);
+        cell_resolve_kernel<false, kTileLdsMax><<<grid_t, kResolveThreads, 0, s>>>(
+            p.g, p.order, p.runs, p.rec_mask, H, W, S, bitset);
     S2C_LAUNCH_CHECK();
     return S2C_OK;
 }
@@ -339,7 +386,6 @@ namespace {
 struct WorkspaceBuffers {
     void* p[BinWorkspace::kSlots] = {};
     size_t n[BinWorkspace::kSlots] = {};
-    BinStreamCounters counters;
 };
 std::map<std::pair<int, hipStream_t>, WorkspaceBuffers>& workspaces() {
     static std::map<std::pair<int, hipStream_t>, WorkspaceBuffers> m;
@@ -372,26 +418,6 @@ int BinWorkspace::get(hipStream_t s, Slot slot, size_t bytes, void** out) {
     return S2C_OK;
 }
 
-int BinWorkspace::counters(hipStream_t s, BinStreamCounters** out) {
-    int dev = 0;
-    S2C_HIP(hipGetDevice(&dev));
-    BinStreamCounters& c = workspaces()[{dev, s}].counters;
-    if (!c.host) {
-        S2C_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.host), sizeof(int) * kBinCounters,
-                              hipHostMallocPortable));
-        if (hipEventCreateWithFlags(&c.done, hipEventDisableTiming) != hipSuccess) {
-            (void)hipHostFree(c.host);
-            c.host = nullptr;
-            S2C_HIP(hipGetLastError());
-            set_error("dda_cluster_mask: event creation failed");
-            return S2C_ERR_ARG;
-        }
-        c.pending = false;
-    }
-    *out = &c;
-    return S2C_OK;
-}
-
 int BinWorkspace::release(hipStream_t s) {
     int dev = 0;
     S2C_HIP(hipGetDevice(&dev));
@@ -404,31 +430,8 @@ int BinWorkspace::release(hipStream_t s) {
         it->second.p[i] = nullptr;
         it->second.n[i] = 0;
     }
-    BinStreamCounters& c = it->second.counters;
-    if (c.done) S2C_HIP(hipEventDestroy(c.done));
-    if (c.host) S2C_HIP(hipHostFree(c.host));
     m.erase(it);
     return S2C_OK;
-}
-
-namespace {
-std::atomic<long long> g_entry_hint{0}, g_run_hint{0};
-
-long long raise_hint(std::atomic<long long>& h, long long want) {
-    long long cur = h.load(std::memory_order_relaxed);
-    while (want > cur && !h.compare_exchange_weak(cur, want, std::memory_order_relaxed)) {
-    }
-    return dmax(cur, want);
-}
-}  // namespace
-
-long long bin_entry_hint(long long want) { return raise_hint(g_entry_hint, want); }
-long long bin_run_hint(long long want) { return raise_hint(g_run_hint, want); }
-void bin_reset_hints() {
-    g_entry_hint.store(0, std::memory_order_relaxed);
-    g_run_hint.store(0, std::memory_order_relaxed);
-    // counts not read yet would raise the estimates again at the next call
-    for (auto& kv : workspaces()) kv.second.counters.pending = false;
 }
 
 }  // namespace s2c

@@ -1,25 +1,27 @@
 // Screen-tile binning for the fused primitive -> cluster-bitset pass.
 //
-// One walk over the primitives.  A workgroup stages its fragments in LDS
-// as packed 32-bit entries (cluster << local_bits | local pixel) together
-// with the LDS hash slot of their tile, counts them per tile, and writes
-// them into its own fixed region of the entry buffer grouped by tile.  For
-// every (workgroup, tile) pair it writes a run record (tile counter slot,
-// index of the run within that slot, region offset, length) into its own
-// fixed record region; the index comes from one global atomic on the
-// tile's run counter, privatized kBinGroups ways.  A scan of the run
-// counters and a scatter put the runs in tile order, and one workgroup per
-// tile gathers its runs, ORs the entries into an LDS copy of the tile's
-// bitset and writes the tile once with coalesced stores (no clear of the
-// bitset, no global OR atomics).
+// One walk over the primitives.  A workgroup ORs each fragment into an LDS
+// bit mask of its (cell, cluster) key, where a cell is a square of up to
+// 16 x 16 pixels inside a screen tile; the key's mask is found through an
+// LDS hash, and a fragment that repeats a pixel and cluster already seen by
+// the workgroup sets no new bit.  After the walk the workgroup appends one
+// record per key (its tile, and its cluster, cell and mask) to a flat
+// record list, with one global atomic per workgroup.
 //
-// A workgroup that needs more than its fixed regions takes space from
-// overflow areas; their sizes come from a running estimate, and the use is
-// counted even past the capacity.  A pass whose areas were too small
-// resolves to zeros and the same call finishes with one global OR per
-// fragment (a kernel that exits at once when the areas sufficed); the host
-// reads the counts of a call when they have arrived, without waiting, and
-// grows the estimate for the calls after it.
+// The records are then put in tile order by a counting sort without global
+// atomics: each sort block counts its kSortChunk records per tile in LDS,
+// a two-level scan down the (sort block, tile) count matrix gives every
+// sort block its first position within each tile, a scan of the tile
+// totals gives the tiles' offsets, and each sort block places its records,
+// ranked within their tile by LDS atomics.  One workgroup per tile then ORs
+// its records' mask pixels into an LDS copy of the tile's bitset and writes
+// the tile once with coalesced stores (no clear of the bitset, no global OR
+// atomics).
+//
+// A workgroup with more distinct keys than its table holds writes no record
+// and lists itself; a kernel behind the resolve walks the listed workgroups
+// again and ORs their fragments directly into the written bitset (it exits
+// at once when the list is empty).  The host reads nothing back.
 #pragma once
 
 #include <mutex>
@@ -30,19 +32,24 @@ namespace s2c {
 
 // LDS bitset of a tile workgroup: the default tile side fits kTileLdsBytes
 // (several resolve workgroups per CU); an explicitly chosen side may use up
-// to kTileLdsMax (one per CU, fewer (workgroup, tile) runs).
+// to kTileLdsMax (one per CU, fewer records per tile).
 constexpr int kTileLdsBytes = 32768;
 constexpr int kTileLdsMax = 131072;
-constexpr int kHashLog2 = 10;
-constexpr int kHash = 1 << kHashLog2;  // LDS table slots per workgroup
-constexpr int kStage = 4096;           // fragments staged in LDS per workgroup
-constexpr int kBinGroups = 8;          // run-counter privatization per tile
+constexpr int kHashLog2 = 9;
+constexpr int kHash = 1 << kHashLog2;  // (cell, cluster) keys per workgroup
+constexpr int kHashProbes = 64;        // probes before a workgroup counts as full
+constexpr int kCellLog2Max = 4;        // cells of at most 16 x 16 pixels
+constexpr int kMaskWordsMax = 8;       // words of a 16 x 16 cell mask
 
 struct TileGrid {
     int ts_log2;   // tile side = 1 << ts_log2
     int tx, ty;    // tiles per row / column
     long long nt;  // B * tx * ty
-    int local_bits;
+    int cs_log2;   // cell side = 1 << cs_log2 = min(tile side, 16)
+    int cy;        // cell rows per view
+    int col_bits;  // bits of a cell column in a key
+    int kb;        // bits of a cluster id in a key
+    int mw_log2;   // log2 of the 32-bit words of a cell mask
 };
 
 // Bytes of a tile bitset of side 1 << l with S words per pixel.
@@ -50,9 +57,19 @@ inline long long tile_bitset_bytes(int l, int S) {
     return (static_cast<long long>(1) << (2 * l)) * S * 4;
 }
 
+// Bits needed to write x >= 0.
+inline int bit_width(long long x) {
+    int n = 0;
+    while (x > 0) {
+        ++n;
+        x >>= 1;
+    }
+    return n;
+}
+
 // Largest power-of-two side (<= 64) whose bitset fits kTileLdsBytes, shrunk
 // while the grid has fewer than `min_tiles` tiles.  Returns false when even
-// 4 x 4 does not fit or the packed entry cannot hold the cluster ids.
+// 4 x 4 does not fit or a (cell, cluster) key does not fit 31 bits.
 // force_lg > 0 fixes the side to 1 << force_lg, whose bitset may use up to
 // kTileLdsMax (false when it does not fit).
 inline bool choose_tiles(int B, int H, int W, int S, long long min_tiles, TileGrid& g,
@@ -74,25 +91,48 @@ inline bool choose_tiles(int B, int H, int W, int S, long long min_tiles, TileGr
         g.tx = (W + side - 1) / side;
         g.ty = (H + side - 1) / side;
         g.nt = static_cast<long long>(B) * g.tx * g.ty;
-        g.local_bits = 2 * l;
+        g.cs_log2 = l < kCellLog2Max ? l : kCellLog2Max;
+        const int cs = 1 << g.cs_log2;
+        g.cy = (H + cs - 1) / cs;
+        g.col_bits = bit_width((W + cs - 1) / cs - 1);
+        g.mw_log2 = 2 * g.cs_log2 > 5 ? 2 * g.cs_log2 - 5 : 0;
     };
     grid(lg);
     while (g.nt < min_tiles && lg > 3) grid(--lg);
-    // packed entries (cluster << local_bits | local pixel) stay below 2^22,
-    // leaving 10 bits for a hash slot in the stage (always true while the
-    // tile bitset fits kTileLdsMax)
-    const long long max_cluster = static_cast<long long>(S) * 32;
-    if (max_cluster > (1ll << (22 - g.local_bits))) return false;
-    return g.nt * kBinGroups + 2 < INT_MAX;
+    g.kb = bit_width(static_cast<long long>(S) * 32 - 1);
+    // keys ((b * cy + cell row) << col_bits | cell column) << kb | cluster
+    // are non-negative ints; the cluster and the cell's place in its tile
+    // share one int of a record (cluster below 2^16)
+    const int low = g.col_bits + g.kb;
+    if (g.kb > 16 || low >= 31 || static_cast<long long>(B) * g.cy > (1ll << (31 - low)))
+        return false;
+    return g.nt + 2 < INT_MAX;
 }
 
-__device__ __forceinline__ int tile_of(const TileGrid& g, int b, int h, int w) {
-    return (b * g.ty + (h >> g.ts_log2)) * g.tx + (w >> g.ts_log2);
+// ------------------------------------------------------------ cell keys
+
+__device__ __forceinline__ int cell_key(const TileGrid& g, int b, int h, int w, int c) {
+    const int row = b * g.cy + (h >> g.cs_log2);
+    return (((row << g.col_bits) | (w >> g.cs_log2)) << g.kb) | c;
 }
 
-__device__ __forceinline__ unsigned local_of(const TileGrid& g, int h, int w) {
-    const int m = (1 << g.ts_log2) - 1;
-    return static_cast<unsigned>(((h & m) << g.ts_log2) | (w & m));
+// Pixel of (h, w) within its cell, row-major.
+__device__ __forceinline__ unsigned cell_local(const TileGrid& g, int h, int w) {
+    const int m = (1 << g.cs_log2) - 1;
+    return static_cast<unsigned>(((h & m) << g.cs_log2) | (w & m));
+}
+
+// The tile of a key's cell and the cell's row-major place in that tile.
+__device__ __forceinline__ void cell_tile(const TileGrid& g, int key, int& tile, int& cell) {
+    const int rc = key >> g.kb;
+    const int col = rc & ((1 << g.col_bits) - 1);
+    const int row = rc >> g.col_bits;
+    const int b = row / g.cy;
+    const int cr = row - b * g.cy;
+    const int sh = g.ts_log2 - g.cs_log2;
+    tile = (b * g.ty + (cr >> sh)) * g.tx + (col >> sh);
+    const int f = (1 << sh) - 1;
+    cell = ((cr & f) << sh) | (col & f);
 }
 
 // ------------------------------------------------------------ LDS hash
@@ -101,132 +141,107 @@ __device__ __forceinline__ int hash_home(int key) {
     return static_cast<int>((static_cast<unsigned>(key) * 2654435761u) >> (32 - kHashLog2));
 }
 
-// Slot holding `key` (inserted if absent), or -1 when the table is full.
+// Slot holding `key` (inserted if absent), or -1 after kHashProbes probes.
+// The first probe is inline, the rest a rolled loop (the walk inlines this
+// at every call site).
 __device__ __forceinline__ int hash_insert(int* keys, int key) {
     int s = hash_home(key);
-    for (int p = 0; p < kHash; ++p) {
-        const int prev = atomicCAS(&keys[s], -1, key);
+    int prev = atomicCAS(&keys[s], -1, key);
+    if (prev == -1 || prev == key) return s;
+#pragma unroll 1
+    for (int p = 1; p < kHashProbes; ++p) {
+        s = (s + 1) & (kHash - 1);
+        prev = atomicCAS(&keys[s], -1, key);
         if (prev == -1 || prev == key) return s;
-        s = (s + 1) & (kHash - 1);
     }
     return -1;
-}
-
-// Slot holding `key`, or -1 if absent.
-__device__ __forceinline__ int hash_find(const int* keys, int key) {
-    int s = hash_home(key);
-    for (int p = 0; p < kHash; ++p) {
-        const int k = keys[s];
-        if (k == key) return s;
-        if (k == -1) return -1;
-        s = (s + 1) & (kHash - 1);
-    }
-    return -1;
-}
-
-// One slot per active lane from an LDS or global counter, with one atomic
-// per wave: the active lanes get consecutive indices in lane order.
-__device__ __forceinline__ int wave_append(int* counter) {
-    const unsigned long long mask = __ballot(1);
-    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-    int base = 0;
-    if (lane_id() == leader) base = atomicAdd(counter, __popcll(mask));
-    base = __shfl(base, leader, kWave);
-    return base + __popcll(mask & lanemask_lt());
 }
 
 // ------------------------------------------------------------ buffers
 
-// Run record of one (workgroup, tile) pair, as the stage pass writes it.
-// `slot` is the tile's counter slot: tile * kBinGroups + group.
-struct BinRecord {
-    int slot, k, src, n;
-};
-// The same run in tile order: entries[src .. src + n).
-struct BinRun {
-    int src, n;
-};
+// A record is its tile and info (the cluster id | the cell's row-major
+// place in its tile << 16) as an int2, and its cell mask (1 << mw_log2 of
+// 8 words used) as two uint4; in tile order it is (record index, info).
 
-// Run counters are privatized per tile over kBinGroups workgroup groups
-// (blockIdx % kBinGroups), so a hot tile's counter is not one address.
-// Each workgroup owns a fixed region of kStage entries and kBinRecs run
-// records; the rare workgroup that needs more takes space from the
-// overflow areas behind the fixed regions.
-constexpr int kBinRecs = 512;
+// A stage workgroup writes at most kHash records (one per table slot) into
+// its own region of kBinRecs.
+constexpr int kBinRecs = kHash;
 
-// Global counters of the overflow areas.
-enum BinCounter { kEntriesOvf = 0, kRunsOvf = 1, kBinCounters = 4 };
+// Resolve order classes (tiles by record count, powers of 4, heaviest first).
+constexpr int kOrderClasses = 8;
+
+// Counting sort of the records by tile: sort block j takes the regions of
+// stage workgroups [j * kSortChunk, (j + 1) * kSortChunk), one wave per
+// workgroup at a time, and counts their records per tile in LDS,
+// kSortTiles tiles at a time; the count matrix is scanned down its columns
+// in kSortSegs segments.
+constexpr int kSortThreads = 1024;
+constexpr int kSortChunk = 64;
+constexpr int kSortTiles = 16384;  // 64 KB of LDS counters
+constexpr int kSortSegs = 16;
+
+// Global counters.
+enum BinCounter { kOvfBlocks = 0, kBinCounters = 4 };
+
+// Outputs of a stage workgroup.
+struct BinOut {
+    int* counters;    // kBinCounters ints
+    int* ovf_blocks;  // workgroups whose table overflowed
+    int* block_recs;  // records of each workgroup
+    int2* rec_ti;     // (tile, info) of each record
+    uint4* rec_mask;  // two uint4 per record
+};
 
 struct BinPlan {
     TileGrid g;
-    long long blocks = 0;          // workgroups of the stage pass
-    int* runs_per_slot = nullptr;  // nt*kBinGroups + 1 ints, shifted by one; offsets after the scan
-    int* counters = nullptr;       // kBinCounters ints
-    int* block_runs = nullptr;     // runs in each workgroup's fixed record region
-    int4* order = nullptr;         // nt: resolve order (tile, first run, run end)
-    int* scratch = nullptr;        // scan scratch
-    unsigned* entries = nullptr;   // blocks*kStage fixed + ovf_entries
-    BinRecord* records = nullptr;  // blocks*kBinRecs fixed + ovf_runs
-    BinRun* runs = nullptr;        // same size as records
-    int ovf_entries = 0, ovf_runs = 0;
-    long long slots() const { return g.nt * kBinGroups; }
-    long long fixed_entries() const { return blocks * kStage; }
-    long long fixed_runs() const { return blocks * kBinRecs; }
+    long long blocks = 0;       // workgroups of the stage pass
+    int* counters = nullptr;    // kBinCounters ints
+    int* offs = nullptr;        // nt + 2 ints: tile totals shifted by one; offsets after the scan
+    int* ovf_blocks = nullptr;  // `blocks` ints
+    int* block_recs = nullptr;  // `blocks` ints
+    int4* order = nullptr;      // nt: resolve order (tile, first record, record end)
+    int* scratch = nullptr;     // scan scratch
+    int* segs = nullptr;        // kSortSegs * nt segment sums of the count matrix
+    int* classes = nullptr;     // kOrderClasses * class_blocks(): tiles per order class
+    int* counts = nullptr;      // sort_blocks() * nt: counts, then first positions
+    int2* rec_ti = nullptr;     // records(): blocks regions of kBinRecs
+    uint4* rec_mask = nullptr;  // 2 * records()
+    int2* runs = nullptr;       // records(): (record, info) in tile order
+    long long records() const { return blocks * kBinRecs; }
+    long long sort_blocks() const { return (blocks + kSortChunk - 1) / kSortChunk; }
+    long long class_blocks() const { return (g.nt + kBlock - 1) / kBlock; }
+    BinOut out() const { return BinOut{counters, ovf_blocks, block_recs, rec_ti, rec_mask}; }
 };
 
-// Device ints of the fixed part of a plan (counters, run counters, scan
-// scratch, per-workgroup run counts).
+// Device ints of a plan besides its masks and runs (counters, tile offsets,
+// scan scratch, resolve order, overflow list, count matrix, record tiles).
 size_t bin_plan_ints(const TileGrid& g, long long blocks);
-// Carves the fixed part out of `ints`.
+// Carves them out of `ints`.
 void bin_carve(BinPlan& p, int* ints);
-// Zeroes the run counters and the global counters.
+// Zeroes the global counters and the tile totals.
 int bin_clear(const BinPlan& p, hipStream_t s);
-// Queues the copy of the counters into `host` (pinned).
-int bin_counters_async(const BinPlan& p, int* host, hipStream_t s);
-// The overflow-area use of a stage pass from its copied counters; false
-// when a count wrapped past 2^31.
-bool bin_overflow_use(const int* host, long long* entries, long long* runs);
-// Run offsets per tile, runs scattered into tile order, then one workgroup
-// per tile ORs its entries into LDS and writes the tile bitset.  If the
-// stage pass overflowed its areas, every tile is written as zeros instead.
+// The records sorted into tile order, then one workgroup per tile ORs their
+// masks into LDS and writes the tile bitset.
 int bin_resolve(const BinPlan& p, int H, int W, int S, int* bitset, hipStream_t s);
-
-// Pinned copy of the last call's counters on a stream, the event after that
-// copy, and whether the copy has not been read yet.
-struct BinStreamCounters {
-    int* host = nullptr;
-    hipEvent_t done = nullptr;
-    bool pending = false;
-};
 
 // Device buffers of the binned pass kept per (device, stream) across calls,
 // so that a call allocates nothing once its sizes have been seen.  Calls on
 // one stream are ordered, so they can share the buffers; `lock` must be
 // held from the first get() to the last launch of a call, so that two
 // threads enqueueing on one stream cannot interleave (the lock is never held
-// across a wait for the device, except to free a block that grows).  Buffers only grow: a
-// larger size waits for the stream, frees the old block and allocates a new
-// one.  They are kept until the process ends or release() (for the
-// 50k-strand, 4-view 1024^2 workload about 0.6 GB per stream).
+// across a wait for the device, except to free a block that grows).  Buffers
+// only grow: a larger size waits for the stream, frees the old block and
+// allocates a new one.  They are kept until the process ends or release()
+// (for the 50k-strand, 4-view 1024^2 workload about 1 GB per stream).
 struct BinWorkspace {
-    enum Slot { kPlanInts = 0, kEntries = 1, kRecords = 2, kRuns = 3, kSlots = 4 };
+    enum Slot { kPlanInts = 0, kMasks = 1, kRuns = 2, kSlots = 3 };
     static std::mutex& lock();
     // Device memory of at least `bytes` for `slot`, for calls on stream s of
     // the current device.
     static int get(hipStream_t s, Slot slot, size_t bytes, void** out);
-    // The pinned counter copy of stream s on the current device (allocated
-    // on first use).
-    static int counters(hipStream_t s, BinStreamCounters** out);
     // Frees the buffers of stream s on the current device, after waiting for
     // the stream.
     static int release(hipStream_t s);
 };
-
-// Running estimates of the overflow-area sizes, per process.
-long long bin_entry_hint(long long want);
-long long bin_run_hint(long long want);
-// Back to the smallest estimates, forgetting counts not read yet (hold
-// BinWorkspace::lock()).
-void bin_reset_hints();
 
 }  // namespace s2c

@@ -155,82 +155,84 @@ dda_mask_direct_kernel(long long P, int H, int W, int V, const float* __restrict
     dda_mask_direct(P, H, W, V, pos, E, edges, edge2cluster, S, bitset, blockIdx.x);
 }
 
-// Behind the binned pass: when its overflow areas were too small (the
-// resolve then wrote every tile as zeros), OR every fragment directly, the
-// grid striding over the walk's blocks; otherwise every workgroup exits at
-// once (a small grid, so the common case costs a few microseconds).
-constexpr int kFallbackGrid = 1024;
+// Behind the binned pass: OR every fragment of the stage workgroups whose
+// key table overflowed (listed in `ovf_blocks`, counters[kOvfBlocks] of
+// them) directly into the resolved bitset, the grid striding over the
+// list; with an empty list every workgroup exits at once (a small grid, so
+// the common case costs a few microseconds).
+constexpr int kFallbackGrid = 256;
 
 __global__ void __launch_bounds__(kBlock)
 dda_mask_fallback_kernel(long long P, int H, int W, int V, const float* __restrict__ pos, int E,
                          const int* __restrict__ edges, const int* __restrict__ edge2cluster,
                          int S, int* __restrict__ bitset, const int* __restrict__ counters,
-                         int ovf_entry_cap, int ovf_cap, long long blocks) {
-    const int ce = counters[kEntriesOvf], cr = counters[kRunsOvf];
-    if (ce >= 0 && ce <= ovf_entry_cap && cr >= 0 && cr <= ovf_cap) return;  // workgroup-uniform
-    for (long long blk = blockIdx.x; blk < blocks; blk += gridDim.x)
-        dda_mask_direct(P, H, W, V, pos, E, edges, edge2cluster, S, bitset, blk);
+                         const int* __restrict__ ovf_blocks) {
+    const int n = counters[kOvfBlocks];  // workgroup-uniform
+    for (int i = blockIdx.x; i < n; i += gridDim.x)
+        dda_mask_direct(P, H, W, V, pos, E, edges, edge2cluster, S, bitset, ovf_blocks[i]);
 }
 
-// One global OR per listed fragment (rasterize.cu:474-498) into a zeroed
-// bitset, with ListSource's checks; grid-striding.
-__device__ __forceinline__ void list_mask_or(long long N, int B, int H, int W,
-                                             const int* __restrict__ frag_pix,
-                                             const float* __restrict__ frag_attrs,
-                                             const int* __restrict__ prim2cluster, unsigned kmax,
-                                             int S, int* __restrict__ bitset) {
-    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
-    for (long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; i < N;
-         i += stride) {
-        const int prim = f2i(frag_attrs[i * 4 + 3]) - 1;
-        if (prim < 0) continue;
-        const int c = prim2cluster[prim];
-        const int b = frag_pix[i * 3 + 0], h = frag_pix[i * 3 + 1], w = frag_pix[i * 3 + 2];
-        if (static_cast<unsigned>(c) >= kmax || static_cast<unsigned>(b) >= static_cast<unsigned>(B) ||
-            static_cast<unsigned>(h) >= static_cast<unsigned>(H) ||
-            static_cast<unsigned>(w) >= static_cast<unsigned>(W))
-            continue;
-        const long long idx = ((static_cast<long long>(b) * H + h) * W + w) * S + (c >> 5);
-        atomicOr(&bitset[idx], static_cast<int>(1u << (c & 31)));
-    }
+// One global OR for fragment i of a list (rasterize.cu:474-498), with
+// ListSource's checks.
+__device__ __forceinline__ void list_or_one(long long i, int B, int H, int W,
+                                            const int* __restrict__ frag_pix,
+                                            const float* __restrict__ frag_attrs,
+                                            const int* __restrict__ prim2cluster, unsigned kmax,
+                                            int S, int* __restrict__ bitset) {
+    const int prim = f2i(frag_attrs[i * 4 + 3]) - 1;
+    if (prim < 0) return;
+    const int c = prim2cluster[prim];
+    const int b = frag_pix[i * 3 + 0], h = frag_pix[i * 3 + 1], w = frag_pix[i * 3 + 2];
+    if (static_cast<unsigned>(c) >= kmax || static_cast<unsigned>(b) >= static_cast<unsigned>(B) ||
+        static_cast<unsigned>(h) >= static_cast<unsigned>(H) ||
+        static_cast<unsigned>(w) >= static_cast<unsigned>(W))
+        return;
+    const long long idx = ((static_cast<long long>(b) * H + h) * W + w) * S + (c >> 5);
+    atomicOr(&bitset[idx], static_cast<int>(1u << (c & 31)));
 }
 
+// Into a zeroed bitset, grid-striding over the list.
 __global__ void __launch_bounds__(kBlock)
 list_mask_direct_kernel(long long N, int B, int H, int W, const int* __restrict__ frag_pix,
                         const float* __restrict__ frag_attrs,
                         const int* __restrict__ prim2cluster, unsigned kmax, int S,
                         int* __restrict__ bitset) {
-    list_mask_or(N, B, H, W, frag_pix, frag_attrs, prim2cluster, kmax, S, bitset);
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    for (long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; i < N;
+         i += stride)
+        list_or_one(i, B, H, W, frag_pix, frag_attrs, prim2cluster, kmax, S, bitset);
 }
 
+// Fragments per workgroup of the list's stage pass.
+constexpr int kListFrags = 4096;
+
+// The list's counterpart of dda_mask_fallback_kernel: the kListFrags
+// fragments of each listed stage workgroup.
 __global__ void __launch_bounds__(kBlock)
 list_mask_fallback_kernel(long long N, int B, int H, int W, const int* __restrict__ frag_pix,
                           const float* __restrict__ frag_attrs,
                           const int* __restrict__ prim2cluster, unsigned kmax, int S,
                           int* __restrict__ bitset, const int* __restrict__ counters,
-                          int ovf_entry_cap, int ovf_cap) {
-    const int ce = counters[kEntriesOvf], cr = counters[kRunsOvf];
-    if (ce >= 0 && ce <= ovf_entry_cap && cr >= 0 && cr <= ovf_cap) return;  // workgroup-uniform
-    list_mask_or(N, B, H, W, frag_pix, frag_attrs, prim2cluster, kmax, S, bitset);
+                          const int* __restrict__ ovf_blocks) {
+    const int n = counters[kOvfBlocks];  // workgroup-uniform
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const long long f0 = static_cast<long long>(ovf_blocks[i]) * kListFrags;
+        const long long f1 = dmin(f0 + kListFrags, N);
+        for (long long f = f0 + threadIdx.x; f < f1; f += kBlock)
+            list_or_one(f, B, H, W, frag_pix, frag_attrs, prim2cluster, kmax, S, bitset);
+    }
 }
 
-// Binned form: one walk.  Each thread stages its fragments in its own
-// kStage / kBlock words of LDS as slot << 22 | entry (the tile's LDS hash
-// slot and the packed entry), and the workgroup counts fragments per slot
-// with one LDS atomic per run of same-tile fragments.  After the walk the
-// slots' counts are scanned into offsets within the workgroup's fixed
-// entry region, a run record per (workgroup, tile) pair goes to its fixed
-// record region at an index from a block scan, each thread puts its column
-// back into the stage in slot order (one LDS atomic reserving each run's
-// span), and the workgroup copies the stage to its region with coalesced
-// stores.  A workgroup whose table overflows (> kHash distinct tiles) makes
-// one run per fragment in the overflow areas; one whose thread has more
-// fragments than its stage share takes its region from the overflow area
-// and walks again to place them.  Nothing is written past the overflow
-// capacities; their use is counted regardless, so the host can size them
-// and launch again.  (choose_tiles keeps entries below 2^22.)
-constexpr int kPerThread = kStage / kBlock;
-constexpr unsigned kEntryMask = (1u << 22) - 1u;
+// Binned form: one walk.  Each fragment sets its pixel's bit in the LDS
+// mask of its (cell, cluster) key (cell_key), the key's slot found in a
+// kHash-slot LDS table once per change of key along a thread's fragments;
+// a fragment repeating a pixel and cluster of the workgroup sets no new
+// bit.  After the walk a block scan of the used slots gives each key its
+// record in the workgroup's fixed region, where its mask is copied; its
+// index among its tile's records comes from one global atomic on the
+// tile's counter.  A workgroup whose table overflows writes no record and
+// lists itself for the fallback kernel.  Cluster ids outside [0, kmax) set
+// no bit.
 
 // The fragment sources of the binned pass.  walk(emit) calls
 // emit(b, h, w, cluster) for every fragment of the workgroup's share of the
@@ -254,13 +256,13 @@ struct EdgeSource {
     __device__ __forceinline__ void reset() const { *ndeferred = 0; }
 };
 
-// kStage consecutive fragments of a list (rasterize.cu:474-498) in
+// kListFrags consecutive fragments of a list (rasterize.cu:474-498) in
 // kListRounds rounds; in each, thread t takes 4 consecutive fragments, read
 // with 16-byte loads (a wave's loads cover 3 KB of frag_pix and 4 KB of
 // frag_attrs contiguously), so a thread's fragments of a round mostly share
 // a tile.  A fragment with prim id <= 0, a pixel outside the B x H x W
 // image or a cluster beyond the bitset sets no bit.
-constexpr int kListRounds = kPerThread / 4;
+constexpr int kListRounds = kListFrags / (4 * kBlock);
 
 struct ListSource {
     long long N;
@@ -273,7 +275,7 @@ struct ListSource {
     __device__ __forceinline__ void walk(Emit&& emit) const {
 #pragma unroll 1
         for (int r = 0; r < kListRounds; ++r) {
-            const long long i0 = static_cast<long long>(blockIdx.x) * kStage +
+            const long long i0 = static_cast<long long>(blockIdx.x) * kListFrags +
                                  static_cast<long long>(r) * (kBlock * 4) +
                                  static_cast<long long>(threadIdx.x) * 4;
             int pix[12], prim[4], c[4];
@@ -317,218 +319,106 @@ struct ListSource {
     __device__ __forceinline__ void reset() const {}
 };
 
-// The stage pass over one workgroup's fragments from `src`.
+// The stage pass over one workgroup's fragments from `src`.  `keys` and
+// `mask` are LDS (kHash ints, kHash << mw_log2 words), `sh` LDS scratch of
+// block_excl_scan.
 template <class Source>
-__device__ __forceinline__ void bin_stage(const Source& src, TileGrid g,
-                                          int* __restrict__ runs_per_slot,
-                                          int* __restrict__ counters,
-                                          int* __restrict__ block_runs,
-                                          unsigned* __restrict__ entries, int ovf_entries,
-                                          BinRecord* __restrict__ records, int ovf_runs,
-                                          int* keys, int* vals, unsigned* stage, int& spill,
-                                          int& full, int& base_e, int& base_r, int* sh) {
-    const long long fixed_e = static_cast<long long>(gridDim.x) * kStage;
-    const long long fixed_r = static_cast<long long>(gridDim.x) * kBinRecs;
-    const int group = static_cast<int>(blockIdx.x % kBinGroups);
-    for (int s = threadIdx.x; s < kHash; s += kBlock) {
-        keys[s] = -1;
-        vals[s] = 0;
-    }
+__device__ __forceinline__ void cell_stage(const Source& src, const TileGrid g, unsigned kmax,
+                                           const BinOut out, int* keys, unsigned* mask,
+                                           int& full, int* sh) {
+    const int mwl = g.mw_log2;
+    for (int s = threadIdx.x; s < kHash; s += kBlock) keys[s] = -1;
+    for (int i = threadIdx.x; i < (kHash << mwl) / 4; i += kBlock)
+        reinterpret_cast<uint4*>(mask)[i] = make_uint4(0u, 0u, 0u, 0u);
     if (threadIdx.x == 0) {
         src.reset();
-        spill = full = 0;
+        full = 0;
     }
     __syncthreads();
-    // thread t's i-th staged fragment lives at i * kBlock + t: consecutive
-    // lanes touch consecutive LDS words
-    int cur = -1, slot = -1, len = 0, mine = 0;
-    auto flush = [&]() {
-        if (slot >= 0 && len > 0) atomicAdd(&vals[slot], len);
-        len = 0;
-    };
+    int cur = -1, row = -1;  // current key and its mask's first word (-1: none)
     src.walk([&](int b, int h, int w, int c) {
-                   const int key = tile_of(g, b, h, w);
-                   if (key != cur) {
-                       flush();
-                       cur = key;
-                       slot = full ? -1 : hash_insert(keys, key);
-                       if (slot < 0) full = 1;
-                   }
-                   ++len;
-                   if (mine < kPerThread)
-                       stage[mine * kBlock + threadIdx.x] =
-                           (static_cast<unsigned>(slot) << 22) |
-                           (static_cast<unsigned>(c) << g.local_bits) | local_of(g, h, w);
-                   else
-                       spill = 1;
-                   ++mine;
-               });
-    flush();
-    __syncthreads();
-    if (full) {  // workgroup-uniform: one run per fragment, in the overflow areas
-        if (threadIdx.x == 0) {
-            src.reset();
-            block_runs[blockIdx.x] = 0;
+        if (static_cast<unsigned>(c) >= kmax) return;
+        const int key = cell_key(g, b, h, w, c);
+        if (key != cur) {
+            cur = key;
+            const int slot = full ? -1 : hash_insert(keys, key);
+            if (slot < 0) full = 1;
+            row = slot < 0 ? -1 : slot << mwl;
         }
-        __syncthreads();
-        src.walk([&](int b, int h, int w, int c) {
-                       const int ks = tile_of(g, b, h, w) * kBinGroups + group;
-                       const int j = atomicAdd(&counters[kEntriesOvf], 1);
-                       const int r = atomicAdd(&counters[kRunsOvf], 1);
-                       const int k = atomicAdd(&runs_per_slot[ks + 1], 1);
-                       if (j < ovf_entries)
-                           entries[fixed_e + j] =
-                               (static_cast<unsigned>(c) << g.local_bits) | local_of(g, h, w);
-                       if (r < ovf_runs)
-                           records[fixed_r + r] =
-                               BinRecord{ks, k, static_cast<int>(fixed_e + j), 1};
-                   });
+        if (row >= 0) {
+            const unsigned l = cell_local(g, h, w);
+            atomicOr(&mask[row + static_cast<int>(l >> 5)], 1u << (l & 31u));
+        }
+    });
+    __syncthreads();
+    if (full) {  // workgroup-uniform
+        if (threadIdx.x == 0) {
+            out.ovf_blocks[atomicAdd(&out.counters[kOvfBlocks], 1)] = static_cast<int>(blockIdx.x);
+            out.block_recs[blockIdx.x] = 0;
+        }
         return;
     }
-    // slot offsets within the region (kHash / kBlock slots per thread)
     constexpr int per_thread = kHash / kBlock;
-    int cnt[per_thread];
-    int here = 0, used = 0;
+    int key[per_thread], used = 0;
 #pragma unroll
     for (int j = 0; j < per_thread; ++j) {
-        cnt[j] = vals[threadIdx.x * per_thread + j];
-        here += cnt[j];
-        used += keys[threadIdx.x * per_thread + j] >= 0;
-    }
-    int total;
-    int run = block_excl_scan(here, sh, &total);
-#pragma unroll
-    for (int j = 0; j < per_thread; ++j) {
-        vals[threadIdx.x * per_thread + j] = run;  // now the slot's offset
-        run += cnt[j];
+        key[j] = keys[threadIdx.x * per_thread + j];
+        used += key[j] >= 0;
     }
     int nrec;
     int rec = block_excl_scan(used, sh, &nrec);  // this thread's first record
-    const bool staged = spill == 0;  // workgroup-uniform
-    if (threadIdx.x == 0) {
-        base_e = staged ? static_cast<int>(blockIdx.x) * kStage
-                        : static_cast<int>(fixed_e) + atomicAdd(&counters[kEntriesOvf], total);
-        const bool in_fixed = nrec <= kBinRecs;
-        base_r = in_fixed ? static_cast<int>(blockIdx.x) * kBinRecs
-                          : static_cast<int>(fixed_r) + atomicAdd(&counters[kRunsOvf], nrec);
-        block_runs[blockIdx.x] = in_fixed ? nrec : 0;
-    }
-    __syncthreads();
-    const int be = base_e, br = base_r;
-    const bool e_fits = staged || static_cast<long long>(be) + total <= fixed_e + ovf_entries;
-    // run records; then the slots' keys become per-slot cursors
+    if (threadIdx.x == 0) out.block_recs[blockIdx.x] = nrec;
+    const long long first = static_cast<long long>(blockIdx.x) * kBinRecs;
+    const int cmask = (1 << g.kb) - 1;
 #pragma unroll
     for (int j = 0; j < per_thread; ++j) {
-        const int s = threadIdx.x * per_thread + j;
-        const int key = keys[s];
-        if (key < 0) continue;
-        const int ks = key * kBinGroups + group;
-        const int k = atomicAdd(&runs_per_slot[ks + 1], 1);
-        const long long r = static_cast<long long>(br) + rec++;
-        if (r < fixed_r + ovf_runs) records[r] = BinRecord{ks, k, be + vals[s], cnt[j]};
-    }
-    if (!e_fits) return;
-    if (staged) {
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < per_thread; ++j) keys[threadIdx.x * per_thread + j] = 0;
-        // The thread's column in registers; after the barrier the stage is
-        // free.  Each run of same-slot entries reserves its span at its last
-        // entry, a backward pass puts every entry below its run's end in the
-        // stage, now in slot order, and the workgroup copies the stage to
-        // its region with coalesced 16-byte stores.
-        unsigned v[kPerThread];
-#pragma unroll
-        for (int i = 0; i < kPerThread; ++i) v[i] = i < mine ? stage[i * kBlock + threadIdx.x] : 0u;
-        __syncthreads();
-        int end[kPerThread];
-        int len = 0;
-#pragma unroll
-        for (int i = 0; i < kPerThread; ++i) {
-            end[i] = 0;
-            if (i < mine) {
-                const unsigned sl = v[i] >> 22;
-                ++len;
-                const bool last = i + 1 == mine || (i + 1 < kPerThread && (v[i + 1] >> 22) != sl);
-                if (last) {  // end > 0: the run holds at least this entry
-                    end[i] = vals[sl] + atomicAdd(&keys[sl], len) + len;
-                    len = 0;
-                }
-            }
+        if (key[j] < 0) continue;
+        int tile, cell;
+        cell_tile(g, key[j], tile, cell);
+        const long long r = first + rec++;
+        const unsigned* m = mask + ((threadIdx.x * per_thread + j) << mwl);
+        uint4 m0, m1;
+        if (mwl == 3) {  // 16 x 16 cells: all 8 words
+            m0 = reinterpret_cast<const uint4*>(m)[0];
+            m1 = reinterpret_cast<const uint4*>(m)[1];
+        } else {
+            unsigned v[kMaskWordsMax] = {};
+            for (int i = 0; i < (1 << mwl); ++i) v[i] = m[i];
+            m0 = make_uint4(v[0], v[1], v[2], v[3]);
+            m1 = make_uint4(v[4], v[5], v[6], v[7]);
         }
-        int at = 0;
-#pragma unroll
-        for (int i = kPerThread - 1; i >= 0; --i) {
-            if (i < mine) {
-                if (end[i]) at = end[i];
-                stage[--at] = v[i] & kEntryMask;
-            }
-        }
-        __syncthreads();
-        // total <= kStage here; the region is kStage entries from a 16-byte
-        // aligned start (be is a multiple of kStage)
-        uint4* dst = reinterpret_cast<uint4*>(entries + be);
-        const uint4* src = reinterpret_cast<const uint4*>(stage);
-        for (int q = threadIdx.x; q < (total + 3) / 4; q += kBlock) dst[q] = src[q];
-        return;
+        out.rec_ti[r] = make_int2(tile, (key[j] & cmask) | (cell << 16));
+        out.rec_mask[2 * r] = m0;
+        out.rec_mask[2 * r + 1] = m1;
     }
-    // more fragments than the stage holds: walk again and place directly;
-    // the stage words now serve as per-slot cursors
-    __syncthreads();
-    for (int s = threadIdx.x; s < kHash; s += kBlock) stage[s] = 0u;
-    if (threadIdx.x == 0) src.reset();
-    __syncthreads();
-    cur = -1;
-    slot = -1;
-    src.walk([&](int b, int h, int w, int c) {
-                   const int key = tile_of(g, b, h, w);
-                   if (key != cur) {
-                       cur = key;
-                       slot = hash_find(keys, key);
-                   }
-                   if (slot < 0) return;
-                   const int p = static_cast<int>(atomicAdd(&stage[slot], 1u));
-                   entries[be + vals[slot] + p] =
-                       (static_cast<unsigned>(c) << g.local_bits) | local_of(g, h, w);
-               });
 }
 
-// LDS of a stage workgroup (25.6 KB with the walk's deferred list).
+// LDS of a stage workgroup (18 KB, with the walk's deferred list 19 KB).
 #define S2C_STAGE_LDS                                                                  \
-    __shared__ int keys[kHash]; /* tile of each slot, then per-slot cursors */         \
-    __shared__ int vals[kHash]; /* fragments per slot, then slot offsets */            \
-    __shared__ unsigned stage[kStage];                                                 \
-    __shared__ int spill, full, base_e, base_r;                                        \
+    __shared__ int keys[kHash];     /* (cell, cluster) key of each slot */              \
+    __shared__ __align__(16) unsigned mask[kHash * kMaskWordsMax]; /* slot masks */     \
+    __shared__ int full;                                                               \
     __shared__ int sh[kBlock / kWave + 1];
 
-// 80 VGPRs (6 waves per SIMD, which the 25.6 KB of LDS also allows)
-// instead of the 82 the compiler picks unconstrained (5 waves); no spills.
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8)))
-dda_bin_stage_kernel(long long P, int H, int W, int V, const float* __restrict__ pos, int E,
-                     const int* __restrict__ edges, const int* __restrict__ edge2cluster,
-                     TileGrid g, int* __restrict__ runs_per_slot, int* __restrict__ counters,
-                     int* __restrict__ block_runs, unsigned* __restrict__ entries, int ovf_entries,
-                     BinRecord* __restrict__ records, int ovf_runs) {
+__global__ void __launch_bounds__(kBlock)
+dda_cell_stage_kernel(long long P, int H, int W, int V, const float* __restrict__ pos, int E,
+                      const int* __restrict__ edges, const int* __restrict__ edge2cluster,
+                      unsigned kmax, TileGrid g, BinOut out) {
     S2C_STAGE_LDS
     __shared__ int deferred[kDeferred];
     __shared__ int ndeferred;
     const EdgeSource src{P, H, W, V, pos, E, edges, edge2cluster, deferred, &ndeferred};
-    bin_stage(src, g, runs_per_slot, counters, block_runs, entries, ovf_entries, records,
-              ovf_runs, keys, vals, stage, spill, full, base_e, base_r, sh);
+    cell_stage(src, g, kmax, out, keys, mask, full, sh);
 }
 
-// The stage pass over a fragment list: kStage fragments per workgroup.
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8)))
-list_bin_stage_kernel(long long N, int B, int H, int W, const int* __restrict__ frag_pix,
-                      const float* __restrict__ frag_attrs, const int* __restrict__ prim2cluster,
-                      unsigned kmax, TileGrid g, int* __restrict__ runs_per_slot,
-                      int* __restrict__ counters, int* __restrict__ block_runs,
-                      unsigned* __restrict__ entries, int ovf_entries,
-                      BinRecord* __restrict__ records, int ovf_runs) {
+// The stage pass over a fragment list: kListFrags fragments per workgroup.
+__global__ void __launch_bounds__(kBlock)
+list_cell_stage_kernel(long long N, int B, int H, int W, const int* __restrict__ frag_pix,
+                       const float* __restrict__ frag_attrs, const int* __restrict__ prim2cluster,
+                       unsigned kmax, TileGrid g, BinOut out) {
     S2C_STAGE_LDS
     const ListSource src{N, B, H, W, frag_pix, frag_attrs, prim2cluster, kmax};
-    bin_stage(src, g, runs_per_slot, counters, block_runs, entries, ovf_entries, records,
-              ovf_runs, keys, vals, stage, spill, full, base_e, base_r, sh);
+    cell_stage(src, g, kmax, out, keys, mask, full, sh);
 }
 #undef S2C_STAGE_LDS
 
@@ -547,62 +437,28 @@ namespace {
 
 // The binned pass after its tile grid is chosen: workspace, counters,
 // stage (stage(plan) launches the stage kernel over plan.blocks
-// workgroups), resolve, and the device-side fallback (fallback(plan)
-// launches a kernel that ORs every fragment into the zeroed bitset when the
-// overflow areas were too small, and exits at once otherwise).
+// workgroups), resolve, and the fallback behind it (fallback(plan) launches
+// the kernel that ORs the fragments of the workgroups whose key table
+// overflowed).  Nothing waits for the device.
 template <class Stage, class Fallback>
 int binned_mask(BinPlan& plan, int H, int W, int num_slots, int* bitset, hipStream_t s,
                 Stage&& stage, Fallback&& fallback) {
     std::lock_guard<std::mutex> held(BinWorkspace::lock());
-    void* work = nullptr;
+    void *work = nullptr, *mbuf = nullptr, *obuf = nullptr;
     if (int r = BinWorkspace::get(s, BinWorkspace::kPlanInts,
                                   sizeof(int) * bin_plan_ints(plan.g, plan.blocks), &work))
         return r;
     bin_carve(plan, static_cast<int*>(work));
-    // No host wait: the overflow-area use of the last call on this stream
-    // grows the running estimates once its copy has arrived, and a call
-    // whose areas are too small finishes with direct ORs on the device.
-    BinStreamCounters* cnt = nullptr;
-    if (int r = BinWorkspace::counters(s, &cnt)) return r;
-    if (cnt->pending) {
-        const hipError_t q = hipEventQuery(cnt->done);
-        if (q == hipSuccess) {
-            long long ne = 0, nr = 0;
-            if (bin_overflow_use(cnt->host, &ne, &nr)) {
-                bin_entry_hint(ne + ne / 8 + 4 * kStage);
-                bin_run_hint(nr + nr / 8 + 4 * kHash);
-            }
-            cnt->pending = false;
-        } else if (q != hipErrorNotReady) {
-            S2C_HIP(q);
-        }
-    }
-    const long long want_e = dmin(bin_entry_hint(4 * kStage), static_cast<long long>(INT_MAX / 2));
-    const long long want_r = dmin(bin_run_hint(4 * kHash), static_cast<long long>(INT_MAX / 2));
-    plan.ovf_entries = static_cast<int>(want_e);
-    plan.ovf_runs = static_cast<int>(want_r);
-    void *ebuf = nullptr, *rbuf = nullptr, *obuf = nullptr;
-    if (int r = BinWorkspace::get(
-            s, BinWorkspace::kEntries,
-            sizeof(unsigned) * static_cast<size_t>(plan.fixed_entries() + want_e), &ebuf))
+    const size_t recs = static_cast<size_t>(plan.records());
+    if (int r = BinWorkspace::get(s, BinWorkspace::kMasks, 2 * sizeof(uint4) * recs, &mbuf))
         return r;
-    if (int r = BinWorkspace::get(
-            s, BinWorkspace::kRecords,
-            sizeof(BinRecord) * static_cast<size_t>(plan.fixed_runs() + want_r), &rbuf))
+    if (int r = BinWorkspace::get(s, BinWorkspace::kRuns, sizeof(int2) * recs, &obuf))
         return r;
-    if (int r = BinWorkspace::get(
-            s, BinWorkspace::kRuns,
-            sizeof(BinRun) * static_cast<size_t>(plan.fixed_runs() + want_r), &obuf))
-        return r;
-    plan.entries = static_cast<unsigned*>(ebuf);
-    plan.records = static_cast<BinRecord*>(rbuf);
-    plan.runs = static_cast<BinRun*>(obuf);
+    plan.rec_mask = static_cast<uint4*>(mbuf);
+    plan.runs = static_cast<int2*>(obuf);
     if (int r = bin_clear(plan, s)) return r;
     stage(plan);
     S2C_LAUNCH_CHECK();
-    if (int r = bin_counters_async(plan, cnt->host, s)) return r;
-    S2C_HIP(hipEventRecord(cnt->done, s));
-    cnt->pending = true;
     if (int r = bin_resolve(plan, H, W, num_slots, bitset, s)) return r;
     fallback(plan);
     S2C_LAUNCH_CHECK();
@@ -633,7 +489,8 @@ extern "C" int s2c_dda_cluster_mask_tiled(int B, int H, int W, int V, const floa
     S2C_CHECK(tiled || tile_log2 == 0 || P == 0,
               "dda_cluster_mask: the requested tile bitset does not fit LDS");
     if (!tiled) {
-        // Empty input, or a tile bitset too large for LDS: clear + direct ORs.
+        // Empty input, a tile bitset too large for LDS, or (cell, cluster)
+        // keys beyond 31 bits: clear + direct ORs.
         if (int r = fill_u32(bitset, 0u, words, s)) return r;
         if (P == 0) return S2C_OK;
         dda_mask_direct_kernel<<<static_cast<unsigned>(plan.blocks), kBlock, 0, s>>>(
@@ -641,19 +498,19 @@ extern "C" int s2c_dda_cluster_mask_tiled(int B, int H, int W, int V, const floa
         S2C_LAUNCH_CHECK();
         return S2C_OK;
     }
-    S2C_CHECK(plan.blocks * kStage < INT_MAX / 2 && plan.blocks * kBinRecs < INT_MAX / 2,
-              "dda_cluster_mask: too many primitives for 32-bit entry offsets");
+    S2C_CHECK(plan.records() < INT_MAX / 2 && plan.sort_blocks() * plan.g.nt < INT_MAX / 2,
+              "dda_cluster_mask: too many primitives for 32-bit record indices");
+    const unsigned kmax = static_cast<unsigned>(num_slots) * 32u;
     return binned_mask(
         plan, H, W, num_slots, bitset, s,
         [&](const BinPlan& p) {
-            dda_bin_stage_kernel<<<static_cast<unsigned>(p.blocks), kBlock, 0, s>>>(
-                P, H, W, V, pos, E, edges, edge2cluster, p.g, p.runs_per_slot, p.counters,
-                p.block_runs, p.entries, p.ovf_entries, p.records, p.ovf_runs);
+            dda_cell_stage_kernel<<<static_cast<unsigned>(p.blocks), kBlock, 0, s>>>(
+                P, H, W, V, pos, E, edges, edge2cluster, kmax, p.g, p.out());
         },
         [&](const BinPlan& p) {
             dda_mask_fallback_kernel<<<fallback_grid(p.blocks), kBlock, 0, s>>>(
                 P, H, W, V, pos, E, edges, edge2cluster, num_slots, bitset, p.counters,
-                p.ovf_entries, p.ovf_runs, p.blocks);
+                p.ovf_blocks);
         });
 }
 
@@ -669,7 +526,7 @@ extern "C" int s2c_fragment_cluster_mask(int B, int H, int W, int num_frags, con
     const unsigned kmax = static_cast<unsigned>(dmin(static_cast<long long>(num_slots) * 32,
                                                      static_cast<long long>(UINT_MAX)));
     BinPlan plan;
-    plan.blocks = (N + kStage - 1) / kStage;
+    plan.blocks = (N + kListFrags - 1) / kListFrags;
     const bool tiled = N > 0 && choose_tiles(B, H, W, num_slots, 4096, plan.g);
     if (!tiled) {
         if (int r = fill_u32(bitset, 0u, words, s)) return r;
@@ -679,26 +536,19 @@ extern "C" int s2c_fragment_cluster_mask(int B, int H, int W, int num_frags, con
         S2C_LAUNCH_CHECK();
         return S2C_OK;
     }
-    S2C_CHECK(plan.blocks * kStage < INT_MAX / 2 && plan.blocks * kBinRecs < INT_MAX / 2,
-              "fragment_cluster_mask: too many fragments for 32-bit entry offsets");
+    S2C_CHECK(plan.records() < INT_MAX / 2 && plan.sort_blocks() * plan.g.nt < INT_MAX / 2,
+              "fragment_cluster_mask: too many fragments for 32-bit record indices");
     return binned_mask(
         plan, H, W, num_slots, bitset, s,
         [&](const BinPlan& p) {
-            list_bin_stage_kernel<<<static_cast<unsigned>(p.blocks), kBlock, 0, s>>>(
-                N, B, H, W, frag_pix, frag_attrs, prim2cluster, kmax, p.g, p.runs_per_slot,
-                p.counters, p.block_runs, p.entries, p.ovf_entries, p.records, p.ovf_runs);
+            list_cell_stage_kernel<<<static_cast<unsigned>(p.blocks), kBlock, 0, s>>>(
+                N, B, H, W, frag_pix, frag_attrs, prim2cluster, kmax, p.g, p.out());
         },
         [&](const BinPlan& p) {
             list_mask_fallback_kernel<<<fallback_grid(p.blocks), kBlock, 0, s>>>(
                 N, B, H, W, frag_pix, frag_attrs, prim2cluster, kmax, num_slots, bitset,
-                p.counters, p.ovf_entries, p.ovf_runs);
+                p.counters, p.ovf_blocks);
         });
-}
-
-extern "C" int s2c_dda_cluster_mask_reset_estimates(void) {
-    std::lock_guard<std::mutex> held(BinWorkspace::lock());
-    bin_reset_hints();
-    return S2C_OK;
 }
 
 extern "C" int s2c_dda_cluster_mask_release(void* stream) {

@@ -186,26 +186,27 @@ def test_dda_mask_long_scattered_edges(dev):
     np.testing.assert_array_equal(got, want)
 
 
-def test_dda_mask_overflow_fallback_then_sized(dev):
-    """With the overflow-area estimate reset, a call whose workgroups spill
-    far past the smallest areas finishes with direct ORs on the device (the
-    call never waits for the device); once its use has been read back, the
-    next call is sized for it and takes the binned path.  Both masks equal
-    the oracle's."""
-    import capi
-
-    report("dda->mask overflow areas too small, then sized from the first call")
+def test_dda_mask_table_overflow_twice(dev):
+    """Workgroups of long lines (every one past its key table: their
+    fragments are OR-ed behind the binned pass) alternating with workgroups
+    of short ones (binned), twice in a row on the same buffers; both masks
+    equal the oracle's."""
+    report("dda->mask key-table overflow mixed with binned workgroups, twice")
     rng = np.random.default_rng(37)
-    pos, edges = random_lines(rng, 2, 6000, 5000, 1024, 1024, persp=False)
+    pos_l, edges_l = random_lines(rng, 1, 3000, 2048, 1024, 1024, persp=False)
+    pos_s, edges_s = _segments(rng, 2048, 1.5, 3.0, 160, 1024, 1024)
+    pos = np.concatenate([pos_l, pos_s], 1)
+    edges_s = edges_s + pos_l.shape[1]
+    edges = np.concatenate([edges_l[:1024], edges_s[:1024], edges_l[1024:], edges_s[1024:]])
     e2c = rng.integers(0, 100, size=edges.shape[0]).astype(np.int32)
+    e2c[1024:2048] %= 4  # the short lines' workgroups: few clusters
+    e2c[3072:] %= 4
+    keys = _workgroup_keys(pos, edges, e2c, (1024, 1024))
+    assert keys[0] > 512 and keys[2] > 512 and keys[1] < 400 and keys[3] < 400
     want = oracle.dda_cluster_mask((1024, 1024), pos, edges, e2c, 100, threads=8)
-    torch.cuda.synchronize()
-    assert capi.lib().s2c_dda_cluster_mask_reset_estimates() == 0
-    got = fused_dda_mask(dev, (1024, 1024), pos, edges, e2c, 100)  # areas of 16K entries
-    np.testing.assert_array_equal(got, want)
-    torch.cuda.synchronize()
-    got = fused_dda_mask(dev, (1024, 1024), pos, edges, e2c, 100)  # areas sized from the use
-    np.testing.assert_array_equal(got, want)
+    for _ in range(2):
+        got = fused_dda_mask(dev, (1024, 1024), pos, edges, e2c, 100)
+        np.testing.assert_array_equal(got, want)
 
 
 def _segments(rng, n, lo, hi, region, H, W):
@@ -224,25 +225,38 @@ def _segments(rng, n, lo, hi, region, H, W):
     return pos, np.arange(2 * n, dtype=np.int32).reshape(n, 2)
 
 
-@pytest.mark.parametrize("case", ["records_overflow", "stage_spill", "thread_runs", "long_runs"])
+def _workgroup_keys(pos, edges, e2c, res, cell=16):
+    """Distinct (16 x 16 cell, cluster) keys of each 1,024-edge workgroup
+    of the fused pass over view 0 (the oracle's fragments)."""
+    fp, fa = oracle.dda_compute_fragments(res, pos[:1], edges)
+    eid = fa[:, 3].astype(np.int64) - 1
+    k = eid >= 0
+    key = (((eid[k] // 1024) * 4096 + fp[k, 1] // cell) * 4096 + fp[k, 2] // cell) * 65536 \
+        + e2c[eid[k]]
+    return np.bincount(np.unique(key) // (4096 * 4096 * 65536))
+
+
+@pytest.mark.parametrize("case", ["near_capacity", "table_overflow", "cells_in_large_tiles"])
 def test_dda_mask_workgroup_paths(dev, case):
-    """Workgroups (1,024 edges each) touching 513-1,024 distinct tiles (their
-    run records go to the overflow area); workgroups whose threads have more
-    fragments than their LDS stage share (placed by a second walk); threads
-    whose four edges make more than four runs of same-tile fragments (the
-    runs held in registers enter the LDS hash in the middle of the walk);
-    and threads whose runs reach the 127-fragment cap of a register run
-    (64-pixel tiles at 64 clusters, edges of 40-60 pixels inside one
-    tile)."""
+    """Workgroups (1,024 edges each) with 370-390 distinct (cell, cluster)
+    keys in their 512-slot table (long probe sequences); with about 1,170
+    (the table overflows: fragments OR-ed behind the binned pass); and
+    64-pixel tiles of 4 x 4 cells (40-60 pixel edges inside one tile, 16
+    clusters), so records carry the cell's place in the tile."""
     report(f"dda->mask workgroup path: {case}")
     rng = np.random.default_rng(5)
-    lo, hi, region = {"records_overflow": (1.5, 3.0, 448), "stage_spill": (10.0, 14.0, 320),
-                      "thread_runs": (30.0, 60.0, 512), "long_runs": (40.0, 60.0, 63)}[case]
+    lo, hi, region, K = {"near_capacity": (1.5, 3.0, 160, 4),
+                         "table_overflow": (1.5, 3.0, 448, 64),
+                         "cells_in_large_tiles": (40.0, 60.0, 63, 16)}[case]
     pos, edges = _segments(rng, 4096, lo, hi, region, 1024, 1024)
-    e2c = rng.integers(0, 64, size=edges.shape[0]).astype(np.int32)
+    e2c = rng.integers(0, K, size=edges.shape[0]).astype(np.int32)
+    keys = _workgroup_keys(pos, edges, e2c, (1024, 1024))
+    lo_k, hi_k = {"near_capacity": (350, 420), "table_overflow": (1000, 4096),
+                  "cells_in_large_tiles": (200, 300)}[case]
+    assert lo_k <= keys.min() and keys.max() <= hi_k, keys
     want = oracle.dda_cluster_mask((1024, 1024), pos, edges, e2c, 64, threads=8)
     got = fused_dda_mask(dev, (1024, 1024), pos, edges, e2c, 64,
-                         tile_log2=6 if case == "long_runs" else 0)
+                         tile_log2=6 if case == "cells_in_large_tiles" else 0)
     np.testing.assert_array_equal(got, want)
 
 
@@ -354,14 +368,11 @@ def test_fragment_mask_invalid_entries(dev):
     np.testing.assert_array_equal(list_mask(dev, B, res, fp, fa, p2c_bad, K), want)
 
 
-def test_fragment_mask_overflow_fallback_then_sized(dev):
+def test_fragment_mask_table_overflow_twice(dev):
     """Fragments at random pixels of 4 views at 1024^2: every workgroup's
-    4,096 fragments touch more tiles than its LDS table holds, so all go to
-    the overflow areas; from the smallest estimate the pass falls back to
-    direct ORs on the device, then runs binned with the areas sized."""
-    import capi
-
-    report("fragment-list mask overflow fallback, then sized")
+    4,096 fragments have more distinct (cell, cluster) keys than its table
+    holds, so all are OR-ed behind the binned pass; twice in a row."""
+    report("fragment-list mask key-table overflow, twice")
     rng = np.random.default_rng(91)
     res, B, K, n = (1024, 1024), 4, 256, 300000
     fp = np.stack([rng.integers(0, B, n), rng.integers(0, res[0], n),
@@ -370,11 +381,8 @@ def test_fragment_mask_overflow_fallback_then_sized(dev):
     fa[:, 3] = rng.integers(1, 1000, n)
     p2c = rng.integers(0, K, size=1000).astype(np.int32)
     want = oracle.cluster_mask_from_fragments(res, fp, fa, p2c, K, B=B)
-    torch.cuda.synchronize()
-    assert capi.lib().s2c_dda_cluster_mask_reset_estimates() == 0
-    np.testing.assert_array_equal(list_mask(dev, B, res, fp, fa, p2c, K), want)
-    torch.cuda.synchronize()
-    np.testing.assert_array_equal(list_mask(dev, B, res, fp, fa, p2c, K), want)
+    for _ in range(2):
+        np.testing.assert_array_equal(list_mask(dev, B, res, fp, fa, p2c, K), want)
 
 
 @pytest.mark.slow
