@@ -91,18 +91,17 @@ __device__ __forceinline__ int order_class(int w) {
 
 // In place: counts[j * nt + t] becomes the number of records of tile t in
 // sort blocks before j.  The last segment writes the tile's total into
-// offs[t + 1] (offs is scanned into tile offsets next) and counts its
+// offs[t], the sum of its block's totals into bsums[blockIdx.x], and its
 // block's tiles per order class into classes[k * gridDim.x + blockIdx.x].
 __global__ void __launch_bounds__(kBlock)
 bin_colprefix_kernel(long long nt, long long nb, int* __restrict__ counts,
                      const int* __restrict__ segs, int* __restrict__ offs,
-                     int* __restrict__ classes) {
+                     int* __restrict__ bsums, int* __restrict__ classes) {
     __shared__ int hist[kOrderClasses];
-    const bool last = blockIdx.y == kSortSegs - 1;  // workgroup-uniform
-    if (last && threadIdx.x < kOrderClasses) hist[threadIdx.x] = 0;
-    if (last) __syncthreads();
-    const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
-    if (t < nt) {
+    __shared__ int sh[kBlock / kWave + 1];
+    if (blockIdx.y != kSortSegs - 1) {
+        const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+        if (t >= nt) return;
         long long j0, j1;
         sort_segment(nb, blockIdx.y, j0, j1);
         int acc = 0;
@@ -112,61 +111,60 @@ bin_colprefix_kernel(long long nt, long long nb, int* __restrict__ counts,
             counts[j * nt + t] = acc;
             acc += c;
         }
-        if (last) {
-            offs[t + 1] = acc;
-            atomicAdd(&hist[order_class(acc)], 1);
-        }
+        return;
     }
-    if (!last) return;
-    __syncthreads();
-    if (threadIdx.x < kOrderClasses) classes[threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
-}
-
-// Exclusive scan of the n class counts in place (class-major, so each
-// (class, block) gets the first order position of its tiles).  One
-// workgroup of kOrderThreads.
-constexpr int kOrderThreads = 1024;
-
-__global__ void __launch_bounds__(kOrderThreads)
-bin_class_scan_kernel(long long n, int* __restrict__ classes) {
-    __shared__ int sh[kOrderThreads / kWave + 1];
-    int carry = 0;
-    for (long long c0 = 0; c0 < n; c0 += kOrderThreads) {
-        const long long i = c0 + threadIdx.x;
-        const int v = i < n ? classes[i] : 0;
-        const int incl = wave_incl_scan(v);
-        if (lane_id() == kWave - 1) sh[threadIdx.x / kWave] = incl;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int acc = 0;
-            for (int w = 0; w < kOrderThreads / kWave; ++w) {
-                const int x = sh[w];
-                sh[w] = acc;
-                acc += x;
-            }
-            sh[kOrderThreads / kWave] = acc;
-        }
-        __syncthreads();
-        if (i < n) classes[i] = carry + sh[threadIdx.x / kWave] + incl - v;
-        carry += sh[kOrderThreads / kWave];
-        __syncthreads();
-    }
-}
-
-// The resolve order: (tile, first record, record end) of each tile at its
-// class's next position (order within a class arbitrary).  Same grid as
-// the class counts.
-__global__ void __launch_bounds__(kBlock)
-bin_order_kernel(long long nt, const int* __restrict__ offs, const int* __restrict__ classes,
-                 int4* __restrict__ order) {
-    __shared__ int next[kOrderClasses];
-    if (threadIdx.x < kOrderClasses) next[threadIdx.x] = classes[threadIdx.x * gridDim.x + blockIdx.x];
+    if (threadIdx.x < kOrderClasses) hist[threadIdx.x] = 0;
     __syncthreads();
     const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    int acc = 0;
+    if (t < nt) {
+        long long j0, j1;
+        sort_segment(nb, blockIdx.y, j0, j1);
+        for (int k = 0; k < kSortSegs - 1; ++k) acc += segs[k * nt + t];
+        for (long long j = j0; j < j1; ++j) {
+            const int c = counts[j * nt + t];
+            counts[j * nt + t] = acc;
+            acc += c;
+        }
+        offs[t] = acc;
+        atomicAdd(&hist[order_class(acc)], 1);
+    }
+    int total;
+    block_excl_scan(acc, sh, &total);  // contains the barrier before hist is read
+    if (threadIdx.x < kOrderClasses) classes[threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
+    if (threadIdx.x == 0) bsums[blockIdx.x] = total;
+}
+
+// Exclusive scans in place by one workgroup of the n class counts
+// (class-major, so each (class, block) gets the first order position of its
+// tiles) and of the nblk block sums of tile totals.
+__global__ void __launch_bounds__(kWgScanThreads)
+bin_scan_kernel(long long n, int* __restrict__ classes, long long nblk, int* __restrict__ bsums) {
+    wg_exclusive_scan(classes, n);
+    __syncthreads();
+    wg_exclusive_scan(bsums, nblk);
+}
+
+// Tile offsets and the resolve order: offs[t] becomes the first record of
+// tile t (its block's offset plus a block scan of the totals; offs[nt] the
+// record count), and (tile, first record, record end) goes to its class's
+// next position (order within a class arbitrary).  Same grid as the class
+// counts.
+__global__ void __launch_bounds__(kBlock)
+bin_order_kernel(long long nt, int* __restrict__ offs, const int* __restrict__ bsums,
+                 const int* __restrict__ classes, int4* __restrict__ order) {
+    __shared__ int next[kOrderClasses];
+    __shared__ int sh[kBlock / kWave + 1];
+    if (threadIdx.x < kOrderClasses) next[threadIdx.x] = classes[threadIdx.x * gridDim.x + blockIdx.x];
+    const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    const int w = t < nt ? offs[t] : 0;
+    int total;
+    const int a = bsums[blockIdx.x] + block_excl_scan(w, sh, &total);  // barrier inside
     if (t >= nt) return;
-    const int a = offs[t], e = offs[t + 1];
-    const int pos = atomicAdd(&next[order_class(e - a)], 1);
-    order[pos] = make_int4(static_cast<int>(t), a, e, 0);
+    offs[t] = a;
+    if (t == nt - 1) offs[nt] = a + w;
+    const int pos = atomicAdd(&next[order_class(w)], 1);
+    order[pos] = make_int4(static_cast<int>(t), a, a + w, 0);
 }
 
 // Sort block j places its records at offs[t] + (its first position in tile
@@ -196,7 +194,7 @@ bin_place_kernel(long long nt, long long blocks, const int* __restrict__ block_r
 // bitset at every pixel the word sets.  The LDS word of (pixel l, word j)
 // is l*S + (j ^ (l & smask)): with S a power of two, the pixels of one
 // cluster's mask (same j) fall in different banks.
-template <bool VEC4, int kLds>
+template <bool VEC4, int kLds, bool ONE_CELL>
 __global__ void __launch_bounds__(kResolveThreads)
 cell_resolve_kernel(TileGrid g, const int4* __restrict__ order, const int2* __restrict__ runs,
                     const uint4* __restrict__ rec_mask, int H, int W, int S,
@@ -239,12 +237,15 @@ cell_resolve_kernel(TileGrid g, const int4* __restrict__ order, const int2* __re
             const int ox = (cell & ((1 << cpl) - 1)) << cs;
             const unsigned bit = 1u << (c & 31u);
             const unsigned j = c >> 5;
+            const unsigned wbase = static_cast<unsigned>(word) << 5;
             do {
-                const int p = (word << 5) + __ffs(v) - 1;  // pixel within the cell
+                const unsigned p = wbase + static_cast<unsigned>(__ffs(v) - 1);  // pixel in the cell
                 v &= v - 1u;
-                const unsigned l = static_cast<unsigned>(((oy + (p >> cs)) << g.ts_log2) |
-                                                         (ox + (p & ((1 << cs) - 1))));
-                atomicOr(&tile[l * S + (j ^ (l & smask))], bit);
+                // with one cell per tile the cell's pixel is the tile's
+                const unsigned l = ONE_CELL ? p
+                                            : static_cast<unsigned>(((oy + (p >> cs)) << g.ts_log2) |
+                                                                    (ox + (p & ((1 << cs) - 1))));
+                atomicOr(&tile[__umul24(l, static_cast<unsigned>(S)) + (j ^ (l & smask))], bit);
             } while (v != 0u);
         }
     }
@@ -308,38 +309,30 @@ size_t bin_plan_ints(const TileGrid& g, long long blocks) {
     BinPlan p;
     p.g = g;
     p.blocks = blocks;
-    return static_cast<size_t>(scan_scratch_ints(g.nt + 1) + 4 + 4 * g.nt + kBinCounters +
-                               (g.nt + 2) + 2 * blocks + kSortSegs * g.nt +
-                               kOrderClasses * p.class_blocks() + p.sort_blocks() * g.nt +
-                               2 * p.records() + 8);
+    return static_cast<size_t>(4 * g.nt + (g.nt + 1) + blocks + kSortSegs * g.nt +
+                               (kOrderClasses + 1) * p.class_blocks() + 2 +
+                               p.sort_blocks() * g.nt + 2 * p.records() + 8);
 }
 
 void bin_carve(BinPlan& p, int* ints) {
     const long long nt = p.g.nt;
-    p.scratch = ints;  // int64 values first: 8-byte aligned
-    long long off = static_cast<long long>(scan_scratch_ints(nt + 1));
-    off = (off + 3) & ~3ll;
-    p.order = reinterpret_cast<int4*>(ints + off);  // 16-byte aligned
+    long long off = 0;
+    p.order = reinterpret_cast<int4*>(ints);  // 16-byte aligned
     off += 4 * nt;
-    p.counters = ints + off;  // counters then tile totals: one contiguous clear
-    p.offs = p.counters + kBinCounters;
-    off += kBinCounters + nt + 2;
-    p.ovf_blocks = ints + off;
-    off += p.blocks;
+    p.offs = ints + off;
+    off += nt + 1;
     p.block_recs = ints + off;
     off += p.blocks;
     p.segs = ints + off;
     off += kSortSegs * nt;
     p.classes = ints + off;
-    off += kOrderClasses * p.class_blocks();
+    off += kOrderClasses * p.class_blocks() + 1;
+    p.bsums = ints + off;
+    off += p.class_blocks() + 1;
     p.counts = ints + off;
     off += p.sort_blocks() * nt;
     off = (off + 1) & ~1ll;
     p.rec_ti = reinterpret_cast<int2*>(ints + off);  // 8-byte aligned
-}
-
-int bin_clear(const BinPlan& p, hipStream_t s) {
-    return fill_u32(p.counters, 0u, kBinCounters + p.g.nt + 2, s);
 }
 
 int bin_resolve(const BinPlan& p, int H, int W, int S, int* bitset, hipStream_t s) {
@@ -353,31 +346,37 @@ int bin_resolve(const BinPlan& p, int H, int W, int S, int* bitset, hipStream_t 
     bin_colsum_kernel<<<dim3(cb, kSortSegs), kBlock, 0, s>>>(nt, nb, p.counts, p.segs);
     S2C_LAUNCH_CHECK();
     bin_colprefix_kernel<<<dim3(cb, kSortSegs), kBlock, 0, s>>>(nt, nb, p.counts, p.segs, p.offs,
-                                                                p.classes);
+                                                                p.bsums, p.classes);
     S2C_LAUNCH_CHECK();
-    if (int r = inclusive_scan_inplace(p.offs, nt + 1, p.scratch, s)) return r;
-    bin_class_scan_kernel<<<1, kOrderThreads, 0, s>>>(kOrderClasses * p.class_blocks(), p.classes);
+    bin_scan_kernel<<<1, kWgScanThreads, 0, s>>>(kOrderClasses * p.class_blocks(), p.classes,
+                                                 p.class_blocks(), p.bsums);
     S2C_LAUNCH_CHECK();
-    bin_order_kernel<<<cb, kBlock, 0, s>>>(nt, p.offs, p.classes, p.order);
+    bin_order_kernel<<<cb, kBlock, 0, s>>>(nt, p.offs, p.bsums, p.classes, p.order);
     S2C_LAUNCH_CHECK();
     bin_place_kernel<<<static_cast<unsigned>(nb), kSortThreads, lds, s>>>(
         nt, p.blocks, p.block_recs, p.rec_ti, p.counts, p.offs, p.runs);
     S2C_LAUNCH_CHECK();
     const bool vec = (S % 4 == 0) && ((reinterpret_cast<uintptr_t>(bitset) & 15) == 0);
     const bool small = tile_bitset_bytes(p.g.ts_log2, S) <= kTileLdsBytes;
+    const bool one = p.g.cs_log2 == p.g.ts_log2;
     const unsigned grid_t = static_cast<unsigned>(nt);
-    if (vec && small)
-        cell_resolve_kernel<true, kTileLdsBytes><<<grid_t, kResolveThreads, 0, s>>>(
-            p.g, p.order, p.runs, p.rec_mask, H, W, S, bitset);
-    else if (vec)
-        cell_resolve_kernel<true, kTileLdsMax><<<grid_t, kResolveThreads, 0, s>>>(
-            p.g, p.order, p.runs, p.rec_mask, H, W, S, bitset);
-    else if (small)
-        cell_resolve_kernel<false, kTileLdsBytes><<<grid_t, kResolveThreads, 0, s>>>(
-            p.g, p.order, p.runs, p.rec_mask, H, W, S, bitset);
-    else
-        cell_resolve_kernel<false, kTileLdsMax><<<grid_t, kResolveThreads, 0, s>>>(
-            p.g, p.order, p.runs, p.rec_mask, H, W, S, bitset);
+#define S2C_RESOLVE(V, L, O)                                                      \
+    cell_resolve_kernel<V, L, O><<<grid_t, kResolveThreads, 0, s>>>(p.g, p.order, p.runs, \
+                                                                  p.rec_mask, H, W, S, bitset)
+    if (vec && small) {
+        if (one) S2C_RESOLVE(true, kTileLdsBytes, true);
+        else S2C_RESOLVE(true, kTileLdsBytes, false);
+    } else if (vec) {
+        if (one) S2C_RESOLVE(true, kTileLdsMax, true);
+        else S2C_RESOLVE(true, kTileLdsMax, false);
+    } else if (small) {
+        if (one) S2C_RESOLVE(false, kTileLdsBytes, true);
+        else S2C_RESOLVE(false, kTileLdsBytes, false);
+    } else {
+        if (one) S2C_RESOLVE(false, kTileLdsMax, true);
+        else S2C_RESOLVE(false, kTileLdsMax, false);
+    }
+#undef S2C_RESOLVE
     S2C_LAUNCH_CHECK();
     return S2C_OK;
 }

@@ -4,24 +4,25 @@
 // bit mask of its (cell, cluster) key, where a cell is a square of up to
 // 16 x 16 pixels inside a screen tile; the key's mask is found through an
 // LDS hash, and a fragment that repeats a pixel and cluster already seen by
-// the workgroup sets no new bit.  After the walk the workgroup appends one
-// record per key (its tile, and its cluster, cell and mask) to a flat
-// record list, with one global atomic per workgroup.
+// the workgroup sets no new bit.  After the walk the workgroup writes one
+// record per key (its tile, and its cluster, cell and mask) into its own
+// region (no global atomics).
 //
 // The records are then put in tile order by a counting sort without global
-// atomics: each sort block counts its kSortChunk records per tile in LDS,
-// a two-level scan down the (sort block, tile) count matrix gives every
-// sort block its first position within each tile, a scan of the tile
-// totals gives the tiles' offsets, and each sort block places its records,
-// ranked within their tile by LDS atomics.  One workgroup per tile then ORs
+// atomics: each sort block counts the records of its kSortChunk stage
+// workgroups per tile in LDS, a two-level scan down the (sort block, tile)
+// count matrix gives every sort block its first position within each tile,
+// block scans of the tile totals give the tiles' offsets, and each sort
+// block places its records, ranked within their tile by LDS atomics.  One workgroup per tile then ORs
 // its records' mask pixels into an LDS copy of the tile's bitset and writes
 // the tile once with coalesced stores (no clear of the bitset, no global OR
 // atomics).
 //
 // A workgroup with more distinct keys than its table holds writes no record
-// and lists itself; a kernel behind the resolve walks the listed workgroups
-// again and ORs their fragments directly into the written bitset (it exits
-// at once when the list is empty).  The host reads nothing back.
+// and marks its record count -1; a kernel behind the resolve walks the
+// marked workgroups again and ORs their fragments directly into the written
+// bitset.  Nothing is cleared before a pass and the host reads nothing
+// back.
 #pragma once
 
 #include <mutex>
@@ -180,14 +181,9 @@ constexpr int kSortChunk = 64;
 constexpr int kSortTiles = 16384;  // 64 KB of LDS counters
 constexpr int kSortSegs = 16;
 
-// Global counters.
-enum BinCounter { kOvfBlocks = 0, kBinCounters = 4 };
-
 // Outputs of a stage workgroup.
 struct BinOut {
-    int* counters;    // kBinCounters ints
-    int* ovf_blocks;  // workgroups whose table overflowed
-    int* block_recs;  // records of each workgroup
+    int* block_recs;  // records of each workgroup; -1 when its table overflowed
     int2* rec_ti;     // (tile, info) of each record
     uint4* rec_mask;  // two uint4 per record
 };
@@ -195,14 +191,12 @@ struct BinOut {
 struct BinPlan {
     TileGrid g;
     long long blocks = 0;       // workgroups of the stage pass
-    int* counters = nullptr;    // kBinCounters ints
-    int* offs = nullptr;        // nt + 2 ints: tile totals shifted by one; offsets after the scan
-    int* ovf_blocks = nullptr;  // `blocks` ints
+    int* offs = nullptr;        // nt + 1 ints: tile totals, then tile offsets
     int* block_recs = nullptr;  // `blocks` ints
     int4* order = nullptr;      // nt: resolve order (tile, first record, record end)
-    int* scratch = nullptr;     // scan scratch
     int* segs = nullptr;        // kSortSegs * nt segment sums of the count matrix
-    int* classes = nullptr;     // kOrderClasses * class_blocks(): tiles per order class
+    int* classes = nullptr;     // kOrderClasses * class_blocks() + 1: tiles per order class
+    int* bsums = nullptr;       // class_blocks() + 1: record totals per class block
     int* counts = nullptr;      // sort_blocks() * nt: counts, then first positions
     int2* rec_ti = nullptr;     // records(): blocks regions of kBinRecs
     uint4* rec_mask = nullptr;  // 2 * records()
@@ -210,16 +204,14 @@ struct BinPlan {
     long long records() const { return blocks * kBinRecs; }
     long long sort_blocks() const { return (blocks + kSortChunk - 1) / kSortChunk; }
     long long class_blocks() const { return (g.nt + kBlock - 1) / kBlock; }
-    BinOut out() const { return BinOut{counters, ovf_blocks, block_recs, rec_ti, rec_mask}; }
+    BinOut out() const { return BinOut{block_recs, rec_ti, rec_mask}; }
 };
 
-// Device ints of a plan besides its masks and runs (counters, tile offsets,
-// scan scratch, resolve order, overflow list, count matrix, record tiles).
+// Device ints of a plan besides its masks and runs (tile offsets, resolve
+// order, record counts, count matrix and its scans, record tiles).
 size_t bin_plan_ints(const TileGrid& g, long long blocks);
 // Carves them out of `ints`.
 void bin_carve(BinPlan& p, int* ints);
-// Zeroes the global counters and the tile totals.
-int bin_clear(const BinPlan& p, hipStream_t s);
 // The records sorted into tile order, then one workgroup per tile ORs their
 // masks into LDS and writes the tile bitset.
 int bin_resolve(const BinPlan& p, int H, int W, int S, int* bitset, hipStream_t s);

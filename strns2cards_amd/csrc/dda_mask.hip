@@ -156,20 +156,40 @@ dda_mask_direct_kernel(long long P, int H, int W, int V, const float* __restrict
 }
 
 // Behind the binned pass: OR every fragment of the stage workgroups whose
-// key table overflowed (listed in `ovf_blocks`, counters[kOvfBlocks] of
-// them) directly into the resolved bitset, the grid striding over the
-// list; with an empty list every workgroup exits at once (a small grid, so
-// the common case costs a few microseconds).
+// key table overflowed (record count -1) directly into the resolved
+// bitset.  Workgroup i checks stage workgroups [i * kBlock, (i + 1) * kBlock)
+// with one load per thread (grid-striding), so when none overflowed the
+// kernel costs a few microseconds.
 constexpr int kFallbackGrid = 256;
+
+// Calls f(b) for every overflowed stage workgroup b of this workgroup's
+// windows (workgroup-uniform).
+template <class F>
+__device__ __forceinline__ void for_overflowed(long long blocks, const int* __restrict__ block_recs,
+                                               F&& f) {
+    __shared__ int list[kBlock];
+    __shared__ int nlist;
+    for (long long w0 = static_cast<long long>(blockIdx.x) * kBlock; w0 < blocks;
+         w0 += static_cast<long long>(gridDim.x) * kBlock) {
+        if (threadIdx.x == 0) nlist = 0;
+        __syncthreads();
+        const long long b = w0 + threadIdx.x;
+        if (b < blocks && block_recs[b] < 0) list[atomicAdd(&nlist, 1)] = static_cast<int>(b);
+        __syncthreads();
+        const int n = nlist;
+        for (int i = 0; i < n; ++i) f(static_cast<long long>(list[i]));
+        __syncthreads();  // list and nlist are rewritten by the next window
+    }
+}
 
 __global__ void __launch_bounds__(kBlock)
 dda_mask_fallback_kernel(long long P, int H, int W, int V, const float* __restrict__ pos, int E,
                          const int* __restrict__ edges, const int* __restrict__ edge2cluster,
-                         int S, int* __restrict__ bitset, const int* __restrict__ counters,
-                         const int* __restrict__ ovf_blocks) {
-    const int n = counters[kOvfBlocks];  // workgroup-uniform
-    for (int i = blockIdx.x; i < n; i += gridDim.x)
-        dda_mask_direct(P, H, W, V, pos, E, edges, edge2cluster, S, bitset, ovf_blocks[i]);
+                         int S, int* __restrict__ bitset, long long blocks,
+                         const int* __restrict__ block_recs) {
+    for_overflowed(blocks, block_recs, [&](long long blk) {
+        dda_mask_direct(P, H, W, V, pos, E, edges, edge2cluster, S, bitset, blk);
+    });
 }
 
 // One global OR for fragment i of a list (rasterize.cu:474-498), with
@@ -207,20 +227,19 @@ list_mask_direct_kernel(long long N, int B, int H, int W, const int* __restrict_
 constexpr int kListFrags = 4096;
 
 // The list's counterpart of dda_mask_fallback_kernel: the kListFrags
-// fragments of each listed stage workgroup.
+// fragments of each overflowed stage workgroup.
 __global__ void __launch_bounds__(kBlock)
 list_mask_fallback_kernel(long long N, int B, int H, int W, const int* __restrict__ frag_pix,
                           const float* __restrict__ frag_attrs,
                           const int* __restrict__ prim2cluster, unsigned kmax, int S,
-                          int* __restrict__ bitset, const int* __restrict__ counters,
-                          const int* __restrict__ ovf_blocks) {
-    const int n = counters[kOvfBlocks];  // workgroup-uniform
-    for (int i = blockIdx.x; i < n; i += gridDim.x) {
-        const long long f0 = static_cast<long long>(ovf_blocks[i]) * kListFrags;
+                          int* __restrict__ bitset, long long blocks,
+                          const int* __restrict__ block_recs) {
+    for_overflowed(blocks, block_recs, [&](long long blk) {
+        const long long f0 = blk * kListFrags;
         const long long f1 = dmin(f0 + kListFrags, N);
         for (long long f = f0 + threadIdx.x; f < f1; f += kBlock)
             list_or_one(f, B, H, W, frag_pix, frag_attrs, prim2cluster, kmax, S, bitset);
-    }
+    });
 }
 
 // Binned form: one walk.  Each fragment sets its pixel's bit in the LDS
@@ -228,11 +247,10 @@ list_mask_fallback_kernel(long long N, int B, int H, int W, const int* __restric
 // kHash-slot LDS table once per change of key along a thread's fragments;
 // a fragment repeating a pixel and cluster of the workgroup sets no new
 // bit.  After the walk a block scan of the used slots gives each key its
-// record in the workgroup's fixed region, where its mask is copied; its
-// index among its tile's records comes from one global atomic on the
-// tile's counter.  A workgroup whose table overflows writes no record and
-// lists itself for the fallback kernel.  Cluster ids outside [0, kmax) set
-// no bit.
+// record in the workgroup's own region: (tile, cluster | cell << 16) and
+// the key's mask.  A workgroup whose table overflows writes no record and
+// marks its count -1 for the fallback kernel.  Cluster ids outside
+// [0, kmax) set no bit.
 
 // The fragment sources of the binned pass.  walk(emit) calls
 // emit(b, h, w, cluster) for every fragment of the workgroup's share of the
@@ -352,10 +370,7 @@ __device__ __forceinline__ void cell_stage(const Source& src, const TileGrid g, 
     });
     __syncthreads();
     if (full) {  // workgroup-uniform
-        if (threadIdx.x == 0) {
-            out.ovf_blocks[atomicAdd(&out.counters[kOvfBlocks], 1)] = static_cast<int>(blockIdx.x);
-            out.block_recs[blockIdx.x] = 0;
-        }
+        if (threadIdx.x == 0) out.block_recs[blockIdx.x] = -1;
         return;
     }
     constexpr int per_thread = kHash / kBlock;
@@ -435,8 +450,8 @@ extern "C" int s2c_dda_cluster_mask(int B, int H, int W, int V, const float* pos
 
 namespace {
 
-// The binned pass after its tile grid is chosen: workspace, counters,
-// stage (stage(plan) launches the stage kernel over plan.blocks
+// The binned pass after its tile grid is chosen: workspace, stage
+// (stage(plan) launches the stage kernel over plan.blocks
 // workgroups), resolve, and the fallback behind it (fallback(plan) launches
 // the kernel that ORs the fragments of the workgroups whose key table
 // overflowed).  Nothing waits for the device.
@@ -456,7 +471,6 @@ int binned_mask(BinPlan& plan, int H, int W, int num_slots, int* bitset, hipStre
         return r;
     plan.rec_mask = static_cast<uint4*>(mbuf);
     plan.runs = static_cast<int2*>(obuf);
-    if (int r = bin_clear(plan, s)) return r;
     stage(plan);
     S2C_LAUNCH_CHECK();
     if (int r = bin_resolve(plan, H, W, num_slots, bitset, s)) return r;
@@ -466,7 +480,8 @@ int binned_mask(BinPlan& plan, int H, int W, int num_slots, int* bitset, hipStre
 }
 
 unsigned fallback_grid(long long blocks) {
-    return static_cast<unsigned>(dmin(blocks, static_cast<long long>(kFallbackGrid)));
+    return static_cast<unsigned>(
+        dmin((blocks + kBlock - 1) / kBlock, static_cast<long long>(kFallbackGrid)));
 }
 
 }  // namespace
@@ -509,8 +524,8 @@ extern "C" int s2c_dda_cluster_mask_tiled(int B, int H, int W, int V, const floa
         },
         [&](const BinPlan& p) {
             dda_mask_fallback_kernel<<<fallback_grid(p.blocks), kBlock, 0, s>>>(
-                P, H, W, V, pos, E, edges, edge2cluster, num_slots, bitset, p.counters,
-                p.ovf_blocks);
+                P, H, W, V, pos, E, edges, edge2cluster, num_slots, bitset, p.blocks,
+                p.block_recs);
         });
 }
 
@@ -547,7 +562,7 @@ extern "C" int s2c_fragment_cluster_mask(int B, int H, int W, int num_frags, con
         [&](const BinPlan& p) {
             list_mask_fallback_kernel<<<fallback_grid(p.blocks), kBlock, 0, s>>>(
                 N, B, H, W, frag_pix, frag_attrs, prim2cluster, kmax, num_slots, bitset,
-                p.counters, p.ovf_blocks);
+                p.blocks, p.block_recs);
         });
 }
 
