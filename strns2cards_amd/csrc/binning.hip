@@ -237,9 +237,9 @@ cell_resolve_kernel(TileGrid g, const int4* __restrict__ order, const int2* __re
             const int ox = (cell & ((1 << cpl) - 1)) << cs;
             const unsigned bit = 1u << (c & 31u);
             const unsigned j = c >> 5;
-            const unsigned wbase = static_cast<unsigned>(word) << 5;
             do {
-                const unsigned p = wbase + static_cast<unsigned>(__ffs(v) - 1);  // pixel in the cell
+                const unsigned p =  // pixel in the cell
+                    cell_pixel(cs, static_cast<unsigned>(word), static_cast<unsigned>(__ffs(v) - 1));
                 v &= v - 1u;
                 // with one cell per tile the cell's pixel is the tile's
                 const unsigned l = ONE_CELL ? p

@@ -364,8 +364,10 @@ __device__ __forceinline__ void cell_stage(const Source& src, const TileGrid g, 
             row = slot < 0 ? -1 : slot << mwl;
         }
         if (row >= 0) {
-            const unsigned l = cell_local(g, h, w);
-            atomicOr(&mask[row + static_cast<int>(l >> 5)], 1u << (l & 31u));
+            int word;
+            unsigned bit;
+            cell_bit(g, h, w, word, bit);
+            atomicOr(&mask[row + word], 1u << bit);
         }
     });
     __syncthreads();

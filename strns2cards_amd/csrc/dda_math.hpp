@@ -43,35 +43,29 @@ __device__ __forceinline__ int dda_span(int H, int W, float4 p0, float4 p1, DdaS
     wrote = false;
     if (dx == 0.f && dy == 0.f) return 0;
     wrote = true;
-    if (fabsf(dx) >= fabsf(dy)) {
-        const float s = dy / dx;
-        sp.axis = 0.f;
-        sp.slope = s;
-        if (x0 > x1) {
-            float t = x0; x0 = x1; x1 = t;
-            t = y0; y0 = y1; y1 = t;
-        }
-        const float sx = dmax(floorf(x0), 0.f);
-        const float sy = y0 + s * (sx - x0);
-        const int i0 = f2i(sx);
-        const int i1 = dmin(f2i(floorf(x1)), W - 1) + 1;
-        sp.sx = sx; sp.sy = sy; sp.off0 = x0 - sx; sp.off1 = x1 - sx;
-        return dmax(i1 - i0, 0);
-    } else {
-        const float s = dx / dy;
-        sp.axis = 1.f;
-        sp.slope = s;
-        if (y0 > y1) {
-            float t = x0; x0 = x1; x1 = t;
-            t = y0; y0 = y1; y1 = t;
-        }
-        const float sy = dmax(floorf(y0), 0.f);
-        const float sx = x0 + s * (sy - y0);
-        const int i0 = f2i(sy);
-        const int i1 = dmin(f2i(floorf(y1)), H - 1) + 1;
-        sp.sx = sx; sp.sy = sy; sp.off0 = y0 - sy; sp.off1 = y1 - sy;
-        return dmax(i1 - i0, 0);
-    }
+    // dda.cu's two branches (x-major when |dx| >= |dy|, else y-major) as
+    // one sequence on the selected major / minor coordinates: the same
+    // operations on the same values, without the divergent branch.
+    const bool ym = !(fabsf(dx) >= fabsf(dy));  // y-major (also for NaN, as in dda.cu)
+    float a0 = ym ? y0 : x0, b0 = ym ? x0 : y0;   // major, minor of vertex 0
+    float a1 = ym ? y1 : x1, b1 = ym ? x1 : y1;
+    const float s = (ym ? dx : dy) / (ym ? dy : dx);
+    sp.axis = ym ? 1.f : 0.f;
+    sp.slope = s;
+    const bool flip = a0 > a1;  // start from the lower major coordinate
+    const float ta = a0;
+    a0 = flip ? a1 : a0;
+    b0 = flip ? b1 : b0;
+    a1 = flip ? ta : a1;
+    const float sa = dmax(floorf(a0), 0.f);
+    const float sb = b0 + s * (sa - a0);
+    const int i0 = f2i(sa);
+    const int i1 = dmin(f2i(floorf(a1)), (ym ? H : W) - 1) + 1;
+    sp.sx = ym ? sb : sa;
+    sp.sy = ym ? sa : sb;
+    sp.off0 = a0 - sa;
+    sp.off1 = a1 - sa;
+    return dmax(i1 - i0, 0);
 }
 
 // dda.cu:124-144.
@@ -165,7 +159,9 @@ __device__ __forceinline__ bool dda_pixel(int H, int W, const DdaSpan& sp, int k
     const float y = sp.sy + ddy;
     h = f2i(floorf(y));
     w = f2i(floorf(x));
-    return !(h < 0 || h >= H || w < 0 || w >= W);
+    // 0 <= h < H and 0 <= w < W, as one unsigned compare each
+    return static_cast<unsigned>(h) < static_cast<unsigned>(H) &&
+           static_cast<unsigned>(w) < static_cast<unsigned>(W);
 }
 
 // Survival of candidate k for a cluster mask: the pixel test, then the
