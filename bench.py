@@ -923,9 +923,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": traffic_from_profiles(),
-                     "kernel": "s2c_dda_cluster_mask entry point (fill_u32_kernel, "
-                               "dda_bin_stage_kernel, counter readback, scan_*, "
-                               "bin_order_kernel, tile_resolve_kernel)",
+                     "kernel": "s2c_dda_cluster_mask entry point (dda_cell_stage_kernel, "
+                               "bin_count/colsum/colprefix/scan/order/place kernels, "
+                               "cell_resolve_kernel, dda_mask_fallback_kernel)",
                      "avg_launch_ms": round(ms_launch, 4)},
         "cpu_baseline": cpu,
         "extra": {"card_raster_mask_ms": round(ms_tri, 4), "antialias_grad_ms": round(ms_aa, 4),
