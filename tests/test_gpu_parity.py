@@ -260,6 +260,25 @@ def test_dda_mask_workgroup_paths(dev, case):
     np.testing.assert_array_equal(got, want)
 
 
+@pytest.mark.parametrize("tile_log2", [2, 3])
+def test_dda_mask_many_tiles(dev, tile_log2):
+    """Screen tiles of 4 x 4 and 8 x 8 pixels at 1040 x 1024: 66,560 and
+    16,640 tiles, more than the counting sort's LDS counters hold at once
+    (16,384), so its count and placement passes take several rounds; the
+    cells are the tiles, with 1- and 2-word masks."""
+    report(f"dda->mask many tiles, tile side {1 << tile_log2}")
+    rng = np.random.default_rng(61 + tile_log2)
+    res = (1040, 1024)
+    pos, edges = random_lines(rng, 1, 3000, 6000, *res, persp=True)
+    seg_pos, seg_edges = _segments(rng, 4096, 2.0, 40.0, 1000, *res)
+    pos = np.concatenate([pos, seg_pos], 1)
+    edges = np.concatenate([edges, seg_edges + 3000])
+    e2c = rng.integers(0, 32, size=edges.shape[0]).astype(np.int32)
+    want = oracle.dda_cluster_mask(res, pos, edges, e2c, 32, threads=8)
+    got = fused_dda_mask(dev, res, pos, edges, e2c, 32, tile_log2=tile_log2)
+    np.testing.assert_array_equal(got, want)
+
+
 def test_dda_mask_nonfinite_and_huge_vertices(dev):
     """Vertices at +-inf, NaN, 1e30 and w = 0 beside ordinary ones.  An
     infinite x or y makes dda.cu:124-144's t1 NaN, so those fragments fail
