@@ -261,7 +261,8 @@ int s2c_dda_cluster_mask(int B, int H, int W, int V, const float* pos, int E,
 /* The same with the screen-tile side of the binning fixed to
  * 1 << tile_log2 (2..6; 0 = chosen automatically, as above).  The result
  * does not depend on it; a side whose tile bitset (side^2 * num_slots
- * words) exceeds 128 KB of LDS is an S2C_ERR_ARG. */
+ * words) exceeds 128 KB of LDS, or whose (cell, cluster) keys need more
+ * than 31 bits, is an S2C_ERR_ARG. */
 int s2c_dda_cluster_mask_tiled(int B, int H, int W, int V, const float* pos, int E,
                                const int* edges, const int* edge2cluster, int num_slots,
                                int tile_log2, int* bitset, void* stream);

@@ -504,7 +504,8 @@ extern "C" int s2c_dda_cluster_mask_tiled(int B, int H, int W, int V, const floa
     plan.blocks = (P + kPrimsPerBlock - 1) / kPrimsPerBlock;
     const bool tiled = P > 0 && choose_tiles(B, H, W, num_slots, 4096, plan.g, tile_log2);
     S2C_CHECK(tiled || tile_log2 == 0 || P == 0,
-              "dda_cluster_mask: the requested tile bitset does not fit LDS");
+              "dda_cluster_mask: the requested tile side does not fit this shape (its tile "
+              "bitset exceeds LDS, or the (cell, cluster) keys exceed 31 bits)");
     if (!tiled) {
         // Empty input, a tile bitset too large for LDS, or (cell, cluster)
         // keys beyond 31 bits: clear + direct ORs.
