@@ -112,9 +112,12 @@ inline bool choose_tiles(int B, int H, int W, int S, long long min_tiles, TileGr
 
 // ------------------------------------------------------------ cell keys
 
+// ((b * cy + h / cs) << col_bits | w / cs) << kb | c.  The fields do not
+// overlap, so the key is a sum whose first term is fixed along an edge.
 __device__ __forceinline__ int cell_key(const TileGrid& g, int b, int h, int w, int c) {
-    const int row = b * g.cy + (h >> g.cs_log2);
-    return (((row << g.col_bits) | (w >> g.cs_log2)) << g.kb) | c;
+    const int a = g.col_bits + g.kb;
+    const int base = ((b * g.cy) << a) | c;
+    return base + ((h >> g.cs_log2) << a) + ((w >> g.cs_log2) << g.kb);
 }
 
 // Word and bit of pixel (h, w) in its cell's mask.  A 16 x 16 cell spreads
@@ -123,15 +126,16 @@ __device__ __forceinline__ int cell_key(const TileGrid& g, int b, int h, int w, 
 // several words: the resolve takes the bits of a word one at a time, each
 // wave as long as its fullest word, and this cuts those iterations by 43%
 // against row pairs per word at the full workload.  Smaller cells are
-// row-major.
+// row-major.  For 16 x 16 cells (x ^ y) & 7 == (w ^ h) & 7 and the bit is
+// (2h + ((w >> 3) & 1)) mod 32, so neither needs x or y.
 __device__ __forceinline__ void cell_bit(const TileGrid& g, int h, int w, int& word,
                                          unsigned& bit) {
-    const int m = (1 << g.cs_log2) - 1;
-    const int y = h & m, x = w & m;
     if (g.cs_log2 == kCellLog2Max) {
-        word = (x ^ y) & 7;
-        bit = static_cast<unsigned>((y << 1) | (x >> 3));
+        word = (w ^ h) & 7;
+        bit = ((static_cast<unsigned>(h) << 1) | ((static_cast<unsigned>(w) >> 3) & 1u)) & 31u;
     } else {
+        const int m = (1 << g.cs_log2) - 1;
+        const int y = h & m, x = w & m;
         const unsigned l = static_cast<unsigned>((y << g.cs_log2) | x);
         word = static_cast<int>(l >> 5);
         bit = l & 31u;
@@ -159,8 +163,14 @@ __device__ __forceinline__ void cell_tile(const TileGrid& g, int key, int& tile,
 
 // ------------------------------------------------------------ LDS hash
 
+// Fibonacci hashing of the key's low 24 bits: the top kHashLog2 bits of
+// key * floor(2^24 / golden ratio) mod 2^24, with a 24-bit multiply (full
+// rate, where a 32-bit multiply is quarter rate).  Keys that differ only
+// above bit 23 (other views of one workgroup) share a home slot and are
+// told apart by the probes.
 __device__ __forceinline__ int hash_home(int key) {
-    return static_cast<int>((static_cast<unsigned>(key) * 2654435761u) >> (32 - kHashLog2));
+    const unsigned p = __umul24(static_cast<unsigned>(key), 0x9E3779u);
+    return static_cast<int>((p >> (24 - kHashLog2)) & (kHash - 1));
 }
 
 // Slot holding `key` (inserted if absent), or -1 after kHashProbes probes.

@@ -353,7 +353,8 @@ __device__ __forceinline__ void cell_stage(const Source& src, const TileGrid g, 
         full = 0;
     }
     __syncthreads();
-    int cur = -1, row = -1;  // current key and its mask's first word (-1: none)
+    // current key and the byte offset of its mask in `mask` (-1: none)
+    int cur = -1, row = -1;
     src.walk([&](int b, int h, int w, int c) {
         if (static_cast<unsigned>(c) >= kmax) return;
         const int key = cell_key(g, b, h, w, c);
@@ -361,13 +362,14 @@ __device__ __forceinline__ void cell_stage(const Source& src, const TileGrid g, 
             cur = key;
             const int slot = full ? -1 : hash_insert(keys, key);
             if (slot < 0) full = 1;
-            row = slot < 0 ? -1 : slot << mwl;
+            row = slot < 0 ? -1 : slot << (mwl + 2);
         }
         if (row >= 0) {
             int word;
             unsigned bit;
             cell_bit(g, h, w, word, bit);
-            atomicOr(&mask[row + word], 1u << bit);
+            atomicOr(reinterpret_cast<unsigned*>(reinterpret_cast<char*>(mask) + row) + word,
+                     1u << bit);
         }
     });
     __syncthreads();
