@@ -166,8 +166,8 @@ __device__ __forceinline__ void cell_tile(const TileGrid& g, int key, int& tile,
 // Fibonacci hashing of the key's low 24 bits: the top kHashLog2 bits of
 // key * floor(2^24 / golden ratio) mod 2^24, with a 24-bit multiply (full
 // rate, where a 32-bit multiply is quarter rate).  Keys that differ only
-// above bit 23 (other views of one workgroup) share a home slot and are
-// told apart by the probes.
+// above bit 23 (in the high bits of the cell row, such as one cell seen in
+// two views) share a home slot and are told apart by the probes.
 __device__ __forceinline__ int hash_home(int key) {
     const unsigned p = __umul24(static_cast<unsigned>(key), 0x9E3779u);
     return static_cast<int>((p >> (24 - kHashLog2)) & (kHash - 1));
