@@ -279,6 +279,33 @@ def test_dda_mask_many_tiles(dev, tile_log2):
     np.testing.assert_array_equal(got, want)
 
 
+def test_dda_mask_keys_beyond_24_bits(dev):
+    """Ten views of the same 20 short segments (one workgroup of 200 edges)
+    at 64 x 64 with 16,384 clusters: 4 x 4 cells, 14 cluster bits, so a key
+    is ((b * 16 + cell row) << 4 | cell column) << 14 | cluster and the keys
+    of views b and b + 4 differ only above bit 23.  The key table's hash
+    reads the low 24 bits, so those keys share their home slot and are
+    placed by probing; the mask equals the oracle's."""
+    report("dda->mask keys that differ only above bit 23")
+    rng = np.random.default_rng(73)
+    res, K, B = (64, 64), 16384, 10
+    pos1, edges = _segments(rng, 20, 1.5, 3.0, 60, *res)
+    pos = np.repeat(pos1, B, axis=0)
+    e2c = rng.integers(0, K, size=edges.shape[0]).astype(np.int32)
+    fp, fa = oracle.dda_compute_fragments(res, pos, edges)
+    eid = fa[:, 3].astype(np.int64) - 1
+    ok = eid >= 0
+    key = ((((fp[ok, 0].astype(np.int64) * 16 + fp[ok, 1] // 4) << 4) | (fp[ok, 2] // 4)) << 14) \
+        | e2c[eid[ok] % edges.shape[0]]
+    key = np.unique(key)
+    low, n = np.unique(key & 0xFFFFFF, return_counts=True)
+    assert key.size <= 400 and key.max() >= 1 << 24, key.size
+    assert (n >= 2).sum() * 3 >= low.size, (n >= 2).sum()  # most home slots shared
+    want = oracle.dda_cluster_mask(res, pos, edges, e2c, K, threads=8)
+    got = fused_dda_mask(dev, res, pos, edges, e2c, K)
+    np.testing.assert_array_equal(got, want)
+
+
 def test_dda_mask_nonfinite_and_huge_vertices(dev):
     """Vertices at +-inf, NaN, 1e30 and w = 0 beside ordinary ones.  An
     infinite x or y makes dda.cu:124-144's t1 NaN, so those fragments fail
