@@ -431,6 +431,27 @@ def test_fragment_mask_table_overflow_twice(dev):
         np.testing.assert_array_equal(list_mask(dev, B, res, fp, fa, p2c, K), want)
 
 
+def test_fragment_mask_keys_beyond_24_bits(dev):
+    """The fragment-list form of test_dda_mask_keys_beyond_24_bits: the ten
+    views' fragments interleaved (fragment i of each view in turn), all in
+    one 4,096-fragment workgroup, so keys that differ only above bit 23
+    share home slots of its table; the mask equals the oracle's."""
+    report("fragment-list mask keys that differ only above bit 23")
+    rng = np.random.default_rng(73)
+    res, K, B = (64, 64), 16384, 10
+    pos1, edges = _segments(rng, 20, 1.5, 3.0, 60, *res)
+    pos = np.repeat(pos1, B, axis=0)
+    e2c = rng.integers(0, K, size=edges.shape[0]).astype(np.int32)
+    fp, fa = oracle.dda_compute_fragments(res, pos, edges)
+    n = fp.shape[0]
+    assert n % B == 0 and n <= 4096, n  # the same fragments in every view
+    fp = np.ascontiguousarray(fp.reshape(B, n // B, 3).transpose(1, 0, 2).reshape(n, 3))
+    fa = np.ascontiguousarray(fa.reshape(B, n // B, 4).transpose(1, 0, 2).reshape(n, 4))
+    assert (fp[:B, 0] == np.arange(B)).all()
+    want = oracle.cluster_mask_from_fragments(res, fp, fa, e2c, K, B=B)
+    np.testing.assert_array_equal(list_mask(dev, B, res, fp, fa, e2c, K), want)
+
+
 @pytest.mark.slow
 def test_dda_mask_full_workload(dev):
     report("dda->mask full workload (50k strands x 100, 4 views 1024^2, K=1024)")
