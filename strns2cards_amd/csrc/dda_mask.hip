@@ -55,8 +55,11 @@ __device__ __forceinline__ void walk_edges(long long P, int H, int W, int V,
                                            long long blk = blockIdx.x) {
     const long long q0 = blk * kPrimsPerBlock +
                          static_cast<long long>(threadIdx.x) * kPrimsPerThread;
-    // All loads of the thread's edges are issued before any is used (one
-    // memory round trip for kPrimsPerThread edges instead of one per edge).
+    // The vertex and cluster loads of all kPrimsPerThread edges are issued
+    // before the first walk, so the walk of an edge overlaps the later
+    // edges' loads.  (Each edge load waits inside its own branch; issuing
+    // them unconditionally, all in flight at once, raised the VGPRs past
+    // 72 and measured no faster.)
     int b0 = 0, e0 = 0;
     if (q0 < P) {
         b0 = static_cast<int>(q0 / E);
