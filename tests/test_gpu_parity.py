@@ -844,3 +844,45 @@ def test_smooth_and_skinning(dev):
     call("s2c_skinning", 300, 40, T(x, dev), T(s, dev), T(h, dev), T(cn, dev), T(R, dev), out, 0.01)
     want, ab = oracle.skinning(x, s, h, cn, R, 0.01, True)
     elementwise_close(out.cpu().numpy(), want, ab, k=8, what="skinning")
+
+
+def test_smooth_and_skinning_edge_cases(dev):
+    """smooth_strands / skinning (curve_utils.cu:135-166) where the weights
+    are special: strands of 1 and 2 points; repeated points (zero arc-length
+    steps, so weight 1 between distinct points); beta = 10, where every
+    point of a strand weighs about 1; beta = 1e-4, where each point keeps
+    little but its repeats; and beta = 0, where the weights are
+    exp(-(0 * inf)) = NaN and the weight-sum test fails, so smoothing leaves
+    the zero-filled output as it is and skinning keeps each point; beside a
+    4,096 x 100 batch."""
+    report("smooth_strands / skinning edge cases")
+    rng = np.random.default_rng(29)
+    for N, P in ((5, 1), (7, 2), (64, 40), (4096, 100)):
+        x = np.cumsum(rng.normal(scale=0.004, size=(N, P, 3)), 1).astype(np.float32)
+        if P == 40:
+            x[:, 10:15] = x[:, 10:11]  # five equal points: zero arc-length steps
+            x[:, 30:32] = x[:, 29:30]
+        s = oracle.compute_arclength(x)
+        h = x + rng.normal(scale=1e-3, size=x.shape).astype(np.float32)
+        cn = x + rng.normal(scale=1e-3, size=x.shape).astype(np.float32)
+        q, _ = np.linalg.qr(rng.normal(size=(N, P, 3, 3)))
+        R = np.ascontiguousarray(q.astype(np.float32))
+        for beta in ((0.0, 1e-4, 0.01, 10.0) if N < 4096 else (0.01,)):
+            what = f"N={N} P={P} beta={beta}"
+            out = torch.zeros(x.shape, device=dev)
+            call("s2c_smooth_strands", N, P, T(x, dev), T(s, dev), out, beta)
+            want, ab = oracle.smooth_strands(x, s, beta, True)
+            if beta == 0.0:
+                assert not want.any()
+                np.testing.assert_array_equal(out.cpu().numpy(), want, err_msg=what)
+            else:
+                elementwise_close(out.cpu().numpy(), want, ab, k=8, what="smooth " + what)
+            out = torch.zeros(x.shape, device=dev)
+            call("s2c_skinning", N, P, T(x, dev), T(s, dev), T(h, dev), T(cn, dev), T(R, dev),
+                 out, beta)
+            want, ab = oracle.skinning(x, s, h, cn, R, beta, True)
+            if beta == 0.0:  # skinning keeps the point itself where the sum test fails
+                np.testing.assert_array_equal(want, x)
+                np.testing.assert_array_equal(out.cpu().numpy(), want, err_msg=what)
+            else:
+                elementwise_close(out.cpu().numpy(), want, ab, k=8, what="skinning " + what)
